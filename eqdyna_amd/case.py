@@ -1,0 +1,372 @@
+"""Case description and input-file writer for EQdyna-style rupture cases.
+
+A ``Case`` holds the same parameters as the reference's ``par`` object
+(scripts/defaultParameters.py:8-256, defaults taken from there) and writes the
+fixed-order text inputs the reference's Fortran host reads
+(formats: scripts/case.setup:10-76, readers: src/readInputFiles.f90:29-186).
+
+On-fault fields are held as ``ofv[k, iz, ix]`` -- the reference's
+``par.on_fault_vars[iz, ix, k]`` with the slot index first -- on the
+(dip, strike) grid of the fault.  The 24 fields the reference reads
+(src/netcdf_io.f90:41-64) are written as one raw little-endian float64 file
+``on_fault_vars_input.bin`` whose Fortran shape is (nfx, nfz, 24).
+"""
+from __future__ import annotations
+
+import copy
+import math
+import os
+from dataclasses import dataclass, field
+from typing import List, Optional
+
+import numpy as np
+
+# on_fault_vars slots in the order of the 24 netCDF variables
+# (scripts/case.setup:163-186 write them; src/netcdf_io.f90:41-64 read them).
+ON_FAULT_SLOTS = [1, 2, 3, 9, 10, 11, 12, 13, 14, 15, 16, 17, 18, 19, 40, 41, 42,
+                  46, 8, 7, 20, 5, 4, 49]
+
+
+def _s(v) -> str:
+    """Python's str(), which is what scripts/case.setup writes."""
+    return str(v)
+
+
+@dataclass
+class Case:
+    # switches (scripts/defaultParameters.py:11-12, 74-106)
+    mode: int = 1
+    C_elastic: int = 1
+    C_nuclea: int = 1
+    C_degen: float = 0
+    insertFaultType: int = 0
+    friclaw: int = 5
+    ntotft: int = 1
+    nucfault: int = 1
+    tpv: int = 105
+    output_plastic: int = 0
+    outputGroundMotion: int = 0
+    outputFinalSurfDisp: int = 0
+    nx: int = 1  # MPI ranks along x, y, z
+    ny: int = 1
+    nz: int = 1
+    term: float = 5.0
+    dt: float = 0.5 * 500.0 / 6.0e3
+    nmat: int = 1
+    n2mat: int = 3
+    roumax: float = 2.67e3
+    rhow: float = 1.0e3
+    gamar: float = 0.66
+    rdampk: float = 0.1
+    vmaxPML: float = 6.0e3
+    xsource: float = -4.0e3
+    ysource: float = 0.0
+    zsource: float = -7.5e3
+    nucR: float = 1.5e3
+    nucRuptVel: float = -9999.0
+    nucdtau0: float = 50.0e6
+    nucT: float = 1.0
+    str1ToFaultAngle: float = 45.0
+    devStrToStrVertRatio: float = 0.33
+    bulk: float = 0.75
+    coheplas: float = 5.0e6
+    strike: float = 270.0
+    dip: float = 90.0
+    slipRateThres: float = 0.001
+    # model geometry (scripts/defaultParameters.py:15-41)
+    xmin: float = -42.0e3
+    xmax: float = 42.0e3
+    ymin: float = -20.0e3
+    ymax: float = 22.0e3
+    zmin: float = -42.0e3
+    zmax: float = 0.0
+    nuni_y_minus: int = 5
+    nuni_y_plus: int = 5
+    enlarging_ratio: float = 1.025
+    dx: float = 500.0
+    dy: float = 500.0
+    dz: float = 500.0
+    fxmin: float = -22.0e3
+    fxmax: float = 22.0e3
+    fymin: float = 0.0
+    fymax: float = 0.0
+    fzmin: float = -22.0e3
+    fzmax: float = 0.0
+    # material: (vp, vs, rho) or rows of (bottom depth, vp, vs, rho)
+    vp: float = 6.0e3
+    vs: float = 3.464e3
+    rou: float = 2.67e3
+    mat: Optional[np.ndarray] = None
+    # stations: on-fault (x, z) km, off-fault (x, y, z) km
+    st_on_fault: List[List[float]] = field(default_factory=list)
+    st_off_fault: List[List[float]] = field(default_factory=list)
+    # on-fault fields, ofv[k, iz, ix]
+    ofv: Optional[np.ndarray] = None
+    # module knobs of the reference (src/globalvar.f90:16-75)
+    nPML: int = 6
+    R: float = 0.01
+    C_Q: int = 0
+    C_hg: int = 1
+    rdampm: float = 0.0
+    kapa_hg: float = 0.1
+    name: str = "case"
+
+    # ------------------------------------------------------------- geometry
+    @property
+    def nfx(self) -> int:
+        return int(round((self.fxmax - self.fxmin) / self.dx + 1))
+
+    @property
+    def nfz(self) -> int:
+        return int(round((self.fzmax - self.fzmin) / self.dz + 1))
+
+    @property
+    def fx(self) -> np.ndarray:
+        return np.linspace(self.fxmin, self.fxmax, self.nfx)
+
+    @property
+    def fz(self) -> np.ndarray:
+        return np.linspace(self.fzmin, self.fzmax, self.nfz)
+
+    @property
+    def nstep(self) -> int:
+        # src/readInputFiles.f90:184: nstep = idnint(totalSimuTime/dt)
+        return int(np.rint(self.term / self.dt))
+
+    def new_ofv(self) -> np.ndarray:
+        self.ofv = np.zeros((100, self.nfz, self.nfx))
+        return self.ofv
+
+    def copy(self) -> "Case":
+        return copy.deepcopy(self)
+
+    # ------------------------------------------------------------- writers
+    def write_inputs(self, d: str, dump: bool = False) -> None:
+        """Write bGlobal/bModelGeometry/bFaultGeometry/bMaterial/bStations.txt,
+        on_fault_vars_input.bin and bKnobs.txt into directory ``d``."""
+        os.makedirs(d, exist_ok=True)
+        p = self
+        g = [p.mode, p.C_elastic, p.C_nuclea, p.C_degen, p.insertFaultType, p.friclaw,
+             p.ntotft, p.nucfault, p.tpv, p.output_plastic, p.outputGroundMotion,
+             p.outputFinalSurfDisp]
+        lines = [_s(v) for v in g]
+        lines += [" ", f"{p.nx} {p.ny} {p.nz}", " ", _s(p.term), _s(p.dt), " ",
+                  f"{p.nmat} {p.n2mat}", f"{_s(p.roumax)} {_s(p.rhow)} {_s(p.gamar)}",
+                  f"{_s(p.rdampk)} {_s(p.vmaxPML)}", " ",
+                  f"{_s(p.xsource)} {_s(p.ysource)} {_s(p.zsource)}",
+                  f"{_s(p.nucR)} {_s(p.nucRuptVel)} {_s(p.nucdtau0)} {_s(p.nucT)}",
+                  f"{_s(p.str1ToFaultAngle)} {_s(p.devStrToStrVertRatio)}",
+                  f"{_s(p.bulk)} {_s(p.coheplas)}", f"{_s(p.strike)} {_s(p.dip)}",
+                  _s(p.slipRateThres)]
+        with open(os.path.join(d, "bGlobal.txt"), "w") as f:
+            f.write("\n".join(lines) + "\n")
+        with open(os.path.join(d, "bModelGeometry.txt"), "w") as f:
+            f.write(f"{_s(p.xmin)}  {_s(p.xmax)}\n{_s(p.ymin)}  {_s(p.ymax)}\n"
+                    f"{_s(p.zmin)}  {_s(p.zmax)}\n \n{p.nuni_y_minus} {p.nuni_y_plus}\n"
+                    f"{_s(p.enlarging_ratio)}\n{_s(p.dx)} {_s(p.dy)} {_s(p.dz)}\n")
+        with open(os.path.join(d, "bFaultGeometry.txt"), "w") as f:
+            for i in range(p.ntotft):
+                f.write(f"For fault No. {i + 1}\n{_s(p.fxmin)}  {_s(p.fxmax)}\n"
+                        f"{_s(p.fymin)}  {_s(p.fymax)}\n{_s(p.fzmin)}  {_s(p.fzmax)}\n")
+        with open(os.path.join(d, "bMaterial.txt"), "w") as f:
+            if p.nmat == 1:
+                f.write(f"{_s(p.vp)} {_s(p.vs)} {_s(p.rou)}\n")
+            else:
+                for i in range(p.nmat):
+                    f.write(" ".join(_s(float(v)) for v in p.mat[i, :4]) + "\n")
+        with open(os.path.join(d, "bStations.txt"), "w") as f:
+            f.write(f"{len(p.st_off_fault)}\n{len(p.st_on_fault)}\n \n")
+            for s in p.st_on_fault:
+                f.write(f"{_s(s[0])} {_s(s[1])}\n")
+            f.write(" \n")
+            for s in p.st_off_fault:
+                f.write(f"{_s(s[0])} {_s(s[1])} {_s(s[2])}\n")
+        if p.ofv is None:
+            raise ValueError("case has no on-fault fields (ofv)")
+        sel = np.ascontiguousarray(p.ofv[ON_FAULT_SLOTS], dtype="<f8")  # (24, nfz, nfx)
+        sel.tofile(os.path.join(d, "on_fault_vars_input.bin"))
+        with open(os.path.join(d, "bKnobs.txt"), "w") as f:
+            f.write(f"nPML {p.nPML}\nR {_s(float(p.R))}\nC_Q {p.C_Q}\nC_hg {p.C_hg}\n"
+                    f"rdampm {_s(float(p.rdampm))}\nkapa_hg {_s(float(p.kapa_hg))}\n"
+                    f"dump {1 if dump else 0}\n")
+
+
+# ---------------------------------------------------------------- helpers
+def state_steady_state(a, b, d0, v0, r0, shear, norm, slip_rate, friclaw):
+    """Steady-state RSF state variable, the closed form of scripts/lib.py:17-24."""
+    if friclaw == 3:
+        tmp = a * math.log(2.0 * math.sinh(abs(shear / norm / a))) - r0 - math.log(slip_rate / v0)
+        return d0 / v0 * math.exp(tmp / b)
+    return a * math.log(2.0 * v0 / slip_rate * math.sinh(abs(shear / norm / a)))
+
+
+def shear_steady_state(a, b, v0, r0, load_rate, norm, slip_rate):
+    """Steady-state shear stress of the ageing law, scripts/lib.py:11-14."""
+    return -norm * a * math.asinh(slip_rate / 2.0 / v0 * math.exp((r0 + b * math.log(v0 / load_rate)) / a))
+
+
+def boxcar_b1(x, ww, w):
+    """Smooth boxcar B1 of TPV104/105 (scripts/lib.py:27-34)."""
+    ax = abs(x)
+    if ax <= ww:
+        return 1.0
+    if ax < ww + w:
+        return 0.5 * (1.0 + math.tanh(w / (ax - ww - w) + w / (ax - ww)))
+    return 0.0
+
+
+# ---------------------------------------------------------------- presets
+def tpv8() -> Case:
+    """SCEC TPV8 as configured by case_input/test.tpv8/user_defined_params.py
+    (the case behind test.reference.results/test.tpv8).  roumax and vmaxPML keep
+    the class defaults of scripts/defaultParameters.py:46-48 because that file
+    evaluates them before the case overrides vp/rou."""
+    c = Case(name="tpv8")
+    c.xmin, c.xmax = -22.0e3, 22.0e3
+    c.ymin, c.ymax = -10.0e3, 12.0e3
+    c.zmin, c.zmax = -22.0e3, 0.0e3
+    c.fxmin, c.fxmax = -15.0e3, 15.0e3
+    c.fymin, c.fymax = 0.0, 0.0
+    c.fzmin, c.fzmax = -15.0e3, 0.0e3
+    c.xsource, c.ysource, c.zsource = 0.0, 0.0, -12.0e3
+    c.dx = c.dy = c.dz = 500.0
+    c.nmat = 1
+    c.vp, c.vs, c.rou = 5716, 3300, 2700
+    c.term = 5.0
+    c.dt = 0.5 * c.dx / c.vp
+    c.friclaw = 1
+    c.tpv = 8
+    c.nucR = 1.5e3
+    c.nx, c.ny, c.nz = 2, 2, 1
+    o = c.new_ofv()
+    fs, fd, d0, coh = 0.76, 0.448, 0.5, 1.0e6
+    for ix, x in enumerate(c.fx):
+        for iz, z in enumerate(c.fz):
+            o[1, iz, ix] = fs
+            if abs(abs(x) - c.fxmax) < 0.01 or abs(z - c.fzmin) < 0.01:
+                o[1, iz, ix] = 1000.0
+            o[2, iz, ix] = fd
+            o[3, iz, ix] = d0
+            o[4, iz, ix] = coh
+            o[7, iz, ix] = 7378.0 * z
+            o[8, iz, ix] = abs(0.55 * o[7, iz, ix])
+            if abs(x - c.xsource) <= 1.5e3 and abs(z - c.zsource) <= 1.5e3:
+                o[8, iz, ix] = 1e6 + abs(1.005 * 0.76 * o[7, iz, ix])
+    c.st_on_fault = [[0.0, 0.0], [0.0, -4.5], [0.0, -7.5], [0.0, -12.0], [4.5, -7.5],
+                     [12.0, -7.5], [4.5, 0.0], [12.0, 0.0]]
+    c.st_off_fault = [[0, 1, 0], [0, -1, 0], [0, 2, 0], [0, -2, 0], [0, 3, 0], [0, -3, 0],
+                      [12, 6, 0], [12, -3, 0], [-12, 3, 0], [0, -0.5, -0.3], [0, 0.5, -0.3],
+                      [0, -1, -0.3], [0, 1, -0.3], [12, -3, -12], [12, 3, -12]]
+    return c
+
+
+def box_sw(nx_half: int = 6, nz: int = 6, ny_half: int = 3, d: float = 200.0,
+           npml: int = 3, steps: int = 40, friclaw: int = 1, layered: bool = False,
+           name: str = "box_sw") -> Case:
+    """Small homogeneous (or layered) box with a vertical split-node fault at
+    y=0, slip-weakening (friclaw 1) or time-weakening (friclaw 2) friction, an
+    over-stressed nucleation patch, PML on five faces and a free surface."""
+    c = Case(name=name)
+    c.dx = c.dy = c.dz = d
+    fx = nx_half * d
+    c.fxmin, c.fxmax = -fx, fx
+    c.fymin, c.fymax = 0.0, 0.0
+    c.fzmin, c.fzmax = -nz * d, 0.0
+    c.xmin, c.xmax = -fx - 0.5 * d, fx + 0.5 * d
+    c.ymin, c.ymax = -ny_half * d - 0.5 * d, ny_half * d + 0.5 * d
+    c.zmin, c.zmax = -nz * d - 0.5 * d, 0.0
+    c.nuni_y_minus = ny_half
+    c.nuni_y_plus = ny_half
+    c.enlarging_ratio = 1.0
+    c.nPML = npml
+    c.R = 0.01
+    c.vp, c.vs, c.rou = 6000.0, 3464.0, 2670.0
+    c.roumax, c.vmaxPML = 2670.0, 6000.0
+    if layered:
+        c.nmat, c.n2mat = 3, 4
+        c.mat = np.array([[0.4 * nz * d, 5000.0, 2887.0, 2500.0],
+                          [0.8 * nz * d, 5600.0, 3233.0, 2600.0],
+                          [1.0e6, 6000.0, 3464.0, 2670.0]])
+    c.dt = 0.25 * d / 6000.0
+    c.term = steps * c.dt
+    c.friclaw = friclaw
+    c.tpv = 8 if friclaw == 1 else 201
+    c.C_nuclea = 0 if friclaw == 1 else 1
+    c.xsource, c.ysource, c.zsource = 0.0, 0.0, -0.5 * nz * d
+    c.nucR = 2.5 * d
+    c.nucRuptVel = 1500.0
+    o = c.new_ofv()
+    for ix, x in enumerate(c.fx):
+        for iz, z in enumerate(c.fz):
+            o[1, iz, ix] = 0.60
+            if abs(abs(x) - c.fxmax) < 0.01 or abs(z - c.fzmin) < 0.01:
+                o[1, iz, ix] = 1.0e4
+            o[2, iz, ix] = 0.45
+            o[3, iz, ix] = 0.02
+            o[5, iz, ix] = 0.05
+            o[7, iz, ix] = -100.0e6
+            o[8, iz, ix] = 55.0e6
+            if abs(x - c.xsource) <= 1.5 * d and abs(z - c.zsource) <= 1.5 * d:
+                o[8, iz, ix] = 62.0e6
+    c.st_on_fault = [[0.0, c.zsource / 1e3], [2 * d / 1e3, c.zsource / 1e3], [0.0, 0.0]]
+    c.st_off_fault = [[0.0, 2 * d / 1e3, 0.0], [2 * d / 1e3, -2 * d / 1e3, 0.0]]
+    return c
+
+
+def box_rsf(friclaw: int = 4, nx_half: int = 6, nz: int = 6, ny_half: int = 3, d: float = 200.0,
+            npml: int = 3, steps: int = 60, a_lo: float = 0.008, a_hi: float = 0.016,
+            seed: int = 1, name: str = "box_rsf") -> Case:
+    """Small box with rate-and-state friction (ageing law 3 or slip law with
+    strong velocity weakening 4), TPV104-style smooth nucleation, and a seeded
+    spatially varying direct-effect parameter a in [a_lo, a_hi]."""
+    c = box_sw(nx_half, nz, ny_half, d, npml, steps, 1, False, name)
+    c.friclaw = friclaw
+    c.tpv = 104
+    c.C_nuclea = 1
+    c.nucR = 3.0 * d
+    c.nucT = 10.0 * c.dt
+    c.nucdtau0 = 40.0e6
+    rng = np.random.default_rng(seed)
+    o = c.new_ofv()
+    b, L, f0, v0, fw, vw = 0.012, 0.02, 0.6, 1.0e-6, 0.1, 0.1
+    vini = 1.0e-12 if friclaw != 3 else 1.0e-9
+    for ix, x in enumerate(c.fx):
+        for iz, z in enumerate(c.fz):
+            a = a_lo + (a_hi - a_lo) * rng.random()
+            o[1, iz, ix], o[2, iz, ix], o[3, iz, ix] = 0.6, 0.45, 0.3
+            o[7, iz, ix] = -60.0e6
+            o[8, iz, ix] = 24.0e6
+            if friclaw == 3:
+                o[8, iz, ix] = shear_steady_state(a, b, v0, f0, vini, o[7, iz, ix], vini)
+            o[9, iz, ix] = a
+            o[10, iz, ix] = b
+            o[11, iz, ix] = L
+            o[12, iz, ix] = v0
+            o[13, iz, ix] = f0
+            o[14, iz, ix] = fw
+            o[15, iz, ix] = vw
+            o[46, iz, ix] = vini
+            if friclaw == 3:
+                o[20, iz, ix] = L / vini  # ageing-law steady state theta = L/V
+            else:
+                o[20, iz, ix] = state_steady_state(a, b, L, v0, f0, o[8, iz, ix], o[7, iz, ix], vini, friclaw)
+    return c
+
+
+def box_plastic(steps: int = 40, friclaw: int = 1, name: str = "box_plastic") -> Case:
+    """Small box with Drucker-Prager viscoplasticity (C_elastic = 0): gravity,
+    lithostatic initial stress (src/meshgen.f90:976-996) and no on-fault
+    initial tractions, as the reference defines the plastic mode."""
+    c = box_sw(steps=steps, friclaw=friclaw, name=name)
+    c.C_elastic = 0
+    c.coheplas = 1.0e6
+    c.bulk = 0.75
+    c.str1ToFaultAngle = 45.0
+    c.devStrToStrVertRatio = 0.6
+    c.gamar = 0.66
+    c.rhow = 1.0e3
+    c.tpv = 8
+    o = c.ofv
+    o[1, :, :] = np.where(o[1] > 100.0, o[1], 0.30)
+    o[2, :, :] = 0.20
+    return c
