@@ -1,0 +1,1005 @@
+// C-ABI of the eqdyna_amd time step: takes the reference's globalvar arrays,
+// builds the device layout (see eqd_device.h), runs steps, returns state.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "../../include/eqdyna_amd.h"
+#include "eqd_device.h"
+
+namespace eqd {
+template <typename R>
+hipError_t launch_elem_regular(const RegElems<R> &, const NodeState &, const StepConsts &, bool, bool, int, bool,
+                               hipStream_t);
+hipError_t launch_elem_pml(const PmlElems &, const NodeState &, const StepConsts &, int, bool, hipStream_t);
+template <typename R>
+hipError_t launch_node3(const NodeState &, const int32_t *, int64_t, const R *, int64_t, const double *, int64_t,
+                        double, int, int, hipStream_t);
+hipError_t launch_node_pml(const NodeState &, const PmlNodes &, const double *, int64_t, double, int, int,
+                           hipStream_t);
+hipError_t launch_update_from_acc(const NodeState &, const int32_t *, int64_t, const PmlNodes &, double, int,
+                                  hipStream_t);
+template <typename R>
+hipError_t launch_fault(const FaultState &, const NodeState &, const StepConsts &, const R *, int64_t,
+                        const double *, int64_t, int, double, int, hipStream_t);
+hipError_t launch_station(const OffStations &, const NodeState &, int, double, hipStream_t);
+}  // namespace eqd
+
+using namespace eqd;
+
+namespace {
+
+struct FieldSpec {
+    bool is_int;
+    int64_t count;
+};
+
+struct KTimer {
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+    size_t used = 0;
+    double total_ms = 0;
+    int64_t launches = 0;
+};
+
+}  // namespace
+
+struct eqd_handle {
+    eqd_scalars s{};
+    std::string err;
+    std::map<int, std::vector<double>> fd;   // f64 fields as handed over
+    std::map<int, std::vector<int32_t>> fi;  // i32 fields
+    bool committed = false;
+    int32_t nt = 0;
+    double time = 0.0;
+    bool pending = true;  // an update from acc is due at the start of the next step
+    hipStream_t stream = nullptr;
+    std::vector<void *> allocs;
+    int64_t bytes = 0;
+    // host-side copies of the layout needed for downloads
+    std::vector<int32_t> reg_ids, pml_ids;     // element ids (0-based) per class
+    std::vector<int32_t> node3_all, node3_free, pml_node_ids;
+    std::vector<int32_t> pml_node_index;       // node -> PML node index or -1
+    int32_t n_reg = 0, n_pml = 0, nreg_stress = 6;
+    // device objects
+    NodeState N{};
+    RegElems<double> E64{};
+    RegElems<float> E32{};
+    PmlElems P{};
+    PmlNodes PN{};
+    FaultState F{};
+    OffStations S{};
+    StepConsts C{};
+    int32_t *d_node3_free = nullptr, *d_node3_all = nullptr;
+    bool timing = false;
+    std::map<std::string, KTimer> timers;
+};
+
+static int fail(eqd_handle *h, int code, const char *fmt, ...)
+{
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    if (h) h->err = buf;
+    return code;
+}
+
+#define HIPCHK(h, x)                                                                              \
+    do {                                                                                          \
+        hipError_t e_ = (x);                                                                      \
+        if (e_ != hipSuccess) return fail(h, EQD_EHIP, "%s: %s", #x, hipGetErrorString(e_));      \
+    } while (0)
+
+static bool field_spec(const eqd_scalars &s, int f, FieldSpec &o)
+{
+    const int64_t nn = s.nn, ne = s.ne, neq = s.neq, nf = s.nf;
+    const int64_t non = s.non > 0 ? s.non : 1;
+    switch (f) {
+    case EQD_MESHCOOR: o = {false, 3 * nn}; return true;
+    case EQD_EQNUMINDEXARR: o = {true, s.size_eqidx}; return true;
+    case EQD_EQNUMSTARTINDEXLOC: o = {true, nn}; return true;
+    case EQD_NUMOFDOFPERNODEARR: o = {true, nn}; return true;
+    case EQD_NODEELEMIDRELATION: o = {true, 8 * ne}; return true;
+    case EQD_ELEMTYPEARR: o = {true, ne}; return true;
+    case EQD_MAT: o = {false, 5 * ne}; return true;
+    case EQD_ELESHP: o = {false, 24 * ne}; return true;
+    case EQD_ELEDET: o = {false, ne}; return true;
+    case EQD_ELEMASS: o = {false, 24 * ne}; return true;
+    case EQD_SS: o = {false, 6 * ne}; return true;
+    case EQD_PHI: o = {false, 32 * ne}; return true;
+    case EQD_STRESSCOMPINDEXARR: o = {true, ne}; return true;
+    case EQD_ELEPOREP: o = {false, ne}; return true;
+    case EQD_NODALMASSARR: o = {false, neq}; return true;
+    case EQD_FNMS: o = {false, nn}; return true;
+    case EQD_NSMP: o = {true, 2 * nf}; return true;
+    case EQD_UN: case EQD_US: case EQD_UD: o = {false, 3 * nf}; return true;
+    case EQD_ARN: o = {false, nf}; return true;
+    case EQD_ANONFS: o = {true, 3 * (int64_t)s.non}; return true;
+    case EQD_IDHIST: o = {true, 3 * 6 * (int64_t)s.noff}; return true;
+    case EQD_STRESSARR: o = {false, s.size_stress}; return true;
+    case EQD_PSTRAIN: o = {false, ne}; return true;
+    case EQD_V1: o = {false, neq}; return true;
+    case EQD_VELARR: case EQD_DISPARR: o = {false, 3 * nn}; return true;
+    case EQD_NODALFORCEARR: o = {false, neq}; return true;
+    case EQD_FRIC: o = {false, 100 * nf}; return true;
+    case EQD_FNFT: o = {false, nf}; return true;
+    case EQD_ONFAULTHIST: o = {false, 12 * (int64_t)s.nstep * non}; return true;
+    case EQD_OFFFAULTHIST: o = {false, (6 * (int64_t)s.noff + 1) * s.nstep}; return true;
+    default: return false;
+    }
+}
+
+template <typename T>
+static int dalloc(eqd_handle *h, T **p, int64_t count)
+{
+    *p = nullptr;
+    if (count <= 0) return 0;
+    void *q = nullptr;
+    const size_t b = sizeof(T) * (size_t)count;
+    if (hipMalloc(&q, b) != hipSuccess) return fail(h, EQD_EOOM, "hipMalloc of %zu bytes failed", b);
+    h->allocs.push_back(q);
+    h->bytes += (int64_t)b;
+    *p = (T *)q;
+    return 0;
+}
+
+template <typename T>
+static int dupload(eqd_handle *h, T **p, const std::vector<T> &v)
+{
+    int rc = dalloc(h, p, (int64_t)v.size());
+    if (rc) return rc;
+    if (!v.empty()) HIPCHK(h, hipMemcpy(*p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
+    return 0;
+}
+
+// comdampv, src/comdampv.f90:1-120 (region 14 compares y with xmax0 as the
+// reference does).  Returns d_x, d_y, d_z.
+static void pml_node_damping(const eqd_scalars &s, double x2, double y, double z, double d[3])
+{
+    const double xmax0 = s.PMLb[0], xmin0 = s.PMLb[1], ymax0 = s.PMLb[2], ymin0 = s.PMLb[3], zmin0 = s.PMLb[4];
+    double damp[3];
+    damp[2] = z <= zmin0 ? std::fabs(z - zmin0) : 0.0;
+    if (x2 >= xmax0 && y >= ymax0) { damp[0] = std::fabs(x2 - xmax0); damp[1] = std::fabs(y - ymax0); }
+    else if (x2 >= xmax0 && y <= ymin0) { damp[0] = std::fabs(x2 - xmax0); damp[1] = std::fabs(y - ymin0); }
+    else if (x2 <= xmin0 && y <= ymin0) { damp[0] = std::fabs(x2 - xmin0); damp[1] = std::fabs(y - ymin0); }
+    else if (x2 <= xmin0 && y >= xmax0) { damp[0] = std::fabs(x2 - xmin0); damp[1] = std::fabs(y - ymax0); }
+    else if (x2 >= xmax0 && y > ymin0 && y < ymax0) { damp[0] = std::fabs(x2 - xmax0); damp[1] = 0.0; }
+    else if (y <= ymin0 && x2 > xmin0 && x2 < xmax0) { damp[0] = 0.0; damp[1] = std::fabs(y - ymin0); }
+    else if (x2 <= xmin0 && y > ymin0 && y < ymax0) { damp[0] = std::fabs(x2 - xmin0); damp[1] = 0.0; }
+    else if (y >= ymax0 && x2 > xmin0 && x2 < xmax0) { damp[0] = 0.0; damp[1] = std::fabs(y - ymax0); }
+    else { damp[0] = 0.0; damp[1] = 0.0; }
+    for (int i = 0; i < 3; i++) {
+        const double delta = (double)s.nPML * s.PMLb[5 + i];
+        d[i] = 3.0 * s.vmaxPML / 2.0 / delta * std::log(1.0 / s.R) * std::pow(damp[i] / delta, 2.0);
+    }
+}
+
+// element-centre damping of calcPMLElemKU, src/assembleGlobalKU.f90:130-213
+static void pml_elem_damping(const eqd_scalars &s, const double xc[3], double d[3])
+{
+    const double xmax2 = s.PMLb[0], xmin2 = s.PMLb[1], ymax2 = s.PMLb[2], ymin2 = s.PMLb[3], zmin2 = s.PMLb[4];
+    double damps[3] = {0.0, 0.0, 0.0};
+    if (xc[2] < zmin2 || xc[2] > zmin2) {
+        damps[2] = xc[2] < zmin2 ? std::fabs(xc[2] - zmin2) : 0.0;
+        if (xc[0] > xmax2 && xc[1] > ymax2) { damps[0] = std::fabs(xc[0] - xmax2); damps[1] = std::fabs(xc[1] - ymax2); }
+        else if (xc[0] > xmax2 && xc[1] < ymin2) { damps[0] = std::fabs(xc[0] - xmax2); damps[1] = std::fabs(xc[1] - ymin2); }
+        else if (xc[0] < xmin2 && xc[1] < ymin2) { damps[0] = std::fabs(xc[0] - xmin2); damps[1] = std::fabs(xc[1] - ymin2); }
+        else if (xc[0] < xmin2 && xc[1] > xmax2) { damps[0] = std::fabs(xc[0] - xmin2); damps[1] = std::fabs(xc[1] - ymax2); }
+        else if (xc[0] > xmax2 && xc[1] > ymin2 && xc[1] < ymax2) { damps[0] = std::fabs(xc[0] - xmax2); damps[1] = 0.0; }
+        else if (xc[1] < ymin2 && xc[0] > xmin2 && xc[0] < xmax2) { damps[0] = 0.0; damps[1] = std::fabs(xc[1] - ymin2); }
+        else if (xc[0] < xmin2 && xc[1] > ymin2 && xc[1] < ymax2) { damps[0] = std::fabs(xc[0] - xmin2); damps[1] = 0.0; }
+        else if (xc[1] > ymax2 && xc[0] > xmin2 && xc[0] < xmax2) { damps[0] = 0.0; damps[1] = std::fabs(xc[1] - ymax2); }
+    }
+    for (int i = 0; i < 3; i++) {
+        const double delta = (double)s.nPML * s.PMLb[5 + i];
+        d[i] = 3 * s.vmaxPML / 2 / delta * std::log(1 / s.R) * std::pow(damps[i] / delta, (double)2.0f);
+    }
+}
+
+// qconstant, src/qconstant.f90:3-34: the coefficient literals are default
+// REAL there, so each is the float value; x**(-0.5) is 1/sqrt(x).
+static void qconstant(double Q, int k, double pi, double *rtaok, double *rwk, double *c1)
+{
+    static const float taok[8] = {1.72333e-3f, 1.80701e-3f, 5.38887e-3f, 1.99322e-2f,
+                                  8.49833e-2f, 4.09335e-1f, 2.05951f, 13.2629f};
+    static const float alfk[8] = {1.66958e-2f, 3.81644e-2f, 9.84666e-3f, -1.36803e-2f,
+                                  -2.85125e-2f, -5.37309e-2f, -6.65035e-2f, -1.33696e-1f};
+    static const float betk[8] = {8.98758e-2f, 6.84635e-2f, 9.67052e-2f, 1.20172e-1f,
+                                  1.30728e-1f, 1.38746e-1f, 1.40705e-1f, 2.14647e-1f};
+    double kapa = (double)3.071f + (double)1.433f * std::pow(Q, (double)(-1.158f)) * std::log(Q / 5.0);
+    kapa = kapa / (1.0 + (double)0.415f * Q);
+    *rwk = kapa * (kapa * (double)alfk[k - 1] + (double)betk[k - 1]);
+    *rtaok = (double)taok[k - 1];
+    const double ref = 2.0 * pi;
+    const double ak0 = 1.0 - *rwk * 8.0 / (1.0 + (*rtaok * ref) * (*rtaok * ref));
+    const double bk0 = *rwk * 8.0 * ref * *rtaok / (1.0 + (*rtaok * ref) * (*rtaok * ref));
+    const double rs = 1.0 / std::sqrt(ak0 * ak0 + bk0 * bk0);
+    *c1 = (0.5 * rs) * (1.0 + ak0 * rs);
+}
+
+extern "C" {
+
+int eqd_create(const eqd_scalars *s, eqd_handle **out)
+{
+    if (!s || !out) return EQD_EINVAL;
+    *out = nullptr;
+    if (s->nn <= 0 || s->ne <= 0 || s->neq <= 0 || s->nf < 0 || s->non < 0 || s->noff < 0 || s->nstep <= 0)
+        return EQD_EINVAL;
+    if (s->precision != EQD_PREC_FP64 && s->precision != EQD_PREC_FP32) return EQD_EINVAL;
+    eqd_handle *h = new eqd_handle();
+    h->s = *s;
+    *out = h;
+    return EQD_OK;
+}
+
+int eqd_set(eqd_handle *h, int32_t field, const void *host, int64_t count)
+{
+    if (!h) return EQD_EINVAL;
+    FieldSpec sp;
+    if (!field_spec(h->s, field, sp)) return fail(h, EQD_EINVAL, "unknown field %d", field);
+    if (count != sp.count)
+        return fail(h, EQD_EINVAL, "field %d: count %lld, expected %lld", field, (long long)count, (long long)sp.count);
+    if (h->committed) return fail(h, EQD_ESTATE, "field %d: arrays are set before eqd_commit", field);
+    if (count > 0 && !host) return fail(h, EQD_EINVAL, "field %d: null pointer", field);
+    if (sp.is_int) {
+        auto &v = h->fi[field];
+        v.assign((const int32_t *)host, (const int32_t *)host + count);
+    } else {
+        auto &v = h->fd[field];
+        v.assign((const double *)host, (const double *)host + count);
+    }
+    return EQD_OK;
+}
+
+static const double *F64(eqd_handle *h, int f) { auto it = h->fd.find(f); return it == h->fd.end() ? nullptr : it->second.data(); }
+static const int32_t *I32(eqd_handle *h, int f) { auto it = h->fi.find(f); return it == h->fi.end() ? nullptr : it->second.data(); }
+
+int eqd_commit(eqd_handle *h)
+{
+    if (!h) return EQD_EINVAL;
+    if (h->committed) return fail(h, EQD_ESTATE, "already committed");
+    const eqd_scalars &s = h->s;
+    const int required[] = {EQD_MESHCOOR, EQD_EQNUMINDEXARR, EQD_EQNUMSTARTINDEXLOC, EQD_NUMOFDOFPERNODEARR,
+                            EQD_NODEELEMIDRELATION, EQD_ELEMTYPEARR, EQD_MAT, EQD_ELESHP, EQD_ELEDET, EQD_ELEMASS,
+                            EQD_SS, EQD_PHI, EQD_STRESSCOMPINDEXARR, EQD_ELEPOREP, EQD_NODALMASSARR, EQD_FNMS,
+                            EQD_NSMP, EQD_UN, EQD_US, EQD_UD, EQD_ARN, EQD_STRESSARR, EQD_PSTRAIN, EQD_V1,
+                            EQD_FRIC, EQD_FNFT};
+    for (int f : required) {
+        FieldSpec sp;
+        field_spec(s, f, sp);
+        const bool have = sp.is_int ? h->fi.count(f) : h->fd.count(f);
+        if (!have && sp.count > 0) return fail(h, EQD_EINVAL, "field %d was not set", f);
+    }
+    if (s.non > 0 && !h->fi.count(EQD_ANONFS)) return fail(h, EQD_EINVAL, "anonfs was not set");
+    if (s.noff > 0 && !h->fi.count(EQD_IDHIST)) return fail(h, EQD_EINVAL, "idhist was not set");
+    if (s.C_Q == 1 && s.C_elastic == 0) return fail(h, EQD_EINVAL, "Q model can only work with elastic code");
+    if (s.friclaw < 1 || s.friclaw > 4)
+        return fail(h, EQD_EINVAL, "friclaw %d is not supported (1-4; thermal pressurization is not built)", s.friclaw);
+    HIPCHK(h, hipSetDevice(s.device));
+    HIPCHK(h, hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+
+    const int64_t nn = s.nn, ne = s.ne;
+    const double *coor = F64(h, EQD_MESHCOOR);
+    const int32_t *eqi = I32(h, EQD_EQNUMINDEXARR), *esl = I32(h, EQD_EQNUMSTARTINDEXLOC),
+                  *ndof = I32(h, EQD_NUMOFDOFPERNODEARR), *ien = I32(h, EQD_NODEELEMIDRELATION),
+                  *etype = I32(h, EQD_ELEMTYPEARR), *sci = I32(h, EQD_STRESSCOMPINDEXARR), *nsmp = I32(h, EQD_NSMP);
+    const double *mat = F64(h, EQD_MAT), *shp = F64(h, EQD_ELESHP), *det = F64(h, EQD_ELEDET),
+                 *emass = F64(h, EQD_ELEMASS), *ss = F64(h, EQD_SS), *phi = F64(h, EQD_PHI),
+                 *porep = F64(h, EQD_ELEPOREP), *nmass = F64(h, EQD_NODALMASSARR), *fnms = F64(h, EQD_FNMS),
+                 *stress = F64(h, EQD_STRESSARR), *pstrain = F64(h, EQD_PSTRAIN), *v1 = F64(h, EQD_V1);
+    const double *vel0 = F64(h, EQD_VELARR), *disp0 = F64(h, EQD_DISPARR), *force0 = F64(h, EQD_NODALFORCEARR);
+
+    auto eq_of = [&](int64_t node, int j) -> int32_t {  // node 0-based, j 1-based
+        const int64_t idx = (int64_t)esl[node] + j;     // 1-based position in eqNumIndexArr
+        if (idx < 1 || idx > s.size_eqidx) return -2;
+        return eqi[idx - 1];
+    };
+
+    // ---------------- nodes
+    std::vector<double> mass(nn, 0.0);
+    std::vector<uint8_t> is_split(nn, 0);
+    for (int64_t i = 0; i < s.nf; i++) {
+        for (int j = 0; j < 2; j++) {
+            const int32_t nd = nsmp[i * 2 + j];
+            if (nd < 1 || nd > nn) return fail(h, EQD_EINVAL, "nsmp(%d,%lld) = %d out of range", j + 1, (long long)i + 1, nd);
+            if (ndof[nd - 1] != 3) return fail(h, EQD_EINVAL, "split node %d is not a 3-component node", nd);
+            is_split[nd - 1] = 1;
+        }
+    }
+    h->pml_node_index.assign(nn, -1);
+    std::vector<double> vel(3 * nn, 0.0), disp(3 * nn, 0.0), acc(3 * nn, 0.0);
+    std::vector<double> pdamp, pv12, pacc;
+    for (int64_t n = 0; n < nn; n++) {
+        for (int k = 0; k < 3; k++) {
+            vel[k * nn + n] = vel0 ? vel0[n * 3 + k] : 0.0;
+            disp[k * nn + n] = disp0 ? disp0[n * 3 + k] : 0.0;
+        }
+        if (ndof[n] == 3) {
+            for (int j = 1; j <= 3; j++) {
+                const int32_t e = eq_of(n, j);
+                if (e < 1 || e > s.neq)
+                    return fail(h, EQD_EINVAL, "3-component node %lld has equation %d (fixed 3-component nodes are not supported)",
+                                (long long)n + 1, e);
+                if (j == 1) mass[n] = nmass[e - 1];
+                else if (nmass[e - 1] != mass[n])
+                    return fail(h, EQD_EINVAL, "node %lld: unequal component masses", (long long)n + 1);
+                vel[(j - 1) * nn + n] = v1[e - 1];
+                acc[(j - 1) * nn + n] = force0 ? force0[e - 1] : 0.0;
+            }
+            h->node3_all.push_back((int32_t)n);
+            if (!is_split[n]) h->node3_free.push_back((int32_t)n);
+        } else if (ndof[n] == 12) {
+            const int32_t e1 = eq_of(n, 1);
+            if (e1 == -1) {
+                for (int j = 2; j <= 12; j++)
+                    if (eq_of(n, j) != -1) return fail(h, EQD_EINVAL, "node %lld: partly fixed PML node", (long long)n + 1);
+                continue;  // fixed boundary node: velocity and displacement stay as given
+            }
+            const int32_t q = (int32_t)h->pml_node_ids.size();
+            h->pml_node_index[n] = q;
+            h->pml_node_ids.push_back((int32_t)n);
+            for (int j = 1; j <= 12; j++) {
+                const int32_t e = eq_of(n, j);
+                if (e < 1 || e > s.neq) return fail(h, EQD_EINVAL, "node %lld: partly fixed PML node", (long long)n + 1);
+                if (j == 1) mass[n] = nmass[e - 1];
+                else if (nmass[e - 1] != mass[n])
+                    return fail(h, EQD_EINVAL, "node %lld: unequal component masses", (long long)n + 1);
+                pv12.push_back(v1[e - 1]);
+                pacc.push_back(force0 ? force0[e - 1] : 0.0);
+            }
+            double d[3];
+            pml_node_damping(s, coor[n * 3], coor[n * 3 + 1], coor[n * 3 + 2], d);
+            for (int k = 0; k < 3; k++) {
+                if (d[k] < 0.0) return fail(h, EQD_EINVAL, "wrong dv");
+                pdamp.push_back(d[k]);
+            }
+        } else {
+            return fail(h, EQD_EINVAL, "node %lld has %d components", (long long)n + 1, ndof[n]);
+        }
+    }
+    const int64_t npn = (int64_t)h->pml_node_ids.size();
+    // transpose per-PML-node arrays to component-major
+    auto tr = [](const std::vector<double> &a, int64_t cnt, int w) {
+        std::vector<double> b(a.size());
+        for (int64_t q = 0; q < cnt; q++)
+            for (int k = 0; k < w; k++) b[k * cnt + q] = a[q * w + k];
+        return b;
+    };
+    pv12 = tr(pv12, npn, 12);
+    pacc = tr(pacc, npn, 12);
+    pdamp = tr(pdamp, npn, 3);
+
+    // ---------------- elements
+    const int nst = s.C_Q == 1 ? 12 : 6;
+    h->nreg_stress = nst;
+    for (int64_t e = 0; e < ne; e++) {
+        const int t = etype[e];
+        if (t == 1 || t > 10) h->reg_ids.push_back((int32_t)e);
+        else if (t == 2) h->pml_ids.push_back((int32_t)e);
+        else return fail(h, EQD_EINVAL, "element %lld has type %d", (long long)e + 1, t);
+        for (int a = 0; a < 8; a++) {
+            const int32_t nd = ien[e * 8 + a];
+            if (nd < 1 || nd > nn) return fail(h, EQD_EINVAL, "element %lld node %d out of range", (long long)e + 1, nd);
+            if ((t == 1 || t > 10) && ndof[nd - 1] != 3)
+                return fail(h, EQD_EINVAL, "regular element %lld touches a PML node", (long long)e + 1);
+        }
+        const int need = t == 2 ? 21 : 12;
+        if (sci[e] < 0 || (int64_t)sci[e] + need > s.size_stress + (t == 2 ? 0 : 6))
+            return fail(h, EQD_EINVAL, "element %lld: stress index out of range", (long long)e + 1);
+    }
+    const int64_t nr = (int64_t)h->reg_ids.size(), np = (int64_t)h->pml_ids.size();
+    if (nr > (int64_t)kEntIdxMask || np > (int64_t)kEntIdxMask) return fail(h, EQD_EINVAL, "too many elements in one class");
+    h->n_reg = (int32_t)nr;
+    h->n_pml = (int32_t)np;
+    const bool fp32 = s.precision == EQD_PREC_FP32;
+
+    // regular class SoA
+    {
+        std::vector<int32_t> r_ien(8 * nr);
+        std::vector<double> r_shp(24 * nr), r_det(nr), r_lam(nr), r_mu(nr), r_vp(nr), r_rho(nr), r_st(nst * nr),
+            r_ss(6 * nr), r_phi(32 * nr), r_em(24 * nr), r_porep(nr), r_pstrain(nr);
+        std::vector<uint8_t> r_q(s.C_Q == 1 ? nr : 0);
+        for (int64_t t = 0; t < nr; t++) {
+            const int64_t e = h->reg_ids[t];
+            for (int a = 0; a < 8; a++) r_ien[a * nr + t] = ien[e * 8 + a] - 1;
+            for (int i = 0; i < 24; i++) {
+                r_shp[i * nr + t] = shp[e * 24 + i];
+                r_em[i * nr + t] = emass[e * 24 + i];
+            }
+            r_det[t] = det[e];
+            r_vp[t] = mat[0 * ne + e];
+            r_rho[t] = mat[2 * ne + e];
+            r_lam[t] = mat[3 * ne + e];
+            r_mu[t] = mat[4 * ne + e];
+            for (int i = 0; i < 6; i++) r_ss[i * nr + t] = ss[e * 6 + i];
+            for (int m = 0; m < 4; m++)
+                for (int a = 0; a < 8; a++) r_phi[(m * 8 + a) * nr + t] = phi[e * 32 + m * 8 + a];
+            for (int i = 0; i < nst; i++) {
+                const int64_t idx = (int64_t)sci[e] + i;
+                r_st[i * nr + t] = idx < s.size_stress ? stress[idx] : 0.0;
+            }
+            r_porep[t] = porep[e];
+            r_pstrain[t] = pstrain[e];
+            if (s.C_Q == 1) {
+                // Q law and relaxation mechanism of src/calcElemKU.f90:78-103
+                double xc[3] = {0, 0, 0};
+                for (int i = 0; i < 3; i++)
+                    for (int a = 0; a < 8; a++) xc[i] = xc[i] + coor[(int64_t)(ien[e * 8 + a] - 1) * 3 + i];
+                for (int i = 0; i < 3; i++) xc[i] = xc[i] / 8.0;
+                const int qsel = xc[2] > -1000.0 ? 0 : 1;
+                const int ip = (int)((xc[0] - (s.PMLb[1] + s.dx / 2)) / s.dx + 1);
+                const int iq = (int)((xc[1] - (s.PMLb[3] + s.dx / 2)) / s.dx + 1);
+                const int ir = (int)((xc[2] - (s.PMLb[4] + s.dx / 2)) / s.dx + 1);
+                const int k = 1 + ip % 2 + 2 * (iq % 2) + 4 * (ir % 2);
+                if (k < 1 || k > 8) return fail(h, EQD_EINVAL, "element %lld: Q mechanism %d", (long long)e + 1, k);
+                r_q[t] = (uint8_t)(qsel * 8 + (k - 1));
+            }
+        }
+        std::vector<double> qtab(16 * 6, 0.0);
+        if (s.C_Q == 1) {
+            for (int qsel = 0; qsel < 2; qsel++)
+                for (int k = 1; k <= 8; k++) {
+                    const double Qs = qsel == 0 ? 10.0 : 50.0, Qp = qsel == 0 ? 20.0 : 100.0;
+                    double taok, wkp, wks, cv, cs;
+                    qconstant(Qp, k, s.pi, &taok, &wkp, &cv);
+                    qconstant(Qs, k, s.pi, &taok, &wks, &cs);
+                    double *row = &qtab[(qsel * 8 + k - 1) * 6];
+                    row[0] = std::exp(-s.dt / taok);
+                    row[1] = 1 - std::exp(-s.dt / taok);
+                    row[2] = wkp * 8.0;
+                    row[3] = wks * 8.0;
+                    row[4] = cv;
+                    row[5] = cs;
+                }
+        }
+        int rc;
+        int32_t *d_ien;
+        if ((rc = dupload(h, &d_ien, r_ien))) return rc;
+        uint8_t *d_q = nullptr;
+        double *d_qtab = nullptr, *d_porep, *d_pstrain;
+        if ((rc = dupload(h, &d_q, r_q))) return rc;
+        if ((rc = dupload(h, &d_qtab, qtab))) return rc;
+        if ((rc = dupload(h, &d_porep, r_porep))) return rc;
+        if ((rc = dupload(h, &d_pstrain, r_pstrain))) return rc;
+        auto fill = [&](auto &E) -> int {
+            using R = std::remove_pointer_t<decltype(E.ef)>;
+            auto up = [&](const R **dst, const std::vector<double> &src) -> int {
+                std::vector<R> c(src.begin(), src.end());
+                R *p;
+                int r2 = dupload(h, &p, c);
+                *dst = p;
+                return r2;
+            };
+            int r2;
+            E.n = nr;
+            E.ien = d_ien;
+            if ((r2 = up(&E.shp, r_shp)) || (r2 = up(&E.det, r_det)) || (r2 = up(&E.lam, r_lam)) ||
+                (r2 = up(&E.mu, r_mu)) || (r2 = up(&E.vp, r_vp)) || (r2 = up(&E.rho, r_rho)) || (r2 = up(&E.ss, r_ss)) ||
+                (r2 = up(&E.phi, r_phi)))
+                return r2;
+            if (s.rdampm != 0.0 || s.C_elastic == 0) {
+                if ((r2 = up(&E.emass, r_em))) return r2;
+            }
+            std::vector<R> stc(r_st.begin(), r_st.end());
+            R *p;
+            if ((r2 = dupload(h, &p, stc))) return r2;
+            E.stress = p;
+            if ((r2 = dalloc(h, &p, 24 * nr))) return r2;
+            E.ef = p;
+            E.porep = d_porep;
+            E.pstrain = d_pstrain;
+            E.qidx = d_q;
+            E.qtab = d_qtab;
+            return 0;
+        };
+        if (fp32) {
+            if ((rc = fill(h->E32))) return rc;
+        } else {
+            if ((rc = fill(h->E64))) return rc;
+        }
+    }
+    // PML class SoA
+    {
+        std::vector<int32_t> p_ien(8 * np);
+        std::vector<double> p_shp(24 * np), p_det(np), p_lam(np), p_mu(np), p_vp(np), p_rho(np), p_st(21 * np),
+            p_ss(6 * np), p_phi(32 * np), p_em(24 * np), p_damp(3 * np);
+        std::vector<uint8_t> p_mask(np);
+        for (int64_t p = 0; p < np; p++) {
+            const int64_t e = h->pml_ids[p];
+            uint8_t m = 0;
+            double xc[3] = {0, 0, 0};
+            for (int a = 0; a < 8; a++) {
+                const int32_t nd = ien[e * 8 + a] - 1;
+                p_ien[a * np + p] = nd;
+                if (ndof[nd] == 12) m |= (uint8_t)(1u << a);
+                for (int i = 0; i < 3; i++) xc[i] = xc[i] + coor[(int64_t)nd * 3 + i];
+            }
+            for (int i = 0; i < 3; i++) xc[i] = xc[i] / 8;
+            p_mask[p] = m;
+            double d[3];
+            pml_elem_damping(s, xc, d);
+            for (int i = 0; i < 3; i++) p_damp[i * np + p] = d[i];
+            for (int i = 0; i < 24; i++) {
+                p_shp[i * np + p] = shp[e * 24 + i];
+                p_em[i * np + p] = emass[e * 24 + i];
+            }
+            p_det[p] = det[e];
+            p_vp[p] = mat[0 * ne + e];
+            p_rho[p] = mat[2 * ne + e];
+            p_lam[p] = mat[3 * ne + e];
+            p_mu[p] = mat[4 * ne + e];
+            for (int i = 0; i < 6; i++) p_ss[i * np + p] = ss[e * 6 + i];
+            for (int mm = 0; mm < 4; mm++)
+                for (int a = 0; a < 8; a++) p_phi[(mm * 8 + a) * np + p] = phi[e * 32 + mm * 8 + a];
+            for (int i = 0; i < 21; i++) p_st[i * np + p] = stress[(int64_t)sci[e] + i];
+        }
+        PmlElems &P = h->P;
+        P.n = np;
+        int rc;
+        int32_t *pi;
+        uint8_t *pm;
+        double *d;
+        if ((rc = dupload(h, &pi, p_ien))) return rc;
+        P.ien = pi;
+        if ((rc = dupload(h, &pm, p_mask))) return rc;
+        P.mask12 = pm;
+        if ((rc = dupload(h, &d, p_shp))) return rc; P.shp = d;
+        if ((rc = dupload(h, &d, p_det))) return rc; P.det = d;
+        if ((rc = dupload(h, &d, p_lam))) return rc; P.lam = d;
+        if ((rc = dupload(h, &d, p_mu))) return rc; P.mu = d;
+        if ((rc = dupload(h, &d, p_vp))) return rc; P.vp = d;
+        if ((rc = dupload(h, &d, p_rho))) return rc; P.rho = d;
+        if ((rc = dupload(h, &d, p_st))) return rc; P.stress = d;
+        if ((rc = dupload(h, &d, p_ss))) return rc; P.ss = d;
+        if ((rc = dupload(h, &d, p_phi))) return rc; P.phi = d;
+        if ((rc = dupload(h, &d, p_damp))) return rc; P.damp = d;
+        if (s.rdampm != 0.0 || s.C_elastic == 0) {
+            if ((rc = dupload(h, &d, p_em))) return rc;
+            P.emass = d;
+        }
+        if ((rc = dalloc(h, &d, 96 * np))) return rc;
+        P.ef = d;
+    }
+
+    // ---------------- node-to-element gather table (ascending element order)
+    {
+        std::vector<int32_t> cls_index(ne);
+        for (int64_t t = 0; t < nr; t++) cls_index[h->reg_ids[t]] = (int32_t)t;
+        for (int64_t p = 0; p < np; p++) cls_index[h->pml_ids[p]] = (int32_t)p;
+        std::vector<int32_t> rowptr(nn + 1, 0);
+        for (int64_t e = 0; e < ne; e++)
+            for (int a = 0; a < 8; a++) rowptr[ien[e * 8 + a]]++;  // node id is 1-based: counts land at [node]
+        for (int64_t n = 0; n < nn; n++) rowptr[n + 1] += rowptr[n];
+        std::vector<int32_t> fillp(rowptr.begin(), rowptr.end() - 1);
+        std::vector<uint32_t> ent(rowptr[nn]);
+        for (int64_t e = 0; e < ne; e++) {
+            const bool pml = etype[e] == 2;
+            for (int a = 0; a < 8; a++) {
+                const int32_t nd = ien[e * 8 + a] - 1;
+                ent[fillp[nd]++] = (pml ? kEntPml : 0u) | ((uint32_t)a << kEntLocalShift) | (uint32_t)cls_index[e];
+            }
+        }
+        for (int64_t q = 0; q < npn; q++) {
+            const int32_t n = h->pml_node_ids[q];
+            for (int j = rowptr[n]; j < rowptr[n + 1]; j++)
+                if (!(ent[j] & kEntPml)) return fail(h, EQD_EINVAL, "PML node %d touches a regular element", n + 1);
+        }
+        int rc;
+        int32_t *d_rp;
+        uint32_t *d_ent;
+        if ((rc = dupload(h, &d_rp, rowptr))) return rc;
+        if ((rc = dupload(h, &d_ent, ent))) return rc;
+        h->N.rowptr = d_rp;
+        h->N.ent = d_ent;
+    }
+    {
+        int rc;
+        double *d;
+        h->N.nn = nn;
+        if ((rc = dupload(h, &d, vel))) return rc; h->N.vel = d;
+        if ((rc = dupload(h, &d, disp))) return rc; h->N.disp = d;
+        if ((rc = dupload(h, &d, acc))) return rc; h->N.acc = d;
+        if ((rc = dupload(h, &d, mass))) return rc; h->N.mass = d;
+        int32_t *fl;
+        if ((rc = dalloc(h, &fl, 2))) return rc;
+        HIPCHK(h, hipMemset(fl, 0, 2 * sizeof(int32_t)));
+        h->N.nanflag = fl;
+        if ((rc = dupload(h, &h->d_node3_free, h->node3_free))) return rc;
+        if ((rc = dupload(h, &h->d_node3_all, h->node3_all))) return rc;
+        PmlNodes &PN = h->PN;
+        PN.n = npn;
+        int32_t *pn;
+        if ((rc = dupload(h, &pn, h->pml_node_ids))) return rc; PN.node = pn;
+        if ((rc = dupload(h, &d, pdamp))) return rc; PN.damp = d;
+        if ((rc = dupload(h, &d, pv12))) return rc; PN.v12 = d;
+        if ((rc = dupload(h, &d, pacc))) return rc; PN.acc = d;
+    }
+
+    // ---------------- fault
+    {
+        const int64_t nf = s.nf;
+        FaultState &F = h->F;
+        F.nf = nf;
+        F.nstep = s.nstep;
+        std::vector<int32_t> fn(2 * nf), sta(nf, -1);
+        std::vector<double> un(3 * nf), us(3 * nf), ud(3 * nf), xyz(3 * nf), fms(nf), fmm(nf), fr(100 * nf);
+        const double *pun = F64(h, EQD_UN), *pus = F64(h, EQD_US), *pud = F64(h, EQD_UD), *fric = F64(h, EQD_FRIC);
+        for (int64_t i = 0; i < nf; i++) {
+            fn[i] = nsmp[i * 2] - 1;
+            fn[nf + i] = nsmp[i * 2 + 1] - 1;
+            for (int k = 0; k < 3; k++) {
+                un[k * nf + i] = pun[i * 3 + k];
+                us[k * nf + i] = pus[i * 3 + k];
+                ud[k * nf + i] = pud[i * 3 + k];
+                xyz[k * nf + i] = coor[(int64_t)fn[i] * 3 + k];
+            }
+            fms[i] = fnms[fn[i]];
+            fmm[i] = fnms[fn[nf + i]];
+            for (int k = 0; k < 100; k++) fr[k * nf + i] = fric[i * 100 + k];
+        }
+        if (s.non > 0) {
+            const int32_t *an = I32(h, EQD_ANONFS);
+            // the reference records station j at every pair with anonfs(1,j) == pair on fault 1
+            for (int64_t j = 0; j < s.non; j++) {
+                const int32_t pair = an[j * 3], flt = an[j * 3 + 2];
+                if (flt != 1) continue;
+                if (pair < 1 || pair > nf) return fail(h, EQD_EINVAL, "anonfs(1,%lld) out of range", (long long)j + 1);
+                if (sta[pair - 1] >= 0) return fail(h, EQD_EINVAL, "two on-fault stations share node pair %d", pair);
+                sta[pair - 1] = (int32_t)j;
+            }
+        }
+        int rc;
+        int32_t *pi;
+        double *d;
+        if ((rc = dupload(h, &pi, fn))) return rc; F.nsmp = pi;
+        if ((rc = dupload(h, &pi, sta))) return rc; F.station = pi;
+        if ((rc = dupload(h, &d, un))) return rc; F.un = d;
+        if ((rc = dupload(h, &d, us))) return rc; F.us = d;
+        if ((rc = dupload(h, &d, ud))) return rc; F.ud = d;
+        if ((rc = dupload(h, &d, std::vector<double>(F64(h, EQD_ARN), F64(h, EQD_ARN) + nf)))) return rc; F.arn = d;
+        if ((rc = dupload(h, &d, fms))) return rc; F.fms = d;
+        if ((rc = dupload(h, &d, fmm))) return rc; F.fmm = d;
+        if ((rc = dupload(h, &d, xyz))) return rc; F.xyz = d;
+        if ((rc = dupload(h, &d, fr))) return rc; F.fric = d;
+        if ((rc = dupload(h, &d, std::vector<double>(F64(h, EQD_FNFT), F64(h, EQD_FNFT) + nf)))) return rc; F.fnft = d;
+        const int64_t non = s.non > 0 ? s.non : 1;
+        if ((rc = dalloc(h, &d, non * (int64_t)s.nstep * 12))) return rc;
+        HIPCHK(h, hipMemset(d, 0, sizeof(double) * non * s.nstep * 12));
+        F.hist = d;
+    }
+    // ---------------- off-fault stations
+    {
+        OffStations &S = h->S;
+        S.nrow = 6 * s.noff;
+        if (S.nrow > 0) {
+            const int32_t *idh = I32(h, EQD_IDHIST);
+            std::vector<int32_t> t(3 * S.nrow);
+            for (int i = 0; i < S.nrow; i++) {
+                t[i] = idh[i * 3] - 1;
+                t[S.nrow + i] = idh[i * 3 + 1] - 1;
+                t[2 * S.nrow + i] = idh[i * 3 + 2];
+                if (t[i] < 0 || t[i] >= nn || t[S.nrow + i] < 0 || t[S.nrow + i] > 2)
+                    return fail(h, EQD_EINVAL, "idhist row %d out of range", i + 1);
+            }
+            int rc;
+            int32_t *pi;
+            double *d;
+            if ((rc = dupload(h, &pi, t))) return rc;
+            S.idh = pi;
+            const int64_t cnt = (int64_t)(S.nrow + 1) * s.nstep;
+            if ((rc = dalloc(h, &d, cnt))) return rc;
+            HIPCHK(h, hipMemset(d, 0, sizeof(double) * cnt));
+            S.hist = d;
+        }
+    }
+    // ---------------- constants
+    StepConsts &C = h->C;
+    C.dt = s.dt;
+    C.w = s.w;
+    C.rdampk = s.rdampk;
+    C.rdampm = s.rdampm;
+    C.gterm = (1.0 - (double)s.C_elastic) * s.grav * (s.roumax - (s.gamar + 1.0) * s.rhow) / s.roumax;
+    C.ccosphi = s.ccosphi;
+    C.sinphi = s.sinphi;
+    C.exp_dt_tv = s.C_elastic == 0 ? std::exp(-s.dt / s.tv) : 0.0;
+    C.kapa_hg = s.kapa_hg;
+    C.slipRateThres = s.slipRateThres;
+    C.xsource = s.xsource; C.ysource = s.ysource; C.zsource = s.zsource;
+    C.nucR = s.nucR; C.nucRuptVel = s.nucRuptVel; C.nucdtau0 = s.nucdtau0; C.nucT = s.nucT;
+    C.C_elastic = s.C_elastic; C.C_Q = s.C_Q; C.C_hg = s.C_hg; C.C_nuclea = s.C_nuclea; C.friclaw = s.friclaw;
+    C.TPV = s.TPV; C.insertFaultType = s.insertFaultType; C.nucfault = s.nucfault;
+    HIPCHK(h, hipDeviceSynchronize());
+    h->committed = true;
+    h->pending = true;
+    // the host copies of bulk inputs are no longer needed
+    for (int f : {EQD_ELESHP, EQD_ELEMASS, EQD_PHI, EQD_SS, EQD_MAT}) h->fd.erase(f);
+    return EQD_OK;
+}
+
+}  // extern "C"
+
+static void timed_begin(eqd_handle *h, const char *name, hipEvent_t *stop)
+{
+    *stop = nullptr;
+    if (!h->timing) return;
+    KTimer &t = h->timers[name];
+    if (t.used == t.ev.size()) {
+        hipEvent_t a, b;
+        (void)hipEventCreate(&a);
+        (void)hipEventCreate(&b);
+        t.ev.push_back({a, b});
+    }
+    (void)hipEventRecord(t.ev[t.used].first, h->stream);
+    *stop = t.ev[t.used].second;
+    t.used++;
+}
+
+static void timed_end(eqd_handle *h, hipEvent_t stop)
+{
+    if (stop) (void)hipEventRecord(stop, h->stream);
+}
+
+template <typename R>
+static int run_steps(eqd_handle *h, const RegElems<R> &E, int32_t nsteps)
+{
+    const eqd_scalars &s = h->s;
+    const double dt = s.dt;
+    const bool q = s.C_Q == 1, dp = s.C_elastic == 0, mass = s.rdampm != 0.0 || s.C_elastic == 0;
+    hipStream_t st = h->stream;
+    hipEvent_t ev;
+    if (h->pending) {
+        HIPCHK(h, launch_update_from_acc(h->N, h->d_node3_all, (int64_t)h->node3_all.size(), h->PN, dt, h->nt + 1, st));
+        h->pending = false;
+    }
+    for (int32_t k = 0; k < nsteps; k++) {
+        h->nt += 1;
+        h->time = h->time + dt;
+        const int nt = h->nt;
+        const int mode = (k == nsteps - 1) ? 1 : 0;
+        if (nt > s.nstep) return fail(h, EQD_EINVAL, "step %d exceeds nstep %d (station history length)", nt, s.nstep);
+        timed_begin(h, "station", &ev);
+        HIPCHK(h, launch_station(h->S, h->N, nt, h->time, st));
+        timed_end(h, ev);
+        timed_begin(h, "element", &ev);
+        HIPCHK(h, launch_elem_regular<R>(E, h->N, h->C, q, dp, s.C_hg, mass, st));
+        timed_end(h, ev);
+        timed_begin(h, "element_pml", &ev);
+        HIPCHK(h, launch_elem_pml(h->P, h->N, h->C, s.C_hg, mass, st));
+        timed_end(h, ev);
+        timed_begin(h, "fault", &ev);
+        HIPCHK(h, launch_fault<R>(h->F, h->N, h->C, E.ef, E.n, h->P.ef, h->P.n, nt, h->time, mode, st));
+        timed_end(h, ev);
+        timed_begin(h, "node", &ev);
+        HIPCHK(h, launch_node3<R>(h->N, h->d_node3_free, (int64_t)h->node3_free.size(), E.ef, E.n, h->P.ef, h->P.n, dt,
+                                  mode, nt + 1, st));
+        timed_end(h, ev);
+        timed_begin(h, "node_pml", &ev);
+        HIPCHK(h, launch_node_pml(h->N, h->PN, h->P.ef, h->P.n, dt, mode, nt + 1, st));
+        timed_end(h, ev);
+    }
+    h->pending = true;
+    return EQD_OK;
+}
+
+extern "C" int eqd_run(eqd_handle *h, int32_t nsteps)
+{
+    if (!h) return EQD_EINVAL;
+    if (!h->committed) return fail(h, EQD_ESTATE, "eqd_run before eqd_commit");
+    if (nsteps < 0) return fail(h, EQD_EINVAL, "nsteps < 0");
+    if (nsteps == 0) return EQD_OK;
+    HIPCHK(h, hipSetDevice(h->s.device));
+    for (auto &kv : h->timers) {
+        kv.second.used = 0;
+        kv.second.total_ms = 0;
+        kv.second.launches = 0;
+    }
+    int rc = h->s.precision == EQD_PREC_FP32 ? run_steps(h, h->E32, nsteps) : run_steps(h, h->E64, nsteps);
+    if (rc) return rc;
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    if (h->timing) {
+        for (auto &kv : h->timers) {
+            KTimer &t = kv.second;
+            for (size_t i = 0; i < t.used; i++) {
+                float ms = 0;
+                (void)hipEventElapsedTime(&ms, t.ev[i].first, t.ev[i].second);
+                t.total_ms += ms;
+            }
+            t.launches = (int64_t)t.used;
+        }
+    }
+    int32_t flag[2];
+    HIPCHK(h, hipMemcpy(flag, h->N.nanflag, sizeof(flag), hipMemcpyDeviceToHost));
+    if (flag[0] != 0) return fail(h, EQD_ENAN, "Velocity NaN at node %d at time step %d", flag[0], flag[1]);
+    return EQD_OK;
+}
+
+template <typename T>
+static int d2h(eqd_handle *h, std::vector<T> &v, const T *d, int64_t count)
+{
+    v.resize(count);
+    if (count > 0) HIPCHK(h, hipMemcpy(v.data(), d, sizeof(T) * count, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+extern "C" {
+
+int eqd_get(eqd_handle *h, int32_t field, void *host, int64_t count)
+{
+    if (!h) return EQD_EINVAL;
+    if (!h->committed) return fail(h, EQD_ESTATE, "eqd_get before eqd_commit");
+    FieldSpec sp;
+    if (!field_spec(h->s, field, sp)) return fail(h, EQD_EINVAL, "unknown field %d", field);
+    if (count != sp.count) return fail(h, EQD_EINVAL, "field %d: count %lld, expected %lld", field, (long long)count, (long long)sp.count);
+    HIPCHK(h, hipSetDevice(h->s.device));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    const eqd_scalars &s = h->s;
+    const int64_t nn = s.nn, nf = s.nf;
+    double *out = (double *)host;
+    int rc;
+    switch (field) {
+    case EQD_VELARR:
+    case EQD_DISPARR: {
+        std::vector<double> v;
+        if ((rc = d2h(h, v, field == EQD_VELARR ? h->N.vel : h->N.disp, 3 * nn))) return rc;
+        for (int64_t n = 0; n < nn; n++)
+            for (int k = 0; k < 3; k++) out[n * 3 + k] = v[k * nn + n];
+        return EQD_OK;
+    }
+    case EQD_V1:
+    case EQD_NODALFORCEARR: {
+        // v1 of 3-component nodes is velArr; of PML nodes the 12 split velocities
+        const bool isv = field == EQD_V1;
+        std::vector<double> v, p;
+        if ((rc = d2h(h, v, isv ? h->N.vel : h->N.acc, 3 * nn))) return rc;
+        if ((rc = d2h(h, p, isv ? h->PN.v12 : h->PN.acc, 12 * h->PN.n))) return rc;
+        const auto &orig = h->fd[field];  // values for equations the device does not own
+        for (int64_t i = 0; i < s.neq; i++) out[i] = orig.empty() ? 0.0 : orig[i];
+        const int32_t *eqi = I32(h, EQD_EQNUMINDEXARR), *esl = I32(h, EQD_EQNUMSTARTINDEXLOC);
+        for (int32_t n : h->node3_all)
+            for (int k = 0; k < 3; k++) out[eqi[esl[n] + k] - 1] = v[k * nn + n];
+        const int64_t np = h->PN.n;
+        for (int64_t q = 0; q < np; q++) {
+            const int32_t n = h->pml_node_ids[q];
+            for (int k = 0; k < 12; k++) out[eqi[esl[n] + k] - 1] = p[k * np + q];
+        }
+        return EQD_OK;
+    }
+    case EQD_FRIC: {
+        std::vector<double> v;
+        if ((rc = d2h(h, v, h->F.fric, 100 * nf))) return rc;
+        for (int64_t i = 0; i < nf; i++)
+            for (int k = 0; k < 100; k++) out[i * 100 + k] = v[k * nf + i];
+        return EQD_OK;
+    }
+    case EQD_FNFT: {
+        if (nf > 0) HIPCHK(h, hipMemcpy(out, h->F.fnft, sizeof(double) * nf, hipMemcpyDeviceToHost));
+        return EQD_OK;
+    }
+    case EQD_ONFAULTHIST: {
+        if (count > 0) HIPCHK(h, hipMemcpy(out, h->F.hist, sizeof(double) * count, hipMemcpyDeviceToHost));
+        return EQD_OK;
+    }
+    case EQD_OFFFAULTHIST: {
+        if (count > 0) HIPCHK(h, hipMemcpy(out, h->S.hist, sizeof(double) * count, hipMemcpyDeviceToHost));
+        return EQD_OK;
+    }
+    case EQD_STRESSARR: {
+        const auto &orig = h->fd[EQD_STRESSARR];
+        for (int64_t i = 0; i < s.size_stress; i++) out[i] = orig[i];
+        const int32_t *sci = I32(h, EQD_STRESSCOMPINDEXARR);
+        const int64_t nr = h->n_reg, np = h->n_pml;
+        std::vector<double> v;
+        if (s.precision == EQD_PREC_FP32) {
+            std::vector<float> f;
+            if ((rc = d2h(h, f, h->E32.stress, (int64_t)h->nreg_stress * nr))) return rc;
+            v.assign(f.begin(), f.end());
+        } else if ((rc = d2h(h, v, h->E64.stress, (int64_t)h->nreg_stress * nr))) {
+            return rc;
+        }
+        for (int64_t t = 0; t < nr; t++) {
+            const int64_t e = h->reg_ids[t];
+            for (int i = 0; i < h->nreg_stress; i++) {
+                const int64_t idx = (int64_t)sci[e] + i;
+                if (idx < s.size_stress) out[idx] = v[i * nr + t];
+            }
+        }
+        if ((rc = d2h(h, v, h->P.stress, 21 * np))) return rc;
+        for (int64_t p = 0; p < np; p++) {
+            const int64_t e = h->pml_ids[p];
+            for (int i = 0; i < 21; i++) out[(int64_t)sci[e] + i] = v[i * np + p];
+        }
+        return EQD_OK;
+    }
+    case EQD_PSTRAIN: {
+        const auto &orig = h->fd[EQD_PSTRAIN];
+        for (int64_t i = 0; i < s.ne; i++) out[i] = orig[i];
+        std::vector<double> v;
+        const double *src = s.precision == EQD_PREC_FP32 ? h->E32.pstrain : h->E64.pstrain;
+        if ((rc = d2h(h, v, src, h->n_reg))) return rc;
+        for (int64_t t = 0; t < h->n_reg; t++) out[h->reg_ids[t]] = v[t];
+        return EQD_OK;
+    }
+    default: {
+        // inputs are returned as handed over
+        if (sp.is_int) {
+            auto it = h->fi.find(field);
+            if (it == h->fi.end()) return fail(h, EQD_EINVAL, "field %d is not available", field);
+            std::memcpy(host, it->second.data(), sizeof(int32_t) * count);
+        } else {
+            auto it = h->fd.find(field);
+            if (it == h->fd.end()) return fail(h, EQD_EINVAL, "field %d is not available after commit", field);
+            std::memcpy(host, it->second.data(), sizeof(double) * count);
+        }
+        return EQD_OK;
+    }
+    }
+}
+
+int eqd_clock(eqd_handle *h, int32_t *nt, double *time_elapsed)
+{
+    if (!h) return EQD_EINVAL;
+    if (nt) *nt = h->nt;
+    if (time_elapsed) *time_elapsed = h->time;
+    return EQD_OK;
+}
+
+int eqd_nan_info(eqd_handle *h, int32_t *node, int32_t *nt)
+{
+    if (!h || !h->committed) return EQD_EINVAL;
+    int32_t flag[2];
+    HIPCHK(h, hipMemcpy(flag, h->N.nanflag, sizeof(flag), hipMemcpyDeviceToHost));
+    if (node) *node = flag[0];
+    if (nt) *nt = flag[1];
+    return EQD_OK;
+}
+
+int eqd_kernel_time(eqd_handle *h, const char *name, double *avg_ms, int64_t *launches)
+{
+    if (!h || !name) return EQD_EINVAL;
+    auto it = h->timers.find(name);
+    if (it == h->timers.end() || it->second.launches == 0) {
+        if (avg_ms) *avg_ms = 0.0;
+        if (launches) *launches = 0;
+        return EQD_OK;
+    }
+    if (avg_ms) *avg_ms = it->second.total_ms / (double)it->second.launches;
+    if (launches) *launches = it->second.launches;
+    return EQD_OK;
+}
+
+int eqd_set_timing(eqd_handle *h, int32_t on)
+{
+    if (!h) return EQD_EINVAL;
+    h->timing = on != 0;
+    return EQD_OK;
+}
+
+int64_t eqd_device_bytes(eqd_handle *h) { return h ? h->bytes : 0; }
+
+const char *eqd_last_error(eqd_handle *h) { return h ? h->err.c_str() : "null handle"; }
+
+int eqd_destroy(eqd_handle *h)
+{
+    if (!h) return EQD_OK;
+    (void)hipSetDevice(h->s.device);
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    for (void *p : h->allocs) (void)hipFree(p);
+    for (auto &kv : h->timers)
+        for (auto &e : kv.second.ev) {
+            (void)hipEventDestroy(e.first);
+            (void)hipEventDestroy(e.second);
+        }
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+    return EQD_OK;
+}
+
+}  // extern "C"

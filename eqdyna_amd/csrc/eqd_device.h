@@ -1,0 +1,100 @@
+// Device-side data layout of the eqdyna_amd time step.
+//
+// Everything the step touches stays resident in HBM in structure-of-arrays
+// form, indexed so that consecutive threads read consecutive addresses:
+//  * elements are split by class (regular hex/wedge vs PML) and renumbered
+//    within the class in ascending reference order; component c of element t
+//    lives at base[c * n + t];
+//  * nodal arrays are component-major, base[k * nn + node];
+//  * the element-force buffers hold, per element and local node, the force the
+//    element contributes to that node; a node-to-element table (CSR, ascending
+//    element order) lets each node sum its contributions without atomics.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace eqd {
+
+// CSR entry: bit 31 = PML element, bits 28..30 = local node a, bits 0..27 =
+// element index within its class.
+constexpr uint32_t kEntPml = 0x80000000u;
+constexpr uint32_t kEntIdxMask = 0x0FFFFFFFu;
+constexpr int kEntLocalShift = 28;
+
+struct StepConsts {
+    double dt, w, rdampk, rdampm, gterm, ccosphi, sinphi, exp_dt_tv, kapa_hg;
+    double slipRateThres, xsource, ysource, zsource, nucR, nucRuptVel, nucdtau0, nucT;
+    int32_t C_elastic, C_Q, C_hg, C_nuclea, friclaw, TPV, insertFaultType, nucfault;
+};
+
+struct NodeState {
+    int64_t nn;
+    double *vel;   // [3][nn]  velArr (== v1 for 3-dof nodes)
+    double *disp;  // [3][nn]  dispArr
+    double *acc;   // [3][nn]  pending F/m of 3-dof nodes
+    const double *mass;    // [nn] nodal mass (nodalMassArr of the node's equations)
+    const int32_t *rowptr; // [nn+1]
+    const uint32_t *ent;   // CSR entries
+    int32_t *nanflag;      // [2] node+1, step
+};
+
+template <typename R>
+struct RegElems {
+    int64_t n;
+    const int32_t *ien;   // [8][n]   0-based node ids
+    const R *shp;         // [24][n]  dN_a/dx_k at (a*3+k)
+    const R *det;         // [n]      eledet
+    const R *lam, *mu;    // [n]
+    const R *vp, *rho;    // [n]      (viscous hourglass only)
+    R *stress;            // [6 or 12][n]
+    const R *ss;          // [6][n]
+    const R *phi;         // [32][n]  Phi_a of mode m at (m*8+a)
+    const R *emass;       // [24][n]  (inertia/gravity term only)
+    const double *porep;  // [n]      (plastic only)
+    double *pstrain;      // [n]      (plastic only)
+    const uint8_t *qidx;  // [n]      (Q only) row of qtab
+    const double *qtab;   // [16][6]  exp(-dt/tau), 1-exp(-dt/tau), 8*w_p, 8*w_s, c_p, c_s
+    R *ef;                // [24][n]  force on local node a, component c at (a*3+c)
+};
+
+struct PmlElems {
+    int64_t n;
+    const int32_t *ien;    // [8][n]
+    const double *shp;     // [24][n]
+    const double *det, *lam, *mu, *vp, *rho;
+    double *stress;        // [21][n]: 15 split + 6 initial
+    const double *ss, *phi, *emass;
+    const double *damp;    // [3][n] element-centre damping d_x, d_y, d_z
+    const uint8_t *mask12; // [n] bit a: local node a carries 12 components
+    double *ef;            // [96][n] slot (a*12+k); 3-component nodes use k=0..2
+};
+
+struct PmlNodes {
+    int64_t n;
+    const int32_t *node;  // [n]
+    const double *damp;   // [3][n] d_x, d_y, d_z at the node
+    double *v12;          // [12][n] split velocities (v1 of the 12 equations)
+    double *acc;          // [12][n] pending F/m
+};
+
+struct FaultState {
+    int64_t nf;
+    const int32_t *nsmp;    // [2][nf] slave, master (0-based node ids)
+    const double *un, *us, *ud; // [3][nf]
+    const double *arn;      // [nf]
+    const double *fms, *fmm; // [nf] fnms of slave and master
+    const double *xyz;      // [3][nf] slave coordinates
+    double *fric;           // [100][nf]
+    double *fnft;           // [nf]
+    const int32_t *station; // [nf] on-fault station (0-based) or -1
+    double *hist;           // [non][nstep][12]
+    int32_t nstep;
+};
+
+struct OffStations {
+    int32_t nrow;          // 6 * noff
+    const int32_t *idh;    // [3][nrow]: node (0-based), component (0-based), kind (1 disp, 2 vel)
+    double *hist;          // [nstep][nrow+1]
+};
+
+}  // namespace eqd

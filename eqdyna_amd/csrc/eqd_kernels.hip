@@ -1,0 +1,1039 @@
+// gfx950 kernels of the eqdyna_amd explicit time step.
+//
+// One step of the reference (src/driver.f90:9-34) becomes
+//   station sample -> element kernels -> fault kernel -> node kernels
+// where the node kernels fuse "F/m" of this step with the central-difference
+// update of the next one (velDispUpdate, src/driver.f90:89-155), so the
+// assembled force never round-trips through HBM.  The summation order inside
+// every element and every node follows the reference; the only regrouping is
+// that an element's stiffness and hourglass forces on a node are combined
+// before the node sums its elements (the reference sums all element
+// stiffness forces first, then all hourglass forces).
+//
+// Compiled with -ffp-contract=off: no FMA contraction, so each rounded
+// operation is the one the reference performs.
+#include "eqd_device.h"
+
+namespace eqd {
+
+template <typename R>
+__device__ __forceinline__ R sqr(R x) { return x * x; }
+
+// ------------------------------------------------------------------------
+// Regular (type 1, 11, 12, 13) elements: calcElemKU + calcElemMass + hrglss
+// for one element per thread.
+//   src/assembleGlobalKU.f90:13-35, src/calcElemKU.f90:3-191,
+//   src/calcElemMass.f90:1-19, src/hrglss.f90:13-98
+// kQ: C_Q == 1; kDP: C_elastic == 0; kHG: C_hg (1 stiffness, 2 viscous);
+// kMass: inertia/gravity term present (rdampm != 0 or plastic).
+template <typename R, bool kQ, bool kDP, int kHG, bool kMass>
+__global__ void __launch_bounds__(256) elem_regular_kernel(RegElems<R> E, NodeState N, StepConsts C)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= E.n) return;
+    const int64_t n = E.n, nn = N.nn;
+    int nd[8];
+#pragma unroll
+    for (int a = 0; a < 8; a++) nd[a] = E.ien[a * n + t];
+    R vl[24], dl[24];
+#pragma unroll
+    for (int a = 0; a < 8; a++) {
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            vl[a * 3 + k] = (R)N.vel[k * nn + nd[a]];
+            dl[a * 3 + k] = (R)N.disp[k * nn + nd[a]];
+        }
+    }
+    R shp[24];
+#pragma unroll
+    for (int i = 0; i < 24; i++) shp[i] = E.shp[i * n + t];
+    const R lam = E.lam[t], miu = E.mu[t];
+    R sr[6] = {0, 0, 0, 0, 0, 0}, strain[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int a = 0; a < 8; a++) {
+        const R n1 = shp[a * 3], n2 = shp[a * 3 + 1], n3 = shp[a * 3 + 2];
+        const int j1 = a * 3, j2 = j1 + 1, j3 = j1 + 2;
+        sr[0] = sr[0] + n1 * vl[j1];
+        sr[1] = sr[1] + n2 * vl[j2];
+        sr[2] = sr[2] + n3 * vl[j3];
+        sr[3] = sr[3] + n3 * vl[j2] + n2 * vl[j3];
+        sr[4] = sr[4] + n3 * vl[j1] + n1 * vl[j3];
+        sr[5] = sr[5] + n2 * vl[j1] + n1 * vl[j2];
+        if (kQ) {
+            strain[0] = strain[0] + n1 * dl[j1];
+            strain[1] = strain[1] + n2 * dl[j2];
+            strain[2] = strain[2] + n3 * dl[j3];
+            strain[3] = strain[3] + n3 * dl[j2] + n2 * dl[j3];
+            strain[4] = strain[4] + n3 * dl[j1] + n1 * dl[j3];
+            strain[5] = strain[5] + n2 * dl[j1] + n1 * dl[j2];
+        }
+    }
+    const R cdiag = lam + (R)2 * miu;
+    R srt[6];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        R acc = 0;
+#pragma unroll
+        for (int j = 0; j < 3; j++) acc = acc + (i == j ? cdiag : lam) * sr[j];
+        srt[i] = acc;
+    }
+#pragma unroll
+    for (int i = 3; i < 6; i++) srt[i] = miu * sr[i];
+
+    R stress[6], strdev[6];
+#pragma unroll
+    for (int i = 0; i < 6; i++) stress[i] = E.stress[i * n + t];
+    if (!kQ) {
+#pragma unroll
+        for (int i = 0; i < 6; i++) {
+            stress[i] = stress[i] + srt[i] * (R)C.dt;
+            strdev[i] = stress[i];
+        }
+    } else {
+        const double *q = E.qtab + 6 * E.qidx[t];
+        const R e1 = (R)q[0], e2 = (R)q[1], wkp = (R)q[2], wks = (R)q[3], cv = (R)q[4], cs = (R)q[5];
+        const R miuu = miu * cs;
+        const R Mu = (lam + (R)2 * miu) * cv;
+        const R vols = strain[0] + strain[1] + strain[2];
+        R anestr1[6], anestr[6];
+#pragma unroll
+        for (int i = 0; i < 6; i++) anestr1[i] = E.stress[(6 + i) * n + t];
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+            anestr[i] = e1 * anestr1[i] + e2 * ((R)2 * miuu * strain[i] * wks + (Mu * wkp - (R)2 * miuu * wks) * vols);
+#pragma unroll
+        for (int i = 3; i < 6; i++) anestr[i] = e1 * anestr1[i] + e2 * (miuu * strain[i] * wks);
+#pragma unroll
+        for (int i = 0; i < 6; i++) E.stress[(6 + i) * n + t] = anestr[i];
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+            stress[i] = (R)2 * miuu * strain[i] + (Mu - (R)2 * miuu) * vols - (R)0.5 * (anestr[i] + anestr1[i]);
+#pragma unroll
+        for (int i = 3; i < 6; i++)
+            stress[i] = (R)2 * miuu * strain[i] / (R)2 - (R)0.5 * (anestr[i] + anestr1[i]);
+#pragma unroll
+        for (int i = 0; i < 6; i++) strdev[i] = stress[i];
+    }
+    if (kDP) {
+        // Drucker-Prager viscoplastic return, src/calcElemKU.f90:133-167
+        const R strmea = (stress[0] + stress[1] + stress[2]) / (R)3;
+#pragma unroll
+        for (int i = 0; i < 3; i++) strdev[i] = stress[i] - strmea;
+        R taomax = (R)0.5 * (sqr(strdev[0]) + sqr(strdev[1]) + sqr(strdev[2])) + sqr(strdev[3]) +
+                   sqr(strdev[4]) + sqr(strdev[5]);
+        taomax = sqrt(taomax);
+        R yield = (R)C.ccosphi - (R)C.sinphi * (strmea + (R)E.porep[t]);
+        if (yield < (R)0) yield = 0;
+        R pstrmag = 0;
+        if (taomax > yield) {
+            const R rjust = yield / taomax + ((R)1 - yield / taomax) * (R)C.exp_dt_tv;
+            R pstrinc[6];
+#pragma unroll
+            for (int i = 0; i < 6; i++) {
+                stress[i] = strdev[i] * rjust;
+                pstrinc[i] = (strdev[i] - stress[i]) / miu;
+                if (i < 3) stress[i] = stress[i] + strmea;
+            }
+            const R pstrmea = (pstrinc[0] + pstrinc[1] + pstrinc[2]) / (R)3;
+#pragma unroll
+            for (int i = 0; i < 6; i++) pstrinc[i] = pstrinc[i] - pstrmea;
+            R pm = (R)0.5 * (sqr(pstrinc[0]) + sqr(pstrinc[1]) + sqr(pstrinc[2])) + sqr(pstrinc[3]) +
+                   sqr(pstrinc[4]) + sqr(pstrinc[5]);
+            pstrmag = sqrt(pm);
+        }
+        E.pstrain[t] = E.pstrain[t] + (double)pstrmag;
+    }
+#pragma unroll
+    for (int i = 0; i < 6; i++) E.stress[i * n + t] = stress[i];
+
+    // f = -m*al (calcElemMass), then + (-det*w) * B^T (sigma + rdampk*sigma_dot)
+    R f[24];
+    if (kMass) {
+#pragma unroll
+        for (int a = 0; a < 8; a++) {
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                R al = (R)C.rdampm * vl[a * 3 + k];
+                if (k == 2) al = al + (R)C.gterm;
+                f[a * 3 + k] = (R)0 - al * E.emass[(a * 3 + k) * n + t];
+            }
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < 24; i++) f[i] = 0;
+    }
+    const R temp = -E.det[t] * (R)C.w;
+    R st[6];
+#pragma unroll
+    for (int i = 0; i < 6; i++) st[i] = temp * (stress[i] + (R)C.rdampk * srt[i]);
+#pragma unroll
+    for (int a = 0; a < 8; a++) {
+        const R n1 = shp[a * 3], n2 = shp[a * 3 + 1], n3 = shp[a * 3 + 2];
+        f[a * 3 + 0] = f[a * 3 + 0] + (n1 * st[0] + n3 * st[4] + n2 * st[5]);
+        f[a * 3 + 1] = f[a * 3 + 1] + (n2 * st[1] + n3 * st[3] + n1 * st[5]);
+        f[a * 3 + 2] = f[a * 3 + 2] + (n3 * st[2] + n2 * st[3] + n1 * st[4]);
+    }
+
+    if (kHG == 1) {
+        // Kosloff-Frazier stiffness hourglass, src/hrglss.f90:20-55
+        R ss[6];
+#pragma unroll
+        for (int i = 0; i < 6; i++) ss[i] = E.ss[i * n + t];
+#pragma unroll
+        for (int i = 0; i < 24; i++) dl[i] = dl[i] + (R)C.rdampk * vl[i];
+#pragma unroll
+        for (int m = 0; m < 4; m++) {
+            R ph[8];
+#pragma unroll
+            for (int a = 0; a < 8; a++) ph[a] = E.phi[(m * 8 + a) * n + t];
+            R pd0 = 0, pd1 = 0, pd2 = 0;
+#pragma unroll
+            for (int a = 0; a < 8; a++) {
+                pd0 = pd0 + ph[a] * dl[a * 3 + 0];
+                pd1 = pd1 + ph[a] * dl[a * 3 + 1];
+                pd2 = pd2 + ph[a] * dl[a * 3 + 2];
+            }
+            const R g0 = ss[0] * pd0 + ss[1] * pd1 + ss[2] * pd2;
+            const R g1 = ss[1] * pd0 + ss[3] * pd1 + ss[4] * pd2;
+            const R g2 = ss[2] * pd0 + ss[4] * pd1 + ss[5] * pd2;
+#pragma unroll
+            for (int a = 0; a < 8; a++) {
+                f[a * 3 + 0] = f[a * 3 + 0] - ph[a] * g0;
+                f[a * 3 + 1] = f[a * 3 + 1] - ph[a] * g1;
+                f[a * 3 + 2] = f[a * 3 + 2] - ph[a] * g2;
+            }
+        }
+    } else if (kHG == 2) {
+        // viscous hourglass, src/hrglss.f90:57-97
+        constexpr int fi[4][8] = {{1, 1, -1, -1, -1, -1, 1, 1}, {1, -1, -1, 1, -1, 1, 1, -1},
+                                  {1, -1, 1, -1, 1, -1, 1, -1}, {1, -1, 1, -1, -1, 1, -1, 1}};
+        const R coef = (R)0.25 * (R)C.kapa_hg * E.rho[t] * E.vp[t] * (R)pow((double)E.det[t] * C.w, 2.0 / 3.0);
+        R q[3][4];
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                R acc = 0;
+#pragma unroll
+                for (int k = 0; k < 8; k++) acc = acc + vl[k * 3 + i] * (R)fi[j][k];
+                q[i][j] = acc;
+            }
+#pragma unroll
+        for (int k = 0; k < 8; k++)
+#pragma unroll
+            for (int i = 0; i < 3; i++) {
+                R fv = 0;
+#pragma unroll
+                for (int j = 0; j < 4; j++) fv = fv - coef * q[i][j] * (R)fi[j][k];
+                f[k * 3 + i] = f[k * 3 + i] + fv;
+            }
+    }
+#pragma unroll
+    for (int i = 0; i < 24; i++) E.ef[i * n + t] = f[i];
+}
+
+// ------------------------------------------------------------------------
+// PML elements: calcPMLElemKU + calcElemMass + hrglss, one element per thread.
+//   src/assembleGlobalKU.f90:36-64, 70-346; src/hrglss.f90
+template <int kHG, bool kMass>
+__global__ void __launch_bounds__(256) elem_pml_kernel(PmlElems E, NodeState N, StepConsts C)
+{
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= E.n) return;
+    const int64_t n = E.n, nn = N.nn;
+    int nd[8];
+#pragma unroll
+    for (int a = 0; a < 8; a++) nd[a] = E.ien[a * n + p];
+    double va[24];
+#pragma unroll
+    for (int a = 0; a < 8; a++)
+#pragma unroll
+        for (int k = 0; k < 3; k++) va[a * 3 + k] = N.vel[k * nn + nd[a]];
+    double shp[24];
+#pragma unroll
+    for (int i = 0; i < 24; i++) shp[i] = E.shp[i * n + p];
+    const double lam = E.lam[p], miu = E.mu[p], det = E.det[p];
+    double sr[6] = {0, 0, 0, 0, 0, 0};
+    double Dx_vx = 0, Dy_vy = 0, Dz_vz = 0, Dx_vy = 0, Dy_vx = 0, Dx_vz = 0, Dz_vx = 0, Dy_vz = 0, Dz_vy = 0;
+#pragma unroll
+    for (int a = 0; a < 8; a++) {
+        const double n1 = shp[a * 3], n2 = shp[a * 3 + 1], n3 = shp[a * 3 + 2];
+        const int j1 = a * 3, j2 = j1 + 1, j3 = j1 + 2;
+        sr[0] = sr[0] + n1 * va[j1];
+        sr[1] = sr[1] + n2 * va[j2];
+        sr[2] = sr[2] + n3 * va[j3];
+        sr[3] = sr[3] + n3 * va[j2] + n2 * va[j3];
+        sr[4] = sr[4] + n3 * va[j1] + n1 * va[j3];
+        sr[5] = sr[5] + n2 * va[j1] + n1 * va[j2];
+        Dx_vx = Dx_vx + n1 * va[j1];
+        Dy_vy = Dy_vy + n2 * va[j2];
+        Dz_vz = Dz_vz + n3 * va[j3];
+        Dx_vy = Dx_vy + n1 * va[j2];
+        Dy_vx = Dy_vx + n2 * va[j1];
+        Dx_vz = Dx_vz + n1 * va[j3];
+        Dz_vx = Dz_vx + n3 * va[j1];
+        Dy_vz = Dy_vz + n2 * va[j3];
+        Dz_vy = Dz_vy + n3 * va[j2];
+    }
+    const double cdiag = lam + 2.0 * miu;
+    double srt[6];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        double acc = 0;
+#pragma unroll
+        for (int j = 0; j < 3; j++) acc = acc + (i == j ? cdiag : lam) * sr[j];
+        srt[i] = acc;
+    }
+#pragma unroll
+    for (int i = 3; i < 6; i++) srt[i] = miu * sr[i];
+
+    const double dt = C.dt;
+    const double d0 = E.damp[p], d1 = E.damp[n + p], d2 = E.damp[2 * n + p];
+    const double l2m = lam + 2 * miu;
+    const double coef[15] = {l2m, lam, lam, lam, l2m, lam, lam, lam, l2m, miu, miu, miu, miu, miu, miu};
+    const double grad[15] = {Dx_vx, Dy_vy, Dz_vz, Dx_vx, Dy_vy, Dz_vz, Dx_vx, Dy_vy, Dz_vz,
+                             Dx_vy, Dy_vx, Dx_vz, Dz_vx, Dy_vz, Dz_vy};
+    constexpr int dir[15] = {0, 1, 2, 0, 1, 2, 0, 1, 2, 0, 1, 0, 2, 1, 2};
+    double sv[15];
+#pragma unroll
+    for (int c = 0; c < 15; c++) {
+        const double dd = dir[c] == 0 ? d0 : (dir[c] == 1 ? d1 : d2);
+        double s = E.stress[c * n + p];
+        s = coef[c] * grad[c] + (1 / dt - dd / 2) * s;
+        s = s / (1 / dt + dd / 2);
+        sv[c] = s;
+        E.stress[c * n + p] = s;
+    }
+    const double sxx = sv[0] + sv[1] + sv[2];
+    const double syy = sv[3] + sv[4] + sv[5];
+    const double szz = sv[6] + sv[7] + sv[8];
+    const double sxy = sv[9] + sv[10];
+    const double sxz = sv[11] + sv[12];
+    const double syz = sv[13] + sv[14];
+    double s0[6];
+#pragma unroll
+    for (int k = 0; k < 6; k++) s0[k] = E.stress[(15 + k) * n + p] + C.rdampk * srt[k];
+    const double dw = det * C.w;
+    const unsigned mask = E.mask12[p];
+
+    // hourglass forces per local node (subtracted mode by mode)
+    double hg[8][3];
+#pragma unroll
+    for (int a = 0; a < 8; a++) hg[a][0] = hg[a][1] = hg[a][2] = 0;
+    double dlh[24];
+    if (kHG == 1 || kHG == 2) {
+#pragma unroll
+        for (int a = 0; a < 8; a++)
+#pragma unroll
+            for (int k = 0; k < 3; k++) dlh[a * 3 + k] = N.disp[k * nn + nd[a]] + C.rdampk * va[a * 3 + k];
+    }
+
+#pragma unroll
+    for (int a = 0; a < 8; a++) {
+        const double g1 = shp[a * 3], g2 = shp[a * 3 + 1], g3 = shp[a * 3 + 2];
+        double fi[12];
+        if (kMass) {
+#pragma unroll
+            for (int k = 0; k < 9; k++) fi[k] = 0.0;
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                double al = C.rdampm * va[a * 3 + k];
+                if (k == 2) al = al + C.gterm;
+                fi[9 + k] = 0.0 - al * E.emass[(a * 3 + k) * n + p];
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < 12; k++) fi[k] = 0.0;
+        }
+        fi[0] = fi[0] - dw * g1 * sxx;
+        fi[1] = fi[1] - dw * g2 * sxy;
+        fi[2] = fi[2] - dw * g3 * sxz;
+        fi[3] = fi[3] - dw * g1 * sxy;
+        fi[4] = fi[4] - dw * g2 * syy;
+        fi[5] = fi[5] - dw * g3 * syz;
+        fi[6] = fi[6] - dw * g1 * sxz;
+        fi[7] = fi[7] - dw * g2 * syz;
+        fi[8] = fi[8] - dw * g3 * szz;
+        fi[9] = fi[9] - dw * (g1 * s0[0] + g3 * s0[4] + g2 * s0[5]);
+        fi[10] = fi[10] - dw * (g2 * s0[1] + g3 * s0[3] + g1 * s0[5]);
+        fi[11] = fi[11] - dw * (g3 * s0[2] + g2 * s0[3] + g1 * s0[4]);
+        if (mask & (1u << a)) {
+#pragma unroll
+            for (int k = 0; k < 12; k++) E.ef[(a * 12 + k) * n + p] = fi[k];
+        } else {
+            E.ef[(a * 12 + 0) * n + p] = fi[0] + fi[1] + fi[2] + fi[9];
+            E.ef[(a * 12 + 1) * n + p] = fi[3] + fi[4] + fi[5] + fi[10];
+            E.ef[(a * 12 + 2) * n + p] = fi[6] + fi[7] + fi[8] + fi[11];
+        }
+    }
+
+    if (kHG == 1) {
+        double ss[6];
+#pragma unroll
+        for (int i = 0; i < 6; i++) ss[i] = E.ss[i * n + p];
+#pragma unroll
+        for (int m = 0; m < 4; m++) {
+            double ph[8];
+#pragma unroll
+            for (int a = 0; a < 8; a++) ph[a] = E.phi[(m * 8 + a) * n + p];
+            double pd0 = 0, pd1 = 0, pd2 = 0;
+#pragma unroll
+            for (int a = 0; a < 8; a++) {
+                pd0 = pd0 + ph[a] * dlh[a * 3 + 0];
+                pd1 = pd1 + ph[a] * dlh[a * 3 + 1];
+                pd2 = pd2 + ph[a] * dlh[a * 3 + 2];
+            }
+            const double q0 = ss[0] * pd0 + ss[1] * pd1 + ss[2] * pd2;
+            const double q1 = ss[1] * pd0 + ss[3] * pd1 + ss[4] * pd2;
+            const double q2 = ss[2] * pd0 + ss[4] * pd1 + ss[5] * pd2;
+#pragma unroll
+            for (int a = 0; a < 8; a++) {
+                hg[a][0] = hg[a][0] - ph[a] * q0;
+                hg[a][1] = hg[a][1] - ph[a] * q1;
+                hg[a][2] = hg[a][2] - ph[a] * q2;
+            }
+        }
+    } else if (kHG == 2) {
+        constexpr int fi[4][8] = {{1, 1, -1, -1, -1, -1, 1, 1}, {1, -1, -1, 1, -1, 1, 1, -1},
+                                  {1, -1, 1, -1, 1, -1, 1, -1}, {1, -1, 1, -1, -1, 1, -1, 1}};
+        const double coef2 = 0.25 * C.kapa_hg * E.rho[p] * E.vp[p] * pow(det * C.w, 2.0 / 3.0);
+        double q[3][4];
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                double acc = 0;
+#pragma unroll
+                for (int k = 0; k < 8; k++) acc = acc + va[k * 3 + i] * (double)fi[j][k];
+                q[i][j] = acc;
+            }
+#pragma unroll
+        for (int k = 0; k < 8; k++)
+#pragma unroll
+            for (int i = 0; i < 3; i++) {
+                double fv = 0;
+#pragma unroll
+                for (int j = 0; j < 4; j++) fv = fv - coef2 * q[i][j] * (double)fi[j][k];
+                hg[k][i] = fv;
+            }
+    }
+    if (kHG == 1 || kHG == 2) {
+        // hourglass goes to components 10-12 of a PML node, 1-3 of a 3-component node
+#pragma unroll
+        for (int a = 0; a < 8; a++) {
+            const int base = (mask & (1u << a)) ? 9 : 0;
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                double *slot = E.ef + (a * 12 + base + k) * n + p;
+                *slot = *slot + hg[a][k];
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------------
+// Gather of one 3-component node: sum of its elements' forces in ascending
+// element order (assembleGlobalKU's scatter-add, src/assembleGlobalKU.f90:28-63).
+template <typename R>
+__device__ __forceinline__ void gather3(const NodeState &N, int node, const R *ef, int64_t nreg,
+                                        const double *efp, int64_t npe, double F[3])
+{
+    F[0] = 0.0; F[1] = 0.0; F[2] = 0.0;
+    const int beg = N.rowptr[node], end = N.rowptr[node + 1];
+    for (int j = beg; j < end; j++) {
+        const uint32_t e = N.ent[j];
+        const int64_t idx = e & kEntIdxMask;
+        const int a = (e >> kEntLocalShift) & 7;
+        if (e & kEntPml) {
+            F[0] = F[0] + efp[(a * 12 + 0) * npe + idx];
+            F[1] = F[1] + efp[(a * 12 + 1) * npe + idx];
+            F[2] = F[2] + efp[(a * 12 + 2) * npe + idx];
+        } else {
+            F[0] = F[0] + (double)ef[(a * 3 + 0) * nreg + idx];
+            F[1] = F[1] + (double)ef[(a * 3 + 1) * nreg + idx];
+            F[2] = F[2] + (double)ef[(a * 3 + 2) * nreg + idx];
+        }
+    }
+}
+
+__device__ __forceinline__ void flag_nan(const NodeState &N, int node, int step, double v0, double v1, double v2)
+{
+    if (v0 != v0 || v1 != v1 || v2 != v2) {
+        if (N.nanflag[0] == 0) {
+            N.nanflag[0] = node + 1;
+            N.nanflag[1] = step;
+        }
+    }
+}
+
+// Node kernel for 3-component nodes that are not split nodes.
+// mode 0: gather, a = F/m, then the update of the next step (fused);
+// mode 1: gather, store a = F/m (the last step of a run).
+template <typename R>
+__global__ void __launch_bounds__(256) node3_kernel(NodeState N, const int32_t *list, int64_t nlist, const R *ef,
+                                                    int64_t nreg, const double *efp, int64_t npe, double dt,
+                                                    int mode, int next_step)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nlist) return;
+    const int node = list[i];
+    const int64_t nn = N.nn;
+    double F[3];
+    gather3<R>(N, node, ef, nreg, efp, npe, F);
+    const double m = N.mass[node];
+    if (mode == 1) {
+#pragma unroll
+        for (int k = 0; k < 3; k++) N.acc[k * nn + node] = F[k] / m;
+        return;
+    }
+    double vn[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const double a = F[k] / m;
+        const double v = N.vel[k * nn + node] + a * dt;
+        N.vel[k * nn + node] = v;
+        N.disp[k * nn + node] = N.disp[k * nn + node] + v * dt;
+        vn[k] = v;
+    }
+    flag_nan(N, node, next_step, vn[0], vn[1], vn[2]);
+}
+
+// 12-component (PML) nodes; the gather reads only PML element forces because
+// regular elements never touch a PML node (checked on the host).
+// src/driver.f90:107-141 for the damped split update.
+__global__ void __launch_bounds__(256) node_pml_kernel(NodeState N, PmlNodes P, const double *efp, int64_t npe,
+                                                       double dt, int mode, int next_step)
+{
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= P.n) return;
+    const int node = P.node[q];
+    const int64_t nn = N.nn, np = P.n;
+    double F[12];
+#pragma unroll
+    for (int k = 0; k < 12; k++) F[k] = 0.0;
+    const int beg = N.rowptr[node], end = N.rowptr[node + 1];
+    for (int j = beg; j < end; j++) {
+        const uint32_t e = N.ent[j];
+        const int64_t idx = e & kEntIdxMask;
+        const int a = (e >> kEntLocalShift) & 7;
+#pragma unroll
+        for (int k = 0; k < 12; k++) F[k] = F[k] + efp[(a * 12 + k) * npe + idx];
+    }
+    const double m = N.mass[node];
+    if (mode == 1) {
+#pragma unroll
+        for (int k = 0; k < 12; k++) P.acc[k * np + q] = F[k] / m;
+        return;
+    }
+    const double d[3] = {P.damp[q], P.damp[np + q], P.damp[2 * np + q]};
+    double v[12];
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        const double dv = d[k % 3];
+        v[k] = (F[k] / m + P.v12[k * np + q] * (1.0 / dt - dv / 2.0)) / (1.0 / dt + dv / 2.0);
+    }
+#pragma unroll
+    for (int k = 9; k < 12; k++) v[k] = P.v12[k * np + q] + F[k] / m * dt;
+#pragma unroll
+    for (int k = 0; k < 12; k++) P.v12[k * np + q] = v[k];
+    double vel[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        vel[k] = v[3 * k] + v[3 * k + 1] + v[3 * k + 2] + v[9 + k];
+        N.vel[k * nn + node] = vel[k];
+        N.disp[k * nn + node] = N.disp[k * nn + node] + vel[k] * dt;
+    }
+    flag_nan(N, node, next_step, vel[0], vel[1], vel[2]);
+}
+
+// Pending update at the start of a run: v += a*dt from the stored F/m
+// (velDispUpdate with the previous step's nodalForceArr).
+__global__ void __launch_bounds__(256) update3_from_acc_kernel(NodeState N, const int32_t *list, int64_t nlist,
+                                                               double dt, int next_step)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nlist) return;
+    const int node = list[i];
+    const int64_t nn = N.nn;
+    double vn[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const double v = N.vel[k * nn + node] + N.acc[k * nn + node] * dt;
+        N.vel[k * nn + node] = v;
+        N.disp[k * nn + node] = N.disp[k * nn + node] + v * dt;
+        vn[k] = v;
+    }
+    flag_nan(N, node, next_step, vn[0], vn[1], vn[2]);
+}
+
+__global__ void __launch_bounds__(256) update_pml_from_acc_kernel(NodeState N, PmlNodes P, double dt, int next_step)
+{
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= P.n) return;
+    const int node = P.node[q];
+    const int64_t nn = N.nn, np = P.n;
+    const double d[3] = {P.damp[q], P.damp[np + q], P.damp[2 * np + q]};
+    double v[12];
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        const double dv = d[k % 3];
+        v[k] = (P.acc[k * np + q] + P.v12[k * np + q] * (1.0 / dt - dv / 2.0)) / (1.0 / dt + dv / 2.0);
+    }
+#pragma unroll
+    for (int k = 9; k < 12; k++) v[k] = P.v12[k * np + q] + P.acc[k * np + q] * dt;
+#pragma unroll
+    for (int k = 0; k < 12; k++) P.v12[k * np + q] = v[k];
+    double vel[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        vel[k] = v[3 * k] + v[3 * k + 1] + v[3 * k + 2] + v[9 + k];
+        N.vel[k * nn + node] = vel[k];
+        N.disp[k * nn + node] = N.disp[k * nn + node] + vel[k] * dt;
+    }
+    flag_nan(N, node, next_step, vel[0], vel[1], vel[2]);
+}
+
+// ------------------------------------------------------------------------
+// Friction laws, src/fric.f90
+__device__ __forceinline__ double slip_weak(double slip, const double *f, int64_t nf, int64_t i)
+{
+    const double fs = f[0 * nf + i], fd = f[1 * nf + i], d0 = f[2 * nf + i];
+    double xmu = 0.0;
+    if (fabs(slip) < 1.0e-10)
+        xmu = fs;
+    else if (slip < d0)
+        xmu = fs - (fs - fd) * slip / d0;
+    if (slip >= d0) xmu = fd;
+    return xmu;
+}
+
+__device__ __forceinline__ double time_weak(double trupt, const double *f, int64_t nf, int64_t i)
+{
+    const double fs = f[0 * nf + i], fd = f[1 * nf + i], t0 = f[4 * nf + i];
+    if (trupt <= 0.0) return fs;
+    if (trupt < t0) return fs - (fs - fd) * trupt / t0;
+    return fd;
+}
+
+struct RsfParams {
+    double A, B, L, f0, V0, fw, Vw;
+};
+
+// rate_state_ageing_law, src/fric.f90:39-61
+__device__ __forceinline__ void ageing_law(double V2, double &theta, const RsfParams &P, double dt, double &xmu,
+                                           double &dxmudv)
+{
+    const double tmpc = 1.0 / (2.0 * P.V0) * exp((P.f0 + P.B * log(P.V0 * theta / P.L)) / P.A);
+    const double tmp = (V2 + 1.0e-30) * tmpc;
+    xmu = P.A * log(tmp + sqrt(tmp * tmp + 1.0));
+    dxmudv = P.A * tmpc / sqrt(1.0 + tmp * tmp);
+    theta = P.L / V2 + (theta - P.L / V2) * exp(-V2 * dt / P.L);
+}
+
+// rate_state_slip_law, src/fric.f90:63-95
+__device__ __forceinline__ void slip_law(double V2, double &psi, const RsfParams &P, double dt, double &xmu,
+                                         double &dxmudv)
+{
+    const double tmpc = 1.0 / (2.0 * P.V0) * exp(psi / P.A);
+    const double tmp = (V2 + 1.0e-30) * tmpc;
+    xmu = P.A * log(tmp + sqrt(tmp * tmp + 1.0));
+    dxmudv = P.A * tmpc / sqrt(1.0 + tmp * tmp);
+    const double fLV = P.f0 - (P.B - P.A) * log(V2 / P.V0);
+    const double r = V2 / P.Vw;
+    double r8 = r * r;
+#pragma unroll
+    for (int k = 2; k < 8; k++) r8 = r * r8;
+    const double fss = P.fw + (fLV - P.fw) / pow(1.0 + r8, 0.125);
+    const double fssa = fss / P.A;
+    const double psiss = P.A * log(2.0 * P.V0 / V2 * (exp(fssa) - exp(-fssa)) / 2.0);
+    psi = psiss + (psi - psiss) * exp(-V2 * dt / P.L);
+}
+
+// ------------------------------------------------------------------------
+// Split-node fault, one node pair per thread: getNsdSlipSliprateTraction,
+// solveSWTW / solveRSF + NewtonRaphson, on-fault stations, rupture time
+// (src/faulting.f90:3-541), then F/m and (mode 0) the next step's update of
+// the pair's two nodes.
+template <typename R>
+__global__ void __launch_bounds__(128) fault_kernel(FaultState Fs, NodeState N, StepConsts C, const R *ef,
+                                                    int64_t nreg, const double *efp, int64_t npe, int nt,
+                                                    double time, int mode)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= Fs.nf) return;
+    const int64_t nf = Fs.nf, nn = N.nn;
+    double *fr = Fs.fric;
+#define FR(k) fr[(int64_t)((k) - 1) * nf + i]
+    const double dt = C.dt, Ce = (double)C.C_elastic;
+    const int ns = Fs.nsmp[i], nm = Fs.nsmp[nf + i];
+    const double un[3] = {Fs.un[i], Fs.un[nf + i], Fs.un[2 * nf + i]};
+    const double us[3] = {Fs.us[i], Fs.us[nf + i], Fs.us[2 * nf + i]};
+    const double ud[3] = {Fs.ud[i], Fs.ud[nf + i], Fs.ud[2 * nf + i]};
+    const double arn = Fs.arn[i];
+    double Fsl[3], Fma[3];
+    gather3<R>(N, ns, ef, nreg, efp, npe, Fsl);
+    gather3<R>(N, nm, ef, nreg, efp, npe, Fma);
+    double q[3][2][3];  // [xyz][slave/master][force, vel, disp]
+    for (int k = 0; k < 3; k++) {
+        q[k][0][0] = Fsl[k];
+        q[k][1][0] = Fma[k];
+        q[k][0][1] = N.vel[k * nn + ns];
+        q[k][1][1] = N.vel[k * nn + nm];
+        q[k][0][2] = N.disp[k * nn + ns];
+        q[k][1][2] = N.disp[k * nn + nm];
+    }
+    double nsd[3][2][3];
+    for (int j = 0; j < 3; j++)
+        for (int s = 0; s < 2; s++) {
+            nsd[0][s][j] = q[0][s][j] * un[0] + q[1][s][j] * un[1] + q[2][s][j] * un[2];
+            nsd[1][s][j] = q[0][s][j] * us[0] + q[1][s][j] * us[1] + q[2][s][j] * us[2];
+            nsd[2][s][j] = q[0][s][j] * ud[0] + q[1][s][j] * ud[1] + q[2][s][j] * ud[2];
+        }
+    double slip[4], rate[4], T[4], T0[3];
+    T0[0] = FR(7);
+    T0[1] = FR(8) + 0.0;
+    T0[2] = FR(49);
+    const double ms = Fs.fms[i], mm = Fs.fmm[i];
+    const double totalMass = (ms + mm) * arn;
+    for (int j = 0; j < 3; j++) slip[j] = nsd[j][1][2] - nsd[j][0][2];
+    slip[3] = sqrt(slip[0] * slip[0] + slip[1] * slip[1] + slip[2] * slip[2]);
+    for (int j = 0; j < 3; j++) rate[j] = nsd[j][1][1] - nsd[j][0][1];
+    rate[3] = sqrt(rate[0] * rate[0] + rate[1] * rate[1] + rate[2] * rate[2]);
+    FR(71) = slip[1];
+    FR(72) = slip[2];
+    FR(73) = slip[0];
+    FR(74) = rate[1];
+    FR(75) = rate[2];
+    if (rate[3] > FR(76)) FR(76) = rate[3];
+    FR(77) = FR(77) + rate[3] * dt;
+    T[0] = (ms * mm * ((nsd[0][1][1] - nsd[0][0][1]) + (nsd[0][1][2] - nsd[0][0][2]) / dt) / dt + ms * nsd[0][1][0] -
+            mm * nsd[0][0][0]) / totalMass + T0[0] * Ce;
+    T[1] = (ms * mm * (nsd[1][1][1] - nsd[1][0][1]) / dt + ms * nsd[1][1][0] - mm * nsd[1][0][0]) / totalMass + T0[1] * Ce;
+    T[2] = (ms * mm * (nsd[2][1][1] - nsd[2][0][1]) / dt + ms * nsd[2][1][0] - mm * nsd[2][0][0]) / totalMass + T0[2] * Ce;
+
+    const double sx = Fs.xyz[i], sy = Fs.xyz[nf + i], sz = Fs.xyz[2 * nf + i];
+    if (C.friclaw >= 3 && C.C_nuclea == 1 && C.nucfault == 1) {
+        // rsfNucleation, src/faulting.f90:367-414
+        double dtau = 0.0;
+        const double radius = sqrt((sx - C.xsource) * (sx - C.xsource) + (sy - C.ysource) * (sy - C.ysource) +
+                                   (sz - C.zsource) * (sz - C.zsource));
+        double Fn = 0.0, G = 1.0;
+        if (radius < C.nucR) Fn = exp(radius * radius / (radius * radius - C.nucR * C.nucR));
+        if (time <= C.nucT) G = exp((time - C.nucT) * (time - C.nucT) / (time * (time - 2.0 * C.nucT)));
+        if (C.TPV == 105 || C.TPV == 104) {
+            dtau = C.nucdtau0 * Fn * G;
+        } else if (C.TPV == 2802) {
+            if (nt == 1) {
+                FR(81) = C.nucdtau0;
+                const double ttao = sqrt(T[1] * T[1] + T[2] * T[2]);
+                const double back = sqrt((rate[1] + FR(26)) * (rate[1] + FR(26)) + (rate[2] + FR(27)) * (rate[2] + FR(27)));
+                FR(20) = FR(9) * log(2.0 * FR(12) / back * sinh(ttao / fabs(T[0]) / FR(9)));
+                FR(23) = fabs(T[0]);
+            }
+            dtau = FR(81) * Fn * G;
+        }
+        T[1] = T[1] + dtau;
+    }
+    T[3] = sqrt(T[1] * T[1] + T[2] * T[2]);
+
+    if (C.friclaw <= 2) {
+        // solveSWTW, src/faulting.f90:136-188
+        double mu = 0.0;
+        if (C.friclaw == 1)
+            mu = slip_weak(FR(77), fr, nf, i);
+        else
+            mu = time_weak(time - Fs.fnft[i], fr, nf, i);
+        if (C.C_nuclea == 1 && C.nucfault == 1) {
+            // swtwNucleation, src/faulting.f90:416-443
+            const double radius = sqrt((sx - C.xsource) * (sx - C.xsource) + (sy - C.ysource) * (sy - C.ysource) +
+                                       (sz - C.zsource) * (sz - C.zsource));
+            double tr = 1.0e9, tc;
+            if (radius <= C.nucR) {
+                if (C.TPV == 201 || C.TPV == 36 || C.TPV == 37)
+                    tr = (radius + 0.081 * C.nucR * (1.0 / (1.0 - (radius / C.nucR) * (radius / C.nucR)) - 1.0)) /
+                         (0.7 * 3464.0);
+                if (C.TPV == 202) tr = radius / C.nucRuptVel;
+            }
+            tc = 1.0;
+            if (time < tr)
+                tc = 0.0;
+            else if ((time < (tr + FR(5))) && (time >= tr))
+                tc = (time - tr) / FR(5);
+            const double cand = FR(1) + (FR(2) - FR(1)) * tc;
+            mu = cand < mu ? cand : mu;
+        }
+        const double eff = (T[0] + FR(6)) > 0 ? 0.0 : T[0] + FR(6);
+        const double trialShear = FR(4) - mu * eff;
+        if (T[3] > trialShear) {
+            T[1] = T[1] * trialShear / T[3];
+            T[2] = T[2] * trialShear / T[3];
+        }
+        for (int j = 0; j < 3; j++) {
+            const double xT = (T[0] * un[j] + T[1] * us[j] + T[2] * ud[j]) * arn;
+            const double xT0 = (T0[0] * un[j] + T0[1] * us[j] + T0[2] * ud[j]) * arn;
+            Fsl[j] = Fsl[j] + xT - xT0 * Ce;
+            Fma[j] = Fma[j] - xT + xT0 * Ce;
+        }
+        FR(78) = T[0];
+        FR(79) = T[1];
+        FR(80) = T[2];
+    } else {
+        // solveRSF + NewtonRaphson, src/faulting.f90:190-327, 459-516
+        const RsfParams P = {FR(9), FR(10), FR(11), FR(13), FR(12), FR(14), FR(15)};
+        double trial[4];
+        if (C.friclaw == 5)
+            T[0] = T[0] + FR(51);
+        else
+            T[0] = T[0] + FR(6);
+        if (C.insertFaultType > 0 && C.C_elastic == 1) {
+            const double max_norm = -40.0e6, min_norm = -10.0e6;
+            if (T[0] >= min_norm)
+                T[0] = min_norm;
+            else if (T[0] <= max_norm)
+                T[0] = max_norm;
+        }
+        if (T[0] > 0.0) T[0] = 0.0;
+        for (int j = 0; j < 3; j++) {
+            slip[j] = slip[j] + FR(25 + j) * time;
+            rate[j] = rate[j] + FR(25 + j);
+        }
+        slip[3] = sqrt(slip[1] * slip[1] + slip[2] * slip[2]);
+        rate[3] = sqrt(rate[1] * rate[1] + rate[2] * rate[2]);
+        double v_trial = rate[3];
+        double theta_pc_tmp = FR(23);
+        // rate_state_normal_stress, src/faulting.f90:35-52
+        double theta_pc_dot = -(v_trial / P.L * (FR(23) - fabs(T[0])));
+        FR(23) = FR(23) + theta_pc_dot * dt;
+        FR(24) = theta_pc_dot;
+        double statetmp = FR(20);
+        double xmu, dxmudv, st20 = FR(20);
+        if (C.friclaw == 3)
+            ageing_law(v_trial, st20, P, dt, xmu, dxmudv);
+        else
+            slip_law(v_trial, st20, P, dt, xmu, dxmudv);
+        FR(20) = st20;
+        const double taoc_old = C.friclaw == 5 ? FR(4) - xmu * T[0] : xmu * theta_pc_tmp;
+        const double mr = mm * ms / (mm + ms);
+        const double Tc = arn * dt / mr;
+        for (int j = 1; j <= 2; j++) trial[j] = T[j] - taoc_old * 0.5 * (rate[j] / rate[3]) + FR(25 + j) / Tc;
+        trial[3] = sqrt(trial[1] * trial[1] + trial[2] * trial[2]);
+        double taoc_new = 0.0, stateTmp = 0.0, thetaPcTmp = 0.0;
+        for (int iv = 1; iv <= 20; iv++) {
+            double rsfeq, drsfeqdv;
+            stateTmp = statetmp;
+            if (C.friclaw == 3)
+                ageing_law(v_trial, stateTmp, P, dt, xmu, dxmudv);
+            else
+                slip_law(v_trial, stateTmp, P, dt, xmu, dxmudv);
+            if (C.friclaw < 5) {
+                thetaPcTmp = theta_pc_tmp;
+                theta_pc_dot = -(v_trial / P.L * (thetaPcTmp - fabs(T[0])));
+                thetaPcTmp = thetaPcTmp + theta_pc_dot * dt;
+                taoc_new = xmu * thetaPcTmp;
+                rsfeq = v_trial + Tc * (taoc_new * 0.5 - trial[3]);
+                drsfeqdv = 1.0 + Tc * (dxmudv * thetaPcTmp) * 0.5;
+            } else {
+                const double tn = T[0] < 0.0 ? T[0] : 0.0;
+                taoc_new = FR(4) - xmu * tn;
+                rsfeq = v_trial + Tc * (taoc_new * 0.5 - trial[3]);
+                drsfeqdv = 1.0 + Tc * (-dxmudv * tn) * 0.5;
+            }
+            if (fabs(rsfeq / drsfeqdv) < 1.0e-14 * fabs(v_trial) && fabs(rsfeq) < 1.0e-6 * fabs(v_trial)) break;
+            const double nv = v_trial - rsfeq / drsfeqdv;
+            if (nv <= 0.0)
+                v_trial = v_trial / 2.0;
+            else
+                v_trial = nv;
+        }
+        if (C.TPV == 105 && v_trial < FR(46)) v_trial = FR(46);
+        FR(20) = stateTmp;
+        FR(23) = thetaPcTmp;
+        for (int j = 1; j <= 2; j++)
+            T[j] = taoc_old * 0.5 * (rate[j] / rate[3]) + taoc_new * 0.5 * (trial[j] / trial[3]);
+        FR(78) = T[0];
+        FR(79) = T[1];
+        FR(80) = T[2];
+        FR(47) = v_trial;
+        FR(48) = sqrt(T[1] * T[1] + T[2] * T[2]);
+        double acc[3], xyzAcc[3], xyzR[3];
+        acc[0] = -rate[0] / dt - slip[0] / dt / dt;
+        acc[1] = (v_trial * (trial[1] / trial[3]) - rate[1]) / dt;
+        acc[2] = (v_trial * (trial[2] / trial[3]) - rate[2]) / dt;
+        for (int j = 0; j < 3; j++) {
+            xyzAcc[j] = acc[0] * un[j] + acc[1] * us[j] + acc[2] * ud[j];
+            xyzR[j] = Fsl[j] + Fma[j];
+        }
+        for (int j = 0; j < 3; j++) {
+            Fsl[j] = (-xyzAcc[j] + xyzR[j] / mm) * mr;
+            Fma[j] = (xyzAcc[j] + xyzR[j] / ms) * mr;
+            FR(31 + j) = q[j][1][1] + (xyzAcc[j] + xyzR[j] / ms) * dt;
+            FR(34 + j) = q[j][0][1] + (-xyzAcc[j] + xyzR[j] / mm) * dt;
+        }
+    }
+    // storeOnFaultStationQuantSCEC, src/faulting.f90:518-541
+    const int sta = Fs.station[i];
+    if (sta >= 0) {
+        double *h = Fs.hist + ((int64_t)sta * Fs.nstep + (nt - 1)) * 12;
+        h[0] = time;
+        h[1] = rate[1];
+        h[2] = rate[2];
+        h[3] = FR(20);
+        h[4] = slip[1];
+        h[5] = slip[2];
+        h[6] = slip[0];
+        h[7] = T[1];
+        h[8] = T[2];
+        h[9] = T[0];
+        h[10] = FR(51) + FR(42);
+        h[11] = FR(52);
+    }
+    // storeRuptureTime, src/faulting.f90:329-341
+    if (Fs.fnft[i] > 5000.0 && rate[3] >= C.slipRateThres) Fs.fnft[i] = time;
+#undef FR
+    // F/m (src/driver.f90:29) and the pair's update for the next step
+    const double msl = N.mass[ns], mma = N.mass[nm];
+    if (mode == 1) {
+        for (int k = 0; k < 3; k++) {
+            N.acc[k * nn + ns] = Fsl[k] / msl;
+            N.acc[k * nn + nm] = Fma[k] / mma;
+        }
+        return;
+    }
+    double vs[3], vm[3];
+    for (int k = 0; k < 3; k++) {
+        vs[k] = q[k][0][1] + Fsl[k] / msl * dt;
+        vm[k] = q[k][1][1] + Fma[k] / mma * dt;
+        N.vel[k * nn + ns] = vs[k];
+        N.vel[k * nn + nm] = vm[k];
+        N.disp[k * nn + ns] = q[k][0][2] + vs[k] * dt;
+        N.disp[k * nn + nm] = q[k][1][2] + vm[k] * dt;
+    }
+    flag_nan(N, ns, nt + 1, vs[0], vs[1], vs[2]);
+    flag_nan(N, nm, nt + 1, vm[0], vm[1], vm[2]);
+}
+
+// storeOffFaultStData, src/driver.f90:157-180
+__global__ void station_kernel(OffStations S, NodeState N, int nt, double time)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r > S.nrow) return;
+    double *col = S.hist + (int64_t)(nt - 1) * (S.nrow + 1);
+    if (r == 0) {
+        col[0] = time;
+        return;
+    }
+    const int i = r - 1;
+    const int node = S.idh[i], k = S.idh[S.nrow + i], kind = S.idh[2 * S.nrow + i];
+    if (kind == 1)
+        col[r] = N.disp[k * N.nn + node];
+    else if (kind == 2)
+        col[r] = N.vel[k * N.nn + node];
+}
+
+// ------------------------------------------------------------------------
+// launchers (host side)
+static inline unsigned grid_for(int64_t n, int block) { return (unsigned)((n + block - 1) / block); }
+
+template <typename R>
+hipError_t launch_elem_regular(const RegElems<R> &E, const NodeState &N, const StepConsts &C, bool q, bool dp,
+                               int hg, bool mass, hipStream_t s)
+{
+    if (E.n == 0) return hipSuccess;
+    const unsigned g = grid_for(E.n, 256);
+#define EQD_LAUNCH(Q, DP, HG, MASS) elem_regular_kernel<R, Q, DP, HG, MASS><<<g, 256, 0, s>>>(E, N, C)
+    if (q && mass) {
+        if (hg == 1) EQD_LAUNCH(true, false, 1, true);
+        else if (hg == 2) EQD_LAUNCH(true, false, 2, true);
+        else EQD_LAUNCH(true, false, 0, true);
+    } else if (q) {
+        if (hg == 1) EQD_LAUNCH(true, false, 1, false);
+        else if (hg == 2) EQD_LAUNCH(true, false, 2, false);
+        else EQD_LAUNCH(true, false, 0, false);
+    } else if (dp) {
+        if (hg == 1) EQD_LAUNCH(false, true, 1, true);
+        else if (hg == 2) EQD_LAUNCH(false, true, 2, true);
+        else EQD_LAUNCH(false, true, 0, true);
+    } else if (mass) {
+        if (hg == 1) EQD_LAUNCH(false, false, 1, true);
+        else if (hg == 2) EQD_LAUNCH(false, false, 2, true);
+        else EQD_LAUNCH(false, false, 0, true);
+    } else {
+        if (hg == 1) EQD_LAUNCH(false, false, 1, false);
+        else if (hg == 2) EQD_LAUNCH(false, false, 2, false);
+        else EQD_LAUNCH(false, false, 0, false);
+    }
+#undef EQD_LAUNCH
+    return hipGetLastError();
+}
+
+hipError_t launch_elem_pml(const PmlElems &E, const NodeState &N, const StepConsts &C, int hg, bool mass, hipStream_t s)
+{
+    if (E.n == 0) return hipSuccess;
+    const unsigned g = grid_for(E.n, 256);
+    if (mass) {
+        if (hg == 1) elem_pml_kernel<1, true><<<g, 256, 0, s>>>(E, N, C);
+        else if (hg == 2) elem_pml_kernel<2, true><<<g, 256, 0, s>>>(E, N, C);
+        else elem_pml_kernel<0, true><<<g, 256, 0, s>>>(E, N, C);
+    } else {
+        if (hg == 1) elem_pml_kernel<1, false><<<g, 256, 0, s>>>(E, N, C);
+        else if (hg == 2) elem_pml_kernel<2, false><<<g, 256, 0, s>>>(E, N, C);
+        else elem_pml_kernel<0, false><<<g, 256, 0, s>>>(E, N, C);
+    }
+    return hipGetLastError();
+}
+
+template <typename R>
+hipError_t launch_node3(const NodeState &N, const int32_t *list, int64_t nlist, const R *ef, int64_t nreg,
+                        const double *efp, int64_t npe, double dt, int mode, int next_step, hipStream_t s)
+{
+    if (nlist == 0) return hipSuccess;
+    node3_kernel<R><<<grid_for(nlist, 256), 256, 0, s>>>(N, list, nlist, ef, nreg, efp, npe, dt, mode, next_step);
+    return hipGetLastError();
+}
+
+hipError_t launch_node_pml(const NodeState &N, const PmlNodes &P, const double *efp, int64_t npe, double dt, int mode,
+                           int next_step, hipStream_t s)
+{
+    if (P.n == 0) return hipSuccess;
+    node_pml_kernel<<<grid_for(P.n, 256), 256, 0, s>>>(N, P, efp, npe, dt, mode, next_step);
+    return hipGetLastError();
+}
+
+hipError_t launch_update_from_acc(const NodeState &N, const int32_t *list3, int64_t n3, const PmlNodes &P, double dt,
+                                  int next_step, hipStream_t s)
+{
+    if (n3 > 0) update3_from_acc_kernel<<<grid_for(n3, 256), 256, 0, s>>>(N, list3, n3, dt, next_step);
+    if (P.n > 0) update_pml_from_acc_kernel<<<grid_for(P.n, 256), 256, 0, s>>>(N, P, dt, next_step);
+    return hipGetLastError();
+}
+
+template <typename R>
+hipError_t launch_fault(const FaultState &F, const NodeState &N, const StepConsts &C, const R *ef, int64_t nreg,
+                        const double *efp, int64_t npe, int nt, double time, int mode, hipStream_t s)
+{
+    if (F.nf == 0) return hipSuccess;
+    fault_kernel<R><<<grid_for(F.nf, 128), 128, 0, s>>>(F, N, C, ef, nreg, efp, npe, nt, time, mode);
+    return hipGetLastError();
+}
+
+hipError_t launch_station(const OffStations &S, const NodeState &N, int nt, double time, hipStream_t s)
+{
+    if (S.nrow == 0) return hipSuccess;
+    station_kernel<<<grid_for(S.nrow + 1, 64), 64, 0, s>>>(S, N, nt, time);
+    return hipGetLastError();
+}
+
+template hipError_t launch_elem_regular<double>(const RegElems<double> &, const NodeState &, const StepConsts &, bool,
+                                                bool, int, bool, hipStream_t);
+template hipError_t launch_elem_regular<float>(const RegElems<float> &, const NodeState &, const StepConsts &, bool,
+                                               bool, int, bool, hipStream_t);
+template hipError_t launch_node3<double>(const NodeState &, const int32_t *, int64_t, const double *, int64_t,
+                                         const double *, int64_t, double, int, int, hipStream_t);
+template hipError_t launch_node3<float>(const NodeState &, const int32_t *, int64_t, const float *, int64_t,
+                                        const double *, int64_t, double, int, int, hipStream_t);
+template hipError_t launch_fault<double>(const FaultState &, const NodeState &, const StepConsts &, const double *,
+                                         int64_t, const double *, int64_t, int, double, int, hipStream_t);
+template hipError_t launch_fault<float>(const FaultState &, const NodeState &, const StepConsts &, const float *,
+                                        int64_t, const double *, int64_t, int, double, int, hipStream_t);
+
+}  // namespace eqd

@@ -1,0 +1,198 @@
+"""Python front end of the eqdyna_amd C-ABI (include/eqdyna_amd.h).
+
+``Solver`` takes the reference's globalvar arrays (a bundle as produced by the
+reference host or by ``eqdyna_amd.mesh``), hands them to libeqd.so, advances
+time steps on the GPU and returns the evolved arrays in the same layout.
+There is no CPU fallback: without the built library or a GPU this raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Dict
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libeqd.so")
+
+EXPORTS = ["eqd_create", "eqd_set", "eqd_commit", "eqd_run", "eqd_get", "eqd_clock", "eqd_nan_info",
+           "eqd_kernel_time", "eqd_set_timing", "eqd_device_bytes", "eqd_last_error", "eqd_destroy"]
+
+FIELDS = {
+    "meshCoor": (1, "f8"), "eqNumIndexArr": (2, "i4"), "eqNumStartIndexLoc": (3, "i4"),
+    "numOfDofPerNodeArr": (4, "i4"), "nodeElemIdRelation": (5, "i4"), "elemTypeArr": (6, "i4"),
+    "mat": (7, "f8"), "eleshp": (8, "f8"), "eledet": (9, "f8"), "elemass": (10, "f8"), "ss": (11, "f8"),
+    "phi": (12, "f8"), "stressCompIndexArr": (13, "i4"), "eleporep": (14, "f8"),
+    "nodalMassArr": (15, "f8"), "fnms": (16, "f8"), "nsmp": (17, "i4"), "un": (18, "f8"),
+    "us": (19, "f8"), "ud": (20, "f8"), "arn": (21, "f8"), "anonfs": (22, "i4"), "idhist": (23, "i4"),
+    "stressArr": (24, "f8"), "pstrain": (25, "f8"), "v1": (26, "f8"), "velArr": (27, "f8"),
+    "dispArr": (28, "f8"), "nodalForceArr": (29, "f8"), "fric": (30, "f8"), "fnft": (31, "f8"),
+    "onFaultQuantHistSCECForm": (32, "f8"), "OffFaultStGramSCEC": (33, "f8"),
+}
+INPUTS = ["meshCoor", "eqNumIndexArr", "eqNumStartIndexLoc", "numOfDofPerNodeArr", "nodeElemIdRelation",
+          "elemTypeArr", "mat", "eleshp", "eledet", "elemass", "ss", "phi", "stressCompIndexArr", "eleporep",
+          "nodalMassArr", "fnms", "nsmp", "un", "us", "ud", "arn", "anonfs", "idhist", "stressArr", "pstrain",
+          "v1", "velArr", "dispArr", "nodalForceArr", "fric", "fnft"]
+OUTPUTS = ["fric", "fnft", "velArr", "dispArr", "v1", "nodalForceArr", "stressArr", "pstrain",
+           "onFaultQuantHistSCECForm", "OffFaultStGramSCEC"]
+
+PREC = {"fp64": 0, "fp32": 1}
+
+
+class Scalars(C.Structure):
+    _fields_ = [(n, C.c_double) for n in
+                ("dt", "rdampk", "rdampm", "w", "grav", "roumax", "rhow", "gamar",
+                 "ccosphi", "sinphi", "tv", "slipRateThres",
+                 "xsource", "ysource", "zsource", "nucR", "nucRuptVel", "nucdtau0", "nucT",
+                 "kapa_hg", "dx", "R", "vmaxPML")] + \
+              [("PMLb", C.c_double * 8), ("pi", C.c_double)] + \
+              [(n, C.c_int32) for n in
+               ("nstep", "nn", "ne", "neq", "size_eqidx", "size_stress", "nf", "non", "noff", "nPML",
+                "C_elastic", "C_Q", "C_hg", "C_nuclea", "friclaw", "TPV", "insertFaultType", "nucfault",
+                "precision", "device")]
+
+
+class EqdError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"eqdyna_amd error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def lib():
+    """Load libeqd.so; raises if it is not built (there is no fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise FileNotFoundError(f"{LIB_PATH} is missing: build it with __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        L.eqd_create.argtypes = [C.POINTER(Scalars), C.POINTER(C.c_void_p)]
+        L.eqd_set.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_int64]
+        L.eqd_commit.argtypes = [C.c_void_p]
+        L.eqd_run.argtypes = [C.c_void_p, C.c_int32]
+        L.eqd_get.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_int64]
+        L.eqd_clock.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_double)]
+        L.eqd_nan_info.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
+        L.eqd_kernel_time.argtypes = [C.c_void_p, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
+        L.eqd_set_timing.argtypes = [C.c_void_p, C.c_int32]
+        L.eqd_device_bytes.argtypes = [C.c_void_p]
+        L.eqd_device_bytes.restype = C.c_int64
+        L.eqd_last_error.argtypes = [C.c_void_p]
+        L.eqd_last_error.restype = C.c_char_p
+        L.eqd_destroy.argtypes = [C.c_void_p]
+        _lib = L
+    return _lib
+
+
+def make_scalars(sc: Dict[str, float], precision: str = "fp64", device: int = 0,
+                 nstep: int | None = None) -> Scalars:
+    s = Scalars()
+    for n in ("dt", "rdampk", "rdampm", "w", "grav", "roumax", "rhow", "gamar", "ccosphi", "sinphi", "tv",
+              "slipRateThres", "xsource", "ysource", "zsource", "nucR", "nucRuptVel", "nucdtau0", "nucT",
+              "kapa_hg", "dx", "R", "vmaxPML", "pi"):
+        setattr(s, n, float(sc[n]))
+    for i in range(8):
+        s.PMLb[i] = float(sc[f"PMLb{i + 1}"])
+    s.nstep = int(nstep if nstep is not None else sc["nstep"])
+    s.nn = int(sc["totalNumOfNodes"])
+    s.ne = int(sc["totalNumOfElements"])
+    s.neq = int(sc["totalNumOfEquations"])
+    s.size_eqidx = int(sc["sizeOfEqNumIndexArr"])
+    s.size_stress = int(sc["sizeOfStressDofIndexArr"])
+    s.nf = int(sc["nftnd"])
+    s.non = int(sc["numOfOnFaultStCount"])
+    s.noff = int(sc["numOfOffFaultStCount"])
+    s.nPML = int(sc["nPML"])
+    for n in ("C_elastic", "C_Q", "C_hg", "C_nuclea", "friclaw", "TPV", "insertFaultType", "nucfault"):
+        setattr(s, n, int(sc[n]))
+    s.precision = PREC[precision]
+    s.device = int(device)
+    return s
+
+
+class Solver:
+    """One GPU-resident rupture scenario."""
+
+    def __init__(self, bundle: Dict[str, np.ndarray], sc: Dict[str, float], precision: str = "fp64",
+                 device: int = 0, nstep: int | None = None, timing: bool = False):
+        self.L = lib()
+        self.sc = dict(sc)
+        self.s = make_scalars(sc, precision, device, nstep)
+        h = C.c_void_p()
+        rc = self.L.eqd_create(C.byref(self.s), C.byref(h))
+        if rc != 0:
+            raise EqdError(rc, "eqd_create rejected the scalars")
+        self.h = h
+        try:
+            for name in INPUTS:
+                if name not in bundle:
+                    continue
+                fid, dt = FIELDS[name]
+                a = np.ascontiguousarray(bundle[name], dtype="<" + dt)
+                self._check(self.L.eqd_set(self.h, fid, a.ctypes.data, a.size))
+            if timing:
+                self._check(self.L.eqd_set_timing(self.h, 1))
+            self._check(self.L.eqd_commit(self.h))
+        except Exception:
+            self.close()
+            raise
+
+    def _check(self, rc: int) -> None:
+        if rc != 0:
+            msg = self.L.eqd_last_error(self.h).decode()
+            raise EqdError(rc, msg)
+
+    def _count(self, name: str) -> tuple:
+        s = self.s
+        non = max(s.non, 1)
+        shapes = {
+            "fric": (s.nf, 100), "fnft": (s.nf,), "velArr": (s.nn, 3), "dispArr": (s.nn, 3),
+            "v1": (s.neq,), "nodalForceArr": (s.neq,), "stressArr": (s.size_stress,), "pstrain": (s.ne,),
+            "onFaultQuantHistSCECForm": (non, s.nstep, 12), "OffFaultStGramSCEC": (s.nstep, 6 * s.noff + 1),
+        }
+        return shapes[name]
+
+    def run(self, nsteps: int) -> None:
+        self._check(self.L.eqd_run(self.h, int(nsteps)))
+
+    def set_timing(self, on: bool) -> None:
+        self._check(self.L.eqd_set_timing(self.h, 1 if on else 0))
+
+    def get(self, name: str) -> np.ndarray:
+        fid, dt = FIELDS[name]
+        out = np.zeros(self._count(name), dtype="<" + dt)
+        self._check(self.L.eqd_get(self.h, fid, out.ctypes.data, out.size))
+        return out
+
+    def outputs(self) -> Dict[str, np.ndarray]:
+        o = {k: self.get(k) for k in OUTPUTS if not (k == "OffFaultStGramSCEC" and self.s.noff == 0)}
+        return o
+
+    def clock(self):
+        nt = C.c_int32()
+        t = C.c_double()
+        self._check(self.L.eqd_clock(self.h, C.byref(nt), C.byref(t)))
+        return nt.value, t.value
+
+    def kernel_time(self, name: str):
+        ms = C.c_double()
+        n = C.c_int64()
+        self._check(self.L.eqd_kernel_time(self.h, name.encode(), C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+    def device_bytes(self) -> int:
+        return int(self.L.eqd_device_bytes(self.h))
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            self.L.eqd_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,143 @@
+/*
+ * eqdyna_amd: MI355X (gfx950) explicit time step for EQdyna-style dynamic
+ * rupture, behind a plain C ABI.
+ *
+ * The reference calls its time loop as an argument-less subroutine,
+ *     call driver                      (src/eqdyna3d.f90:72, body src/driver.f90:3-36)
+ * with all data in module globalvar (src/globalvar.f90:3-103).  This ABI is
+ * what a Fortran iso_c_binding shim replacing driver.f90 binds: the host hands
+ * over globalvar's arrays exactly as they lie in Fortran memory (column-major,
+ * 1-based ids, the layout listed in src/globalvar.f90:78-101), the library
+ * keeps the whole mesh resident in GPU memory, advances steps, and copies the
+ * state back into the same arrays so the reference's output routines
+ * (src/library_output.f90) can write them unchanged.
+ *
+ * Conventions: every function returns 0 on success or a negative EQD_E* code;
+ * eqd_last_error() gives the message.  Handles are not re-entrant; one handle
+ * drives one device stream.
+ */
+#ifndef EQDYNA_AMD_H
+#define EQDYNA_AMD_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EQD_OK 0
+#define EQD_EINVAL (-1)      /* bad argument, size or unsupported combination */
+#define EQD_ENAN (-2)        /* NaN velocity (reference: src/driver.f90:147-152) */
+#define EQD_EHIP (-3)        /* HIP runtime error */
+#define EQD_EOOM (-4)        /* device allocation failed */
+#define EQD_ESTATE (-5)      /* call out of order (e.g. run before commit) */
+
+/* Arithmetic of the bulk (element force + hourglass) path.  The split-node
+ * fault, the rate-state Newton solve and the nodal update are FP64 in every
+ * mode. */
+#define EQD_PREC_FP64 0
+#define EQD_PREC_FP32 1
+
+/* Scalars of module globalvar that the time step reads.  Names are the
+ * reference's; counts are the allocation sizes of the arrays below. */
+typedef struct {
+    double dt, rdampk, rdampm, w, grav, roumax, rhow, gamar;
+    double ccosphi, sinphi, tv, slipRateThres;
+    double xsource, ysource, zsource, nucR, nucRuptVel, nucdtau0, nucT;
+    double kapa_hg, dx, R, vmaxPML, PMLb[8], pi;
+    int32_t nstep;                    /* length of the station histories */
+    int32_t nn, ne, neq;              /* totalNumOfNodes/Elements/Equations */
+    int32_t size_eqidx;               /* sizeOfEqNumIndexArr */
+    int32_t size_stress;              /* sizeOfStressDofIndexArr */
+    int32_t nf;                       /* nftnd(1): split-node pairs */
+    int32_t non, noff;                /* numOfOnFaultStCount, numOfOffFaultStCount */
+    int32_t nPML;
+    int32_t C_elastic, C_Q, C_hg, C_nuclea, friclaw, TPV, insertFaultType, nucfault;
+    int32_t precision;                /* EQD_PREC_* */
+    int32_t device;                   /* HIP device ordinal */
+} eqd_scalars;
+
+/* Array ids for eqd_set / eqd_get.  Shapes are Fortran (column-major). */
+enum eqd_field {
+    EQD_MESHCOOR = 1,            /* f64 (3, nn)          in            */
+    EQD_EQNUMINDEXARR,           /* i32 (size_eqidx)     in            */
+    EQD_EQNUMSTARTINDEXLOC,      /* i32 (nn)             in            */
+    EQD_NUMOFDOFPERNODEARR,      /* i32 (nn)             in            */
+    EQD_NODEELEMIDRELATION,      /* i32 (8, ne)          in            */
+    EQD_ELEMTYPEARR,             /* i32 (ne)             in            */
+    EQD_MAT,                     /* f64 (ne, 5)          in            */
+    EQD_ELESHP,                  /* f64 (3, 8, ne)       in            */
+    EQD_ELEDET,                  /* f64 (ne)             in            */
+    EQD_ELEMASS,                 /* f64 (24, ne)         in            */
+    EQD_SS,                      /* f64 (6, ne)          in            */
+    EQD_PHI,                     /* f64 (8, 4, ne)       in            */
+    EQD_STRESSCOMPINDEXARR,      /* i32 (ne)             in            */
+    EQD_ELEPOREP,                /* f64 (ne)             in            */
+    EQD_NODALMASSARR,            /* f64 (neq)            in            */
+    EQD_FNMS,                    /* f64 (nn)             in            */
+    EQD_NSMP,                    /* i32 (2, nf)          in            */
+    EQD_UN,                      /* f64 (3, nf)          in            */
+    EQD_US,                      /* f64 (3, nf)          in            */
+    EQD_UD,                      /* f64 (3, nf)          in            */
+    EQD_ARN,                     /* f64 (nf)             in            */
+    EQD_ANONFS,                  /* i32 (3, non)         in            */
+    EQD_IDHIST,                  /* i32 (3, 6*noff)      in            */
+    EQD_STRESSARR,               /* f64 (size_stress)    in/out        */
+    EQD_PSTRAIN,                 /* f64 (ne)             in/out        */
+    EQD_V1,                      /* f64 (neq)            in/out        */
+    EQD_VELARR,                  /* f64 (3, nn)          in/out        */
+    EQD_DISPARR,                 /* f64 (3, nn)          in/out        */
+    EQD_NODALFORCEARR,           /* f64 (neq)            in/out (holds F/m between steps) */
+    EQD_FRIC,                    /* f64 (100, nf)        in/out        */
+    EQD_FNFT,                    /* f64 (nf)             in/out        */
+    EQD_ONFAULTHIST,             /* f64 (12, nstep, non) out: onFaultQuantHistSCECForm */
+    EQD_OFFFAULTHIST,            /* f64 (6*noff+1, nstep) out: OffFaultStGramSCEC */
+    EQD_FIELD_COUNT
+};
+
+typedef struct eqd_handle eqd_handle;
+
+/* Replaces the state set up by allocInit/meshgen/assembleGlobalMass
+ * (src/eqdyna3d.f90:54-70): stores the scalars; arrays follow via eqd_set. */
+int eqd_create(const eqd_scalars *s, eqd_handle **h);
+
+/* Hand over one globalvar array (copied; the host buffer may be reused). */
+int eqd_set(eqd_handle *h, int32_t field, const void *host, int64_t count);
+
+/* Validate, build the device layout (class-sorted SoA element arrays, the
+ * node-to-element gather table, PML damping, Q tables) and upload. */
+int eqd_commit(eqd_handle *h);
+
+/* Advance nsteps steps of src/driver.f90:9-34 (velDispUpdate,
+ * storeOffFaultStData, assembleGlobalKU, hrglss, faulting, F/m), continuing the
+ * step counter nt and timeElapsed across calls.  Returns EQD_ENAN if a velocity
+ * becomes NaN; eqd_nan_info then names the node and step. */
+int eqd_run(eqd_handle *h, int32_t nsteps);
+
+/* Copy one array back into reference layout. */
+int eqd_get(eqd_handle *h, int32_t field, void *host, int64_t count);
+
+/* Clock of the time loop (nt, timeElapsed of src/driver.f90:9-11). */
+int eqd_clock(eqd_handle *h, int32_t *nt, double *time_elapsed);
+
+/* Node id (1-based) and step of the first NaN velocity, 0 if none. */
+int eqd_nan_info(eqd_handle *h, int32_t *node, int32_t *nt);
+
+/* Average device time per launch of each kernel over the last eqd_run,
+ * measured with HIP events on the handle's stream (milliseconds), and the
+ * number of launches.  names: "element", "element_pml", "fault", "node",
+ * "node_pml", "station". */
+int eqd_kernel_time(eqd_handle *h, const char *name, double *avg_ms, int64_t *launches);
+
+/* Enable (1) or disable (0) the per-launch event timing used by eqd_kernel_time. */
+int eqd_set_timing(eqd_handle *h, int32_t on);
+
+/* Device bytes held by the handle. */
+int64_t eqd_device_bytes(eqd_handle *h);
+
+const char *eqd_last_error(eqd_handle *h);
+int eqd_destroy(eqd_handle *h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,61 @@
+"""Write the golden fixtures under tests/golden/.
+
+Each fixture is one small rupture case run by the reference EQdyna time step
+(oracle/_ref/eqd_ref, compiled from the reference's own Fortran sources by
+oracle/ref.mk) on one rank: the FP64 input bundle the reference hands to its
+time loop (after meshgen and assembleGlobalMass) and the reference's outputs
+after ``nstep`` steps.  Data only; regenerate with
+
+    python tests/golden/make_fixtures.py
+
+(needs /root/reference and a built oracle/_ref/eqd_ref).
+"""
+import os
+import sys
+import tempfile
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+from eqdyna_amd import case  # noqa: E402
+import refrun  # noqa: E402
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def cases():
+    q = case.box_sw(name="box_q")
+    q.C_Q = 1
+    hg2 = case.box_sw(name="box_hg2")
+    hg2.C_hg = 2
+    return [
+        case.box_sw(name="box_sw"),
+        case.box_sw(friclaw=2, name="box_tw"),
+        case.box_sw(layered=True, name="box_layered"),
+        case.box_rsf(friclaw=3, name="box_rsf3"),
+        case.box_rsf(friclaw=4, name="box_rsf4"),
+        q,
+        case.box_plastic(name="box_plastic"),
+        hg2,
+    ]
+
+
+def main():
+    for c in cases():
+        with tempfile.TemporaryDirectory() as d:
+            b, s, o, _ = refrun.run_reference(c, d, 1, True)
+        arrays = {"in_" + k: v for k, v in b.items()}
+        arrays.update({"out_" + k: v for k, v in o.items()})
+        names = sorted(s)
+        arrays["scalar_names"] = np.array(names)
+        arrays["scalar_values"] = np.array([s[k] for k in names])
+        p = os.path.join(HERE, c.name + ".npz")
+        np.savez_compressed(p, **arrays)
+        print(p, os.path.getsize(p))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,26 @@
+"""The C restatement (oracle/eqd_oracle.c) against the reference's own FP64
+run: the golden fixtures hold the reference's inputs and outputs."""
+import numpy as np
+import pytest
+
+import oracle
+from fixtures import CASES, load_golden
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_oracle_bitwise_vs_reference(name):
+    bundle, sc, ref = load_golden(name)
+    out = oracle.run_oracle(bundle, sc)
+    for k, v in ref.items():
+        assert np.array_equal(out[k], v), f"{name}: {k} differs from the reference"
+
+
+def test_oracle_nan_stop():
+    # a NaN velocity stops the step like src/driver.f90:147-152
+    bundle, sc, _ = load_golden("box_sw")
+    b = dict(bundle)
+    v1 = b["v1"].copy()
+    v1[100] = np.nan
+    b["v1"] = v1
+    with pytest.raises(FloatingPointError):
+        oracle.run_oracle(b, sc, nsteps=2)
