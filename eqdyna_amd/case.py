@@ -370,3 +370,135 @@ def box_plastic(steps: int = 40, friclaw: int = 1, name: str = "box_plastic") ->
     o[1, :, :] = np.where(o[1] > 100.0, o[1], 0.30)
     o[2, :, :] = 0.20
     return c
+
+
+def gardner_rho(vp: float) -> float:
+    """Gardner's relation in SI units, rho = 310 * Vp**0.25 (kg/m^3, m/s)."""
+    return 310.0 * vp ** 0.25
+
+
+def layered_medium(seed: int, n_layers: int = 5, vs_top: float = 2000.0, vs_bottom: float = 3464.0,
+                   max_depth: float = 10.0e3) -> np.ndarray:
+    """Seeded 1-D medium: n_layers rows of (bottom depth, Vp, Vs, rho) with Vs
+    increasing from vs_top to vs_bottom with depth, Vp/Vs = 1.732 and Gardner
+    density; the last layer extends to 1e6 m (the bMaterial.txt row format,
+    src/readInputFiles.f90:177-179, src/meshgen.f90:172-198)."""
+    rng = np.random.default_rng(seed)
+    cuts = np.sort(rng.uniform(0.05, 0.95, n_layers - 1)) * max_depth
+    vs = np.linspace(vs_top, vs_bottom, n_layers)
+    rows = []
+    for i in range(n_layers):
+        depth = float(np.round(cuts[i], 0)) if i < n_layers - 1 else 1.0e6
+        vp = round(1.732 * vs[i], 3)
+        rows.append([depth, vp, float(vs[i]), round(gardner_rho(vp), 3)])
+    return np.array(rows)
+
+
+def _strike_slip_box(d: float, npml: int, name: str) -> Case:
+    """Shared set-up of the BASELINE boxes: a vertical strike-slip split-node
+    fault 16 km along strike, rupturable from 2 to 9.8 km depth (the fault
+    plane must reach the free surface in the reference, src/meshgen.f90:476-531,
+    so the top 2 km are locked with mu_s = 1e4 as TPV8 does), linear slip
+    weakening mu_s = 0.677, mu_d = 0.45, Dc = 0.3 m, sigma_n = -100 MPa,
+    tau_0 = 65 MPa and a 3 x 3 km nucleation patch at 75 MPa."""
+    c = Case(name=name)
+    c.dx = c.dy = c.dz = d
+    c.fxmin, c.fxmax = -8.0e3, 8.0e3
+    c.fymin = c.fymax = 0.0
+    c.fzmin, c.fzmax = -9.8e3, 0.0
+    c.enlarging_ratio = 1.0
+    c.nPML = npml
+    c.R = 0.001  # pairs with nPML = 10 (src/globalvar.f90:23-25)
+    c.friclaw = 1
+    c.tpv = 8
+    c.C_nuclea = 0
+    c.xsource, c.ysource, c.zsource = 0.0, 0.0, -6.0e3
+    c.nucR = 1.5e3
+    o = c.new_ofv()
+    for ix, x in enumerate(c.fx):
+        for iz, z in enumerate(c.fz):
+            o[1, iz, ix] = 0.677
+            if abs(abs(x) - c.fxmax) < 0.01 or abs(z - c.fzmin) < 0.01 or z > -2.0e3 + 0.01:
+                o[1, iz, ix] = 1.0e4
+            o[2, iz, ix] = 0.45
+            o[3, iz, ix] = 0.3
+            o[7, iz, ix] = -100.0e6
+            o[8, iz, ix] = 65.0e6
+            if abs(x - c.xsource) <= 1.5e3 and abs(z - c.zsource) <= 1.5e3:
+                o[8, iz, ix] = 75.0e6
+    xs = [-6.0, -2.0, 2.0, 6.0]
+    zs = [-3.0, -5.0, -7.0, -9.0]
+    c.st_on_fault = [[x, z] for x in xs for z in zs]
+    c.st_off_fault = [[x, y, 0.0] for x in (-6.0, -2.0, 2.0, 6.0) for y in (-3.0, -1.0, 1.0, 3.0)]
+    return c
+
+
+def c1(steps: int = 600) -> Case:
+    """homogeneous elastic box at 200 m, 120 x 60 x 60 =
+    432,000 hexahedra including a 10-element PML on five faces, rho = 2670,
+    Vp = 6000, Vs = 3464, dt = 0.01 s."""
+    c = _strike_slip_box(200.0, 10, "c1")
+    c.xmin, c.xmax = -10.0e3, 10.0e3
+    c.ymin, c.ymax = -4.0e3, 4.0e3
+    c.zmin, c.zmax = -10.0e3, 0.0
+    c.nuni_y_minus = c.nuni_y_plus = 19
+    c.vp, c.vs, c.rou = 6000.0, 3464.0, 2670.0
+    c.roumax, c.vmaxPML = 2670.0, 6000.0
+    c.dt = 0.01
+    c.term = steps * c.dt
+    return c
+
+
+def c2(steps: int = 4000, seed: int = 2, q: bool = True) -> Case:
+    """BASELINE config 2 at 50 m: 480 x 240 x 240 = 27,648,000 hexahedra
+    including a 10-element PML, a seeded five-layer medium (Vs 2000 -> 3464
+    m/s with depth, Vp/Vs = 1.732, Gardner density), the reference's
+    frequency-independent Q (C_Q = 1, src/calcElemKU.f90:77-132) and
+    dt = 0.0025 s."""
+    c = _strike_slip_box(50.0, 10, "c2")
+    c.xmin, c.xmax = -11.5e3, 11.5e3
+    c.ymin, c.ymax = -5.5e3, 5.5e3
+    c.zmin, c.zmax = -11.5e3, 0.0
+    c.nuni_y_minus = c.nuni_y_plus = 76
+    c.nmat, c.n2mat = 5, 4
+    c.mat = layered_medium(seed)
+    c.vp, c.vs, c.rou = c.mat[-1, 1], c.mat[-1, 2], c.mat[-1, 3]
+    c.roumax, c.vmaxPML = float(c.mat[-1, 3]), float(c.mat[-1, 1])
+    c.C_Q = 1 if q else 0
+    c.dt = 0.0025
+    c.term = steps * c.dt
+    return c
+
+
+def c2_sample(steps: int = 100, seed: int = 2, q: bool = True) -> Case:
+    """A bounded sample of the C2 workload for timing the reference on the
+    host: the same 50 m resolution, seeded layered medium, Q and friction on a
+    240 x 120 x 120 = 3,456,000-hexahedron sub-box (fault 8 km x 4.8 km)."""
+    c = _strike_slip_box(50.0, 10, "c2_sample")
+    c.fxmin, c.fxmax = -4.0e3, 4.0e3
+    c.fzmin, c.fzmax = -4.8e3, 0.0
+    c.xmin, c.xmax = -5.5e3, 5.5e3
+    c.ymin, c.ymax = -2.5e3, 2.5e3
+    c.zmin, c.zmax = -5.5e3, 0.0
+    c.nuni_y_minus = c.nuni_y_plus = 35
+    c.xsource, c.zsource = 0.0, -3.0e3
+    o = c.new_ofv()
+    for ix, x in enumerate(c.fx):
+        for iz, z in enumerate(c.fz):
+            o[1, iz, ix] = 0.677
+            if abs(abs(x) - c.fxmax) < 0.01 or abs(z - c.fzmin) < 0.01 or z > -1.0e3 + 0.01:
+                o[1, iz, ix] = 1.0e4
+            o[2, iz, ix], o[3, iz, ix] = 0.45, 0.3
+            o[7, iz, ix], o[8, iz, ix] = -100.0e6, 65.0e6
+            if abs(x - c.xsource) <= 1.5e3 and abs(z - c.zsource) <= 1.5e3:
+                o[8, iz, ix] = 75.0e6
+    c.st_on_fault = [[x, z] for x in (-2.0, 2.0) for z in (-2.0, -4.0)]
+    c.st_off_fault = [[x, y, 0.0] for x in (-2.0, 2.0) for y in (-1.0, 1.0)]
+    c.nmat, c.n2mat = 5, 4
+    c.mat = layered_medium(seed)
+    c.vp, c.vs, c.rou = c.mat[-1, 1], c.mat[-1, 2], c.mat[-1, 3]
+    c.roumax, c.vmaxPML = float(c.mat[-1, 3]), float(c.mat[-1, 1])
+    c.C_Q = 1 if q else 0
+    c.dt = 0.0025
+    c.term = steps * c.dt
+    return c

@@ -53,8 +53,9 @@ struct KTimer {
 struct eqd_handle {
     eqd_scalars s{};
     std::string err;
-    std::map<int, std::vector<double>> fd;   // f64 fields as handed over
-    std::map<int, std::vector<int32_t>> fi;  // i32 fields
+    std::map<int, const void *> src;         // caller's arrays, read during eqd_commit
+    std::map<int, std::vector<double>> fd;   // f64 fields kept after commit
+    std::map<int, std::vector<int32_t>> fi;  // i32 fields kept after commit
     bool committed = false;
     int32_t nt = 0;
     double time = 0.0;
@@ -225,6 +226,32 @@ static void qconstant(double Q, int k, double pi, double *rtaok, double *rwk, do
     *c1 = (0.5 * rs) * (1.0 + ak0 * rs);
 }
 
+// During eqd_commit arrays come from the caller; afterwards from the kept copies.
+static const double *F64(eqd_handle *h, int f)
+{
+    auto k = h->fd.find(f);
+    if (k != h->fd.end()) return k->second.data();
+    auto it = h->src.find(f);
+    return it == h->src.end() ? nullptr : (const double *)it->second;
+}
+static const int32_t *I32(eqd_handle *h, int f)
+{
+    auto k = h->fi.find(f);
+    if (k != h->fi.end()) return k->second.data();
+    auto it = h->src.find(f);
+    return it == h->src.end() ? nullptr : (const int32_t *)it->second;
+}
+template <typename T>
+static void keep_copy(eqd_handle *h, int f, std::map<int, std::vector<T>> &dst)
+{
+    auto it = h->src.find(f);
+    if (it == h->src.end()) return;
+    FieldSpec sp;
+    field_spec(h->s, f, sp);
+    const T *p = (const T *)it->second;
+    dst[f].assign(p, p + sp.count);
+}
+
 extern "C" {
 
 int eqd_create(const eqd_scalars *s, eqd_handle **out)
@@ -249,18 +276,11 @@ int eqd_set(eqd_handle *h, int32_t field, const void *host, int64_t count)
         return fail(h, EQD_EINVAL, "field %d: count %lld, expected %lld", field, (long long)count, (long long)sp.count);
     if (h->committed) return fail(h, EQD_ESTATE, "field %d: arrays are set before eqd_commit", field);
     if (count > 0 && !host) return fail(h, EQD_EINVAL, "field %d: null pointer", field);
-    if (sp.is_int) {
-        auto &v = h->fi[field];
-        v.assign((const int32_t *)host, (const int32_t *)host + count);
-    } else {
-        auto &v = h->fd[field];
-        v.assign((const double *)host, (const double *)host + count);
-    }
+    h->src[field] = host;
     return EQD_OK;
 }
 
-static const double *F64(eqd_handle *h, int f) { auto it = h->fd.find(f); return it == h->fd.end() ? nullptr : it->second.data(); }
-static const int32_t *I32(eqd_handle *h, int f) { auto it = h->fi.find(f); return it == h->fi.end() ? nullptr : it->second.data(); }
+
 
 int eqd_commit(eqd_handle *h)
 {
@@ -275,11 +295,10 @@ int eqd_commit(eqd_handle *h)
     for (int f : required) {
         FieldSpec sp;
         field_spec(s, f, sp);
-        const bool have = sp.is_int ? h->fi.count(f) : h->fd.count(f);
-        if (!have && sp.count > 0) return fail(h, EQD_EINVAL, "field %d was not set", f);
+        if (!h->src.count(f) && sp.count > 0) return fail(h, EQD_EINVAL, "field %d was not set", f);
     }
-    if (s.non > 0 && !h->fi.count(EQD_ANONFS)) return fail(h, EQD_EINVAL, "anonfs was not set");
-    if (s.noff > 0 && !h->fi.count(EQD_IDHIST)) return fail(h, EQD_EINVAL, "idhist was not set");
+    if (s.non > 0 && !h->src.count(EQD_ANONFS)) return fail(h, EQD_EINVAL, "anonfs was not set");
+    if (s.noff > 0 && !h->src.count(EQD_IDHIST)) return fail(h, EQD_EINVAL, "idhist was not set");
     if (s.C_Q == 1 && s.C_elastic == 0) return fail(h, EQD_EINVAL, "Q model can only work with elastic code");
     if (s.friclaw < 1 || s.friclaw > 4)
         return fail(h, EQD_EINVAL, "friclaw %d is not supported (1-4; thermal pressurization is not built)", s.friclaw);
@@ -407,6 +426,8 @@ int eqd_commit(eqd_handle *h)
         std::vector<double> r_shp(24 * nr), r_det(nr), r_lam(nr), r_mu(nr), r_vp(nr), r_rho(nr), r_st(nst * nr),
             r_ss(6 * nr), r_phi(32 * nr), r_em(24 * nr), r_porep(nr), r_pstrain(nr);
         std::vector<uint8_t> r_q(s.C_Q == 1 ? nr : 0);
+        int qbad = 0;
+#pragma omp parallel for schedule(static) reduction(max : qbad)
         for (int64_t t = 0; t < nr; t++) {
             const int64_t e = h->reg_ids[t];
             for (int a = 0; a < 8; a++) r_ien[a * nr + t] = ien[e * 8 + a] - 1;
@@ -439,10 +460,14 @@ int eqd_commit(eqd_handle *h)
                 const int iq = (int)((xc[1] - (s.PMLb[3] + s.dx / 2)) / s.dx + 1);
                 const int ir = (int)((xc[2] - (s.PMLb[4] + s.dx / 2)) / s.dx + 1);
                 const int k = 1 + ip % 2 + 2 * (iq % 2) + 4 * (ir % 2);
-                if (k < 1 || k > 8) return fail(h, EQD_EINVAL, "element %lld: Q mechanism %d", (long long)e + 1, k);
+                if (k < 1 || k > 8) {
+                    qbad = 1;
+                    continue;
+                }
                 r_q[t] = (uint8_t)(qsel * 8 + (k - 1));
             }
         }
+        if (qbad) return fail(h, EQD_EINVAL, "Q relaxation mechanism outside 1..8 (element centre outside the PML box?)");
         std::vector<double> qtab(16 * 6, 0.0);
         if (s.C_Q == 1) {
             for (int qsel = 0; qsel < 2; qsel++)
@@ -512,6 +537,7 @@ int eqd_commit(eqd_handle *h)
         std::vector<double> p_shp(24 * np), p_det(np), p_lam(np), p_mu(np), p_vp(np), p_rho(np), p_st(21 * np),
             p_ss(6 * np), p_phi(32 * np), p_em(24 * np), p_damp(3 * np);
         std::vector<uint8_t> p_mask(np);
+#pragma omp parallel for schedule(static)
         for (int64_t p = 0; p < np; p++) {
             const int64_t e = h->pml_ids[p];
             uint8_t m = 0;
@@ -717,10 +743,13 @@ int eqd_commit(eqd_handle *h)
     C.C_elastic = s.C_elastic; C.C_Q = s.C_Q; C.C_hg = s.C_hg; C.C_nuclea = s.C_nuclea; C.friclaw = s.friclaw;
     C.TPV = s.TPV; C.insertFaultType = s.insertFaultType; C.nucfault = s.nucfault;
     HIPCHK(h, hipDeviceSynchronize());
+    // keep what eqd_get needs to rebuild reference-layout arrays; the caller's
+    // buffers are not referenced after this point
+    for (int f : {EQD_STRESSARR, EQD_PSTRAIN, EQD_V1, EQD_NODALFORCEARR}) keep_copy(h, f, h->fd);
+    for (int f : {EQD_EQNUMINDEXARR, EQD_EQNUMSTARTINDEXLOC, EQD_STRESSCOMPINDEXARR}) keep_copy(h, f, h->fi);
+    h->src.clear();
     h->committed = true;
     h->pending = true;
-    // the host copies of bulk inputs are no longer needed
-    for (int f : {EQD_ELESHP, EQD_ELEMASS, EQD_PHI, EQD_SS, EQD_MAT}) h->fd.erase(f);
     return EQD_OK;
 }
 
@@ -860,7 +889,7 @@ int eqd_get(eqd_handle *h, int32_t field, void *host, int64_t count)
         std::vector<double> v, p;
         if ((rc = d2h(h, v, isv ? h->N.vel : h->N.acc, 3 * nn))) return rc;
         if ((rc = d2h(h, p, isv ? h->PN.v12 : h->PN.acc, 12 * h->PN.n))) return rc;
-        const auto &orig = h->fd[field];  // values for equations the device does not own
+        const auto &orig = h->fd[field];  // values for equations the device does not own (fixed PML nodes)
         for (int64_t i = 0; i < s.neq; i++) out[i] = orig.empty() ? 0.0 : orig[i];
         const int32_t *eqi = I32(h, EQD_EQNUMINDEXARR), *esl = I32(h, EQD_EQNUMSTARTINDEXLOC);
         for (int32_t n : h->node3_all)
@@ -927,19 +956,8 @@ int eqd_get(eqd_handle *h, int32_t field, void *host, int64_t count)
         for (int64_t t = 0; t < h->n_reg; t++) out[h->reg_ids[t]] = v[t];
         return EQD_OK;
     }
-    default: {
-        // inputs are returned as handed over
-        if (sp.is_int) {
-            auto it = h->fi.find(field);
-            if (it == h->fi.end()) return fail(h, EQD_EINVAL, "field %d is not available", field);
-            std::memcpy(host, it->second.data(), sizeof(int32_t) * count);
-        } else {
-            auto it = h->fd.find(field);
-            if (it == h->fd.end()) return fail(h, EQD_EINVAL, "field %d is not available after commit", field);
-            std::memcpy(host, it->second.data(), sizeof(double) * count);
-        }
-        return EQD_OK;
-    }
+    default:
+        return fail(h, EQD_EINVAL, "field %d is an input and is not kept after eqd_commit", field);
     }
 }
 

@@ -1,0 +1,643 @@
+// Structured case builder: the reference host's mesh generation and one-time
+// element precompute, for one rank and a planar vertical fault at y = 0.
+//
+// It produces the same globalvar arrays, in the same layout and bit for bit,
+// as src/meshgen.f90 (nodes, equation numbers, elements, split nodes, fault
+// areas, stations) followed by src/assembleGlobalMass.f90 (shape-function
+// derivatives, Jacobians, lumped masses, hourglass operators), so that cases
+// far larger than the Fortran host can mesh in reasonable time can be handed
+// to the GPU time step.  Element-local precompute runs in parallel (OpenMP);
+// everything that accumulates across elements runs in the reference's order.
+//
+// Two calls: eqdm_size() counts, the caller allocates, eqdm_fill() fills.
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <unordered_map>
+#include <vector>
+
+extern "C" {
+
+typedef struct {
+    // bModelGeometry.txt / bFaultGeometry.txt / bGlobal.txt / bMaterial.txt
+    double xmin, xmax, ymin, ymax, zmin, zmax;
+    int32_t dis4uniF, dis4uniB;
+    double rat, dx, dy, dz;
+    double fxmin, fxmax, fymin, fymax, fzmin, fzmax;
+    double fstrike;      // degrees
+    int32_t nPML;
+    int32_t C_elastic;
+    double roumax, rhow, gamar, grav;
+    double str1ToFaultAngle;  // degrees
+    double devStrToStrVertRatio;
+    int32_t nmat, n2mat;
+    const double *material;  // (nmat, n2mat) row-major (material(i,:) in row i)
+    int32_t n_on;            // on-fault stations (x, z) in metres
+    const double *st_on;
+    int32_t n_off;           // off-fault stations (x, y, z) in metres
+    const double *st_off;
+    double tol;
+    int32_t nthreads;
+} eqdm_params;
+
+typedef struct {
+    int32_t nx, ny, nz;
+    int64_t nn, ne, neq, size_eqidx, size_stress, nf, non, noff;
+    double PMLb[8];
+    double xmin, xmax, ymin, ymax, zmin, zmax;
+} eqdm_sizes;
+
+typedef struct {
+    double *meshCoor;          // (3, nn)
+    int32_t *eqNumIndexArr;    // (size_eqidx)
+    int32_t *eqNumStartIndexLoc, *numOfDofPerNodeArr;  // (nn)
+    int32_t *nodeElemIdRelation;  // (8, ne)
+    int32_t *elemTypeArr;      // (ne)
+    double *mat;               // (ne, 5)
+    double *eleshp;            // (3, 8, ne)
+    double *eledet;            // (ne)
+    double *elemass;           // (24, ne)
+    double *ss;                // (6, ne)
+    double *phi;               // (8, 4, ne)
+    int32_t *stressCompIndexArr;  // (ne)
+    double *stressArr;         // (size_stress)
+    double *nodalMassArr;      // (neq)
+    double *fnms;              // (nn)
+    int32_t *nsmp;             // (2, nf)
+    double *un, *us, *ud;      // (3, nf)
+    double *arn;               // (nf)
+    int32_t *anonfs;           // (3, max(non,1))
+    int32_t *idhist;           // (3, 6*noff)
+} eqdm_arrays;
+
+}  // extern "C"
+
+namespace {
+
+// getLocalOneDimCoorArrAndSize for one rank, src/meshgen.f90:447-546
+static std::vector<double> axis(const eqdm_params &p, int dim, double PMLb[8], double bounds[2])
+{
+    int nuni;
+    double h, front, back, mn, mx;
+    if (dim == 1) {
+        nuni = (int)std::lround((p.fxmax - p.fxmin) / p.dx) + 1;
+        h = p.dx; front = p.fxmin; back = p.fxmax; mn = p.xmin; mx = p.xmax;
+    } else if (dim == 2) {
+        nuni = p.dis4uniF + p.dis4uniB + 1;
+        h = p.dy; front = -(p.dis4uniF * p.dy); back = p.dis4uniB * p.dy; mn = p.ymin; mx = p.ymax;
+    } else {
+        nuni = (int)std::lround((p.fzmax - p.fzmin) / p.dz) + 1;
+        h = p.dz; front = p.fzmin; back = p.fzmax; mn = p.zmin; mx = p.zmax;
+    }
+    const int np = 1000000;
+    double c = front, g = h;
+    int i;
+    for (i = 1; i <= np; i++) {
+        g = g * p.rat;
+        c = c - g;
+        if (c <= mn) break;
+    }
+    const int frontId = i + p.nPML;
+    c = back;
+    g = h;
+    for (i = 1; i <= np; i++) {
+        g = g * p.rat;
+        c = c + g;
+        if (c >= mx) break;
+    }
+    if (dim == 3) i = -p.nPML;
+    const int size = nuni + frontId + i + p.nPML;
+    std::vector<double> a(size + 1);  // 1-based
+    a[frontId + 1] = front;
+    g = h;
+    for (int k = frontId; k >= 1; k--) {
+        g = g * p.rat;
+        a[k] = a[k + 1] - g;
+    }
+    for (int k = frontId + 2; k <= frontId + nuni; k++) a[k] = a[k - 1] + h;
+    if (dim < 3) {
+        g = h;
+        for (int k = frontId + nuni + 1; k <= size; k++) {
+            g = g * p.rat;
+            a[k] = a[k - 1] + g;
+        }
+    }
+    bounds[0] = a[1];
+    bounds[1] = a[size];
+    if (dim == 1) {
+        PMLb[0] = a[size - p.nPML];
+        PMLb[1] = a[p.nPML + 1];
+        PMLb[5] = a[size] - a[size - 1];
+    } else if (dim == 2) {
+        PMLb[2] = a[size - p.nPML];
+        PMLb[3] = a[p.nPML + 1];
+        PMLb[6] = a[size] - a[size - 1];
+    } else {
+        PMLb[4] = a[p.nPML + 1];
+        PMLb[7] = a[2] - a[1];
+    }
+    return a;
+}
+
+struct Grid {
+    std::vector<double> x, y, z;  // 1-based
+    int nx, ny, nz;
+    double PMLb[8];
+    double xmin, xmax, ymin, ymax, zmin, zmax;
+};
+
+static Grid make_grid(const eqdm_params &p)
+{
+    Grid G;
+    double b[2];
+    G.x = axis(p, 1, G.PMLb, b);
+    G.xmin = b[0]; G.xmax = b[1];
+    G.y = axis(p, 2, G.PMLb, b);
+    G.ymin = b[0]; G.ymax = b[1];
+    G.z = axis(p, 3, G.PMLb, b);
+    G.zmin = b[0]; G.zmax = b[1];
+    G.nx = (int)G.x.size() - 1;
+    G.ny = (int)G.y.size() - 1;
+    G.nz = (int)G.z.size() - 1;
+    return G;
+}
+
+static inline bool on_fault(const eqdm_params &p, double x, double y, double z)
+{
+    // checkIsOnFault for a planar vertical fault, src/meshgen.f90:787-803
+    return x >= p.fxmin - p.tol && x <= p.fxmax + p.tol && y >= p.fymin - p.tol && y <= p.fymax + p.tol &&
+           z >= p.fzmin - p.tol && z <= p.fzmax + p.tol && y == 0.0;
+}
+
+static inline bool is_pml_node(const Grid &G, double x, double y, double z)
+{
+    return x > G.PMLb[0] || x < G.PMLb[1] || y > G.PMLb[2] || y < G.PMLb[3] || z < G.PMLb[4];
+}
+
+static inline bool is_fixed(const eqdm_params &p, const Grid &G, double x, double y, double z)
+{
+    return std::fabs(x - G.xmin) < p.tol || std::fabs(x - G.xmax) < p.tol || std::fabs(y - G.ymin) < p.tol ||
+           std::fabs(y - G.ymax) < p.tol || std::fabs(z - G.zmin) < p.tol;
+}
+
+struct Counts {
+    int64_t nn = 0, ne = 0, neq = 0, size_eqidx = 0, size_stress = 0, nf = 0, non = 0, noff = 0;
+};
+
+// off-fault station match of setSurfaceStation, src/meshgen.f90:574-658
+static int match_station(const eqdm_params &p, const Grid &G, int ix, int iy, double x, double y, double z,
+                         std::vector<char> &found)
+{
+    const int nx = G.nx, ny = G.ny;
+    if (!(iy > 1 && iy < ny)) return -1;
+    const int part = (ix > 1 && ix < nx) ? 1 : (ix == 1 ? 2 : (ix == nx ? 3 : 0));
+    if (!part) return -1;
+    for (int i = 0; i < p.n_off; i++) {
+        if (found[i]) continue;
+        const double xs = p.st_off[i * 3], ys = p.st_off[i * 3 + 1], zs = p.st_off[i * 3 + 2];
+        if (!(std::fabs(z - zs) < p.tol)) continue;
+        bool xok;
+        if (part == 1)
+            xok = std::fabs(x - xs) < p.tol ||
+                  (xs > G.x[ix - 1] && xs < x && (x - xs) < (xs - G.x[ix - 1])) ||
+                  (xs > x && xs < G.x[ix + 1] && (xs - x) < (G.x[ix + 1] - xs));
+        else if (part == 2)
+            xok = std::fabs(x - xs) < p.tol || (xs > x && xs < G.x[ix + 1] && (xs - x) < (G.x[ix + 1] - xs));
+        else
+            xok = xs > G.x[ix - 1] && xs < x && (x - xs) < (xs - G.x[ix - 1]);
+        if (!xok) continue;
+        const bool yok = std::fabs(y - ys) < p.tol ||
+                         (ys > G.y[iy - 1] && ys < y && (y - ys) < (ys - G.y[iy - 1])) ||
+                         (ys > y && ys < G.y[iy + 1] && (ys - y) < (G.y[iy + 1] - ys));
+        if (!yok) continue;
+        found[i] = 1;
+        return i;
+    }
+    return -1;
+}
+
+// Node loop of src/meshgen.f90:64-107 (topology only) or, with arrays, the
+// full fill.  Returns counts.
+static Counts topology(const eqdm_params &p, const Grid &G, eqdm_arrays *A, int64_t ne_total)
+{
+    Counts c;
+    const int nx = G.nx, ny = G.ny, nz = G.nz;
+    const int64_t nreg = (int64_t)nx * ny * nz;
+    std::vector<int32_t> plane1((size_t)(ny + 2) * (nz + 1), 0), plane2((size_t)(ny + 2) * (nz + 1), 0);
+    auto P1 = [&](int iy, int iz) -> int32_t & { return plane1[(size_t)iz * (ny + 2) + iy]; };
+    auto P2 = [&](int iy, int iz) -> int32_t & { return plane2[(size_t)iz * (ny + 2) + iy]; };
+    int64_t node = 0, elem = 0, eq = 0, tag = 0, sdof = 0, nf = 0, non = 0, noff = 0;
+    std::unordered_map<int32_t, int32_t> slave_to_master;
+    std::vector<char> found(p.n_off, 0);
+    std::vector<int32_t> stnode;  // OffFaultStNodeIdIndex(2,:)
+    const double pi = 4.0 * std::atan(1.0);
+    const double stk = p.fstrike * pi / 180.0, dip = 90.0 * pi / 180.0;
+    // fault area bookkeeping (fltrc of src/meshgen.f90:770-775)
+    int ixfi = 0, izfi = 0, ifs = 0, ifd = 0;
+    std::vector<int32_t> frc_m, frc_k;  // [ifd][ifs] master node, pair index
+    const int nxf = nx + 1, nzf = nz + 1;
+    if (A) {
+        frc_m.assign((size_t)nxf * nzf, 0);
+        frc_k.assign((size_t)nxf * nzf, 0);
+    }
+    for (int ix = 1; ix <= nx; ix++) {
+        for (int iz = 1; iz <= nz; iz++) {
+            for (int iy = 1; iy <= ny; iy++) {
+                const double x = G.x[ix], y = G.y[iy], z = G.z[iz];
+                node++;
+                P2(iy, iz) = (int32_t)node;
+                const int ndof = is_pml_node(G, x, y, z) ? 12 : 3;
+                const bool fixed = is_fixed(p, G, x, y, z);
+                if (A) {
+                    A->meshCoor[(node - 1) * 3] = x;
+                    A->meshCoor[(node - 1) * 3 + 1] = y;
+                    A->meshCoor[(node - 1) * 3 + 2] = z;
+                    A->eqNumStartIndexLoc[node - 1] = (int32_t)tag;
+                    A->numOfDofPerNodeArr[node - 1] = ndof;
+                }
+                for (int d = 0; d < ndof; d++) {
+                    tag++;
+                    int32_t v = -1;
+                    if (!fixed) v = (int32_t)(++eq);
+                    if (A) A->eqNumIndexArr[tag - 1] = v;
+                }
+                // off-fault stations
+                if (p.n_off > 0) {
+                    const int st = match_station(p, G, ix, iy, x, y, z, found);
+                    if (st >= 0) {
+                        noff++;
+                        stnode.push_back((int32_t)node);
+                    }
+                }
+                // split node (createMasterNode, src/meshgen.f90:805-920)
+                if (on_fault(p, x, y, z)) {
+                    nf++;
+                    const int64_t ms = nreg + nf;
+                    P2(ny + 1, iz) = (int32_t)ms;
+                    slave_to_master[(int32_t)node] = (int32_t)ms;
+                    if (A) {
+                        A->nsmp[(nf - 1) * 2] = (int32_t)node;
+                        A->nsmp[(nf - 1) * 2 + 1] = (int32_t)ms;
+                        A->eqNumStartIndexLoc[ms - 1] = (int32_t)tag;
+                        A->numOfDofPerNodeArr[ms - 1] = 3;
+                        A->meshCoor[(ms - 1) * 3] = x;
+                        A->meshCoor[(ms - 1) * 3 + 1] = y;
+                        A->meshCoor[(ms - 1) * 3 + 2] = z;
+                    }
+                    for (int d = 0; d < 3; d++) {
+                        eq++;
+                        tag++;
+                        if (A) A->eqNumIndexArr[tag - 1] = (int32_t)eq;
+                    }
+                    for (int i = 0; i < p.n_on; i++) {
+                        if (std::fabs(x - p.st_on[i * 2]) < p.tol && std::fabs(z - p.st_on[i * 2 + 1]) < p.tol) {
+                            non++;
+                            if (A) {
+                                A->anonfs[(non - 1) * 3] = (int32_t)nf;
+                                A->anonfs[(non - 1) * 3 + 1] = i + 1;
+                                A->anonfs[(non - 1) * 3 + 2] = 1;
+                            }
+                            break;
+                        }
+                    }
+                    if (A) {
+                        double *un = A->un + (nf - 1) * 3, *us = A->us + (nf - 1) * 3, *ud = A->ud + (nf - 1) * 3;
+                        un[0] = std::cos(stk) * std::sin(dip);
+                        un[1] = -std::sin(stk) * std::sin(dip);
+                        un[2] = std::cos(dip);
+                        us[0] = -std::sin(stk);
+                        us[1] = -std::cos(stk);
+                        us[2] = 0.0;
+                        ud[0] = std::cos(stk) * std::cos(dip);
+                        ud[1] = std::sin(stk) * std::cos(dip);
+                        ud[2] = std::sin(dip);
+                    }
+                    if (ixfi == 0) ixfi = ix;
+                    if (izfi == 0) izfi = iz;
+                    ifs = ix - ixfi + 1;
+                    ifd = iz - izfi + 1;
+                    if (A) {
+                        frc_m[(size_t)ifd * nxf + ifs] = (int32_t)ms;
+                        frc_k[(size_t)ifd * nxf + ifs] = (int32_t)nf;
+                    }
+                }
+                // element (createElement + setElementMaterial + replaceSlaveWithMasterNode)
+                if (ix >= 2 && iy >= 2 && iz >= 2) {
+                    elem++;
+                    int32_t nd[8] = {P1(iy - 1, iz - 1), P2(iy - 1, iz - 1), P2(iy, iz - 1), P1(iy, iz - 1),
+                                     P1(iy - 1, iz),     P2(iy - 1, iz),     P2(iy, iz),     P1(iy, iz)};
+                    double xc[3] = {0.0, 0.0, 0.0};
+                    // the centroid needs coordinates; the nodes of an element all exist already
+                    if (A) {
+                        for (int a = 0; a < 8; a++)
+                            for (int j = 0; j < 3; j++) xc[j] = xc[j] + A->meshCoor[(int64_t)(nd[a] - 1) * 3 + j];
+                    } else {
+                        // same sums from the grid lines (only the PML test needs them here)
+                        const int ixs[8] = {ix - 1, ix, ix, ix - 1, ix - 1, ix, ix, ix - 1};
+                        const int iys[8] = {iy - 1, iy - 1, iy, iy, iy - 1, iy - 1, iy, iy};
+                        const int izs[8] = {iz - 1, iz - 1, iz - 1, iz - 1, iz, iz, iz, iz};
+                        for (int a = 0; a < 8; a++) {
+                            xc[0] = xc[0] + G.x[ixs[a]];
+                            xc[1] = xc[1] + G.y[iys[a]];
+                            xc[2] = xc[2] + G.z[izs[a]];
+                        }
+                    }
+                    for (int j = 0; j < 3; j++) xc[j] = xc[j] / 8.0;
+                    const bool pml = xc[0] > G.PMLb[0] || xc[0] < G.PMLb[1] || xc[1] > G.PMLb[2] ||
+                                     xc[1] < G.PMLb[3] || xc[2] < G.PMLb[4];
+                    const int etype = pml ? 2 : 1;
+                    if (A) {
+                        A->stressCompIndexArr[elem - 1] = (int32_t)sdof;
+                        A->elemTypeArr[elem - 1] = etype;
+                        // setElementMaterial, src/meshgen.f90:158-204
+                        double vp = 0, vs = 0, rho = 0;
+                        if (p.nmat == 1 && p.n2mat == 3) {
+                            vp = p.material[0]; vs = p.material[1]; rho = p.material[2];
+                        } else if (p.nmat > 1 && p.n2mat == 4) {
+                            const double dep = std::fabs(xc[2]);
+                            if (dep < p.material[0]) {
+                                vp = p.material[1]; vs = p.material[2]; rho = p.material[3];
+                            } else {
+                                for (int i = 1; i < p.nmat; i++)
+                                    if (dep < p.material[i * 4] && dep >= p.material[(i - 1) * 4]) {
+                                        vp = p.material[i * 4 + 1]; vs = p.material[i * 4 + 2]; rho = p.material[i * 4 + 3];
+                                    }
+                            }
+                        }
+                        double *m = A->mat + (elem - 1);  // mat(ne, 5), column stride ne
+                        m[0 * ne_total] = vp;
+                        m[1 * ne_total] = vs;
+                        m[2 * ne_total] = rho;
+                        m[4 * ne_total] = vs * vs * rho;
+                        m[3 * ne_total] = vp * vp * rho - 2.0 * m[4 * ne_total];
+                        // replaceSlaveWithMasterNode, src/meshgen.f90:759-785
+                        if (etype == 1 && x > p.fxmin - p.tol && x < p.fxmax + p.dx + p.tol && z > p.fzmin - p.tol &&
+                            y > 0.0 && std::fabs(y - p.dy) < p.tol) {
+                            for (int a = 0; a < 8; a++) {
+                                auto it = slave_to_master.find(nd[a]);
+                                if (it != slave_to_master.end()) nd[a] = it->second;
+                            }
+                        }
+                        for (int a = 0; a < 8; a++) A->nodeElemIdRelation[(elem - 1) * 8 + a] = nd[a];
+                        if (p.C_elastic == 0) {
+                            // setPlasticStress, src/meshgen.f90:976-996
+                            const double depth = -0.5 * (G.z[iz] + G.z[iz - 1]) + 7.3215;
+                            const int et = etype == 2 ? 1 : 0;
+                            const double ang = p.str1ToFaultAngle * pi / 180.0;
+                            const double strVert = -(p.roumax - p.rhow * (p.gamar + 1.0)) * depth * p.grav;
+                            const double devStr = std::fabs(strVert) * p.devStrToStrVertRatio;
+                            double *sa = A->stressArr + sdof;
+                            sa[3 + 15 * et - 1] = strVert;
+                            sa[1 + 15 * et - 1] = strVert - devStr * std::cos(2.0 * ang);
+                            sa[2 + 15 * et - 1] = strVert + devStr * std::cos(2.0 * ang);
+                            sa[6 + 15 * et - 1] = devStr * std::sin(2.0 * ang);
+                        }
+                    }
+                    sdof += pml ? 21 : 12;
+                }
+            }
+        }
+        plane1 = plane2;
+    }
+    c.nn = node + nf;
+    c.ne = elem;
+    c.neq = eq;
+    c.size_eqidx = tag;
+    c.size_stress = sdof;
+    c.nf = nf;
+    c.non = non;
+    c.noff = noff;
+    if (A) {
+        // fault areas, src/meshgen.f90:114-152
+        for (int i = 2; i <= ifd; i++)
+            for (int j = 2; j <= ifs; j++) {
+                const int32_t n1 = frc_m[(size_t)i * nxf + j], n2 = frc_m[(size_t)i * nxf + j - 1],
+                              n3 = frc_m[(size_t)(i - 1) * nxf + j - 1], n4 = frc_m[(size_t)(i - 1) * nxf + j];
+                const int32_t m1 = frc_k[(size_t)i * nxf + j], m2 = frc_k[(size_t)i * nxf + j - 1],
+                              m3 = frc_k[(size_t)(i - 1) * nxf + j - 1], m4 = frc_k[(size_t)(i - 1) * nxf + j];
+                auto X = [&](int32_t n, int k) { return A->meshCoor[(int64_t)(n - 1) * 3 + k]; };
+                auto dist = [&](int32_t a, int32_t b) {
+                    return std::sqrt((X(b, 0) - X(a, 0)) * (X(b, 0) - X(a, 0)) + (X(b, 1) - X(a, 1)) * (X(b, 1) - X(a, 1)) +
+                                     (X(b, 2) - X(a, 2)) * (X(b, 2) - X(a, 2)));
+                };
+                const double aa1 = dist(n1, n2), bb1 = dist(n2, n3), cc1 = dist(n3, n4), dd1 = dist(n4, n1);
+                const double p1 = dist(n2, n4), q1 = dist(n1, n3);
+                const double t = bb1 * bb1 + dd1 * dd1 - aa1 * aa1 - cc1 * cc1;
+                double area = 0.25 * std::sqrt(4 * p1 * p1 * q1 * q1 - t * t);
+                area = 0.25 * area;
+                A->arn[m1 - 1] = A->arn[m1 - 1] + area;
+                A->arn[m2 - 1] = A->arn[m2 - 1] + area;
+                A->arn[m3 - 1] = A->arn[m3 - 1] + area;
+                A->arn[m4 - 1] = A->arn[m4 - 1] + area;
+            }
+        // idhist, src/eqdyna3d.f90:170-188
+        int64_t row = 0;
+        for (int64_t s = 0; s < noff; s++)
+            for (int d = 1; d <= 3; d++)
+                for (int k = 1; k <= 2; k++) {
+                    A->idhist[row * 3] = stnode[s];
+                    A->idhist[row * 3 + 1] = d;
+                    A->idhist[row * 3 + 2] = k;
+                    row++;
+                }
+    }
+    return c;
+}
+
+// vlm, src/library.f90:60-93
+static double vlm(const double xl[3][8])
+{
+    static const int it[8][8] = {{1, 2, 3, 4, 5, 6, 7, 8}, {2, 3, 4, 1, 6, 7, 8, 5}, {3, 4, 1, 2, 7, 8, 5, 6},
+                                 {4, 1, 2, 3, 8, 5, 6, 7}, {5, 8, 7, 6, 1, 4, 3, 2}, {6, 5, 8, 7, 2, 1, 4, 3},
+                                 {7, 6, 5, 8, 3, 2, 1, 4}, {8, 7, 6, 5, 4, 3, 2, 1}};  // it[column][row]
+    double bb[8];
+    for (int i = 0; i < 8; i++) {
+        auto Y = [&](int r) { return xl[1][it[i][r - 1] - 1]; };
+        auto Z = [&](int r) { return xl[2][it[i][r - 1] - 1]; };
+        bb[i] = Y(2) * (Z(6) - Z(3) + Z(5) - Z(4)) + Y(3) * (Z(2) - Z(4)) + Y(4) * (Z(3) - Z(8) + Z(2) - Z(5)) +
+                Y(5) * (Z(8) - Z(6) + Z(4) - Z(2)) + Y(6) * (Z(5) - Z(2)) + Y(8) * (Z(4) - Z(5));
+    }
+    double v = 0.0;
+    for (int i = 0; i < 8; i++) v = v + xl[0][i] * bb[i];
+    return v / 12.0;
+}
+
+// calcGlobalShapeFunc + contm + calcSSPhi4Hrgls for one hexahedron,
+// src/calcGlobalShapeFunc.f90:1-75, src/assembleGlobalMass.f90:328-406.
+// Returns false on a non-positive Jacobian.
+static bool elem_precompute(const double xl[3][8], double rho, double lam, double mu, double w, double shp[24],
+                            double *detp, double emass[24], double ss[6], double phi[32])
+{
+    static const double ac[8][3] = {{-1, -1, -1}, {1, -1, -1}, {1, 1, -1}, {-1, 1, -1},
+                                    {-1, -1, 1},  {1, -1, 1},  {1, 1, 1},  {-1, 1, 1}};
+    const double cst = 1.0 / 8.0;
+    double g[4][8];
+    for (int i = 0; i < 8; i++) {
+        g[3][i] = cst;
+        for (int j = 0; j < 3; j++) g[j][i] = cst * ac[i][j];
+    }
+    double xs[3][3];  // xs[j][i] = d x_j / d xi_i
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            double t = 0.0;
+            for (int k = 0; k < 8; k++) t = t + g[i][k] * xl[j][k];
+            xs[j][i] = t;
+        }
+    const double cof11 = xs[1][1] * xs[2][2] - xs[1][2] * xs[2][1];
+    const double cof12 = xs[1][2] * xs[2][0] - xs[1][0] * xs[2][2];
+    const double cof13 = xs[1][0] * xs[2][1] - xs[1][1] * xs[2][0];
+    const double cof21 = xs[2][1] * xs[0][2] - xs[2][2] * xs[0][1];
+    const double cof22 = xs[2][2] * xs[0][0] - xs[2][0] * xs[0][2];
+    const double cof23 = xs[2][0] * xs[0][1] - xs[2][1] * xs[0][0];
+    const double cof31 = xs[0][1] * xs[1][2] - xs[0][2] * xs[1][1];
+    const double cof32 = xs[0][2] * xs[1][0] - xs[0][0] * xs[1][2];
+    const double cof33 = xs[0][0] * xs[1][1] - xs[0][1] * xs[1][0];
+    const double det = xs[0][0] * cof11 + xs[0][1] * cof12 + xs[0][2] * cof13;
+    if (det <= 0.0) return false;
+    for (int i = 0; i < 8; i++) {
+        const double t1 = g[0][i], t2 = g[1][i], t3 = g[2][i];
+        shp[i * 3 + 0] = (t1 * cof11 + t2 * cof12 + t3 * cof13) / det;
+        shp[i * 3 + 1] = (t1 * cof21 + t2 * cof22 + t3 * cof23) / det;
+        shp[i * 3 + 2] = (t1 * cof31 + t2 * cof32 + t3 * cof33) / det;
+    }
+    // returned xs = reshape(cof)/det: xr[r][c] with columns (cof11,cof12,cof13), (cof21..), (cof31..)
+    const double xr[3][3] = {{cof11 / det, cof21 / det, cof31 / det},
+                             {cof12 / det, cof22 / det, cof32 / det},
+                             {cof13 / det, cof23 / det, cof33 / det}};
+    *detp = det;
+    // contm
+    const double totmas = rho * w * det;
+    double dsum = 0.0, work[8];
+    for (int j = 0; j < 8; j++) {
+        const double t2 = totmas * (g[3][j] * g[3][j]);
+        dsum = dsum + t2;
+        work[j] = 0.0 + t2;
+    }
+    const double t1 = totmas / dsum;
+    for (int j = 0; j < 8; j++) {
+        const double t2 = t1 * work[j];
+        for (int k = 0; k < 3; k++) emass[j * 3 + k] = t2;
+    }
+    // calcSSPhi4Hrgls
+    double xl2[3][8];
+    std::memcpy(xl2, xl, sizeof(xl2));
+    const double vol = vlm(xl2);
+    double ce = mu * (3 * lam + 2 * mu) / (lam + mu);
+    ce = 16.0 * ce / 15.0;
+    const double co = ce * vol / 48.0;
+    auto X = [&](int r, int c) { return xr[r - 1][c - 1]; };
+    ss[0] = co * (X(1, 1) * X(1, 1) + X(2, 1) * X(2, 1) + X(3, 1) * X(3, 1));
+    ss[1] = co * (X(1, 1) * X(1, 2) + X(2, 1) * X(2, 2) + X(3, 1) * X(3, 2));
+    ss[2] = co * (X(1, 1) * X(1, 3) + X(2, 1) * X(2, 3) + X(3, 1) * X(3, 3));
+    ss[3] = co * (X(1, 2) * X(1, 2) + X(2, 2) * X(2, 2) + X(3, 2) * X(3, 2));
+    ss[4] = co * (X(1, 2) * X(1, 3) + X(2, 2) * X(2, 3) + X(3, 2) * X(3, 3));
+    ss[5] = co * (X(1, 3) * X(1, 3) + X(2, 3) * X(2, 3) + X(3, 3) * X(3, 3));
+    static const int ha[4][8] = {{1, 1, -1, -1, -1, -1, 1, 1}, {1, -1, -1, 1, -1, 1, 1, -1},
+                                 {1, -1, 1, -1, 1, -1, 1, -1}, {-1, 1, -1, 1, 1, -1, 1, -1}};
+    for (int m = 0; m < 4; m++) {
+        double ph[8];
+        for (int j = 0; j < 8; j++) {
+            double v = 0.0;
+            for (int k = 0; k < 8; k++)
+                v = v + (double)ha[m][k] * (xl[0][k] * shp[j * 3] + xl[1][k] * shp[j * 3 + 1] + xl[2][k] * shp[j * 3 + 2]);
+            ph[j] = (double)ha[m][j] - v;
+        }
+        double v = 0.0;
+        for (int j = 0; j < 8; j++) v = v + ph[j] * ph[j];
+        v = std::sqrt(v / 8.0);
+        for (int j = 0; j < 8; j++) phi[m * 8 + j] = ph[j] / v;
+    }
+    return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+int eqdm_size(const eqdm_params *p, eqdm_sizes *s)
+{
+    if (!p || !s) return -1;
+    Grid G = make_grid(*p);
+    Counts c = topology(*p, G, nullptr, 0);
+    s->nx = G.nx;
+    s->ny = G.ny;
+    s->nz = G.nz;
+    s->nn = c.nn;
+    s->ne = c.ne;
+    s->neq = c.neq;
+    s->size_eqidx = c.size_eqidx;
+    s->size_stress = c.size_stress;
+    s->nf = c.nf;
+    s->non = c.non;
+    s->noff = c.noff;
+    std::memcpy(s->PMLb, G.PMLb, sizeof(G.PMLb));
+    s->xmin = G.xmin; s->xmax = G.xmax; s->ymin = G.ymin; s->ymax = G.ymax; s->zmin = G.zmin; s->zmax = G.zmax;
+    return 0;
+}
+
+// Arrays must be zero-initialised by the caller and sized by eqdm_size.
+// Returns 0, -2 on a non-positive Jacobian (src/calcGlobalShapeFunc.f90:52-57),
+// -3 on an element without material (src/eqdyna3d.f90:214-224).
+int eqdm_fill(const eqdm_params *p, const eqdm_sizes *s, eqdm_arrays *A, int32_t *bad_elem)
+{
+    if (!p || !s || !A) return -1;
+    Grid G = make_grid(*p);
+    const int64_t ne = s->ne;
+    topology(*p, G, A, ne);
+    // checkMeshMaterial, src/eqdyna3d.f90:214-224
+    for (int64_t e = 0; e < ne; e++) {
+        if (A->mat[e] == 0.0 || A->mat[ne + e] == 0.0 || A->mat[2 * ne + e] == 0.0) {
+            if (bad_elem) *bad_elem = (int32_t)(e + 1);
+            return -3;
+        }
+    }
+    const double w = 8.0;
+    int32_t bad = 0;
+#pragma omp parallel for schedule(static) num_threads(p->nthreads > 0 ? p->nthreads : 1)
+    for (int64_t e = 0; e < ne; e++) {
+        double xl[3][8];
+        for (int a = 0; a < 8; a++) {
+            const int64_t nd = A->nodeElemIdRelation[e * 8 + a] - 1;
+            for (int j = 0; j < 3; j++) xl[j][a] = A->meshCoor[nd * 3 + j];
+        }
+        double det;
+        if (!elem_precompute(xl, A->mat[2 * ne + e], A->mat[3 * ne + e], A->mat[4 * ne + e], w, A->eleshp + e * 24,
+                             &det, A->elemass + e * 24, A->ss + e * 6, A->phi + e * 32)) {
+#pragma omp critical
+            bad = (int32_t)(e + 1);
+            continue;
+        }
+        A->eledet[e] = det;
+    }
+    if (bad) {
+        if (bad_elem) *bad_elem = bad;
+        return -2;
+    }
+    // nodal masses in element order, src/assembleGlobalMass.f90:283-326
+    for (int64_t e = 0; e < ne; e++) {
+        for (int a = 0; a < 8; a++) {
+            const int64_t nd = A->nodeElemIdRelation[e * 8 + a] - 1;
+            const int64_t b = A->eqNumStartIndexLoc[nd];
+            const double *em = A->elemass + e * 24 + a * 3;
+            if (A->numOfDofPerNodeArr[nd] == 12) {
+                for (int k = 0; k < 3; k++) {
+                    for (int j = 3 * k + 1; j <= 3 * k + 3; j++) {
+                        const int32_t q = A->eqNumIndexArr[b + j - 1];
+                        if (q > 0) A->nodalMassArr[q - 1] = A->nodalMassArr[q - 1] + em[k];
+                    }
+                    const int32_t q = A->eqNumIndexArr[b + k + 9];
+                    if (q > 0) A->nodalMassArr[q - 1] = A->nodalMassArr[q - 1] + em[k];
+                }
+            } else {
+                for (int j = 1; j <= 3; j++) {
+                    const int32_t q = A->eqNumIndexArr[b + j - 1];
+                    A->nodalMassArr[q - 1] = A->nodalMassArr[q - 1] + em[j - 1];
+                }
+            }
+            A->fnms[nd] = A->fnms[nd] + em[0];
+        }
+    }
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,214 @@
+"""Build a Case into the reference host's globalvar arrays without the Fortran host.
+
+``build_case(case)`` does what the reference program does between reading its
+inputs and calling the time loop (src/eqdyna3d.f90:33-70) for one rank and a
+planar vertical fault: scalar set-up of readglobal/readmaterial
+(src/readInputFiles.f90:2-186), mesh generation and element precompute
+(libeqd_mesh.so, restating src/meshgen.f90 and src/assembleGlobalMass.f90),
+on-fault fields into fric (src/netcdf_io.f90:70-107) and init_vel
+(src/eqdyna3d.f90:191-212).  It returns (bundle, scalars) in the layout of
+eqdyna_amd.bundle, ready for eqdyna_amd.solver.Solver.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+import subprocess
+from typing import Dict, Tuple
+
+import numpy as np
+
+from .case import ON_FAULT_SLOTS, Case
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libeqd_mesh.so")
+SRC = os.path.join(HERE, "csrc", "eqd_mesh.cpp")
+
+
+class Params(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("xmin", "xmax", "ymin", "ymax", "zmin", "zmax")] + \
+        [("dis4uniF", C.c_int32), ("dis4uniB", C.c_int32)] + \
+        [(n, C.c_double) for n in ("rat", "dx", "dy", "dz", "fxmin", "fxmax", "fymin", "fymax", "fzmin", "fzmax",
+                                   "fstrike")] + \
+        [("nPML", C.c_int32), ("C_elastic", C.c_int32)] + \
+        [(n, C.c_double) for n in ("roumax", "rhow", "gamar", "grav", "str1ToFaultAngle", "devStrToStrVertRatio")] + \
+        [("nmat", C.c_int32), ("n2mat", C.c_int32), ("material", C.c_void_p),
+         ("n_on", C.c_int32), ("st_on", C.c_void_p), ("n_off", C.c_int32), ("st_off", C.c_void_p),
+         ("tol", C.c_double), ("nthreads", C.c_int32)]
+
+
+class Sizes(C.Structure):
+    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32)] + \
+        [(n, C.c_int64) for n in ("nn", "ne", "neq", "size_eqidx", "size_stress", "nf", "non", "noff")] + \
+        [("PMLb", C.c_double * 8)] + [(n, C.c_double) for n in ("xmin", "xmax", "ymin", "ymax", "zmin", "zmax")]
+
+
+ARRAY_FIELDS = ["meshCoor", "eqNumIndexArr", "eqNumStartIndexLoc", "numOfDofPerNodeArr", "nodeElemIdRelation",
+                "elemTypeArr", "mat", "eleshp", "eledet", "elemass", "ss", "phi", "stressCompIndexArr",
+                "stressArr", "nodalMassArr", "fnms", "nsmp", "un", "us", "ud", "arn", "anonfs", "idhist"]
+
+
+class Arrays(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in ARRAY_FIELDS]
+
+
+def build_lib(verbose: bool = False) -> str:
+    cmd = ["g++", "-O2", "-fPIC", "-shared", "-fopenmp", "-ffp-contract=off", "-std=c++17", "-o", LIB_PATH + ".tmp", SRC]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(LIB_PATH + ".tmp", LIB_PATH)
+    return LIB_PATH
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise FileNotFoundError(f"{LIB_PATH} is missing: build it with __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        L.eqdm_size.argtypes = [C.POINTER(Params), C.POINTER(Sizes)]
+        L.eqdm_fill.argtypes = [C.POINTER(Params), C.POINTER(Sizes), C.POINTER(Arrays), C.POINTER(C.c_int32)]
+        _lib = L
+    return _lib
+
+
+def _params(c: Case, nthreads: int, keep: list) -> Params:
+    if c.C_degen > 3.0 or c.insertFaultType > 0 or c.ntotft != 1 or c.fymin != 0.0 or c.fymax != 0.0:
+        raise ValueError("the case builder covers one planar vertical fault at y = 0 "
+                         "(dipping, wedge and rough faults need the Fortran host)")
+    if c.nx * c.ny * c.nz != 1:
+        raise ValueError("the case builder meshes one rank (nx = ny = nz = 1)")
+    p = Params()
+    for n in ("xmin", "xmax", "ymin", "ymax", "zmin", "zmax", "dx", "dy", "dz", "fxmin", "fxmax", "fymin", "fymax",
+              "fzmin", "fzmax", "roumax", "rhow", "gamar", "str1ToFaultAngle", "devStrToStrVertRatio"):
+        setattr(p, n, float(getattr(c, n)))
+    p.dis4uniF, p.dis4uniB = int(c.nuni_y_minus), int(c.nuni_y_plus)
+    p.rat = float(c.enlarging_ratio)
+    p.fstrike = float(c.strike)
+    p.nPML = int(c.nPML)
+    p.C_elastic = int(c.C_elastic)
+    p.grav = 9.8
+    p.nmat, p.n2mat = int(c.nmat), int(c.n2mat)
+    if c.nmat == 1:
+        mat = np.array([c.vp, c.vs, c.rou], dtype=np.float64)
+    else:
+        mat = np.ascontiguousarray(np.asarray(c.mat, dtype=np.float64)[:, :4])
+    keep.append(mat)
+    p.material = mat.ctypes.data
+    # readstations2 converts km to m (src/readInputFiles.f90:262-263)
+    on = np.ascontiguousarray(np.asarray(c.st_on_fault, dtype=np.float64).reshape(-1, 2) * 1000.0)
+    off = np.ascontiguousarray(np.asarray(c.st_off_fault, dtype=np.float64).reshape(-1, 3) * 1000.0)
+    keep += [on, off]
+    p.n_on, p.st_on = len(on), on.ctypes.data if len(on) else None
+    p.n_off, p.st_off = len(off), off.ctypes.data if len(off) else None
+    p.tol = 1.0e-5
+    p.nthreads = int(nthreads)
+    return p
+
+
+def mesh_sizes(c: Case) -> Sizes:
+    keep: list = []
+    p = _params(c, 1, keep)
+    s = Sizes()
+    if lib().eqdm_size(C.byref(p), C.byref(s)) != 0:
+        raise RuntimeError("eqdm_size failed")
+    return s
+
+
+def build_case(c: Case, nthreads: int | None = None) -> Tuple[Dict[str, np.ndarray], Dict[str, float]]:
+    """Mesh and precompute a Case; returns (bundle, scalars) in reference layout."""
+    if nthreads is None:
+        nthreads = min(16, os.cpu_count() or 1)
+    keep: list = []
+    p = _params(c, nthreads, keep)
+    s = Sizes()
+    L = lib()
+    if L.eqdm_size(C.byref(p), C.byref(s)) != 0:
+        raise RuntimeError("eqdm_size failed")
+    nn, ne, neq, nf = s.nn, s.ne, s.neq, s.nf
+    non, noff = s.non, s.noff
+    b = {
+        "meshCoor": np.zeros((nn, 3)), "eqNumIndexArr": np.zeros(s.size_eqidx, np.int32),
+        "eqNumStartIndexLoc": np.zeros(nn, np.int32), "numOfDofPerNodeArr": np.zeros(nn, np.int32),
+        "nodeElemIdRelation": np.zeros((ne, 8), np.int32), "elemTypeArr": np.zeros(ne, np.int32),
+        "mat": np.zeros((5, ne)), "eleshp": np.zeros((ne, 8, 3)), "eledet": np.zeros(ne),
+        "elemass": np.zeros((ne, 24)), "ss": np.zeros((ne, 6)), "phi": np.zeros((ne, 4, 8)),
+        "stressCompIndexArr": np.zeros(ne, np.int32), "stressArr": np.zeros(s.size_stress),
+        "nodalMassArr": np.zeros(neq), "fnms": np.zeros(nn), "nsmp": np.zeros((max(nf, 1), 2), np.int32),
+        "un": np.zeros((max(nf, 1), 3)), "us": np.zeros((max(nf, 1), 3)), "ud": np.zeros((max(nf, 1), 3)),
+        "arn": np.zeros(max(nf, 1)), "anonfs": np.zeros((max(non, 1), 3), np.int32),
+        "idhist": np.zeros((max(6 * noff, 1), 3), np.int32),
+    }
+    A = Arrays()
+    for n in ARRAY_FIELDS:
+        setattr(A, n, b[n].ctypes.data)
+    bad = C.c_int32(0)
+    rc = L.eqdm_fill(C.byref(p), C.byref(s), C.byref(A), C.byref(bad))
+    if rc == -2:
+        raise ValueError(f"Non-positive determinant at element {bad.value}")
+    if rc == -3:
+        raise ValueError(f"Element {bad.value} is not assigned material property")
+    if rc != 0:
+        raise RuntimeError(f"eqdm_fill failed ({rc})")
+    # reference defaults of allocInit for fault arrays (src/eqdyna3d.f90:136-142)
+    nft = max(nf, 1)
+    b["nsmp"] = b["nsmp"][:nft]
+    if nf == 0:
+        b["us"][:] = 1000.0
+    b["eleporep"] = np.zeros(ne)
+    b["pstrain"] = np.zeros(ne)
+    b["v1"] = np.zeros(neq)
+    b["fnft"] = np.full(nft, 99999.0)
+    if non == 0:
+        b["anonfs"] = np.zeros((0, 3), np.int32)
+    if noff == 0:
+        del b["idhist"]
+    # on-fault fields into fric, src/netcdf_io.f90:70-107
+    fric = np.zeros((nft, 100))
+    if nf > 0:
+        sl = b["nsmp"][:, 0] - 1
+        ii = np.floor((b["meshCoor"][sl, 0] - c.fxmin) / c.dx + 0.5).astype(np.int64)
+        jj = np.floor((b["meshCoor"][sl, 2] - c.fzmin) / c.dz + 0.5).astype(np.int64)
+        v = c.ofv[ON_FAULT_SLOTS][:, jj, ii].T  # (nf, 24) in the file's field order
+        for col, slot in enumerate(ON_FAULT_SLOTS):
+            fric[:, slot - 1] = v[:, col]
+        fric[:, 46] = fric[:, 45]
+        fric[:, 24] = 0.0
+        fric[:, 25] = fric[:, 45]
+        fric[:, 26] = 0.0
+        fric[:, 22] = np.abs(fric[:, 6])
+    b["fric"] = fric
+    # init_vel: v1 of split nodes from fric(31:36)
+    if nf > 0:
+        esl, eqi = b["eqNumStartIndexLoc"], b["eqNumIndexArr"]
+        for j, base in ((0, 33), (1, 30)):
+            nodes = b["nsmp"][:, j] - 1
+            for k in range(3):
+                b["v1"][eqi[esl[nodes] + k] - 1] = fric[:, base + k]
+    pi = 4.0 * math.atan(1.0)
+    dt = float(c.dt)
+    sc = {
+        "dt": dt, "nstep": float(c.nstep), "totalNumOfNodes": float(nn), "totalNumOfElements": float(ne),
+        "totalNumOfEquations": float(neq), "sizeOfEqNumIndexArr": float(s.size_eqidx),
+        "sizeOfStressDofIndexArr": float(s.size_stress), "nftnd": float(nf), "nftmx": float(nft),
+        "ntotft": 1.0, "C_elastic": float(c.C_elastic), "C_Q": float(c.C_Q), "C_hg": float(c.C_hg),
+        "C_nuclea": float(c.C_nuclea), "friclaw": float(c.friclaw), "TPV": float(c.tpv),
+        "insertFaultType": float(c.insertFaultType), "nucfault": float(c.nucfault), "nPML": float(c.nPML),
+        "R": float(c.R), "vmaxPML": float(c.vmaxPML), "rdampk": float(c.rdampk) * dt, "rdampm": float(c.rdampm),
+        "w": 8.0, "grav": 9.8, "roumax": float(c.roumax), "rhow": float(c.rhow), "gamar": float(c.gamar),
+        "ccosphi": float(c.coheplas) * math.cos(math.atan(float(c.bulk))),
+        "sinphi": math.sin(math.atan(float(c.bulk))), "tv": 2.0 * float(c.dz) / 3464.0,
+        "slipRateThres": float(c.slipRateThres), "xsource": float(c.xsource), "ysource": float(c.ysource),
+        "zsource": float(c.zsource), "nucR": float(c.nucR), "nucRuptVel": float(c.nucRuptVel),
+        "nucdtau0": float(c.nucdtau0), "nucT": float(c.nucT), "kapa_hg": float(c.kapa_hg), "dx": float(c.dx),
+        "dy": float(c.dy), "dz": float(c.dz), "tol": 1.0e-5, "pi": pi, "numOfOnFaultStCount": float(non),
+        "numOfOffFaultStCount": float(noff), "mode": float(c.mode),
+    }
+    for i in range(8):
+        sc[f"PMLb{i + 1}"] = float(s.PMLb[i])
+    return b, sc

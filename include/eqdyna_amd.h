@@ -100,7 +100,9 @@ typedef struct eqd_handle eqd_handle;
  * (src/eqdyna3d.f90:54-70): stores the scalars; arrays follow via eqd_set. */
 int eqd_create(const eqd_scalars *s, eqd_handle **h);
 
-/* Hand over one globalvar array (copied; the host buffer may be reused). */
+/* Hand over one globalvar array.  The library reads it during eqd_commit and
+ * keeps no reference afterwards, so the buffer must stay valid (and unchanged)
+ * until eqd_commit returns; no copy is made before that. */
 int eqd_set(eqd_handle *h, int32_t field, const void *host, int64_t count);
 
 /* Validate, build the device layout (class-sorted SoA element arrays, the
