@@ -113,6 +113,21 @@ def cpu_baseline_port(steps: int):
             "kind": "port", "sample": f"C1 mesh (432,000 hexahedra), {steps} steps, C restatement on one core"}
 
 
+def calibrate_copy(nbytes: int) -> int:
+    """A plain copy of known size: its FETCH_SIZE / WRITE_SIZE counters give the
+    scale factor between counter readings and bytes moved."""
+    import torch
+    n = nbytes // 4
+    a = torch.empty(n, dtype=torch.float32, device="cuda")
+    a.fill_(1.0)
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": n * 4, "launches": 3}), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -123,7 +138,11 @@ def main() -> int:
     ap.add_argument("--cpu-baseline", default="auto", choices=["auto", "reference", "port", "none"])
     ap.add_argument("--cpu-ranks", type=int, default=16)
     ap.add_argument("--cpu-steps", type=int, default=60)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
+                    help="only run a device-to-device copy of BYTES (for scaling HBM counters) and exit")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        return calibrate_copy(args.calibrate_copy)
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}). "
               "The halo-exchange design for several GPUs is described in DESIGN.md.", file=sys.stderr)

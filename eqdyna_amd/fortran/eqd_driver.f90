@@ -1,0 +1,229 @@
+! Drop-in replacement for the reference's time loop, `subroutine driver`
+! (src/driver.f90:3-36), that runs the explicit time step on the GPU through the
+! eqdyna_amd C ABI (include/eqdyna_amd.h).
+!
+! Link this file instead of src/driver.f90 and the routines only the time loop
+! uses (src/assembleGlobalKU.f90, calcElemKU.f90, calcB.f90, calcElemMass.f90,
+! qconstant.f90, hrglss.f90, faulting.f90, fric.f90, comdampv.f90,
+! thermop.f90), plus libeqd.so.  Everything else -- input readers, meshgen,
+! assembleGlobalMass, init_vel and the output writers -- stays the reference's.
+! On return the module globalvar holds what the reference's own driver leaves
+! there (fric, fnft, velArr, dispArr, v1, nodalForceArr, stressArr, pstrain,
+! the station histories, nt and timeElapsed), so output_frt and the station
+! writers run unchanged.
+!
+! One MPI rank per GPU; several ranks would need the halo exchange of
+! MPI4NodalQuant inside the step, which this build does not do.
+module eqd_abi
+    use iso_c_binding
+    implicit none
+
+    type, bind(C) :: eqd_scalars
+        real(c_double) :: dt, rdampk, rdampm, w, grav, roumax, rhow, gamar
+        real(c_double) :: ccosphi, sinphi, tv, slipRateThres
+        real(c_double) :: xsource, ysource, zsource, nucR, nucRuptVel, nucdtau0, nucT
+        real(c_double) :: kapa_hg, dx, R, vmaxPML, PMLb(8), pi
+        integer(c_int32_t) :: nstep, nn, ne, neq, size_eqidx, size_stress, nf, non, noff, nPML
+        integer(c_int32_t) :: C_elastic, C_Q, C_hg, C_nuclea, friclaw, TPV, insertFaultType, nucfault
+        integer(c_int32_t) :: precision, device
+    end type eqd_scalars
+
+    ! field ids of enum eqd_field
+    integer(c_int32_t), parameter :: EQD_MESHCOOR = 1, EQD_EQNUMINDEXARR = 2, EQD_EQNUMSTARTINDEXLOC = 3, &
+        EQD_NUMOFDOFPERNODEARR = 4, EQD_NODEELEMIDRELATION = 5, EQD_ELEMTYPEARR = 6, EQD_MAT = 7, &
+        EQD_ELESHP = 8, EQD_ELEDET = 9, EQD_ELEMASS = 10, EQD_SS = 11, EQD_PHI = 12, &
+        EQD_STRESSCOMPINDEXARR = 13, EQD_ELEPOREP = 14, EQD_NODALMASSARR = 15, EQD_FNMS = 16, &
+        EQD_NSMP = 17, EQD_UN = 18, EQD_US = 19, EQD_UD = 20, EQD_ARN = 21, EQD_ANONFS = 22, &
+        EQD_IDHIST = 23, EQD_STRESSARR = 24, EQD_PSTRAIN = 25, EQD_V1 = 26, EQD_VELARR = 27, &
+        EQD_DISPARR = 28, EQD_NODALFORCEARR = 29, EQD_FRIC = 30, EQD_FNFT = 31, EQD_ONFAULTHIST = 32, &
+        EQD_OFFFAULTHIST = 33
+
+    interface
+        integer(c_int) function eqd_create(s, h) bind(C, name='eqd_create')
+            import :: eqd_scalars, c_ptr, c_int
+            type(eqd_scalars), intent(in) :: s
+            type(c_ptr), intent(out) :: h
+        end function eqd_create
+        integer(c_int) function eqd_set(h, field, host, count) bind(C, name='eqd_set')
+            import :: c_ptr, c_int, c_int32_t, c_int64_t
+            type(c_ptr), value :: h
+            integer(c_int32_t), value :: field
+            type(*), dimension(*), intent(in) :: host
+            integer(c_int64_t), value :: count
+        end function eqd_set
+        integer(c_int) function eqd_commit(h) bind(C, name='eqd_commit')
+            import :: c_ptr, c_int
+            type(c_ptr), value :: h
+        end function eqd_commit
+        integer(c_int) function eqd_run(h, nsteps) bind(C, name='eqd_run')
+            import :: c_ptr, c_int, c_int32_t
+            type(c_ptr), value :: h
+            integer(c_int32_t), value :: nsteps
+        end function eqd_run
+        integer(c_int) function eqd_get(h, field, host, count) bind(C, name='eqd_get')
+            import :: c_ptr, c_int, c_int32_t, c_int64_t
+            type(c_ptr), value :: h
+            integer(c_int32_t), value :: field
+            type(*), dimension(*) :: host
+            integer(c_int64_t), value :: count
+        end function eqd_get
+        integer(c_int) function eqd_clock(h, nt, t) bind(C, name='eqd_clock')
+            import :: c_ptr, c_int, c_int32_t, c_double
+            type(c_ptr), value :: h
+            integer(c_int32_t), intent(out) :: nt
+            real(c_double), intent(out) :: t
+        end function eqd_clock
+        integer(c_int) function eqd_nan_info(h, node, nt) bind(C, name='eqd_nan_info')
+            import :: c_ptr, c_int, c_int32_t
+            type(c_ptr), value :: h
+            integer(c_int32_t), intent(out) :: node, nt
+        end function eqd_nan_info
+        type(c_ptr) function eqd_last_error(h) bind(C, name='eqd_last_error')
+            import :: c_ptr
+            type(c_ptr), value :: h
+        end function eqd_last_error
+        integer(c_int) function eqd_destroy(h) bind(C, name='eqd_destroy')
+            import :: c_ptr, c_int
+            type(c_ptr), value :: h
+        end function eqd_destroy
+    end interface
+
+contains
+
+    subroutine eqd_check(h, rc, what)
+        type(c_ptr), intent(in) :: h
+        integer(c_int), intent(in) :: rc
+        character(len=*), intent(in) :: what
+        character(kind=c_char), pointer :: msg(:)
+        integer :: i, n
+        type(c_ptr) :: p
+        if (rc == 0) return
+        p = eqd_last_error(h)
+        call c_f_pointer(p, msg, [1024])
+        n = 0
+        do i = 1, 1024
+            if (msg(i) == c_null_char) exit
+            n = i
+        enddo
+        write(*,*) 'eqdyna_amd: ', what, ' failed (', rc, '): ', (msg(i), i = 1, n)
+        stop 4001
+    end subroutine eqd_check
+
+end module eqd_abi
+
+subroutine driver
+    use globalvar
+    use eqd_abi
+    implicit none
+    type(eqd_scalars) :: s
+    type(c_ptr) :: h
+    integer(c_int) :: rc
+    integer(c_int32_t) :: ntc, badnode, badnt, chunk, done
+    real(c_double) :: tc
+    integer(c_int64_t) :: nf, non, noff
+    integer :: k
+
+    nf = nftnd(1)
+    non = min(numOfOnFaultStCount, size(anonfs, 2))
+    noff = numOfOffFaultStCount
+    s%dt = dt; s%rdampk = rdampk; s%rdampm = rdampm; s%w = w; s%grav = grav
+    s%roumax = roumax; s%rhow = rhow; s%gamar = gamar; s%ccosphi = ccosphi; s%sinphi = sinphi
+    s%tv = tv; s%slipRateThres = slipRateThres; s%xsource = xsource; s%ysource = ysource
+    s%zsource = zsource; s%nucR = nucR; s%nucRuptVel = nucRuptVel; s%nucdtau0 = nucdtau0
+    s%nucT = nucT; s%kapa_hg = kapa_hg; s%dx = dx; s%R = R; s%vmaxPML = vmaxPML
+    s%PMLb = PMLb; s%pi = pi
+    s%nstep = nstep; s%nn = totalNumOfNodes; s%ne = totalNumOfElements; s%neq = totalNumOfEquations
+    s%size_eqidx = sizeOfEqNumIndexArr; s%size_stress = sizeOfStressDofIndexArr
+    s%nf = nftnd(1); s%non = numOfOnFaultStCount; s%noff = numOfOffFaultStCount; s%nPML = nPML
+    s%C_elastic = C_elastic; s%C_Q = C_Q; s%C_hg = C_hg; s%C_nuclea = C_nuclea
+    s%friclaw = friclaw; s%TPV = TPV; s%insertFaultType = insertFaultType; s%nucfault = nucfault
+    s%precision = 0; s%device = 0
+    if (ntotft /= 1) stop 'eqdyna_amd: one fault (ntotft = 1) is supported'
+    if (totalNumOfMPIProcs /= 1) stop 'eqdyna_amd: one MPI rank per GPU (npx = npy = npz = 1)'
+
+    rc = eqd_create(s, h)
+    if (rc /= 0) stop 'eqdyna_amd: eqd_create rejected the scalars'
+    call eqd_check(h, eqd_set(h, EQD_MESHCOOR, meshCoor, int(size(meshCoor), c_int64_t)), 'set meshCoor')
+    call eqd_check(h, eqd_set(h, EQD_EQNUMINDEXARR, eqNumIndexArr, int(sizeOfEqNumIndexArr, c_int64_t)), 'set eqNumIndexArr')
+    call eqd_check(h, eqd_set(h, EQD_EQNUMSTARTINDEXLOC, eqNumStartIndexLoc, int(totalNumOfNodes, c_int64_t)), 'set eqNumStartIndexLoc')
+    call eqd_check(h, eqd_set(h, EQD_NUMOFDOFPERNODEARR, numOfDofPerNodeArr, int(totalNumOfNodes, c_int64_t)), 'set numOfDofPerNodeArr')
+    call eqd_check(h, eqd_set(h, EQD_NODEELEMIDRELATION, nodeElemIdRelation, int(size(nodeElemIdRelation), c_int64_t)), 'set IEN')
+    call eqd_check(h, eqd_set(h, EQD_ELEMTYPEARR, elemTypeArr, int(totalNumOfElements, c_int64_t)), 'set elemTypeArr')
+    call eqd_check(h, eqd_set(h, EQD_MAT, mat, int(size(mat), c_int64_t)), 'set mat')
+    call eqd_check(h, eqd_set(h, EQD_ELESHP, eleshp, int(size(eleshp), c_int64_t)), 'set eleshp')
+    call eqd_check(h, eqd_set(h, EQD_ELEDET, eledet, int(totalNumOfElements, c_int64_t)), 'set eledet')
+    call eqd_check(h, eqd_set(h, EQD_ELEMASS, elemass, int(size(elemass), c_int64_t)), 'set elemass')
+    call eqd_check(h, eqd_set(h, EQD_SS, ss, int(size(ss), c_int64_t)), 'set ss')
+    call eqd_check(h, eqd_set(h, EQD_PHI, phi, int(size(phi), c_int64_t)), 'set phi')
+    call eqd_check(h, eqd_set(h, EQD_STRESSCOMPINDEXARR, stressCompIndexArr, int(totalNumOfElements, c_int64_t)), 'set sci')
+    call eqd_check(h, eqd_set(h, EQD_ELEPOREP, eleporep, int(totalNumOfElements, c_int64_t)), 'set eleporep')
+    call eqd_check(h, eqd_set(h, EQD_NODALMASSARR, nodalMassArr, int(totalNumOfEquations, c_int64_t)), 'set mass')
+    call eqd_check(h, eqd_set(h, EQD_FNMS, fnms, int(totalNumOfNodes, c_int64_t)), 'set fnms')
+    call eqd_check(h, eqd_set(h, EQD_NSMP, nsmp, 2 * nf), 'set nsmp')
+    call eqd_check(h, eqd_set(h, EQD_UN, un, 3 * nf), 'set un')
+    call eqd_check(h, eqd_set(h, EQD_US, us, 3 * nf), 'set us')
+    call eqd_check(h, eqd_set(h, EQD_UD, ud, 3 * nf), 'set ud')
+    call eqd_check(h, eqd_set(h, EQD_ARN, arn, nf), 'set arn')
+    if (non > 0) call eqd_check(h, eqd_set(h, EQD_ANONFS, anonfs, 3 * non), 'set anonfs')
+    if (noff > 0) call eqd_check(h, eqd_set(h, EQD_IDHIST, idhist, 3 * 6 * noff), 'set idhist')
+    call eqd_check(h, eqd_set(h, EQD_STRESSARR, stressArr, int(sizeOfStressDofIndexArr, c_int64_t)), 'set stressArr')
+    call eqd_check(h, eqd_set(h, EQD_PSTRAIN, pstrain, int(totalNumOfElements, c_int64_t)), 'set pstrain')
+    call eqd_check(h, eqd_set(h, EQD_V1, v1, int(totalNumOfEquations, c_int64_t)), 'set v1')
+    call eqd_check(h, eqd_set(h, EQD_VELARR, velArr, int(size(velArr), c_int64_t)), 'set velArr')
+    call eqd_check(h, eqd_set(h, EQD_DISPARR, dispArr, int(size(dispArr), c_int64_t)), 'set dispArr')
+    call eqd_check(h, eqd_set(h, EQD_NODALFORCEARR, nodalForceArr, int(totalNumOfEquations, c_int64_t)), 'set F')
+    call eqd_check(h, eqd_set(h, EQD_FRIC, fric, 100 * nf), 'set fric')
+    call eqd_check(h, eqd_set(h, EQD_FNFT, fnft, nf), 'set fnft')
+    call eqd_check(h, eqd_commit(h), 'commit')
+
+    ! Advance in chunks that end where the reference prints progress
+    ! (src/driver.f90:13-17) or writes ground motion (src/driver.f90:30-33).
+    done = 0
+    do while (done < nstep)
+        chunk = nstep - done
+        if (outputGroundMotion == 1) then
+            ! stop after the next step nt with mod(nt,10) == 1
+            chunk = min(chunk, int(mod(1 - mod(done + 1, 10) + 10, 10) + 1, c_int32_t))
+        endif
+        if (me == masterProcsId) then
+            k = done + 1
+            if (mod(k, 100) == 1) then
+                write(*,*) '=     Current time in dynamic rupture (GPU)                         ='
+                write(*,'(X,A,40X,f7.3,4X,A)') '=', timeElapsed + dt, 's'
+            endif
+            chunk = min(chunk, int(100 - mod(done, 100), c_int32_t))
+        endif
+        rc = eqd_run(h, chunk)
+        if (rc == -2) then
+            rc = eqd_nan_info(h, badnode, badnt)
+            write(*,*) 'Velocity NaN at point ', meshCoor(1,badnode), meshCoor(2,badnode), meshCoor(3,badnode), &
+                       ' at time step ', badnt
+            stop
+        endif
+        call eqd_check(h, rc, 'run')
+        done = done + chunk
+        rc = eqd_clock(h, ntc, tc)
+        nt = ntc
+        timeElapsed = tc
+        if (outputGroundMotion == 1 .and. mod(nt, 10) == 1) then
+            call eqd_check(h, eqd_get(h, EQD_VELARR, velArr, int(size(velArr), c_int64_t)), 'get velArr')
+            call eqd_check(h, eqd_get(h, EQD_FRIC, fric, 100 * nf), 'get fric')
+            call output_gm
+            call output_src_evol
+        endif
+    enddo
+
+    call eqd_check(h, eqd_get(h, EQD_FRIC, fric, 100 * nf), 'get fric')
+    call eqd_check(h, eqd_get(h, EQD_FNFT, fnft, nf), 'get fnft')
+    call eqd_check(h, eqd_get(h, EQD_VELARR, velArr, int(size(velArr), c_int64_t)), 'get velArr')
+    call eqd_check(h, eqd_get(h, EQD_DISPARR, dispArr, int(size(dispArr), c_int64_t)), 'get dispArr')
+    call eqd_check(h, eqd_get(h, EQD_V1, v1, int(totalNumOfEquations, c_int64_t)), 'get v1')
+    call eqd_check(h, eqd_get(h, EQD_NODALFORCEARR, nodalForceArr, int(totalNumOfEquations, c_int64_t)), 'get F')
+    call eqd_check(h, eqd_get(h, EQD_STRESSARR, stressArr, int(sizeOfStressDofIndexArr, c_int64_t)), 'get stressArr')
+    call eqd_check(h, eqd_get(h, EQD_PSTRAIN, pstrain, int(totalNumOfElements, c_int64_t)), 'get pstrain')
+    call eqd_check(h, eqd_get(h, EQD_ONFAULTHIST, onFaultQuantHistSCECForm, &
+                              int(size(onFaultQuantHistSCECForm), c_int64_t)), 'get on-fault stations')
+    if (noff > 0) call eqd_check(h, eqd_get(h, EQD_OFFFAULTHIST, OffFaultStGramSCEC, &
+                                             int(size(OffFaultStGramSCEC), c_int64_t)), 'get off-fault stations')
+    rc = eqd_destroy(h)
+end subroutine driver

@@ -1,0 +1,44 @@
+"""The reference's own Fortran host (meshgen, assembleGlobalMass, output
+writers) linked with the GPU time step through the iso_c_binding shim
+(eqdyna_amd/fortran/eqd_driver.f90), against the reference build on the same
+TPV8 case (case_input/test.tpv8), one rank each.  frt.txt is the reference's
+E18.7 text output (src/library_output.f90:160-205)."""
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+import refrun
+from eqdyna_amd import case
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GPU_HOST = os.path.join(ROOT, "eqdyna_amd", "fortran", "_build", "eqdyna_gpu")
+
+
+@pytest.mark.gpu
+def test_reference_host_with_gpu_driver_matches_reference():
+    if not os.path.exists(GPU_HOST) or not refrun.available():
+        pytest.skip("the Fortran host build needs the reference sources (built where they are present)")
+    c = case.tpv8()
+    c.nx = c.ny = c.nz = 1
+    with tempfile.TemporaryDirectory() as dg, tempfile.TemporaryDirectory() as dr:
+        c.write_inputs(dg)
+        print("running the Fortran host with the GPU driver on TPV8", flush=True)
+        p = subprocess.run([GPU_HOST], cwd=dg, capture_output=True, text=True, timeout=300)
+        assert p.returncode == 0 and "EQD_GPU" in p.stdout, p.stdout[-2000:] + p.stderr[-2000:]
+        refrun.run_reference(c, dr, 1, dump=False, timeout=600)
+        g = np.loadtxt(os.path.join(dg, "frt.txt0"))
+        r = np.loadtxt(os.path.join(dr, "frt.txt0"))
+        gs = sorted(f for f in os.listdir(dg) if f.startswith("faultst"))
+        rs = sorted(f for f in os.listdir(dr) if f.startswith("faultst"))
+    assert g.shape == r.shape
+    # rupture times (column 4) identical; every other column within one unit of the
+    # 7th printed digit of that column's largest value
+    assert np.array_equal(g[:, 3], r[:, 3])
+    scale = np.maximum(np.abs(r).max(axis=0), 1e-300)
+    err = (np.abs(g - r) / scale).max(axis=0)
+    print("per-column error", np.array2string(err, precision=2), flush=True)
+    assert err.max() <= 2e-6
+    assert gs == rs
