@@ -36,12 +36,34 @@ __global__ void __launch_bounds__(256) elem_regular_kernel(RegElems<R> E, NodeSt
 #pragma unroll
     for (int a = 0; a < 8; a++) nd[a] = E.ien[a * n + t];
     R vl[24], dl[24];
+    if constexpr (sizeof(R) == sizeof(double)) {
 #pragma unroll
-    for (int a = 0; a < 8; a++) {
+        for (int a = 0; a < 8; a++) {
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                vl[a * 3 + k] = N.vel[k * nn + nd[a]];
+                dl[a * 3 + k] = N.disp[k * nn + nd[a]];
+            }
+        }
+    } else {
+        // FP32 bulk: round velocities and displacements relative to the first
+        // node (formed in FP64).  Strain, strain rate and hourglass modes are
+        // blind to a common translation (sum_a dN_a/dx = 0, Phi orthogonal to
+        // constants), so only the rounding changes, and it is now relative to
+        // the deformation rather than to the absolute motion.
+        double v0[3], u0[3];
 #pragma unroll
         for (int k = 0; k < 3; k++) {
-            vl[a * 3 + k] = (R)N.vel[k * nn + nd[a]];
-            dl[a * 3 + k] = (R)N.disp[k * nn + nd[a]];
+            v0[k] = N.vel[k * nn + nd[0]];
+            u0[k] = N.disp[k * nn + nd[0]];
+        }
+#pragma unroll
+        for (int a = 0; a < 8; a++) {
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                vl[a * 3 + k] = (R)(N.vel[k * nn + nd[a]] - v0[k]);
+                dl[a * 3 + k] = (R)(N.disp[k * nn + nd[a]] - u0[k]);
+            }
         }
     }
     R shp[24];
@@ -153,7 +175,8 @@ __global__ void __launch_bounds__(256) elem_regular_kernel(RegElems<R> E, NodeSt
         for (int a = 0; a < 8; a++) {
 #pragma unroll
             for (int k = 0; k < 3; k++) {
-                R al = (R)C.rdampm * vl[a * 3 + k];
+                // absolute velocity here: the Rayleigh mass term is not translation invariant
+                R al = (R)(C.rdampm * N.vel[k * nn + nd[a]]);
                 if (k == 2) al = al + (R)C.gterm;
                 f[a * 3 + k] = (R)0 - al * E.emass[(a * 3 + k) * n + t];
             }

@@ -1,0 +1,40 @@
+"""FP32 bulk mode (element force + hourglass in FP32; fault traction, Newton
+solve, state and nodal update in FP64) against the reference's FP64 run.
+
+Stated error bars (north_star): rupture time within 0.5*dt on average over the
+ruptured nodes, peak slip rate within 1 % of the FP64 value."""
+import numpy as np
+import pytest
+
+from eqdyna_amd.solver import Solver
+from fixtures import channel_errors, channels, load_golden
+
+ELASTIC = ["box_sw", "box_tw", "box_layered", "box_rsf3", "box_rsf4", "box_q", "box_hg2"]
+
+
+def fp32_errors(out, ref, dt):
+    rt_g, rt_r = out["fnft"], ref["fnft"]
+    both = (rt_g < 5000) & (rt_r < 5000)
+    flips = int(((rt_g < 5000) != (rt_r < 5000)).sum())
+    drt = np.abs(rt_g[both] - rt_r[both]) / dt if both.any() else np.zeros(1)
+    pk_g, pk_r = out["fric"][:, 75], ref["fric"][:, 75]
+    peak_rel = float(np.max(np.abs(pk_g - pk_r)) / max(np.max(np.abs(pk_r)), 1e-300))
+    return {"rupture_time_mean_dt": float(drt.mean()), "rupture_time_max_dt": float(drt.max()),
+            "rupture_flips": flips, "peak_slip_rate_rel": peak_rel}
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ELASTIC)
+def test_fp32_error_vs_fp64_reference(name):
+    print(f"case {name}", flush=True)
+    bundle, sc, ref = load_golden(name)
+    s = Solver(bundle, sc, "fp32")
+    s.run(int(sc["nstep"]))
+    out = s.outputs()
+    s.close()
+    e = fp32_errors(out, ref, sc["dt"])
+    ch = channel_errors(channels(out), channels(ref))
+    worst = max(ch, key=ch.get)
+    print(f"[{name}] fp32 vs fp64 reference: {e}; worst channel {worst} {ch[worst]:.2e}", flush=True)
+    assert e["rupture_time_mean_dt"] <= 0.5
+    assert e["peak_slip_rate_rel"] <= 0.01
