@@ -18,17 +18,19 @@ namespace eqd {
 template <typename R>
 hipError_t launch_elem_regular(const RegElems<R> &, const NodeState &, const StepConsts &, bool, bool, int, bool,
                                hipStream_t);
-hipError_t launch_elem_pml(const PmlElems &, const NodeState &, const StepConsts &, int, bool, hipStream_t);
 template <typename R>
-hipError_t launch_node3(const NodeState &, const int32_t *, int64_t, const R *, int64_t, const double *, int64_t,
+hipError_t launch_elem_pml(const PmlElems<R> &, const NodeState &, const StepConsts &, int, bool, hipStream_t);
+template <typename R>
+hipError_t launch_node3(const NodeState &, const int32_t *, int64_t, const R *, int64_t, const R *, int64_t,
                         double, int, int, hipStream_t);
-hipError_t launch_node_pml(const NodeState &, const PmlNodes &, const double *, int64_t, double, int, int,
+template <typename R>
+hipError_t launch_node_pml(const NodeState &, const PmlNodes &, const R *, int64_t, double, int, int,
                            hipStream_t);
 hipError_t launch_update_from_acc(const NodeState &, const int32_t *, int64_t, const PmlNodes &, double, int,
                                   hipStream_t);
 template <typename R>
 hipError_t launch_fault(const FaultState &, const NodeState &, const StepConsts &, const R *, int64_t,
-                        const double *, int64_t, int, double, int, hipStream_t);
+                        const R *, int64_t, int, double, int, hipStream_t);
 hipError_t launch_station(const OffStations &, const NodeState &, int, double, hipStream_t);
 }  // namespace eqd
 
@@ -72,7 +74,9 @@ struct eqd_handle {
     NodeState N{};
     RegElems<double> E64{};
     RegElems<float> E32{};
-    PmlElems P{};
+    PmlElems<double> P64{};
+    PmlElems<float> P32{};
+    int64_t n_pml_elems = 0;
     PmlNodes PN{};
     FaultState F{};
     OffStations S{};
@@ -567,32 +571,45 @@ int eqd_commit(eqd_handle *h)
                 for (int a = 0; a < 8; a++) p_phi[(mm * 8 + a) * np + p] = phi[e * 32 + mm * 8 + a];
             for (int i = 0; i < 21; i++) p_st[i * np + p] = stress[(int64_t)sci[e] + i];
         }
-        PmlElems &P = h->P;
-        P.n = np;
+        h->n_pml_elems = np;
         int rc;
         int32_t *pi;
         uint8_t *pm;
-        double *d;
         if ((rc = dupload(h, &pi, p_ien))) return rc;
-        P.ien = pi;
         if ((rc = dupload(h, &pm, p_mask))) return rc;
-        P.mask12 = pm;
-        if ((rc = dupload(h, &d, p_shp))) return rc; P.shp = d;
-        if ((rc = dupload(h, &d, p_det))) return rc; P.det = d;
-        if ((rc = dupload(h, &d, p_lam))) return rc; P.lam = d;
-        if ((rc = dupload(h, &d, p_mu))) return rc; P.mu = d;
-        if ((rc = dupload(h, &d, p_vp))) return rc; P.vp = d;
-        if ((rc = dupload(h, &d, p_rho))) return rc; P.rho = d;
-        if ((rc = dupload(h, &d, p_st))) return rc; P.stress = d;
-        if ((rc = dupload(h, &d, p_ss))) return rc; P.ss = d;
-        if ((rc = dupload(h, &d, p_phi))) return rc; P.phi = d;
-        if ((rc = dupload(h, &d, p_damp))) return rc; P.damp = d;
-        if (s.rdampm != 0.0 || s.C_elastic == 0) {
-            if ((rc = dupload(h, &d, p_em))) return rc;
-            P.emass = d;
+        auto fillp = [&](auto &P) -> int {
+            using R = std::remove_pointer_t<decltype(P.ef)>;
+            auto up = [&](const R **dst, const std::vector<double> &src) -> int {
+                std::vector<R> c(src.begin(), src.end());
+                R *q;
+                int r2 = dupload(h, &q, c);
+                *dst = q;
+                return r2;
+            };
+            int r2;
+            P.n = np;
+            P.ien = pi;
+            P.mask12 = pm;
+            if ((r2 = up(&P.shp, p_shp)) || (r2 = up(&P.det, p_det)) || (r2 = up(&P.lam, p_lam)) ||
+                (r2 = up(&P.mu, p_mu)) || (r2 = up(&P.vp, p_vp)) || (r2 = up(&P.rho, p_rho)) || (r2 = up(&P.ss, p_ss)) ||
+                (r2 = up(&P.phi, p_phi)) || (r2 = up(&P.damp, p_damp)))
+                return r2;
+            if (s.rdampm != 0.0 || s.C_elastic == 0) {
+                if ((r2 = up(&P.emass, p_em))) return r2;
+            }
+            std::vector<R> stc(p_st.begin(), p_st.end());
+            R *q;
+            if ((r2 = dupload(h, &q, stc))) return r2;
+            P.stress = q;
+            if ((r2 = dalloc(h, &q, 96 * np))) return r2;
+            P.ef = q;
+            return 0;
+        };
+        if (fp32) {
+            if ((rc = fillp(h->P32))) return rc;
+        } else {
+            if ((rc = fillp(h->P64))) return rc;
         }
-        if ((rc = dalloc(h, &d, 96 * np))) return rc;
-        P.ef = d;
     }
 
     // ---------------- node-to-element gather table (ascending element order)
@@ -777,7 +794,7 @@ static void timed_end(eqd_handle *h, hipEvent_t stop)
 }
 
 template <typename R>
-static int run_steps(eqd_handle *h, const RegElems<R> &E, int32_t nsteps)
+static int run_steps(eqd_handle *h, const RegElems<R> &E, const PmlElems<R> &P, int32_t nsteps)
 {
     const eqd_scalars &s = h->s;
     const double dt = s.dt;
@@ -801,17 +818,17 @@ static int run_steps(eqd_handle *h, const RegElems<R> &E, int32_t nsteps)
         HIPCHK(h, launch_elem_regular<R>(E, h->N, h->C, q, dp, s.C_hg, mass, st));
         timed_end(h, ev);
         timed_begin(h, "element_pml", &ev);
-        HIPCHK(h, launch_elem_pml(h->P, h->N, h->C, s.C_hg, mass, st));
+        HIPCHK(h, launch_elem_pml<R>(P, h->N, h->C, s.C_hg, mass, st));
         timed_end(h, ev);
         timed_begin(h, "fault", &ev);
-        HIPCHK(h, launch_fault<R>(h->F, h->N, h->C, E.ef, E.n, h->P.ef, h->P.n, nt, h->time, mode, st));
+        HIPCHK(h, launch_fault<R>(h->F, h->N, h->C, E.ef, E.n, P.ef, P.n, nt, h->time, mode, st));
         timed_end(h, ev);
         timed_begin(h, "node", &ev);
-        HIPCHK(h, launch_node3<R>(h->N, h->d_node3_free, (int64_t)h->node3_free.size(), E.ef, E.n, h->P.ef, h->P.n, dt,
+        HIPCHK(h, launch_node3<R>(h->N, h->d_node3_free, (int64_t)h->node3_free.size(), E.ef, E.n, P.ef, P.n, dt,
                                   mode, nt + 1, st));
         timed_end(h, ev);
         timed_begin(h, "node_pml", &ev);
-        HIPCHK(h, launch_node_pml(h->N, h->PN, h->P.ef, h->P.n, dt, mode, nt + 1, st));
+        HIPCHK(h, launch_node_pml<R>(h->N, h->PN, P.ef, P.n, dt, mode, nt + 1, st));
         timed_end(h, ev);
     }
     h->pending = true;
@@ -830,7 +847,7 @@ extern "C" int eqd_run(eqd_handle *h, int32_t nsteps)
         kv.second.total_ms = 0;
         kv.second.launches = 0;
     }
-    int rc = h->s.precision == EQD_PREC_FP32 ? run_steps(h, h->E32, nsteps) : run_steps(h, h->E64, nsteps);
+    int rc = h->s.precision == EQD_PREC_FP32 ? run_steps(h, h->E32, h->P32, nsteps) : run_steps(h, h->E64, h->P64, nsteps);
     if (rc) return rc;
     HIPCHK(h, hipStreamSynchronize(h->stream));
     if (h->timing) {
@@ -940,7 +957,13 @@ int eqd_get(eqd_handle *h, int32_t field, void *host, int64_t count)
                 if (idx < s.size_stress) out[idx] = v[i * nr + t];
             }
         }
-        if ((rc = d2h(h, v, h->P.stress, 21 * np))) return rc;
+        if (s.precision == EQD_PREC_FP32) {
+            std::vector<float> f;
+            if ((rc = d2h(h, f, h->P32.stress, 21 * np))) return rc;
+            v.assign(f.begin(), f.end());
+        } else if ((rc = d2h(h, v, h->P64.stress, 21 * np))) {
+            return rc;
+        }
         for (int64_t p = 0; p < np; p++) {
             const int64_t e = h->pml_ids[p];
             for (int i = 0; i < 21; i++) out[(int64_t)sci[e] + i] = v[i * np + p];

@@ -57,16 +57,17 @@ struct RegElems {
     R *ef;                // [24][n]  force on local node a, component c at (a*3+c)
 };
 
+template <typename R>
 struct PmlElems {
     int64_t n;
     const int32_t *ien;    // [8][n]
-    const double *shp;     // [24][n]
-    const double *det, *lam, *mu, *vp, *rho;
-    double *stress;        // [21][n]: 15 split + 6 initial
-    const double *ss, *phi, *emass;
-    const double *damp;    // [3][n] element-centre damping d_x, d_y, d_z
+    const R *shp;          // [24][n]
+    const R *det, *lam, *mu, *vp, *rho;
+    R *stress;             // [21][n]: 15 split + 6 initial
+    const R *ss, *phi, *emass;
+    const R *damp;         // [3][n] element-centre damping d_x, d_y, d_z
     const uint8_t *mask12; // [n] bit a: local node a carries 12 components
-    double *ef;            // [96][n] slot (a*12+k); 3-component nodes use k=0..2
+    R *ef;                 // [96][n] slot (a*12+k); 3-component nodes use k=0..2
 };
 
 struct PmlNodes {

