@@ -155,7 +155,8 @@ def main() -> int:
     nn = int(sc["totalNumOfNodes"])
     t = time.perf_counter()
     s = Solver(bundle, sc, args.precision, device=0, nstep=total)
-    log(f"uploaded ({s.device_bytes() / 1e9:.1f} GB on the device) in {time.perf_counter() - t:.1f} s")
+    layout = s.layout_info()
+    log(f"uploaded ({s.device_bytes() / 1e9:.1f} GB on the device) in {time.perf_counter() - t:.1f} s; layout {layout}")
     del bundle
     s.run(args.warmup)
     s.set_timing(True)
@@ -195,7 +196,7 @@ def main() -> int:
         "data": "synthetic (seeded case generator; no external data)",
         "config": {"workload": desc, "elements": int(sc["totalNumOfElements"]), "nodes": nn,
                    "regular_elements": n_reg, "fault_pairs": int(sc["nftnd"]), "precision": args.precision,
-                   "steps_run": nt},
+                   "steps_run": nt, "operator_classes": {k: v for k, v in layout.items() if "classes" in k}},
         "roofline": roofline, "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

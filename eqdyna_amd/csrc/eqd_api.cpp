@@ -5,10 +5,12 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <string>
 #include <type_traits>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/eqdyna_amd.h"
@@ -77,6 +79,7 @@ struct eqd_handle {
     PmlElems<double> P64{};
     PmlElems<float> P32{};
     int64_t n_pml_elems = 0;
+    int64_t op_classes_reg = 0, phi_classes_reg = 0, op_classes_pml = 0, phi_classes_pml = 0;
     PmlNodes PN{};
     FaultState F{};
     OffStations S{};
@@ -245,6 +248,41 @@ static const int32_t *I32(eqd_handle *h, int f)
     auto it = h->src.find(f);
     return it == h->src.end() ? nullptr : (const int32_t *)it->second;
 }
+// Groups rows whose values are bit-identical: idx[t] is the row's class and
+// table holds one copy per class.  Gives up (returns false) when there are
+// more than max_classes classes, in which case per-element arrays are used.
+template <typename R, typename RowFn>
+static bool classify_rows(int64_t n, int width, RowFn row, int64_t max_classes, std::vector<uint32_t> &idx,
+                          std::vector<R> &table)
+{
+    idx.assign(n, 0);
+    table.clear();
+    std::unordered_map<uint64_t, std::vector<uint32_t>> buckets;
+    std::vector<R> tmp(width);
+    int64_t nclass = 0;
+    for (int64_t t = 0; t < n; t++) {
+        row(t, tmp.data());
+        uint64_t hsh = 1469598103934665603ull;
+        const unsigned char *b = (const unsigned char *)tmp.data();
+        for (size_t i = 0; i < sizeof(R) * width; i++) hsh = (hsh ^ b[i]) * 1099511628211ull;
+        auto &bk = buckets[hsh];
+        int64_t c = -1;
+        for (uint32_t k : bk)
+            if (std::memcmp(table.data() + (size_t)k * width, tmp.data(), sizeof(R) * width) == 0) {
+                c = k;
+                break;
+            }
+        if (c < 0) {
+            if (nclass >= max_classes) return false;
+            c = nclass++;
+            bk.push_back((uint32_t)c);
+            table.insert(table.end(), tmp.begin(), tmp.end());
+        }
+        idx[t] = (uint32_t)c;
+    }
+    return true;
+}
+
 template <typename T>
 static void keep_copy(eqd_handle *h, int f, std::map<int, std::vector<T>> &dst)
 {
@@ -423,6 +461,8 @@ int eqd_commit(eqd_handle *h)
     h->n_reg = (int32_t)nr;
     h->n_pml = (int32_t)np;
     const bool fp32 = s.precision == EQD_PREC_FP32;
+    const char *nc_env = std::getenv("EQD_NO_OPERATOR_CLASSES");
+    const bool no_classes = nc_env && nc_env[0] == '1';
 
     // regular class SoA
     {
@@ -510,10 +550,43 @@ int eqd_commit(eqd_handle *h)
             int r2;
             E.n = nr;
             E.ien = d_ien;
-            if ((r2 = up(&E.shp, r_shp)) || (r2 = up(&E.det, r_det)) || (r2 = up(&E.lam, r_lam)) ||
-                (r2 = up(&E.mu, r_mu)) || (r2 = up(&E.vp, r_vp)) || (r2 = up(&E.rho, r_rho)) || (r2 = up(&E.ss, r_ss)) ||
-                (r2 = up(&E.phi, r_phi)))
+            // operator classes (bit-identical rows share a table row)
+            std::vector<uint32_t> cls;
+            std::vector<R> tab;
+            // EQD_NO_OPERATOR_CLASSES=1 forces per-element arrays (used to test that both give equal results)
+            const int64_t maxc = no_classes ? 0 : std::max<int64_t>(1, nr / 16);
+            if (classify_rows<R>(nr, kOpW, [&](int64_t t, R *row) {
+                    for (int i = 0; i < 24; i++) row[kOpShp + i] = (R)r_shp[i * nr + t];
+                    row[kOpDet] = (R)r_det[t];
+                    row[kOpLam] = (R)r_lam[t];
+                    row[kOpMu] = (R)r_mu[t];
+                    row[kOpVp] = (R)r_vp[t];
+                    row[kOpRho] = (R)r_rho[t];
+                    for (int i = 0; i < 6; i++) row[kOpSs + i] = (R)r_ss[i * nr + t];
+                }, maxc, cls, tab)) {
+                uint32_t *pc;
+                R *pt;
+                if ((r2 = dupload(h, &pc, cls)) || (r2 = dupload(h, &pt, tab))) return r2;
+                E.opc = pc;
+                E.optab = pt;
+                h->op_classes_reg = (int64_t)(tab.size() / kOpW);
+            } else if ((r2 = up(&E.shp, r_shp)) || (r2 = up(&E.det, r_det)) || (r2 = up(&E.lam, r_lam)) ||
+                       (r2 = up(&E.mu, r_mu)) || (r2 = up(&E.vp, r_vp)) || (r2 = up(&E.rho, r_rho)) ||
+                       (r2 = up(&E.ss, r_ss))) {
                 return r2;
+            }
+            if (classify_rows<R>(nr, 32, [&](int64_t t, R *row) {
+                    for (int i = 0; i < 32; i++) row[i] = (R)r_phi[i * nr + t];
+                }, maxc, cls, tab)) {
+                uint32_t *pc;
+                R *pt;
+                if ((r2 = dupload(h, &pc, cls)) || (r2 = dupload(h, &pt, tab))) return r2;
+                E.phic = pc;
+                E.phitab = pt;
+                h->phi_classes_reg = (int64_t)(tab.size() / 32);
+            } else if ((r2 = up(&E.phi, r_phi))) {
+                return r2;
+            }
             if (s.rdampm != 0.0 || s.C_elastic == 0) {
                 if ((r2 = up(&E.emass, r_em))) return r2;
             }
@@ -590,10 +663,42 @@ int eqd_commit(eqd_handle *h)
             P.n = np;
             P.ien = pi;
             P.mask12 = pm;
-            if ((r2 = up(&P.shp, p_shp)) || (r2 = up(&P.det, p_det)) || (r2 = up(&P.lam, p_lam)) ||
-                (r2 = up(&P.mu, p_mu)) || (r2 = up(&P.vp, p_vp)) || (r2 = up(&P.rho, p_rho)) || (r2 = up(&P.ss, p_ss)) ||
-                (r2 = up(&P.phi, p_phi)) || (r2 = up(&P.damp, p_damp)))
+            std::vector<uint32_t> cls;
+            std::vector<R> tab;
+            const int64_t maxc = no_classes ? 0 : std::max<int64_t>(1, np / 16);
+            if (classify_rows<R>(np, kOpWPml, [&](int64_t p, R *row) {
+                    for (int i = 0; i < 24; i++) row[kOpShp + i] = (R)p_shp[i * np + p];
+                    row[kOpDet] = (R)p_det[p];
+                    row[kOpLam] = (R)p_lam[p];
+                    row[kOpMu] = (R)p_mu[p];
+                    row[kOpVp] = (R)p_vp[p];
+                    row[kOpRho] = (R)p_rho[p];
+                    for (int i = 0; i < 6; i++) row[kOpSs + i] = (R)p_ss[i * np + p];
+                    for (int i = 0; i < 3; i++) row[kOpDamp + i] = (R)p_damp[i * np + p];
+                }, maxc, cls, tab)) {
+                uint32_t *pc;
+                R *pt;
+                if ((r2 = dupload(h, &pc, cls)) || (r2 = dupload(h, &pt, tab))) return r2;
+                P.opc = pc;
+                P.optab = pt;
+                h->op_classes_pml = (int64_t)(tab.size() / kOpWPml);
+            } else if ((r2 = up(&P.shp, p_shp)) || (r2 = up(&P.det, p_det)) || (r2 = up(&P.lam, p_lam)) ||
+                       (r2 = up(&P.mu, p_mu)) || (r2 = up(&P.vp, p_vp)) || (r2 = up(&P.rho, p_rho)) ||
+                       (r2 = up(&P.ss, p_ss)) || (r2 = up(&P.damp, p_damp))) {
                 return r2;
+            }
+            if (classify_rows<R>(np, 32, [&](int64_t p, R *row) {
+                    for (int i = 0; i < 32; i++) row[i] = (R)p_phi[i * np + p];
+                }, maxc, cls, tab)) {
+                uint32_t *pc;
+                R *pt;
+                if ((r2 = dupload(h, &pc, cls)) || (r2 = dupload(h, &pt, tab))) return r2;
+                P.phic = pc;
+                P.phitab = pt;
+                h->phi_classes_pml = (int64_t)(tab.size() / 32);
+            } else if ((r2 = up(&P.phi, p_phi))) {
+                return r2;
+            }
             if (s.rdampm != 0.0 || s.C_elastic == 0) {
                 if ((r2 = up(&P.emass, p_em))) return r2;
             }
@@ -1024,6 +1129,20 @@ int eqd_set_timing(eqd_handle *h, int32_t on)
 }
 
 int64_t eqd_device_bytes(eqd_handle *h) { return h ? h->bytes : 0; }
+
+int eqd_layout_info(eqd_handle *h, const char *key, int64_t *value)
+{
+    if (!h || !key || !value) return EQD_EINVAL;
+    const std::string k = key;
+    if (k == "regular_elements") *value = h->n_reg;
+    else if (k == "pml_elements") *value = h->n_pml;
+    else if (k == "op_classes_regular") *value = h->op_classes_reg;
+    else if (k == "phi_classes_regular") *value = h->phi_classes_reg;
+    else if (k == "op_classes_pml") *value = h->op_classes_pml;
+    else if (k == "phi_classes_pml") *value = h->phi_classes_pml;
+    else return fail(h, EQD_EINVAL, "unknown layout key %s", key);
+    return EQD_OK;
+}
 
 const char *eqd_last_error(eqd_handle *h) { return h ? h->err.c_str() : "null handle"; }
 

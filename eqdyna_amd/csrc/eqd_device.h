@@ -55,7 +55,19 @@ struct RegElems {
     const uint8_t *qidx;  // [n]      (Q only) row of qtab
     const double *qtab;   // [16][6]  exp(-dt/tau), 1-exp(-dt/tau), 8*w_p, 8*w_s, c_p, c_s
     R *ef;                // [24][n]  force on local node a, component c at (a*3+c)
+    // Operator classes: elements whose operators are bit-identical share one
+    // table row.  When optab is set, dN/dx, det, lambda, mu, Vp, rho and SS come
+    // from optab[opc[t]] (kOpW values per row, kOp* offsets) instead of the
+    // per-element arrays; likewise Phi from phitab[phic[t]] (32 per row).
+    const uint32_t *opc;
+    const R *optab;
+    const uint32_t *phic;
+    const R *phitab;
 };
+
+// layout of one operator-class row
+constexpr int kOpShp = 0, kOpDet = 24, kOpLam = 25, kOpMu = 26, kOpVp = 27, kOpRho = 28, kOpSs = 29,
+              kOpDamp = 35, kOpW = 35, kOpWPml = 38;
 
 template <typename R>
 struct PmlElems {
@@ -68,6 +80,11 @@ struct PmlElems {
     const R *damp;         // [3][n] element-centre damping d_x, d_y, d_z
     const uint8_t *mask12; // [n] bit a: local node a carries 12 components
     R *ef;                 // [96][n] slot (a*12+k); 3-component nodes use k=0..2
+    // operator classes as in RegElems; a PML row also holds the damping (kOpWPml values)
+    const uint32_t *opc;
+    const R *optab;
+    const uint32_t *phic;
+    const R *phitab;
 };
 
 struct PmlNodes {

@@ -67,10 +67,13 @@ __global__ void __launch_bounds__(256) elem_regular_kernel(RegElems<R> E, NodeSt
             }
         }
     }
+    // operators: one class-table row, or the element's own arrays
+    const R *op = E.optab ? E.optab + (int64_t)E.opc[t] * kOpW : nullptr;
     R shp[24];
 #pragma unroll
-    for (int i = 0; i < 24; i++) shp[i] = E.shp[i * n + t];
-    const R lam = E.lam[t], miu = E.mu[t];
+    for (int i = 0; i < 24; i++) shp[i] = op ? op[kOpShp + i] : E.shp[i * n + t];
+    const R lam = op ? op[kOpLam] : E.lam[t], miu = op ? op[kOpMu] : E.mu[t];
+    const R det = op ? op[kOpDet] : E.det[t];
     R sr[6] = {0, 0, 0, 0, 0, 0}, strain[6] = {0, 0, 0, 0, 0, 0};
 #pragma unroll
     for (int a = 0; a < 8; a++) {
@@ -186,7 +189,7 @@ __global__ void __launch_bounds__(256) elem_regular_kernel(RegElems<R> E, NodeSt
 #pragma unroll
         for (int i = 0; i < 24; i++) f[i] = 0;
     }
-    const R temp = -E.det[t] * (R)C.w;
+    const R temp = -det * (R)C.w;
     R st[6];
 #pragma unroll
     for (int i = 0; i < 6; i++) st[i] = temp * (stress[i] + (R)C.rdampk * srt[i]);
@@ -202,14 +205,15 @@ __global__ void __launch_bounds__(256) elem_regular_kernel(RegElems<R> E, NodeSt
         // Kosloff-Frazier stiffness hourglass, src/hrglss.f90:20-55
         R ss[6];
 #pragma unroll
-        for (int i = 0; i < 6; i++) ss[i] = E.ss[i * n + t];
+        for (int i = 0; i < 6; i++) ss[i] = op ? op[kOpSs + i] : E.ss[i * n + t];
+        const R *pht = E.phitab ? E.phitab + (int64_t)E.phic[t] * 32 : nullptr;
 #pragma unroll
         for (int i = 0; i < 24; i++) dl[i] = dl[i] + (R)C.rdampk * vl[i];
 #pragma unroll
         for (int m = 0; m < 4; m++) {
             R ph[8];
 #pragma unroll
-            for (int a = 0; a < 8; a++) ph[a] = E.phi[(m * 8 + a) * n + t];
+            for (int a = 0; a < 8; a++) ph[a] = pht ? pht[m * 8 + a] : E.phi[(m * 8 + a) * n + t];
             R pd0 = 0, pd1 = 0, pd2 = 0;
 #pragma unroll
             for (int a = 0; a < 8; a++) {
@@ -231,7 +235,8 @@ __global__ void __launch_bounds__(256) elem_regular_kernel(RegElems<R> E, NodeSt
         // viscous hourglass, src/hrglss.f90:57-97
         constexpr int fi[4][8] = {{1, 1, -1, -1, -1, -1, 1, 1}, {1, -1, -1, 1, -1, 1, 1, -1},
                                   {1, -1, 1, -1, 1, -1, 1, -1}, {1, -1, 1, -1, -1, 1, -1, 1}};
-        const R coef = (R)0.25 * (R)C.kapa_hg * E.rho[t] * E.vp[t] * (R)pow((double)E.det[t] * C.w, 2.0 / 3.0);
+        const R rho = op ? op[kOpRho] : E.rho[t], vp = op ? op[kOpVp] : E.vp[t];
+        const R coef = (R)0.25 * (R)C.kapa_hg * rho * vp * (R)pow((double)det * C.w, 2.0 / 3.0);
         R q[3][4];
 #pragma unroll
         for (int i = 0; i < 3; i++)
@@ -284,10 +289,11 @@ __global__ void __launch_bounds__(256) elem_pml_kernel(PmlElems<R> E, NodeState 
     for (int a = 0; a < 8; a++)
 #pragma unroll
         for (int k = 0; k < 3; k++) va[a * 3 + k] = (R)(N.vel[k * nn + nd[a]] - v0[k]);
+    const R *op = E.optab ? E.optab + (int64_t)E.opc[p] * kOpWPml : nullptr;
     R shp[24];
 #pragma unroll
-    for (int i = 0; i < 24; i++) shp[i] = E.shp[i * n + p];
-    const R lam = E.lam[p], miu = E.mu[p], det = E.det[p];
+    for (int i = 0; i < 24; i++) shp[i] = op ? op[kOpShp + i] : E.shp[i * n + p];
+    const R lam = op ? op[kOpLam] : E.lam[p], miu = op ? op[kOpMu] : E.mu[p], det = op ? op[kOpDet] : E.det[p];
     R sr[6] = {0, 0, 0, 0, 0, 0};
     R Dx_vx = 0, Dy_vy = 0, Dz_vz = 0, Dx_vy = 0, Dy_vx = 0, Dx_vz = 0, Dz_vx = 0, Dy_vz = 0, Dz_vy = 0;
 #pragma unroll
@@ -323,7 +329,8 @@ __global__ void __launch_bounds__(256) elem_pml_kernel(PmlElems<R> E, NodeState 
     for (int i = 3; i < 6; i++) srt[i] = miu * sr[i];
 
     const R dt = (R)C.dt;
-    const R d0 = E.damp[p], d1 = E.damp[n + p], d2 = E.damp[2 * n + p];
+    const R d0 = op ? op[kOpDamp] : E.damp[p], d1 = op ? op[kOpDamp + 1] : E.damp[n + p],
+            d2 = op ? op[kOpDamp + 2] : E.damp[2 * n + p];
     const R l2m = lam + (R)2 * miu;
     const R coef[15] = {l2m, lam, lam, lam, l2m, lam, lam, lam, l2m, miu, miu, miu, miu, miu, miu};
     const R grad[15] = {Dx_vx, Dy_vy, Dz_vz, Dx_vx, Dy_vy, Dz_vz, Dx_vx, Dy_vy, Dz_vz,
@@ -368,12 +375,13 @@ __global__ void __launch_bounds__(256) elem_pml_kernel(PmlElems<R> E, NodeState 
             }
         R ss[6];
 #pragma unroll
-        for (int i = 0; i < 6; i++) ss[i] = E.ss[i * n + p];
+        for (int i = 0; i < 6; i++) ss[i] = op ? op[kOpSs + i] : E.ss[i * n + p];
+        const R *pht = E.phitab ? E.phitab + (int64_t)E.phic[p] * 32 : nullptr;
 #pragma unroll
         for (int m = 0; m < 4; m++) {
             R ph[8];
 #pragma unroll
-            for (int a = 0; a < 8; a++) ph[a] = E.phi[(m * 8 + a) * n + p];
+            for (int a = 0; a < 8; a++) ph[a] = pht ? pht[m * 8 + a] : E.phi[(m * 8 + a) * n + p];
             R pd0 = 0, pd1 = 0, pd2 = 0;
 #pragma unroll
             for (int a = 0; a < 8; a++) {
@@ -394,7 +402,8 @@ __global__ void __launch_bounds__(256) elem_pml_kernel(PmlElems<R> E, NodeState 
     } else if (kHG == 2) {
         constexpr int fi[4][8] = {{1, 1, -1, -1, -1, -1, 1, 1}, {1, -1, -1, 1, -1, 1, 1, -1},
                                   {1, -1, 1, -1, 1, -1, 1, -1}, {1, -1, 1, -1, -1, 1, -1, 1}};
-        const R coef2 = (R)0.25 * (R)C.kapa_hg * E.rho[p] * E.vp[p] * (R)pow((double)det * C.w, 2.0 / 3.0);
+        const R rho = op ? op[kOpRho] : E.rho[p], vp = op ? op[kOpVp] : E.vp[p];
+        const R coef2 = (R)0.25 * (R)C.kapa_hg * rho * vp * (R)pow((double)det * C.w, 2.0 / 3.0);
         R q[3][4];
 #pragma unroll
         for (int i = 0; i < 3; i++)

@@ -17,7 +17,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("EQD_LIB", os.path.join(HERE, "libeqd.so"))
 
 EXPORTS = ["eqd_create", "eqd_set", "eqd_commit", "eqd_run", "eqd_get", "eqd_clock", "eqd_nan_info",
-           "eqd_kernel_time", "eqd_set_timing", "eqd_device_bytes", "eqd_last_error", "eqd_destroy"]
+           "eqd_kernel_time", "eqd_set_timing", "eqd_device_bytes", "eqd_layout_info", "eqd_last_error",
+           "eqd_destroy"]
 
 FIELDS = {
     "meshCoor": (1, "f8"), "eqNumIndexArr": (2, "i4"), "eqNumStartIndexLoc": (3, "i4"),
@@ -80,6 +81,7 @@ def lib():
         L.eqd_set_timing.argtypes = [C.c_void_p, C.c_int32]
         L.eqd_device_bytes.argtypes = [C.c_void_p]
         L.eqd_device_bytes.restype = C.c_int64
+        L.eqd_layout_info.argtypes = [C.c_void_p, C.c_char_p, C.POINTER(C.c_int64)]
         L.eqd_last_error.argtypes = [C.c_void_p]
         L.eqd_last_error.restype = C.c_char_p
         L.eqd_destroy.argtypes = [C.c_void_p]
@@ -182,6 +184,15 @@ class Solver:
         n = C.c_int64()
         self._check(self.L.eqd_kernel_time(self.h, name.encode(), C.byref(ms), C.byref(n)))
         return ms.value, n.value
+
+    def layout_info(self) -> Dict[str, int]:
+        out = {}
+        for k in ("regular_elements", "pml_elements", "op_classes_regular", "phi_classes_regular",
+                  "op_classes_pml", "phi_classes_pml"):
+            v = C.c_int64()
+            self._check(self.L.eqd_layout_info(self.h, k.encode(), C.byref(v)))
+            out[k] = v.value
+        return out
 
     def device_bytes(self) -> int:
         return int(self.L.eqd_device_bytes(self.h))

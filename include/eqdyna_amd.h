@@ -136,6 +136,13 @@ int eqd_set_timing(eqd_handle *h, int32_t on);
 /* Device bytes held by the handle. */
 int64_t eqd_device_bytes(eqd_handle *h);
 
+/* Facts about the device layout built by eqd_commit: "regular_elements",
+ * "pml_elements", and the number of operator classes (elements whose stored
+ * operators are bit-identical share one table row; 0 = per-element arrays):
+ * "op_classes_regular", "phi_classes_regular", "op_classes_pml",
+ * "phi_classes_pml".  Returns EQD_EINVAL for an unknown key. */
+int eqd_layout_info(eqd_handle *h, const char *key, int64_t *value);
+
 const char *eqd_last_error(eqd_handle *h);
 int eqd_destroy(eqd_handle *h);
 

@@ -85,3 +85,27 @@ def test_nan_is_reported():
         s.run(2)
     assert e.value.code == -2
     s.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_operator_classes_are_lossless(precision, monkeypatch):
+    # elements with bit-identical operators share a table row; the result must be
+    # bit-identical to per-element operator arrays
+    bundle, sc, _ = load_golden("box_layered")
+    n = int(sc["nstep"])
+    s1 = Solver(bundle, sc, precision)
+    info = s1.layout_info()
+    s1.run(n)
+    o1 = s1.outputs()
+    s1.close()
+    monkeypatch.setenv("EQD_NO_OPERATOR_CLASSES", "1")
+    s2 = Solver(bundle, sc, precision)
+    info2 = s2.layout_info()
+    s2.run(n)
+    o2 = s2.outputs()
+    s2.close()
+    print(f"\n[{precision}] classes {info} / forced off {info2}", flush=True)
+    assert info["op_classes_regular"] > 0 and info2["op_classes_regular"] == 0
+    for k in o1:
+        assert np.array_equal(o1[k], o2[k]), k
