@@ -52,6 +52,10 @@ def build_workload(name: str, steps_total: int):
     elif name == "c1":
         c = case.c1(steps=steps_total)
         desc = "BASELINE config 1: 120x60x60 = 432,000 hexahedra at 200 m incl. 10-element PML, homogeneous, SW fault"
+    elif name == "c3":
+        c = case.c3(steps=steps_total)
+        desc = ("BASELINE config 3: 240x120x120 = 3,456,000 hexahedra at 100 m incl. 10-element PML, seeded fractal "
+                "rough fault (rms/length 0.005, H=1), slip-law RSF with strong velocity weakening, seeded a field")
     else:
         raise SystemExit(f"unknown workload {name}")
     t = time.perf_counter()
@@ -133,7 +137,7 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--workload", default="c2", choices=["c2", "c1"])
+    ap.add_argument("--workload", default="c2", choices=["c2", "c1", "c3"])
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
     ap.add_argument("--cpu-baseline", default="auto", choices=["auto", "reference", "port", "none"])
     ap.add_argument("--cpu-ranks", type=int, default=16)

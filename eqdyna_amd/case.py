@@ -102,6 +102,14 @@ class Case:
     st_off_fault: List[List[float]] = field(default_factory=list)
     # on-fault fields, ofv[k, iz, ix]
     ofv: Optional[np.ndarray] = None
+    # rough fault interface (insertFaultType > 0): grid origin and spacing
+    # (fx_min, fz_min, spacing) and rows (y offset, dy/dx, dy/dz), x-major with
+    # z fastest, shape (nnx * nnz, 3) -- bFault_Rough_Geometry.txt,
+    # src/readInputFiles.f90:268-312, used by src/meshgen.f90:921-962
+    rough_nx: int = 0
+    rough_nz: int = 0
+    rough_origin: tuple = (0.0, 0.0, 0.0)
+    rough: Optional[np.ndarray] = None
     # module knobs of the reference (src/globalvar.f90:16-75)
     nPML: int = 6
     R: float = 0.01
@@ -185,6 +193,14 @@ class Case:
             raise ValueError("case has no on-fault fields (ofv)")
         sel = np.ascontiguousarray(p.ofv[ON_FAULT_SLOTS], dtype="<f8")  # (24, nfz, nfx)
         sel.tofile(os.path.join(d, "on_fault_vars_input.bin"))
+        if p.insertFaultType > 0:
+            if p.rough is None or p.rough.shape != (p.rough_nx * p.rough_nz, 3):
+                raise ValueError("insertFaultType > 0 needs rough rows of shape (rough_nx*rough_nz, 3)")
+            fx0, fz0, sp = p.rough_origin
+            with open(os.path.join(d, "bFault_Rough_Geometry.txt"), "w") as f:
+                f.write(f"{p.rough_nx} {p.rough_nz}\n{_s(float(sp))} {_s(float(fx0))} {_s(float(fz0))}\n")
+                for r in p.rough:
+                    f.write(f"{_s(float(r[0]))} {_s(float(r[1]))} {_s(float(r[2]))}\n")
         with open(os.path.join(d, "bKnobs.txt"), "w") as f:
             f.write(f"nPML {p.nPML}\nR {_s(float(p.R))}\nC_Q {p.C_Q}\nC_hg {p.C_hg}\n"
                     f"rdampm {_s(float(p.rdampm))}\nkapa_hg {_s(float(p.kapa_hg))}\n"
@@ -372,6 +388,100 @@ def box_plastic(steps: int = 40, friclaw: int = 1, name: str = "box_plastic") ->
     return c
 
 
+def box_dip(dip: float = 30.0, nx_half: int = 6, nd: int = 6, pad: int = 3, d: float = 200.0,
+            npml: int = 3, steps: int = 60, name: str = "box_dip") -> Case:
+    """Small box with a planar fault dipping ``dip`` degrees towards y+ on
+    wedge elements (C_degen = dip, src/library_degeneration.f90:3-55): the
+    grid spacing is d along strike and d*cos(dip), d*sin(dip) in y and z so
+    that the fault plane z = -y*tan(dip) runs through grid nodes and cuts the
+    hexahedra along a diagonal (the set-up of case_input/test.tpv36).  Slip
+    weakening with the forced expanding nucleation of TPV36
+    (src/faulting.f90:418-440), along-strike and along-dip initial shear."""
+    c = Case(name=name)
+    s, co = math.sin(math.radians(dip)), math.cos(math.radians(dip))
+    c.dip = dip
+    c.C_degen = dip
+    c.insertFaultType = 0
+    c.dx, c.dy, c.dz = d, d * co, d * s
+    fx = nx_half * d
+    c.fxmin, c.fxmax = -fx, fx
+    c.fymin, c.fymax = 0.0, nd * c.dy
+    c.fzmin, c.fzmax = -nd * c.dz, 0.0
+    c.xmin, c.xmax = -fx - 0.5 * d, fx + 0.5 * d
+    c.nuni_y_minus, c.nuni_y_plus = pad, nd + pad
+    c.ymin, c.ymax = -(pad + 0.5) * c.dy, (nd + pad + 0.5) * c.dy
+    c.zmin, c.zmax = c.fzmin - (pad + 0.5) * c.dz, 0.0
+    c.enlarging_ratio = 1.0
+    c.nPML = npml
+    c.R = 0.01
+    c.vp, c.vs, c.rou = 6000.0, 3464.0, 2670.0
+    c.roumax, c.vmaxPML = 2670.0, 6000.0
+    c.dt = 0.5 * c.dz / 6000.0
+    c.term = steps * c.dt
+    c.friclaw = 1
+    c.tpv = 36
+    c.C_nuclea = 1
+    src = 0.5 * nd * d  # down-dip distance of the hypocentre
+    c.xsource, c.ysource, c.zsource = 0.0, src * co, -src * s
+    c.nucR = 2.5 * d
+    o = c.new_ofv()
+    for ix, x in enumerate(c.fx):
+        for iz, z in enumerate(c.fz):
+            o[1, iz, ix] = 0.60
+            if abs(abs(x) - c.fxmax) < 0.01 or abs(z - c.fzmin) < 0.01:
+                o[1, iz, ix] = 1.0e4
+            o[2, iz, ix] = 0.45
+            o[3, iz, ix] = 0.02
+            o[5, iz, ix] = 0.05
+            o[7, iz, ix] = -100.0e6
+            o[8, iz, ix] = 20.0e6
+            o[49, iz, ix] = 50.0e6
+    c.st_on_fault = [[0.0, c.zsource / 1e3], [2 * d / 1e3, -2 * c.dz / 1e3], [0.0, 0.0]]
+    c.st_off_fault = [[0.0, 2 * c.dy / 1e3, 0.0], [2 * d / 1e3, -2 * c.dy / 1e3, 0.0]]
+    return c
+
+
+def rough_surface(c: Case, amp: float, lx: float, lz: float):
+    """Smooth fault-surface offsets y(x, z) = amp*sin(2*pi*x/lx)*cos(pi*z/lz)
+    with their exact slopes, sampled on the fault grid of ``c`` (spacing dx
+    along strike, dz along dip) as the rows of bFault_Rough_Geometry.txt.
+    (The reference's scripts/generateFaultInterface draws a random fractal
+    surface; a closed-form one keeps the fixture reproducible.)"""
+    xs, zs = c.fx, c.fz
+    rows = []
+    for x in xs:            # x-major, z fastest: row nnz*(ix-1)+iz
+        for z in zs:
+            sx, cx = math.sin(2 * math.pi * x / lx), math.cos(2 * math.pi * x / lx)
+            sz, cz = math.sin(math.pi * z / lz), math.cos(math.pi * z / lz)
+            rows.append([amp * sx * cz, amp * 2 * math.pi / lx * cx * cz, -amp * math.pi / lz * sx * sz])
+    c.rough_nx, c.rough_nz = len(xs), len(zs)
+    c.rough_origin = (float(xs[0]), float(zs[0]), float(c.dx))
+    c.rough = np.array(rows)
+
+
+def box_rough(friclaw: int = 4, amp: float = 20.0, steps: int = 60, name: str = "box_rough") -> Case:
+    """The rate-and-state box on a rough fault (insertFaultType = 1): every
+    node column is shifted in y by the surface offset at its (x, z) and the
+    split-node normals follow the local slopes (src/meshgen.f90:874-889,
+    921-962); elastic, so the normal stress is held in [-40, -10] MPa
+    (src/faulting.f90:208-219)."""
+    c = box_rsf(friclaw=friclaw, steps=steps, name=name)
+    c.insertFaultType = 1
+    rough_surface(c, amp, lx=c.fxmax - c.fxmin, lz=2.0 * (c.fzmax - c.fzmin))
+    # initial normal stress at the cap, so that every compressive change is
+    # capped; shear and state in steady state at that value
+    o = c.ofv
+    capped = -40.0e6
+    for ix in range(c.nfx):
+        for iz in range(c.nfz):
+            o[7, iz, ix] = capped
+            o[8, iz, ix] = 0.4 * -capped
+            if friclaw != 3:
+                o[20, iz, ix] = state_steady_state(o[9, iz, ix], o[10, iz, ix], o[11, iz, ix], o[12, iz, ix],
+                                                   o[13, iz, ix], o[8, iz, ix], capped, o[46, iz, ix], friclaw)
+    return c
+
+
 def gardner_rho(vp: float) -> float:
     """Gardner's relation in SI units, rho = 310 * Vp**0.25 (kg/m^3, m/s)."""
     return 310.0 * vp ** 0.25
@@ -501,4 +611,86 @@ def c2_sample(steps: int = 100, seed: int = 2, q: bool = True) -> Case:
     c.C_Q = 1 if q else 0
     c.dt = 0.0025
     c.term = steps * c.dt
+    return c
+
+
+def fractal_surface(n: int, hurst: float, seed: int) -> np.ndarray:
+    """Seeded self-affine random surface on an n x n grid (unit spacing): a
+    complex Gaussian spectrum shaped to power |k|^-2(H+1), low-pass filtered
+    with a fourth-order Butterworth roll-off at |k| = n/4, the three longest
+    wavelengths (|k|^2 <= 2) removed so the surface carries no overall tilt,
+    transformed back and normalised to zero mean and unit standard deviation
+    (the construction of Dunham, Belanger, Cong and Kozdon, BSSA 2011; cf.
+    scripts/generateFaultInterface:11-52)."""
+    rng = np.random.default_rng(seed)
+    k = np.fft.fftfreq(n, d=1.0 / n)  # integer wavenumbers 0..n/2, -n/2+1..-1
+    ksq = k[:, None] ** 2 + k[None, :] ** 2
+    with np.errstate(divide="ignore"):
+        power = np.where(ksq > 2.0, ksq ** -(hurst + 1.0), 0.0)
+    filt = 1.0 / (1.0 + (ksq / (n / 4.0) ** 2) ** 4)
+    spec = (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))) * np.sqrt(0.5 * power * filt)
+    w = np.real(np.fft.ifft2(spec))
+    return (w - w.mean()) / w.std()
+
+
+def rough_fractal(c: Case, alpha: float, hurst: float, seed: int) -> None:
+    """A fractal fault surface for ``c``: rms height alpha times the fault
+    length along strike, sampled at the fault grid (dx along strike, dz down
+    dip), slopes by centred differences (one-sided at the edges)."""
+    nx, nz = c.nfx, c.nfz
+    n = 1 << int(math.ceil(math.log2(max(nx, nz))))
+    w = fractal_surface(n, hurst, seed)[:nz, :nx]
+    h = (w - w.mean()) / w.std() * alpha * (c.fxmax - c.fxmin)
+    dhdx = np.gradient(h, c.dx, axis=1)
+    dhdz = np.gradient(h, c.dz, axis=0)
+    rows = np.stack([h.T.ravel(), dhdx.T.ravel(), dhdz.T.ravel()], axis=1)  # x-major, z fastest
+    c.rough_nx, c.rough_nz = nx, nz
+    c.rough_origin = (float(c.fxmin), float(c.fzmin), float(c.dx))
+    c.rough = np.ascontiguousarray(rows)
+
+
+def c3(steps: int = 3000, seed: int = 3) -> Case:
+    """BASELINE config 3 at 100 m: 240 x 120 x 120 = 3,456,000 hexahedra
+    including a 10-element PML; homogeneous rock; a seeded fractal rough
+    fault (rms height / length 0.005, H = 1) 16 km along strike and 9.8 km
+    down dip; slip law with strong velocity weakening (friclaw 4): a seeded
+    in [0.008, 0.016] (0.02 in a 1 km velocity-strengthening rim on the
+    sides and bottom), b = 0.012, L = 0.02 m, f0 = 0.6, V0 = 1e-6 m/s,
+    fw = 0.1, Vw = 0.1 m/s; sigma_n = -30 MPa (inside the rough-fault cap of
+    [-40, -10] MPa), tau_0 = 18.75 MPa with the state that makes the initial
+    slip rate 1e-12 m/s, TPV104 smooth nucleation (3 km, 1 s, 15 MPa);
+    dt = 0.005 s."""
+    c = _strike_slip_box(100.0, 10, "c3")
+    c.xmin, c.xmax = -11.0e3, 11.0e3
+    c.ymin, c.ymax = -5.0e3, 5.0e3
+    c.zmin, c.zmax = -11.0e3, 0.0
+    c.nuni_y_minus = c.nuni_y_plus = 49
+    c.vp, c.vs, c.rou = 6000.0, 3464.0, 2670.0
+    c.roumax, c.vmaxPML = 2670.0, 6000.0
+    c.dt = 0.005
+    c.term = steps * c.dt
+    c.friclaw = 4
+    c.tpv = 104
+    c.C_nuclea = 1
+    c.insertFaultType = 1
+    c.xsource, c.ysource, c.zsource = 0.0, 0.0, -6.0e3
+    c.nucR, c.nucT, c.nucdtau0 = 3.0e3, 1.0, 15.0e6
+    rough_fractal(c, 0.005, 1.0, seed)
+    rng = np.random.default_rng(seed + 1000)
+    o = c.new_ofv()
+    b, L, f0, v0, fw, vw, vini = 0.012, 0.02, 0.6, 1.0e-6, 0.1, 0.1, 1.0e-12
+    sn, tau0 = -30.0e6, 18.75e6
+    a_field = 0.008 + 0.008 * rng.random((c.nfz, c.nfx))
+    for ix, x in enumerate(c.fx):
+        for iz, z in enumerate(c.fz):
+            a = a_field[iz, ix]
+            if abs(x) > c.fxmax - 1.0e3 - 0.01 or z < c.fzmin + 1.0e3 + 0.01:
+                a = 0.02
+            o[1, iz, ix], o[2, iz, ix], o[3, iz, ix] = 0.6, 0.45, 0.3
+            o[7, iz, ix] = sn
+            o[8, iz, ix] = tau0
+            o[9, iz, ix], o[10, iz, ix], o[11, iz, ix] = a, b, L
+            o[12, iz, ix], o[13, iz, ix], o[14, iz, ix], o[15, iz, ix] = v0, f0, fw, vw
+            o[46, iz, ix] = vini
+            o[20, iz, ix] = state_steady_state(a, b, L, v0, f0, tau0, sn, vini, 4)
     return c

@@ -1,5 +1,6 @@
 // Structured case builder: the reference host's mesh generation and one-time
-// element precompute, for one rank and a planar vertical fault at y = 0.
+// element precompute, for one rank and one fault: planar vertical at y = 0,
+// planar dipping on wedge elements (C_degen > 3) or rough (insertFaultType > 0).
 //
 // It produces the same globalvar arrays, in the same layout and bit for bit,
 // as src/meshgen.f90 (nodes, equation numbers, elements, split nodes, fault
@@ -38,6 +39,14 @@ typedef struct {
     const double *st_off;
     double tol;
     int32_t nthreads;
+    // dipping fault on wedges: dip in degrees when > 3 (src/readInputFiles.f90:140-145)
+    double C_degen;
+    // rough fault: bFault_Rough_Geometry.txt (src/readInputFiles.f90:268-312);
+    // rough_geo(3, nx*nz) = (y offset, dy/dx, dy/dz), x-major, z fastest
+    int32_t insertFaultType;
+    int32_t rough_nx, rough_nz;
+    double rough_dx, rough_fxmin, rough_fzmin;
+    const double *rough_geo;
 } eqdm_params;
 
 typedef struct {
@@ -162,11 +171,46 @@ static Grid make_grid(const eqdm_params &p)
     return G;
 }
 
+static const double kPi = 4.0 * std::atan(1.0);
+
+// checkIsOnFault, src/meshgen.f90:771-786: the fault box, then y == 0 for a
+// vertical fault or distance to the plane z = -y*tan(dip) below dx/100
 static inline bool on_fault(const eqdm_params &p, double x, double y, double z)
 {
-    // checkIsOnFault for a planar vertical fault, src/meshgen.f90:787-803
-    return x >= p.fxmin - p.tol && x <= p.fxmax + p.tol && y >= p.fymin - p.tol && y <= p.fymax + p.tol &&
-           z >= p.fzmin - p.tol && z <= p.fzmax + p.tol && y == 0.0;
+    if (!(x >= p.fxmin - p.tol && x <= p.fxmax + p.tol && y >= p.fymin - p.tol && y <= p.fymax + p.tol &&
+          z >= p.fzmin - p.tol && z <= p.fzmax + p.tol))
+        return false;
+    if (p.C_degen == 0.0) return y == 0.0;
+    if (p.C_degen > 3.0) {
+        const double tg = std::tan(p.C_degen / 180.0 * kPi);
+        const double d = std::fabs(z + y * tg) / std::sqrt(1.0 + tg * tg);
+        return d < p.dx / 100.0;
+    }
+    return false;
+}
+
+// insertFaultInterface, src/meshgen.f90:921-962: the y coordinate of a node
+// column shifted by the rough surface, and the surface slopes there
+static inline void rough_shift(const eqdm_params &p, double ymin, double ymax, double x, double y, double z,
+                               double *yc, double *pfx, double *pfz)
+{
+    const double fx1 = p.rough_fxmin, fz1 = p.rough_fzmin;
+    const double fx2 = (p.rough_nx - 1) * p.rough_dx + fx1;
+    int ixx = 1, izz = 1;
+    const bool zin = z > fz1 - p.tol;
+    if (x < fx2 + p.tol && x > fx1 - p.tol)
+        ixx = (int)std::lround((x - fx1) / p.dx) + 1;
+    else if (x > fx2 + p.tol)
+        ixx = p.rough_nx;
+    if (zin) izz = (int)std::lround((z - fz1) / p.dz) + 1;
+    const double *r = p.rough_geo + 3 * ((int64_t)p.rough_nz * (ixx - 1) + izz - 1);
+    const double peak = r[0];
+    *pfx = r[1];
+    *pfz = r[2];
+    if (y > -p.tol)
+        *yc = y * (ymax - peak) / ymax + peak;
+    else
+        *yc = y * (peak - ymin) / (-ymin) + peak;
 }
 
 static inline bool is_pml_node(const Grid &G, double x, double y, double z)
@@ -216,6 +260,17 @@ static int match_station(const eqdm_params &p, const Grid &G, int ix, int iy, do
     return -1;
 }
 
+// mat(e, 1:5) = vp, vs, rho, lambda, mu (src/meshgen.f90:196-200)
+static inline void set_mat(eqdm_arrays *A, int64_t e, int64_t ne_total, double vp, double vs, double rho)
+{
+    double *m = A->mat + (e - 1);  // mat(ne, 5), column stride ne
+    m[0 * ne_total] = vp;
+    m[1 * ne_total] = vs;
+    m[2 * ne_total] = rho;
+    m[4 * ne_total] = vs * vs * rho;
+    m[3 * ne_total] = vp * vp * rho - 2.0 * m[4 * ne_total];
+}
+
 // Node loop of src/meshgen.f90:64-107 (topology only) or, with arrays, the
 // full fill.  Returns counts.
 static Counts topology(const eqdm_params &p, const Grid &G, eqdm_arrays *A, int64_t ne_total)
@@ -226,12 +281,16 @@ static Counts topology(const eqdm_params &p, const Grid &G, eqdm_arrays *A, int6
     std::vector<int32_t> plane1((size_t)(ny + 2) * (nz + 1), 0), plane2((size_t)(ny + 2) * (nz + 1), 0);
     auto P1 = [&](int iy, int iz) -> int32_t & { return plane1[(size_t)iz * (ny + 2) + iy]; };
     auto P2 = [&](int iy, int iz) -> int32_t & { return plane2[(size_t)iz * (ny + 2) + iy]; };
+    // y coordinates of the plane nodes (shifted by a rough fault surface)
+    std::vector<double> ypl1((size_t)(ny + 2) * (nz + 1), 0.0), ypl2((size_t)(ny + 2) * (nz + 1), 0.0);
+    auto Y1 = [&](int iy, int iz) -> double & { return ypl1[(size_t)iz * (ny + 2) + iy]; };
+    auto Y2 = [&](int iy, int iz) -> double & { return ypl2[(size_t)iz * (ny + 2) + iy]; };
     int64_t node = 0, elem = 0, eq = 0, tag = 0, sdof = 0, nf = 0, non = 0, noff = 0;
     std::unordered_map<int32_t, int32_t> slave_to_master;
     std::vector<char> found(p.n_off, 0);
     std::vector<int32_t> stnode;  // OffFaultStNodeIdIndex(2,:)
     const double pi = 4.0 * std::atan(1.0);
-    const double stk = p.fstrike * pi / 180.0, dip = 90.0 * pi / 180.0;
+    const double stk = p.fstrike * pi / 180.0, dip = (p.C_degen > 3.0 ? p.C_degen : 90.0) * pi / 180.0;
     // fault area bookkeeping (fltrc of src/meshgen.f90:770-775)
     int ixfi = 0, izfi = 0, ifs = 0, ifd = 0;
     std::vector<int32_t> frc_m, frc_k;  // [ifd][ifs] master node, pair index
@@ -244,13 +303,16 @@ static Counts topology(const eqdm_params &p, const Grid &G, eqdm_arrays *A, int6
         for (int iz = 1; iz <= nz; iz++) {
             for (int iy = 1; iy <= ny; iy++) {
                 const double x = G.x[ix], y = G.y[iy], z = G.z[iz];
+                double yc = y, pfx = 0.0, pfz = 0.0;
+                if (p.insertFaultType > 0) rough_shift(p, G.ymin, G.ymax, x, y, z, &yc, &pfx, &pfz);
                 node++;
                 P2(iy, iz) = (int32_t)node;
+                Y2(iy, iz) = yc;
                 const int ndof = is_pml_node(G, x, y, z) ? 12 : 3;
                 const bool fixed = is_fixed(p, G, x, y, z);
                 if (A) {
                     A->meshCoor[(node - 1) * 3] = x;
-                    A->meshCoor[(node - 1) * 3 + 1] = y;
+                    A->meshCoor[(node - 1) * 3 + 1] = yc;
                     A->meshCoor[(node - 1) * 3 + 2] = z;
                     A->eqNumStartIndexLoc[node - 1] = (int32_t)tag;
                     A->numOfDofPerNodeArr[node - 1] = ndof;
@@ -281,7 +343,7 @@ static Counts topology(const eqdm_params &p, const Grid &G, eqdm_arrays *A, int6
                         A->eqNumStartIndexLoc[ms - 1] = (int32_t)tag;
                         A->numOfDofPerNodeArr[ms - 1] = 3;
                         A->meshCoor[(ms - 1) * 3] = x;
-                        A->meshCoor[(ms - 1) * 3 + 1] = y;
+                        A->meshCoor[(ms - 1) * 3 + 1] = yc;
                         A->meshCoor[(ms - 1) * 3 + 2] = z;
                     }
                     for (int d = 0; d < 3; d++) {
@@ -302,15 +364,29 @@ static Counts topology(const eqdm_params &p, const Grid &G, eqdm_arrays *A, int6
                     }
                     if (A) {
                         double *un = A->un + (nf - 1) * 3, *us = A->us + (nf - 1) * 3, *ud = A->ud + (nf - 1) * 3;
-                        un[0] = std::cos(stk) * std::sin(dip);
-                        un[1] = -std::sin(stk) * std::sin(dip);
-                        un[2] = std::cos(dip);
-                        us[0] = -std::sin(stk);
-                        us[1] = -std::cos(stk);
-                        us[2] = 0.0;
-                        ud[0] = std::cos(stk) * std::cos(dip);
-                        ud[1] = std::sin(stk) * std::cos(dip);
-                        ud[2] = std::sin(dip);
+                        if (p.insertFaultType > 0) {
+                            // normals from the local surface slopes, src/meshgen.f90:874-889
+                            const double nrm = std::sqrt(pfx * pfx + 1.0 + pfz * pfz), sn = std::sqrt(1.0 + pfx * pfx);
+                            un[0] = -pfx / nrm;
+                            un[1] = 1.0 / nrm;
+                            un[2] = -pfz / nrm;
+                            us[0] = 1.0 / sn;
+                            us[1] = pfx / sn;
+                            us[2] = 0.0;
+                            ud[0] = us[1] * un[2] - us[2] * un[1];
+                            ud[1] = us[2] * un[0] - us[0] * un[2];
+                            ud[2] = us[0] * un[1] - us[1] * un[0];
+                        } else {
+                            un[0] = std::cos(stk) * std::sin(dip);
+                            un[1] = -std::sin(stk) * std::sin(dip);
+                            un[2] = std::cos(dip);
+                            us[0] = -std::sin(stk);
+                            us[1] = -std::cos(stk);
+                            us[2] = 0.0;
+                            ud[0] = std::cos(stk) * std::cos(dip);
+                            ud[1] = std::sin(stk) * std::cos(dip);
+                            ud[2] = std::sin(dip);
+                        }
                     }
                     if (ixfi == 0) ixfi = ix;
                     if (izfi == 0) izfi = iz;
@@ -323,32 +399,28 @@ static Counts topology(const eqdm_params &p, const Grid &G, eqdm_arrays *A, int6
                 }
                 // element (createElement + setElementMaterial + replaceSlaveWithMasterNode)
                 if (ix >= 2 && iy >= 2 && iz >= 2) {
+                    // createElement, src/meshgen.f90:700-741
                     elem++;
-                    int32_t nd[8] = {P1(iy - 1, iz - 1), P2(iy - 1, iz - 1), P2(iy, iz - 1), P1(iy, iz - 1),
-                                     P1(iy - 1, iz),     P2(iy - 1, iz),     P2(iy, iz),     P1(iy, iz)};
+                    const int32_t base[8] = {P1(iy - 1, iz - 1), P2(iy - 1, iz - 1), P2(iy, iz - 1), P1(iy, iz - 1),
+                                             P1(iy - 1, iz),     P2(iy - 1, iz),     P2(iy, iz),     P1(iy, iz)};
+                    // centroid from the nodes' coordinates (all nodes of the element exist already)
+                    const int ixs[8] = {ix - 1, ix, ix, ix - 1, ix - 1, ix, ix, ix - 1};
+                    const double ys[8] = {Y1(iy - 1, iz - 1), Y2(iy - 1, iz - 1), Y2(iy, iz - 1), Y1(iy, iz - 1),
+                                          Y1(iy - 1, iz),     Y2(iy - 1, iz),     Y2(iy, iz),     Y1(iy, iz)};
+                    const int izs[8] = {iz - 1, iz - 1, iz - 1, iz - 1, iz, iz, iz, iz};
                     double xc[3] = {0.0, 0.0, 0.0};
-                    // the centroid needs coordinates; the nodes of an element all exist already
-                    if (A) {
-                        for (int a = 0; a < 8; a++)
-                            for (int j = 0; j < 3; j++) xc[j] = xc[j] + A->meshCoor[(int64_t)(nd[a] - 1) * 3 + j];
-                    } else {
-                        // same sums from the grid lines (only the PML test needs them here)
-                        const int ixs[8] = {ix - 1, ix, ix, ix - 1, ix - 1, ix, ix, ix - 1};
-                        const int iys[8] = {iy - 1, iy - 1, iy, iy, iy - 1, iy - 1, iy, iy};
-                        const int izs[8] = {iz - 1, iz - 1, iz - 1, iz - 1, iz, iz, iz, iz};
-                        for (int a = 0; a < 8; a++) {
-                            xc[0] = xc[0] + G.x[ixs[a]];
-                            xc[1] = xc[1] + G.y[iys[a]];
-                            xc[2] = xc[2] + G.z[izs[a]];
-                        }
+                    for (int a = 0; a < 8; a++) {
+                        xc[0] = xc[0] + G.x[ixs[a]];
+                        xc[1] = xc[1] + ys[a];
+                        xc[2] = xc[2] + G.z[izs[a]];
                     }
                     for (int j = 0; j < 3; j++) xc[j] = xc[j] / 8.0;
                     const bool pml = xc[0] > G.PMLb[0] || xc[0] < G.PMLb[1] || xc[1] > G.PMLb[2] ||
                                      xc[1] < G.PMLb[3] || xc[2] < G.PMLb[4];
-                    const int etype = pml ? 2 : 1;
+                    const int64_t e1 = elem;
+                    int etype = pml ? 2 : 1;
                     if (A) {
-                        A->stressCompIndexArr[elem - 1] = (int32_t)sdof;
-                        A->elemTypeArr[elem - 1] = etype;
+                        A->stressCompIndexArr[e1 - 1] = (int32_t)sdof;
                         // setElementMaterial, src/meshgen.f90:158-204
                         double vp = 0, vs = 0, rho = 0;
                         if (p.nmat == 1 && p.n2mat == 3) {
@@ -364,21 +436,53 @@ static Counts topology(const eqdm_params &p, const Grid &G, eqdm_arrays *A, int6
                                     }
                             }
                         }
-                        double *m = A->mat + (elem - 1);  // mat(ne, 5), column stride ne
-                        m[0 * ne_total] = vp;
-                        m[1 * ne_total] = vs;
-                        m[2 * ne_total] = rho;
-                        m[4 * ne_total] = vs * vs * rho;
-                        m[3 * ne_total] = vp * vp * rho - 2.0 * m[4 * ne_total];
-                        // replaceSlaveWithMasterNode, src/meshgen.f90:759-785
-                        if (etype == 1 && x > p.fxmin - p.tol && x < p.fxmax + p.dx + p.tol && z > p.fzmin - p.tol &&
-                            y > 0.0 && std::fabs(y - p.dy) < p.tol) {
+                        set_mat(A, e1, ne_total, vp, vs, rho);
+                    }
+                    sdof += pml ? 21 : 12;
+                    int32_t nd[8];
+                    for (int a = 0; a < 8; a++) nd[a] = base[a];
+                    int64_t last = e1;
+                    if (p.C_degen > 3.0) {
+                        // wedge, src/library_degeneration.f90:3-55: a hexahedron the
+                        // dipping plane cuts along its diagonal becomes two wedges
+                        const double tg = std::tan(p.C_degen / 180.0 * kPi);
+                        const double d = std::fabs(xc[1] * tg + xc[2]) / std::sqrt(1.0 + tg * tg);
+                        if (xc[0] > p.fxmin && xc[0] < p.fxmax && xc[1] > p.fymin && xc[1] < p.fymax &&
+                            xc[2] > p.fzmin && d < p.tol) {
+                            static const int below[8] = {5, 1, 4, 4, 6, 2, 3, 3}, above[8] = {4, 8, 5, 5, 3, 7, 6, 6};
+                            if (A) {
+                                A->elemTypeArr[e1 - 1] = 11;
+                                for (int a = 0; a < 8; a++) A->nodeElemIdRelation[(e1 - 1) * 8 + a] = base[below[a] - 1];
+                            }
+                            elem++;
+                            last = elem;
+                            etype = 12;
+                            for (int a = 0; a < 8; a++) nd[a] = base[above[a] - 1];
+                            if (A) {
+                                A->stressCompIndexArr[last - 1] = (int32_t)sdof;
+                                // the upper wedge takes the first material row as given
+                                set_mat(A, last, ne_total, p.material[0], p.material[1], p.material[2]);
+                            }
+                            sdof += 12;
+                        } else if (etype == 1) {
+                            // an element touching the plane with its nodes 1 or 2 sits on its upper side
+                            const double y1 = ys[0], y2 = ys[1];
+                            if (on_fault(p, G.x[ix - 1], y1, G.z[iz - 1]) || on_fault(p, G.x[ix], y2, G.z[iz - 1]))
+                                etype = 13;
+                        }
+                    }
+                    if (A) {
+                        A->elemTypeArr[last - 1] = etype;
+                        // replaceSlaveWithMasterNode, src/meshgen.f90:743-766
+                        if ((etype == 1 && x > p.fxmin - p.tol && x < p.fxmax + p.dx + p.tol && z > p.fzmin - p.tol &&
+                             y > 0.0 && std::fabs(y - p.dy) < p.tol) ||
+                            etype == 12 || etype == 13) {
                             for (int a = 0; a < 8; a++) {
                                 auto it = slave_to_master.find(nd[a]);
                                 if (it != slave_to_master.end()) nd[a] = it->second;
                             }
                         }
-                        for (int a = 0; a < 8; a++) A->nodeElemIdRelation[(elem - 1) * 8 + a] = nd[a];
+                        for (int a = 0; a < 8; a++) A->nodeElemIdRelation[(last - 1) * 8 + a] = nd[a];
                         if (p.C_elastic == 0) {
                             // setPlasticStress, src/meshgen.f90:976-996
                             const double depth = -0.5 * (G.z[iz] + G.z[iz - 1]) + 7.3215;
@@ -386,18 +490,18 @@ static Counts topology(const eqdm_params &p, const Grid &G, eqdm_arrays *A, int6
                             const double ang = p.str1ToFaultAngle * pi / 180.0;
                             const double strVert = -(p.roumax - p.rhow * (p.gamar + 1.0)) * depth * p.grav;
                             const double devStr = std::fabs(strVert) * p.devStrToStrVertRatio;
-                            double *sa = A->stressArr + sdof;
+                            double *sa = A->stressArr + A->stressCompIndexArr[last - 1];
                             sa[3 + 15 * et - 1] = strVert;
                             sa[1 + 15 * et - 1] = strVert - devStr * std::cos(2.0 * ang);
                             sa[2 + 15 * et - 1] = strVert + devStr * std::cos(2.0 * ang);
                             sa[6 + 15 * et - 1] = devStr * std::sin(2.0 * ang);
                         }
                     }
-                    sdof += pml ? 21 : 12;
                 }
             }
         }
         plane1 = plane2;
+        ypl1 = ypl2;
     }
     c.nn = node + nf;
     c.ne = elem;
@@ -465,8 +569,8 @@ static double vlm(const double xl[3][8])
 // calcGlobalShapeFunc + contm + calcSSPhi4Hrgls for one hexahedron,
 // src/calcGlobalShapeFunc.f90:1-75, src/assembleGlobalMass.f90:328-406.
 // Returns false on a non-positive Jacobian.
-static bool elem_precompute(const double xl[3][8], double rho, double lam, double mu, double w, double shp[24],
-                            double *detp, double emass[24], double ss[6], double phi[32])
+static bool elem_precompute(const double xl[3][8], bool wedge, double rho, double lam, double mu, double w,
+                            double shp[24], double *detp, double emass[24], double ss[6], double phi[32])
 {
     static const double ac[8][3] = {{-1, -1, -1}, {1, -1, -1}, {1, 1, -1}, {-1, 1, -1},
                                     {-1, -1, 1},  {1, -1, 1},  {1, 1, 1},  {-1, 1, 1}};
@@ -475,6 +579,15 @@ static bool elem_precompute(const double xl[3][8], double rho, double lam, doubl
     for (int i = 0; i < 8; i++) {
         g[3][i] = cst;
         for (int j = 0; j < 3; j++) g[j][i] = cst * ac[i][j];
+    }
+    if (wedge) {
+        // degenerate to a wedge: columns 3+4 and 7+8 collapse (src/calcGlobalShapeFunc.f90:19-28)
+        for (int i = 0; i < 4; i++) {
+            g[i][2] = g[i][2] + g[i][3];
+            g[i][3] = 0.0;
+            g[i][6] = g[i][6] + g[i][7];
+            g[i][7] = 0.0;
+        }
     }
     double xs[3][3];  // xs[j][i] = d x_j / d xi_i
     for (int i = 0; i < 3; i++)
@@ -601,8 +714,9 @@ int eqdm_fill(const eqdm_params *p, const eqdm_sizes *s, eqdm_arrays *A, int32_t
             for (int j = 0; j < 3; j++) xl[j][a] = A->meshCoor[nd * 3 + j];
         }
         double det;
-        if (!elem_precompute(xl, A->mat[2 * ne + e], A->mat[3 * ne + e], A->mat[4 * ne + e], w, A->eleshp + e * 24,
-                             &det, A->elemass + e * 24, A->ss + e * 6, A->phi + e * 32)) {
+        const bool wedge = A->elemTypeArr[e] == 11 || A->elemTypeArr[e] == 12;
+        if (!elem_precompute(xl, wedge, A->mat[2 * ne + e], A->mat[3 * ne + e], A->mat[4 * ne + e], w,
+                             A->eleshp + e * 24, &det, A->elemass + e * 24, A->ss + e * 6, A->phi + e * 32)) {
 #pragma omp critical
             bad = (int32_t)(e + 1);
             continue;

@@ -1,8 +1,8 @@
 """Build a Case into the reference host's globalvar arrays without the Fortran host.
 
 ``build_case(case)`` does what the reference program does between reading its
-inputs and calling the time loop (src/eqdyna3d.f90:33-70) for one rank and a
-planar vertical fault: scalar set-up of readglobal/readmaterial
+inputs and calling the time loop (src/eqdyna3d.f90:33-70) for one rank and one
+fault (vertical planar, dipping planar on wedges, or rough): scalar set-up of readglobal/readmaterial
 (src/readInputFiles.f90:2-186), mesh generation and element precompute
 (libeqd_mesh.so, restating src/meshgen.f90 and src/assembleGlobalMass.f90),
 on-fault fields into fric (src/netcdf_io.f90:70-107) and init_vel
@@ -35,7 +35,10 @@ class Params(C.Structure):
         [(n, C.c_double) for n in ("roumax", "rhow", "gamar", "grav", "str1ToFaultAngle", "devStrToStrVertRatio")] + \
         [("nmat", C.c_int32), ("n2mat", C.c_int32), ("material", C.c_void_p),
          ("n_on", C.c_int32), ("st_on", C.c_void_p), ("n_off", C.c_int32), ("st_off", C.c_void_p),
-         ("tol", C.c_double), ("nthreads", C.c_int32)]
+         ("tol", C.c_double), ("nthreads", C.c_int32),
+         ("C_degen", C.c_double), ("insertFaultType", C.c_int32), ("rough_nx", C.c_int32), ("rough_nz", C.c_int32),
+         ("rough_dx", C.c_double), ("rough_fxmin", C.c_double), ("rough_fzmin", C.c_double),
+         ("rough_geo", C.c_void_p)]
 
 
 class Sizes(C.Structure):
@@ -78,9 +81,10 @@ def lib():
 
 
 def _params(c: Case, nthreads: int, keep: list) -> Params:
-    if c.C_degen > 3.0 or c.insertFaultType > 0 or c.ntotft != 1 or c.fymin != 0.0 or c.fymax != 0.0:
-        raise ValueError("the case builder covers one planar vertical fault at y = 0 "
-                         "(dipping, wedge and rough faults need the Fortran host)")
+    if c.ntotft != 1:
+        raise ValueError("the case builder meshes one fault (ntotft = 1)")
+    if c.C_degen <= 3.0 and (c.fymin != 0.0 or c.fymax != 0.0):
+        raise ValueError("a fault off y = 0 needs C_degen > 3 (a dipping fault on wedges)")
     if c.nx * c.ny * c.nz != 1:
         raise ValueError("the case builder meshes one rank (nx = ny = nz = 1)")
     p = Params()
@@ -108,6 +112,16 @@ def _params(c: Case, nthreads: int, keep: list) -> Params:
     p.n_off, p.st_off = len(off), off.ctypes.data if len(off) else None
     p.tol = 1.0e-5
     p.nthreads = int(nthreads)
+    p.C_degen = float(c.C_degen)
+    p.insertFaultType = int(c.insertFaultType)
+    if c.insertFaultType > 0:
+        if c.rough is None or c.rough.shape != (c.rough_nx * c.rough_nz, 3):
+            raise ValueError("insertFaultType > 0 needs rough rows of shape (rough_nx*rough_nz, 3)")
+        rg = np.ascontiguousarray(c.rough, dtype=np.float64)
+        keep.append(rg)
+        p.rough_nx, p.rough_nz = int(c.rough_nx), int(c.rough_nz)
+        p.rough_fxmin, p.rough_fzmin, p.rough_dx = (float(v) for v in c.rough_origin)
+        p.rough_geo = rg.ctypes.data
     return p
 
 

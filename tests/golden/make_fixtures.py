@@ -6,7 +6,7 @@ oracle/ref.mk) on one rank: the FP64 input bundle the reference hands to its
 time loop (after meshgen and assembleGlobalMass) and the reference's outputs
 after ``nstep`` steps.  Data only; regenerate with
 
-    python tests/golden/make_fixtures.py
+    python tests/golden/make_fixtures.py [case names]
 
 (needs /root/reference and a built oracle/_ref/eqd_ref).
 """
@@ -40,22 +40,26 @@ def cases():
         q,
         case.box_plastic(name="box_plastic"),
         hg2,
+        case.box_dip(name="box_dip"),
+        case.box_rough(name="box_rough"),
     ]
 
 
-def main():
+def main(names):
     for c in cases():
+        if names and c.name not in names:
+            continue
         with tempfile.TemporaryDirectory() as d:
             b, s, o, _ = refrun.run_reference(c, d, 1, True)
         arrays = {"in_" + k: v for k, v in b.items()}
         arrays.update({"out_" + k: v for k, v in o.items()})
-        names = sorted(s)
-        arrays["scalar_names"] = np.array(names)
-        arrays["scalar_values"] = np.array([s[k] for k in names])
+        keys = sorted(s)
+        arrays["scalar_names"] = np.array(keys)
+        arrays["scalar_values"] = np.array([s[k] for k in keys])
         p = os.path.join(HERE, c.name + ".npz")
         np.savez_compressed(p, **arrays)
         print(p, os.path.getsize(p))
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:])

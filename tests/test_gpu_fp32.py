@@ -9,7 +9,7 @@ import pytest
 from eqdyna_amd.solver import Solver
 from fixtures import channel_errors, channels, load_golden
 
-ELASTIC = ["box_sw", "box_tw", "box_layered", "box_rsf3", "box_rsf4", "box_q", "box_hg2"]
+ELASTIC = ["box_sw", "box_tw", "box_layered", "box_rsf3", "box_rsf4", "box_q", "box_hg2", "box_dip", "box_rough"]
 
 
 def fp32_errors(out, ref, dt):
