@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <algorithm>
 #include <map>
 #include <string>
 #include <type_traits>
@@ -32,7 +33,8 @@ hipError_t launch_update_from_acc(const NodeState &, const int32_t *, int64_t, c
                                   hipStream_t);
 template <typename R>
 hipError_t launch_fault(const FaultState &, const NodeState &, const StepConsts &, const R *, int64_t,
-                        const R *, int64_t, int, double, int, hipStream_t);
+                        const R *, int64_t, int, double, int, bool, hipStream_t);
+hipError_t launch_mlp_prepare(const float *p, __bf16 *wb, float *w4r, hipStream_t s);
 hipError_t launch_station(const OffStations &, const NodeState &, int, double, hipStream_t);
 }  // namespace eqd
 
@@ -85,6 +87,14 @@ struct eqd_handle {
     OffStations S{};
     StepConsts C{};
     int32_t *d_node3_free = nullptr, *d_node3_all = nullptr;
+    // learned Newton guess: float32 parameters and their bfloat16 operands
+    bool learn = false;
+    float *mlp_p = nullptr, *mlp_w4r = nullptr;
+    __bf16 *mlp_wb = nullptr;
+    // harvest of training rows
+    int32_t harvest_every = 0;
+    float *harvest = nullptr;
+    int64_t harvest_cap = 0, harvest_rows = 0;
     bool timing = false;
     std::map<std::string, KTimer> timers;
 };
@@ -776,9 +786,11 @@ int eqd_commit(eqd_handle *h)
         const int64_t nf = s.nf;
         FaultState &F = h->F;
         F.nf = nf;
+        F.ld = (nf + 127) / 128 * 128;  // the fault kernel's grid, lanes past nf write padding rows
         F.nstep = s.nstep;
+        const int64_t ld = F.ld;
         std::vector<int32_t> fn(2 * nf), sta(nf, -1);
-        std::vector<double> un(3 * nf), us(3 * nf), ud(3 * nf), xyz(3 * nf), fms(nf), fmm(nf), fr(100 * nf);
+        std::vector<double> un(3 * nf), us(3 * nf), ud(3 * nf), xyz(3 * nf), fms(nf), fmm(nf), fr(100 * ld, 0.0);
         const double *pun = F64(h, EQD_UN), *pus = F64(h, EQD_US), *pud = F64(h, EQD_UD), *fric = F64(h, EQD_FRIC);
         for (int64_t i = 0; i < nf; i++) {
             fn[i] = nsmp[i * 2] - 1;
@@ -791,7 +803,7 @@ int eqd_commit(eqd_handle *h)
             }
             fms[i] = fnms[fn[i]];
             fmm[i] = fnms[fn[nf + i]];
-            for (int k = 0; k < 100; k++) fr[k * nf + i] = fric[i * 100 + k];
+            for (int k = 0; k < 100; k++) fr[k * ld + i] = fric[i * 100 + k];
         }
         if (s.non > 0) {
             const int32_t *an = I32(h, EQD_ANONFS);
@@ -817,6 +829,8 @@ int eqd_commit(eqd_handle *h)
         if ((rc = dupload(h, &d, fmm))) return rc; F.fmm = d;
         if ((rc = dupload(h, &d, xyz))) return rc; F.xyz = d;
         if ((rc = dupload(h, &d, fr))) return rc; F.fric = d;
+        if ((rc = dupload(h, &pi, std::vector<int32_t>(5 * ld, 0)))) return rc; F.nstat = pi;
+        F.k_fallback = 0;
         if ((rc = dupload(h, &d, std::vector<double>(F64(h, EQD_FNFT), F64(h, EQD_FNFT) + nf)))) return rc; F.fnft = d;
         const int64_t non = s.non > 0 ? s.non : 1;
         if ((rc = dalloc(h, &d, non * (int64_t)s.nstep * 12))) return rc;
@@ -926,7 +940,15 @@ static int run_steps(eqd_handle *h, const RegElems<R> &E, const PmlElems<R> &P, 
         HIPCHK(h, launch_elem_pml<R>(P, h->N, h->C, s.C_hg, mass, st));
         timed_end(h, ev);
         timed_begin(h, "fault", &ev);
-        HIPCHK(h, launch_fault<R>(h->F, h->N, h->C, E.ef, E.n, P.ef, P.n, nt, h->time, mode, st));
+        FaultState F = h->F;
+        F.harvest = nullptr;
+        if (h->harvest && h->harvest_every > 0 && nt % h->harvest_every == 0 && h->harvest_rows < h->harvest_cap) {
+            F.harvest = h->harvest;
+            F.harvest_base = h->harvest_rows;
+            F.harvest_cap = h->harvest_cap;
+            h->harvest_rows = std::min(h->harvest_cap, h->harvest_rows + F.nf);
+        }
+        HIPCHK(h, launch_fault<R>(F, h->N, h->C, E.ef, E.n, P.ef, P.n, nt, h->time, mode, h->learn, st));
         timed_end(h, ev);
         timed_begin(h, "node", &ev);
         HIPCHK(h, launch_node3<R>(h->N, h->d_node3_free, (int64_t)h->node3_free.size(), E.ef, E.n, P.ef, P.n, dt,
@@ -1025,9 +1047,10 @@ int eqd_get(eqd_handle *h, int32_t field, void *host, int64_t count)
     }
     case EQD_FRIC: {
         std::vector<double> v;
-        if ((rc = d2h(h, v, h->F.fric, 100 * nf))) return rc;
+        const int64_t ld = h->F.ld;
+        if ((rc = d2h(h, v, h->F.fric, 100 * ld))) return rc;
         for (int64_t i = 0; i < nf; i++)
-            for (int k = 0; k < 100; k++) out[i * 100 + k] = v[k * nf + i];
+            for (int k = 0; k < 100; k++) out[i * 100 + k] = v[k * ld + i];
         return EQD_OK;
     }
     case EQD_FNFT: {
@@ -1130,6 +1153,107 @@ int eqd_set_timing(eqd_handle *h, int32_t on)
 
 int64_t eqd_device_bytes(eqd_handle *h) { return h ? h->bytes : 0; }
 
+int eqd_set_guess(eqd_handle *h, const float *params, int64_t nparams, int32_t k_fallback, float gate)
+{
+    if (!h) return EQD_EINVAL;
+    if (!h->committed) return fail(h, EQD_ESTATE, "eqd_set_guess before eqd_commit");
+    if (k_fallback < 0 || k_fallback > 20) return fail(h, EQD_EINVAL, "k_fallback %d outside 0..20", k_fallback);
+    if (!params || nparams == 0) {
+        // plain guess: the reference's solve (k_fallback 0) or the faster
+        // mode's iteration policy without the network (k_fallback > 0)
+        h->learn = false;
+        h->F.k_fallback = k_fallback;
+        return EQD_OK;
+    }
+    if (nparams != EQD_MLP_NPARAMS)
+        return fail(h, EQD_EINVAL, "the network has %d parameters, got %lld", EQD_MLP_NPARAMS, (long long)nparams);
+    if (k_fallback < 1 || k_fallback > 20) return fail(h, EQD_EINVAL, "k_fallback %d outside 1..20", k_fallback);
+    if (!(gate >= 0.0f) || !std::isfinite(gate)) return fail(h, EQD_EINVAL, "gate must be finite and >= 0");
+    if (h->s.friclaw < 3) return fail(h, EQD_EINVAL, "the learned guess needs rate-and-state friction (friclaw 3, 4)");
+    for (int64_t k = 0; k < nparams; k++)
+        if (!std::isfinite(params[k])) return fail(h, EQD_EINVAL, "parameter %lld is not finite", (long long)k);
+    if (params[kOffNorm + 2 * kMlpIn + 1] == 0.0f) return fail(h, EQD_EINVAL, "out_scale is zero");
+    HIPCHK(h, hipSetDevice(h->s.device));
+    int rc;
+    if (!h->mlp_p) {
+        if ((rc = dalloc(h, &h->mlp_p, (int64_t)kMlpNParams))) return rc;
+        if ((rc = dalloc(h, &h->mlp_wb, (int64_t)(64 * 16 + 4 * 4096)))) return rc;
+        if ((rc = dalloc(h, &h->mlp_w4r, (int64_t)64))) return rc;
+    }
+    HIPCHK(h, hipMemcpyAsync(h->mlp_p, params, sizeof(float) * kMlpNParams, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, launch_mlp_prepare(h->mlp_p, h->mlp_wb, h->mlp_w4r, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    MlpNet &M = h->F.net;
+    M.w1 = h->mlp_wb;
+    M.w2 = h->mlp_wb + 64 * 16;
+    M.w3 = M.w2 + 4096;
+    M.w2t = M.w3 + 4096;
+    M.w3t = M.w2t + 4096;
+    M.b1 = h->mlp_p + kOffB1;
+    M.b2 = h->mlp_p + kOffB2;
+    M.b3 = h->mlp_p + kOffB3;
+    M.w4r = h->mlp_w4r;
+    M.b4 = h->mlp_p + kOffB4;
+    M.norm = h->mlp_p + kOffNorm;
+    h->F.k_fallback = k_fallback;
+    h->F.gate = gate;
+    h->learn = true;
+    return EQD_OK;
+}
+
+int eqd_harvest(eqd_handle *h, int32_t every, int64_t capacity)
+{
+    if (!h) return EQD_EINVAL;
+    if (!h->committed) return fail(h, EQD_ESTATE, "eqd_harvest before eqd_commit");
+    if (capacity < 0 || (capacity > 0 && every < 1)) return fail(h, EQD_EINVAL, "bad harvest every/capacity");
+    HIPCHK(h, hipSetDevice(h->s.device));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    if (h->harvest) {
+        (void)hipFree(h->harvest);
+        h->harvest = nullptr;
+    }
+    h->harvest_cap = 0;
+    h->harvest_rows = 0;
+    h->harvest_every = 0;
+    if (capacity == 0) return EQD_OK;
+    if (hipMalloc(&h->harvest, sizeof(float) * 7 * (size_t)capacity) != hipSuccess) {
+        h->harvest = nullptr;
+        return fail(h, EQD_EOOM, "harvest buffer of %lld rows", (long long)capacity);
+    }
+    h->harvest_cap = capacity;
+    h->harvest_every = every;
+    return EQD_OK;
+}
+
+int eqd_harvest_get(eqd_handle *h, float *out, int64_t max_rows, int64_t *rows)
+{
+    if (!h || !rows || max_rows < 0 || (max_rows > 0 && !out)) return EQD_EINVAL;
+    if (!h->committed) return fail(h, EQD_ESTATE, "eqd_harvest_get before eqd_commit");
+    HIPCHK(h, hipSetDevice(h->s.device));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    *rows = h->harvest_rows;
+    const int64_t n = std::min(max_rows, h->harvest_rows);
+    if (n > 0) HIPCHK(h, hipMemcpy(out, h->harvest, sizeof(float) * 7 * n, hipMemcpyDeviceToHost));
+    return EQD_OK;
+}
+
+int eqd_newton_stats(eqd_handle *h, int64_t stats[5])
+{
+    if (!h || !stats) return EQD_EINVAL;
+    if (!h->committed) return fail(h, EQD_ESTATE, "eqd_newton_stats before eqd_commit");
+    HIPCHK(h, hipSetDevice(h->s.device));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    for (int k = 0; k < 5; k++) stats[k] = 0;
+    const int64_t nf = h->F.nf, ld = h->F.ld;
+    if (nf == 0) return EQD_OK;
+    std::vector<int32_t> v;
+    int rc;
+    if ((rc = d2h(h, v, (const int32_t *)h->F.nstat, 5 * ld))) return rc;
+    for (int k = 0; k < 5; k++)
+        for (int64_t i = 0; i < nf; i++) stats[k] += v[k * ld + i];
+    return EQD_OK;
+}
+
 int eqd_layout_info(eqd_handle *h, const char *key, int64_t *value)
 {
     if (!h || !key || !value) return EQD_EINVAL;
@@ -1152,6 +1276,7 @@ int eqd_destroy(eqd_handle *h)
     (void)hipSetDevice(h->s.device);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     for (void *p : h->allocs) (void)hipFree(p);
+    if (h->harvest) (void)hipFree(h->harvest);
     for (auto &kv : h->timers)
         for (auto &e : kv.second.ev) {
             (void)hipEventDestroy(e.first);

@@ -13,6 +13,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "eqd_mlp.h"
+
 namespace eqd {
 
 // CSR entry: bit 31 = PML element, bits 28..30 = local node a, bits 0..27 =
@@ -97,12 +99,23 @@ struct PmlNodes {
 
 struct FaultState {
     int64_t nf;
+    int64_t ld;             // row stride of fric and nstat: nf rounded up to the fault kernel's block
+    int32_t *nstat;         // [5][ld] Newton counters: solves, iterations, fallbacks, unconverged,
+                            // stopped at the roundoff floor
+    // learned Newton guess (when the kernel is launched in learned mode)
+    MlpNet net;
+    int32_t k_fallback;     // 0: the reference's solve; > 0: the faster mode's iteration policy
+    float gate;             // predicted |ln(V/V_prev)| below which the plain guess is kept
+    // harvest of (inputs, converged ln V): rows [harvest_cap][7]; this step's
+    // rows start at harvest_base (no harvest when harvest is null)
+    float *harvest;
+    int64_t harvest_base, harvest_cap;
     const int32_t *nsmp;    // [2][nf] slave, master (0-based node ids)
     const double *un, *us, *ud; // [3][nf]
     const double *arn;      // [nf]
     const double *fms, *fmm; // [nf] fnms of slave and master
     const double *xyz;      // [3][nf] slave coordinates
-    double *fric;           // [100][nf]
+    double *fric;           // [100][ld]
     double *fnft;           // [nf]
     const int32_t *station; // [nf] on-fault station (0-based) or -1
     double *hist;           // [non][nstep][12]

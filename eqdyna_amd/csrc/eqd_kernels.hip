@@ -27,73 +27,102 @@ __device__ __forceinline__ R sqr(R x) { return x * x; }
 //   src/calcElemMass.f90:1-19, src/hrglss.f90:13-98
 // kQ: C_Q == 1; kDP: C_elastic == 0; kHG: C_hg (1 stiffness, 2 viscous);
 // kMass: inertia/gravity term present (rdampm != 0 or plastic).
+//
+// Streaming form: one pass over the eight nodes gathers each node's v and u
+// once and accumulates everything that needs them (strain rate, strain, the
+// four hourglass mode amplitudes), so no per-node array stays live; the
+// stress update follows; a second pass over the nodes forms and stores each
+// node's force.  Every accumulation keeps the reference's order (nodes in
+// ascending order, modes in ascending order), and the register footprint is
+// a few dozen values instead of four 24-vectors.
+// Node-loop unrolling of the regular element kernel: 1 keeps the register
+// footprint (and so occupancy) best; larger values trade it for more loads in
+// flight per wave.
+#ifndef EQD_ELEM_UNROLL
+#define EQD_ELEM_UNROLL 1
+#endif
+
 template <typename R, bool kQ, bool kDP, int kHG, bool kMass>
 __global__ void __launch_bounds__(256) elem_regular_kernel(RegElems<R> E, NodeState N, StepConsts C)
 {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= E.n) return;
     const int64_t n = E.n, nn = N.nn;
-    int nd[8];
-#pragma unroll
-    for (int a = 0; a < 8; a++) nd[a] = E.ien[a * n + t];
-    R vl[24], dl[24];
-    if constexpr (sizeof(R) == sizeof(double)) {
-#pragma unroll
-        for (int a = 0; a < 8; a++) {
-#pragma unroll
-            for (int k = 0; k < 3; k++) {
-                vl[a * 3 + k] = N.vel[k * nn + nd[a]];
-                dl[a * 3 + k] = N.disp[k * nn + nd[a]];
-            }
-        }
-    } else {
-        // FP32 bulk: round velocities and displacements relative to the first
-        // node (formed in FP64).  Strain, strain rate and hourglass modes are
-        // blind to a common translation (sum_a dN_a/dx = 0, Phi orthogonal to
-        // constants), so only the rounding changes, and it is now relative to
-        // the deformation rather than to the absolute motion.
-        double v0[3], u0[3];
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-            v0[k] = N.vel[k * nn + nd[0]];
-            u0[k] = N.disp[k * nn + nd[0]];
-        }
-#pragma unroll
-        for (int a = 0; a < 8; a++) {
-#pragma unroll
-            for (int k = 0; k < 3; k++) {
-                vl[a * 3 + k] = (R)(N.vel[k * nn + nd[a]] - v0[k]);
-                dl[a * 3 + k] = (R)(N.disp[k * nn + nd[a]] - u0[k]);
-            }
-        }
-    }
+    constexpr bool kF32 = sizeof(R) != sizeof(double);
     // operators: one class-table row, or the element's own arrays
     const R *op = E.optab ? E.optab + (int64_t)E.opc[t] * kOpW : nullptr;
-    R shp[24];
+    const R *pht = E.phitab ? E.phitab + (int64_t)E.phic[t] * 32 : nullptr;
+    auto SHP = [&](int i) -> R { return op ? op[kOpShp + i] : E.shp[i * n + t]; };
+    auto PHI = [&](int m, int a) -> R { return pht ? pht[m * 8 + a] : E.phi[(m * 8 + a) * n + t]; };
+    const R rdk = (R)C.rdampk;
+    // FP32 bulk: velocities and displacements relative to the first node
+    // (formed in FP64).  Strain, strain rate and the hourglass modes are
+    // blind to a common translation (sum_a dN_a/dx = 0, Phi orthogonal to
+    // constants), so only the rounding changes, and it is now relative to the
+    // deformation rather than to the absolute motion.
+    double v0[3] = {0, 0, 0}, u0[3] = {0, 0, 0};
+    const int nd0 = E.ien[t];
+    if constexpr (kF32) {
 #pragma unroll
-    for (int i = 0; i < 24; i++) shp[i] = op ? op[kOpShp + i] : E.shp[i * n + t];
-    const R lam = op ? op[kOpLam] : E.lam[t], miu = op ? op[kOpMu] : E.mu[t];
-    const R det = op ? op[kOpDet] : E.det[t];
-    R sr[6] = {0, 0, 0, 0, 0, 0}, strain[6] = {0, 0, 0, 0, 0, 0};
-#pragma unroll
-    for (int a = 0; a < 8; a++) {
-        const R n1 = shp[a * 3], n2 = shp[a * 3 + 1], n3 = shp[a * 3 + 2];
-        const int j1 = a * 3, j2 = j1 + 1, j3 = j1 + 2;
-        sr[0] = sr[0] + n1 * vl[j1];
-        sr[1] = sr[1] + n2 * vl[j2];
-        sr[2] = sr[2] + n3 * vl[j3];
-        sr[3] = sr[3] + n3 * vl[j2] + n2 * vl[j3];
-        sr[4] = sr[4] + n3 * vl[j1] + n1 * vl[j3];
-        sr[5] = sr[5] + n2 * vl[j1] + n1 * vl[j2];
-        if (kQ) {
-            strain[0] = strain[0] + n1 * dl[j1];
-            strain[1] = strain[1] + n2 * dl[j2];
-            strain[2] = strain[2] + n3 * dl[j3];
-            strain[3] = strain[3] + n3 * dl[j2] + n2 * dl[j3];
-            strain[4] = strain[4] + n3 * dl[j1] + n1 * dl[j3];
-            strain[5] = strain[5] + n2 * dl[j1] + n1 * dl[j2];
+        for (int k = 0; k < 3; k++) {
+            v0[k] = N.vel[k * nn + nd0];
+            u0[k] = N.disp[k * nn + nd0];
         }
     }
+    // pass 1: strain rate, strain (Q), hourglass mode amplitudes
+    R sr[6] = {0, 0, 0, 0, 0, 0}, strain[6] = {0, 0, 0, 0, 0, 0};
+    R pd[4][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}};  // kHG 1: sum_a Phi_ma (u + rdampk v)_a
+    R qv[3][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0, 0}};    // kHG 2: sum_a v_a Gamma_ja
+    constexpr int fi[4][8] = {{1, 1, -1, -1, -1, -1, 1, 1}, {1, -1, -1, 1, -1, 1, 1, -1},
+                              {1, -1, 1, -1, 1, -1, 1, -1}, {1, -1, 1, -1, -1, 1, -1, 1}};
+#pragma unroll EQD_ELEM_UNROLL
+    for (int a = 0; a < 8; a++) {
+        const int nd = a == 0 ? nd0 : E.ien[a * n + t];
+        R vl[3], dl[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            if constexpr (kF32) {
+                vl[k] = (R)(N.vel[k * nn + nd] - v0[k]);
+                dl[k] = (R)(N.disp[k * nn + nd] - u0[k]);
+            } else {
+                vl[k] = N.vel[k * nn + nd];
+                dl[k] = N.disp[k * nn + nd];
+            }
+        }
+        const R n1 = SHP(a * 3), n2 = SHP(a * 3 + 1), n3 = SHP(a * 3 + 2);
+        sr[0] = sr[0] + n1 * vl[0];
+        sr[1] = sr[1] + n2 * vl[1];
+        sr[2] = sr[2] + n3 * vl[2];
+        sr[3] = sr[3] + n3 * vl[1] + n2 * vl[2];
+        sr[4] = sr[4] + n3 * vl[0] + n1 * vl[2];
+        sr[5] = sr[5] + n2 * vl[0] + n1 * vl[1];
+        if (kQ) {
+            strain[0] = strain[0] + n1 * dl[0];
+            strain[1] = strain[1] + n2 * dl[1];
+            strain[2] = strain[2] + n3 * dl[2];
+            strain[3] = strain[3] + n3 * dl[1] + n2 * dl[2];
+            strain[4] = strain[4] + n3 * dl[0] + n1 * dl[2];
+            strain[5] = strain[5] + n2 * dl[0] + n1 * dl[1];
+        }
+        if (kHG == 1) {
+            R ut[3];
+#pragma unroll
+            for (int k = 0; k < 3; k++) ut[k] = dl[k] + rdk * vl[k];
+#pragma unroll
+            for (int m = 0; m < 4; m++) {
+                const R ph = PHI(m, a);
+#pragma unroll
+                for (int k = 0; k < 3; k++) pd[m][k] = pd[m][k] + ph * ut[k];
+            }
+        } else if (kHG == 2) {
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+#pragma unroll
+                for (int j = 0; j < 4; j++) qv[i][j] = qv[i][j] + vl[i] * (R)fi[j][a];
+        }
+    }
+    const R lam = op ? op[kOpLam] : E.lam[t], miu = op ? op[kOpMu] : E.mu[t];
+    const R det = op ? op[kOpDet] : E.det[t];
     const R cdiag = lam + (R)2 * miu;
     R srt[6];
 #pragma unroll
@@ -172,99 +201,75 @@ __global__ void __launch_bounds__(256) elem_regular_kernel(RegElems<R> E, NodeSt
 #pragma unroll
     for (int i = 0; i < 6; i++) E.stress[i * n + t] = stress[i];
 
-    // f = -m*al (calcElemMass), then + (-det*w) * B^T (sigma + rdampk*sigma_dot)
-    R f[24];
-    if (kMass) {
-#pragma unroll
-        for (int a = 0; a < 8; a++) {
-#pragma unroll
-            for (int k = 0; k < 3; k++) {
-                // absolute velocity here: the Rayleigh mass term is not translation invariant
-                R al = (R)(C.rdampm * N.vel[k * nn + nd[a]]);
-                if (k == 2) al = al + (R)C.gterm;
-                f[a * 3 + k] = (R)0 - al * E.emass[(a * 3 + k) * n + t];
-            }
-        }
-    } else {
-#pragma unroll
-        for (int i = 0; i < 24; i++) f[i] = 0;
-    }
+    // f = -m*al (calcElemMass), + (-det*w) * B^T (sigma + rdampk*sigma_dot), - hourglass
     const R temp = -det * (R)C.w;
     R st[6];
 #pragma unroll
-    for (int i = 0; i < 6; i++) st[i] = temp * (stress[i] + (R)C.rdampk * srt[i]);
-#pragma unroll
-    for (int a = 0; a < 8; a++) {
-        const R n1 = shp[a * 3], n2 = shp[a * 3 + 1], n3 = shp[a * 3 + 2];
-        f[a * 3 + 0] = f[a * 3 + 0] + (n1 * st[0] + n3 * st[4] + n2 * st[5]);
-        f[a * 3 + 1] = f[a * 3 + 1] + (n2 * st[1] + n3 * st[3] + n1 * st[5]);
-        f[a * 3 + 2] = f[a * 3 + 2] + (n3 * st[2] + n2 * st[3] + n1 * st[4]);
-    }
-
+    for (int i = 0; i < 6; i++) st[i] = temp * (stress[i] + rdk * srt[i]);
+    R g[4][3];  // kHG 1: SS * mode amplitude
+    R coef = 0;
     if (kHG == 1) {
-        // Kosloff-Frazier stiffness hourglass, src/hrglss.f90:20-55
         R ss[6];
 #pragma unroll
         for (int i = 0; i < 6; i++) ss[i] = op ? op[kOpSs + i] : E.ss[i * n + t];
-        const R *pht = E.phitab ? E.phitab + (int64_t)E.phic[t] * 32 : nullptr;
-#pragma unroll
-        for (int i = 0; i < 24; i++) dl[i] = dl[i] + (R)C.rdampk * vl[i];
 #pragma unroll
         for (int m = 0; m < 4; m++) {
-            R ph[8];
-#pragma unroll
-            for (int a = 0; a < 8; a++) ph[a] = pht ? pht[m * 8 + a] : E.phi[(m * 8 + a) * n + t];
-            R pd0 = 0, pd1 = 0, pd2 = 0;
-#pragma unroll
-            for (int a = 0; a < 8; a++) {
-                pd0 = pd0 + ph[a] * dl[a * 3 + 0];
-                pd1 = pd1 + ph[a] * dl[a * 3 + 1];
-                pd2 = pd2 + ph[a] * dl[a * 3 + 2];
-            }
-            const R g0 = ss[0] * pd0 + ss[1] * pd1 + ss[2] * pd2;
-            const R g1 = ss[1] * pd0 + ss[3] * pd1 + ss[4] * pd2;
-            const R g2 = ss[2] * pd0 + ss[4] * pd1 + ss[5] * pd2;
-#pragma unroll
-            for (int a = 0; a < 8; a++) {
-                f[a * 3 + 0] = f[a * 3 + 0] - ph[a] * g0;
-                f[a * 3 + 1] = f[a * 3 + 1] - ph[a] * g1;
-                f[a * 3 + 2] = f[a * 3 + 2] - ph[a] * g2;
-            }
+            g[m][0] = ss[0] * pd[m][0] + ss[1] * pd[m][1] + ss[2] * pd[m][2];
+            g[m][1] = ss[1] * pd[m][0] + ss[3] * pd[m][1] + ss[4] * pd[m][2];
+            g[m][2] = ss[2] * pd[m][0] + ss[4] * pd[m][1] + ss[5] * pd[m][2];
         }
     } else if (kHG == 2) {
-        // viscous hourglass, src/hrglss.f90:57-97
-        constexpr int fi[4][8] = {{1, 1, -1, -1, -1, -1, 1, 1}, {1, -1, -1, 1, -1, 1, 1, -1},
-                                  {1, -1, 1, -1, 1, -1, 1, -1}, {1, -1, 1, -1, -1, 1, -1, 1}};
         const R rho = op ? op[kOpRho] : E.rho[t], vp = op ? op[kOpVp] : E.vp[t];
-        const R coef = (R)0.25 * (R)C.kapa_hg * rho * vp * (R)pow((double)det * C.w, 2.0 / 3.0);
-        R q[3][4];
+        coef = (R)0.25 * (R)C.kapa_hg * rho * vp * (R)pow((double)det * C.w, 2.0 / 3.0);
+    }
+    // pass 2: one node's force at a time
+#pragma unroll EQD_ELEM_UNROLL
+    for (int a = 0; a < 8; a++) {
+        R f[3] = {0, 0, 0};
+        if (kMass) {
+            const int nd = E.ien[a * n + t];
 #pragma unroll
-        for (int i = 0; i < 3; i++)
-#pragma unroll
-            for (int j = 0; j < 4; j++) {
-                R acc = 0;
-#pragma unroll
-                for (int k = 0; k < 8; k++) acc = acc + vl[k * 3 + i] * (R)fi[j][k];
-                q[i][j] = acc;
+            for (int k = 0; k < 3; k++) {
+                // absolute velocity here: the Rayleigh mass term is not translation invariant
+                R al = (R)(C.rdampm * N.vel[k * nn + nd]);
+                if (k == 2) al = al + (R)C.gterm;
+                f[k] = (R)0 - al * E.emass[(a * 3 + k) * n + t];
             }
+        }
+        const R n1 = SHP(a * 3), n2 = SHP(a * 3 + 1), n3 = SHP(a * 3 + 2);
+        f[0] = f[0] + (n1 * st[0] + n3 * st[4] + n2 * st[5]);
+        f[1] = f[1] + (n2 * st[1] + n3 * st[3] + n1 * st[5]);
+        f[2] = f[2] + (n3 * st[2] + n2 * st[3] + n1 * st[4]);
+        if (kHG == 1) {
+            // Kosloff-Frazier stiffness hourglass, src/hrglss.f90:20-55
 #pragma unroll
-        for (int k = 0; k < 8; k++)
+            for (int m = 0; m < 4; m++) {
+                const R ph = PHI(m, a);
+#pragma unroll
+                for (int k = 0; k < 3; k++) f[k] = f[k] - ph * g[m][k];
+            }
+        } else if (kHG == 2) {
+            // viscous hourglass, src/hrglss.f90:57-97
 #pragma unroll
             for (int i = 0; i < 3; i++) {
                 R fv = 0;
 #pragma unroll
-                for (int j = 0; j < 4; j++) fv = fv - coef * q[i][j] * (R)fi[j][k];
-                f[k * 3 + i] = f[k * 3 + i] + fv;
+                for (int j = 0; j < 4; j++) fv = fv - coef * qv[i][j] * (R)fi[j][a];
+                f[i] = f[i] + fv;
             }
-    }
+        }
 #pragma unroll
-    for (int i = 0; i < 24; i++) E.ef[i * n + t] = f[i];
+        for (int k = 0; k < 3; k++) E.ef[(a * 3 + k) * n + t] = f[k];
+    }
 }
 
 // ------------------------------------------------------------------------
 // PML elements: calcPMLElemKU + calcElemMass + hrglss, one element per thread.
 //   src/assembleGlobalKU.f90:36-64, 70-346; src/hrglss.f90
-// The hourglass force is formed first so that each force slot is written once.
+// Streaming like elem_regular_kernel: pass 1 gathers each node once and
+// accumulates the velocity gradients and hourglass mode amplitudes; pass 2
+// forms each node's 12 (or collapsed 3) force slots, hourglass included, and
+// writes every slot once.
 template <typename R, int kHG, bool kMass>
 __global__ void __launch_bounds__(256) elem_pml_kernel(PmlElems<R> E, NodeState N, StepConsts C)
 {
@@ -272,50 +277,71 @@ __global__ void __launch_bounds__(256) elem_pml_kernel(PmlElems<R> E, NodeState 
     if (p >= E.n) return;
     const int64_t n = E.n, nn = N.nn;
     constexpr bool kF64 = sizeof(R) == sizeof(double);
-    int nd[8];
-#pragma unroll
-    for (int a = 0; a < 8; a++) nd[a] = E.ien[a * n + p];
+    const R *op = E.optab ? E.optab + (int64_t)E.opc[p] * kOpWPml : nullptr;
+    const R *pht = E.phitab ? E.phitab + (int64_t)E.phic[p] * 32 : nullptr;
+    auto SHP = [&](int i) -> R { return op ? op[kOpShp + i] : E.shp[i * n + p]; };
+    auto PHI = [&](int m, int a) -> R { return pht ? pht[m * 8 + a] : E.phi[(m * 8 + a) * n + p]; };
     // velocities (relative to node 1 in the FP32 variant, see elem_regular_kernel)
-    R va[24];
     double v0[3] = {0.0, 0.0, 0.0}, u0[3] = {0.0, 0.0, 0.0};
+    const int nd0 = E.ien[p];
     if (!kF64) {
 #pragma unroll
         for (int k = 0; k < 3; k++) {
-            v0[k] = N.vel[k * nn + nd[0]];
-            u0[k] = N.disp[k * nn + nd[0]];
+            v0[k] = N.vel[k * nn + nd0];
+            u0[k] = N.disp[k * nn + nd0];
         }
     }
-#pragma unroll
-    for (int a = 0; a < 8; a++)
-#pragma unroll
-        for (int k = 0; k < 3; k++) va[a * 3 + k] = (R)(N.vel[k * nn + nd[a]] - v0[k]);
-    const R *op = E.optab ? E.optab + (int64_t)E.opc[p] * kOpWPml : nullptr;
-    R shp[24];
-#pragma unroll
-    for (int i = 0; i < 24; i++) shp[i] = op ? op[kOpShp + i] : E.shp[i * n + p];
-    const R lam = op ? op[kOpLam] : E.lam[p], miu = op ? op[kOpMu] : E.mu[p], det = op ? op[kOpDet] : E.det[p];
+    constexpr int fi[4][8] = {{1, 1, -1, -1, -1, -1, 1, 1}, {1, -1, -1, 1, -1, 1, 1, -1},
+                              {1, -1, 1, -1, 1, -1, 1, -1}, {1, -1, 1, -1, -1, 1, -1, 1}};
     R sr[6] = {0, 0, 0, 0, 0, 0};
     R Dx_vx = 0, Dy_vy = 0, Dz_vz = 0, Dx_vy = 0, Dy_vx = 0, Dx_vz = 0, Dz_vx = 0, Dy_vz = 0, Dz_vy = 0;
-#pragma unroll
+    R pd[4][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    R qv[3][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0, 0}};
+#pragma unroll EQD_ELEM_UNROLL
     for (int a = 0; a < 8; a++) {
-        const R n1 = shp[a * 3], n2 = shp[a * 3 + 1], n3 = shp[a * 3 + 2];
-        const int j1 = a * 3, j2 = j1 + 1, j3 = j1 + 2;
-        sr[0] = sr[0] + n1 * va[j1];
-        sr[1] = sr[1] + n2 * va[j2];
-        sr[2] = sr[2] + n3 * va[j3];
-        sr[3] = sr[3] + n3 * va[j2] + n2 * va[j3];
-        sr[4] = sr[4] + n3 * va[j1] + n1 * va[j3];
-        sr[5] = sr[5] + n2 * va[j1] + n1 * va[j2];
-        Dx_vx = Dx_vx + n1 * va[j1];
-        Dy_vy = Dy_vy + n2 * va[j2];
-        Dz_vz = Dz_vz + n3 * va[j3];
-        Dx_vy = Dx_vy + n1 * va[j2];
-        Dy_vx = Dy_vx + n2 * va[j1];
-        Dx_vz = Dx_vz + n1 * va[j3];
-        Dz_vx = Dz_vx + n3 * va[j1];
-        Dy_vz = Dy_vz + n2 * va[j3];
-        Dz_vy = Dz_vy + n3 * va[j2];
+        const int nd = a == 0 ? nd0 : E.ien[a * n + p];
+        R va[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) va[k] = (R)(N.vel[k * nn + nd] - v0[k]);
+        const R n1 = SHP(a * 3), n2 = SHP(a * 3 + 1), n3 = SHP(a * 3 + 2);
+        sr[0] = sr[0] + n1 * va[0];
+        sr[1] = sr[1] + n2 * va[1];
+        sr[2] = sr[2] + n3 * va[2];
+        sr[3] = sr[3] + n3 * va[1] + n2 * va[2];
+        sr[4] = sr[4] + n3 * va[0] + n1 * va[2];
+        sr[5] = sr[5] + n2 * va[0] + n1 * va[1];
+        Dx_vx = Dx_vx + n1 * va[0];
+        Dy_vy = Dy_vy + n2 * va[1];
+        Dz_vz = Dz_vz + n3 * va[2];
+        Dx_vy = Dx_vy + n1 * va[1];
+        Dy_vx = Dy_vx + n2 * va[0];
+        Dx_vz = Dx_vz + n1 * va[2];
+        Dz_vx = Dz_vx + n3 * va[0];
+        Dy_vz = Dy_vz + n2 * va[2];
+        Dz_vy = Dz_vy + n3 * va[1];
+        if (kHG == 1) {
+            R dlh[3];
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                if (kF64)
+                    dlh[k] = (R)(N.disp[k * nn + nd] + C.rdampk * (double)va[k]);
+                else
+                    dlh[k] = (R)((N.disp[k * nn + nd] - u0[k]) + C.rdampk * (N.vel[k * nn + nd] - v0[k]));
+            }
+#pragma unroll
+            for (int m = 0; m < 4; m++) {
+                const R ph = PHI(m, a);
+#pragma unroll
+                for (int k = 0; k < 3; k++) pd[m][k] = pd[m][k] + ph * dlh[k];
+            }
+        } else if (kHG == 2) {
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+#pragma unroll
+                for (int j = 0; j < 4; j++) qv[i][j] = qv[i][j] + va[i] * (R)fi[j][a];
+        }
     }
+    const R lam = op ? op[kOpLam] : E.lam[p], miu = op ? op[kOpMu] : E.mu[p], det = op ? op[kOpDet] : E.det[p];
     const R cdiag = lam + (R)2.0 * miu;
     R srt[6];
 #pragma unroll
@@ -357,114 +383,82 @@ __global__ void __launch_bounds__(256) elem_pml_kernel(PmlElems<R> E, NodeState 
     for (int k = 0; k < 6; k++) s0[k] = E.stress[(15 + k) * n + p] + (R)C.rdampk * srt[k];
     const R dw = det * (R)C.w;
     const unsigned mask = E.mask12[p];
-
-    // hourglass force on each local node, subtracted mode by mode
-    R hg[8][3];
-#pragma unroll
-    for (int a = 0; a < 8; a++) hg[a][0] = hg[a][1] = hg[a][2] = 0;
+    R g[4][3];
+    R coef2 = 0;
     if (kHG == 1) {
-        R dlh[24];
-#pragma unroll
-        for (int a = 0; a < 8; a++)
-#pragma unroll
-            for (int k = 0; k < 3; k++) {
-                if (kF64)
-                    dlh[a * 3 + k] = (R)(N.disp[k * nn + nd[a]] + C.rdampk * (double)va[a * 3 + k]);
-                else
-                    dlh[a * 3 + k] = (R)((N.disp[k * nn + nd[a]] - u0[k]) + C.rdampk * (N.vel[k * nn + nd[a]] - v0[k]));
-            }
         R ss[6];
 #pragma unroll
         for (int i = 0; i < 6; i++) ss[i] = op ? op[kOpSs + i] : E.ss[i * n + p];
-        const R *pht = E.phitab ? E.phitab + (int64_t)E.phic[p] * 32 : nullptr;
 #pragma unroll
         for (int m = 0; m < 4; m++) {
-            R ph[8];
-#pragma unroll
-            for (int a = 0; a < 8; a++) ph[a] = pht ? pht[m * 8 + a] : E.phi[(m * 8 + a) * n + p];
-            R pd0 = 0, pd1 = 0, pd2 = 0;
-#pragma unroll
-            for (int a = 0; a < 8; a++) {
-                pd0 = pd0 + ph[a] * dlh[a * 3 + 0];
-                pd1 = pd1 + ph[a] * dlh[a * 3 + 1];
-                pd2 = pd2 + ph[a] * dlh[a * 3 + 2];
-            }
-            const R q0 = ss[0] * pd0 + ss[1] * pd1 + ss[2] * pd2;
-            const R q1 = ss[1] * pd0 + ss[3] * pd1 + ss[4] * pd2;
-            const R q2 = ss[2] * pd0 + ss[4] * pd1 + ss[5] * pd2;
-#pragma unroll
-            for (int a = 0; a < 8; a++) {
-                hg[a][0] = hg[a][0] - ph[a] * q0;
-                hg[a][1] = hg[a][1] - ph[a] * q1;
-                hg[a][2] = hg[a][2] - ph[a] * q2;
-            }
+            g[m][0] = ss[0] * pd[m][0] + ss[1] * pd[m][1] + ss[2] * pd[m][2];
+            g[m][1] = ss[1] * pd[m][0] + ss[3] * pd[m][1] + ss[4] * pd[m][2];
+            g[m][2] = ss[2] * pd[m][0] + ss[4] * pd[m][1] + ss[5] * pd[m][2];
         }
     } else if (kHG == 2) {
-        constexpr int fi[4][8] = {{1, 1, -1, -1, -1, -1, 1, 1}, {1, -1, -1, 1, -1, 1, 1, -1},
-                                  {1, -1, 1, -1, 1, -1, 1, -1}, {1, -1, 1, -1, -1, 1, -1, 1}};
         const R rho = op ? op[kOpRho] : E.rho[p], vp = op ? op[kOpVp] : E.vp[p];
-        const R coef2 = (R)0.25 * (R)C.kapa_hg * rho * vp * (R)pow((double)det * C.w, 2.0 / 3.0);
-        R q[3][4];
+        coef2 = (R)0.25 * (R)C.kapa_hg * rho * vp * (R)pow((double)det * C.w, 2.0 / 3.0);
+    }
+
+#pragma unroll EQD_ELEM_UNROLL
+    for (int a = 0; a < 8; a++) {
+        // hourglass force on this node, subtracted mode by mode
+        R hg[3] = {0, 0, 0};
+        if (kHG == 1) {
 #pragma unroll
-        for (int i = 0; i < 3; i++)
+            for (int m = 0; m < 4; m++) {
+                const R ph = PHI(m, a);
 #pragma unroll
-            for (int j = 0; j < 4; j++) {
-                R acc = 0;
-#pragma unroll
-                for (int k = 0; k < 8; k++) acc = acc + va[k * 3 + i] * (R)fi[j][k];
-                q[i][j] = acc;
+                for (int k = 0; k < 3; k++) hg[k] = hg[k] - ph * g[m][k];
             }
-#pragma unroll
-        for (int k = 0; k < 8; k++)
+        } else if (kHG == 2) {
 #pragma unroll
             for (int i = 0; i < 3; i++) {
                 R fv = 0;
 #pragma unroll
-                for (int j = 0; j < 4; j++) fv = fv - coef2 * q[i][j] * (R)fi[j][k];
-                hg[k][i] = fv;
+                for (int j = 0; j < 4; j++) fv = fv - coef2 * qv[i][j] * (R)fi[j][a];
+                hg[i] = fv;
             }
-    }
-
-#pragma unroll
-    for (int a = 0; a < 8; a++) {
-        const R g1 = shp[a * 3], g2 = shp[a * 3 + 1], g3 = shp[a * 3 + 2];
-        R fi[12];
+        }
+        const R g1 = SHP(a * 3), g2 = SHP(a * 3 + 1), g3 = SHP(a * 3 + 2);
+        R f[12];
         if (kMass) {
+            const int nd = E.ien[a * n + p];
 #pragma unroll
-            for (int k = 0; k < 9; k++) fi[k] = 0;
+            for (int k = 0; k < 9; k++) f[k] = 0;
 #pragma unroll
             for (int k = 0; k < 3; k++) {
-                R al = (R)(C.rdampm * N.vel[k * nn + nd[a]]);
+                R al = (R)(C.rdampm * N.vel[k * nn + nd]);
                 if (k == 2) al = al + (R)C.gterm;
-                fi[9 + k] = (R)0 - al * E.emass[(a * 3 + k) * n + p];
+                f[9 + k] = (R)0 - al * E.emass[(a * 3 + k) * n + p];
             }
         } else {
 #pragma unroll
-            for (int k = 0; k < 12; k++) fi[k] = 0;
+            for (int k = 0; k < 12; k++) f[k] = 0;
         }
-        fi[0] = fi[0] - dw * g1 * sxx;
-        fi[1] = fi[1] - dw * g2 * sxy;
-        fi[2] = fi[2] - dw * g3 * sxz;
-        fi[3] = fi[3] - dw * g1 * sxy;
-        fi[4] = fi[4] - dw * g2 * syy;
-        fi[5] = fi[5] - dw * g3 * syz;
-        fi[6] = fi[6] - dw * g1 * sxz;
-        fi[7] = fi[7] - dw * g2 * syz;
-        fi[8] = fi[8] - dw * g3 * szz;
-        fi[9] = fi[9] - dw * (g1 * s0[0] + g3 * s0[4] + g2 * s0[5]);
-        fi[10] = fi[10] - dw * (g2 * s0[1] + g3 * s0[3] + g1 * s0[5]);
-        fi[11] = fi[11] - dw * (g3 * s0[2] + g2 * s0[3] + g1 * s0[4]);
+        f[0] = f[0] - dw * g1 * sxx;
+        f[1] = f[1] - dw * g2 * sxy;
+        f[2] = f[2] - dw * g3 * sxz;
+        f[3] = f[3] - dw * g1 * sxy;
+        f[4] = f[4] - dw * g2 * syy;
+        f[5] = f[5] - dw * g3 * syz;
+        f[6] = f[6] - dw * g1 * sxz;
+        f[7] = f[7] - dw * g2 * syz;
+        f[8] = f[8] - dw * g3 * szz;
+        f[9] = f[9] - dw * (g1 * s0[0] + g3 * s0[4] + g2 * s0[5]);
+        f[10] = f[10] - dw * (g2 * s0[1] + g3 * s0[3] + g1 * s0[5]);
+        f[11] = f[11] - dw * (g3 * s0[2] + g2 * s0[3] + g1 * s0[4]);
         if (mask & (1u << a)) {
             // a PML node takes the 9 split components and, in 10-12, the rest
 #pragma unroll
-            for (int k = 0; k < 9; k++) E.ef[(a * 12 + k) * n + p] = fi[k];
+            for (int k = 0; k < 9; k++) E.ef[(a * 12 + k) * n + p] = f[k];
 #pragma unroll
-            for (int k = 0; k < 3; k++) E.ef[(a * 12 + 9 + k) * n + p] = fi[9 + k] + hg[a][k];
+            for (int k = 0; k < 3; k++) E.ef[(a * 12 + 9 + k) * n + p] = f[9 + k] + hg[k];
         } else {
             // a 3-component node takes the collapsed sums (src/assembleGlobalKU.f90:55-61)
-            E.ef[(a * 12 + 0) * n + p] = fi[0] + fi[1] + fi[2] + fi[9] + hg[a][0];
-            E.ef[(a * 12 + 1) * n + p] = fi[3] + fi[4] + fi[5] + fi[10] + hg[a][1];
-            E.ef[(a * 12 + 2) * n + p] = fi[6] + fi[7] + fi[8] + fi[11] + hg[a][2];
+            E.ef[(a * 12 + 0) * n + p] = f[0] + f[1] + f[2] + f[9] + hg[0];
+            E.ef[(a * 12 + 1) * n + p] = f[3] + f[4] + f[5] + f[10] + hg[1];
+            E.ef[(a * 12 + 2) * n + p] = f[6] + f[7] + f[8] + f[11] + hg[2];
         }
     }
 }
@@ -693,16 +687,25 @@ __device__ __forceinline__ void slip_law(double V2, double &psi, const RsfParams
 // solveSWTW / solveRSF + NewtonRaphson, on-fault stations, rupture time
 // (src/faulting.f90:3-541), then F/m and (mode 0) the next step's update of
 // the pair's two nodes.
-template <typename R>
+//
+// kLearn: the rate-and-state Newton starts from the learned guess (one
+// wavefront evaluates the network for its 64 pairs, so every lane stays until
+// then: lanes past the last pair read pair nf-1 and write only their own
+// padding rows of fric, then leave).
+template <typename R, bool kLearn>
 __global__ void __launch_bounds__(128) fault_kernel(FaultState Fs, NodeState N, StepConsts C, const R *ef,
                                                     int64_t nreg, const R *efp, int64_t npe, int nt,
                                                     double time, int mode)
 {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= Fs.nf) return;
-    const int64_t nf = Fs.nf, nn = N.nn;
+    __shared__ __attribute__((aligned(16))) __bf16 mlp_lds0[kLearn ? 2 * 64 * kLd0 : 8];
+    __shared__ __attribute__((aligned(16))) __bf16 mlp_lds[kLearn ? 2 * 64 * kLd : 8];
+    const int64_t i0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (!kLearn && i0 >= Fs.nf) return;
+    const bool active = i0 < Fs.nf;
+    const int64_t i = active ? i0 : Fs.nf - 1;
+    const int64_t nf = Fs.nf, nn = N.nn, ld = Fs.ld;
     double *fr = Fs.fric;
-#define FR(k) fr[(int64_t)((k) - 1) * nf + i]
+#define FR(k) fr[(int64_t)((k) - 1) * ld + i0]
     const double dt = C.dt, Ce = (double)C.C_elastic;
     const int ns = Fs.nsmp[i], nm = Fs.nsmp[nf + i];
     const double un[3] = {Fs.un[i], Fs.un[nf + i], Fs.un[2 * nf + i]};
@@ -779,9 +782,9 @@ __global__ void __launch_bounds__(128) fault_kernel(FaultState Fs, NodeState N, 
         // solveSWTW, src/faulting.f90:136-188
         double mu = 0.0;
         if (C.friclaw == 1)
-            mu = slip_weak(FR(77), fr, nf, i);
+            mu = slip_weak(FR(77), fr, ld, i0);
         else
-            mu = time_weak(time - Fs.fnft[i], fr, nf, i);
+            mu = time_weak(time - Fs.fnft[i], fr, ld, i0);
         if (C.C_nuclea == 1 && C.nucfault == 1) {
             // swtwNucleation, src/faulting.f90:416-443
             const double radius = sqrt((sx - C.xsource) * (sx - C.xsource) + (sy - C.ysource) * (sy - C.ysource) +
@@ -857,32 +860,94 @@ __global__ void __launch_bounds__(128) fault_kernel(FaultState Fs, NodeState N, 
         for (int j = 1; j <= 2; j++) trial[j] = T[j] - taoc_old * 0.5 * (rate[j] / rate[3]) + FR(25 + j) / Tc;
         trial[3] = sqrt(trial[1] * trial[1] + trial[2] * trial[2]);
         double taoc_new = 0.0, stateTmp = 0.0, thetaPcTmp = 0.0;
-        for (int iv = 1; iv <= 20; iv++) {
-            double rsfeq, drsfeqdv;
-            stateTmp = statetmp;
-            if (C.friclaw == 3)
-                ageing_law(v_trial, stateTmp, P, dt, xmu, dxmudv);
-            else
-                slip_law(v_trial, stateTmp, P, dt, xmu, dxmudv);
-            if (C.friclaw < 5) {
-                thetaPcTmp = theta_pc_tmp;
-                theta_pc_dot = -(v_trial / P.L * (thetaPcTmp - fabs(T[0])));
-                thetaPcTmp = thetaPcTmp + theta_pc_dot * dt;
-                taoc_new = xmu * thetaPcTmp;
-                rsfeq = v_trial + Tc * (taoc_new * 0.5 - trial[3]);
-                drsfeqdv = 1.0 + Tc * (dxmudv * thetaPcTmp) * 0.5;
-            } else {
-                const double tn = T[0] < 0.0 ? T[0] : 0.0;
-                taoc_new = FR(4) - xmu * tn;
-                rsfeq = v_trial + Tc * (taoc_new * 0.5 - trial[3]);
-                drsfeqdv = 1.0 + Tc * (-dxmudv * tn) * 0.5;
+        // NewtonRaphson, src/faulting.f90:459-516, from v0 for at most maxit
+        // iterations; 1 when the convergence test was met, 0 if not; its =
+        // evaluations.  With roundoff_stop, also 2 when the residual is within
+        // 16 ulps of the size of its terms (no iterate can then do better).
+        auto newton = [&](double v0, int maxit, int &its, bool roundoff_stop) -> int {
+            v_trial = v0;
+            for (int iv = 1; iv <= maxit; iv++) {
+                its = iv;
+                double rsfeq, drsfeqdv;
+                stateTmp = statetmp;
+                if (C.friclaw == 3)
+                    ageing_law(v_trial, stateTmp, P, dt, xmu, dxmudv);
+                else
+                    slip_law(v_trial, stateTmp, P, dt, xmu, dxmudv);
+                if (C.friclaw < 5) {
+                    thetaPcTmp = theta_pc_tmp;
+                    theta_pc_dot = -(v_trial / P.L * (thetaPcTmp - fabs(T[0])));
+                    thetaPcTmp = thetaPcTmp + theta_pc_dot * dt;
+                    taoc_new = xmu * thetaPcTmp;
+                    rsfeq = v_trial + Tc * (taoc_new * 0.5 - trial[3]);
+                    drsfeqdv = 1.0 + Tc * (dxmudv * thetaPcTmp) * 0.5;
+                } else {
+                    const double tn = T[0] < 0.0 ? T[0] : 0.0;
+                    taoc_new = FR(4) - xmu * tn;
+                    rsfeq = v_trial + Tc * (taoc_new * 0.5 - trial[3]);
+                    drsfeqdv = 1.0 + Tc * (-dxmudv * tn) * 0.5;
+                }
+                if (fabs(rsfeq / drsfeqdv) < 1.0e-14 * fabs(v_trial) && fabs(rsfeq) < 1.0e-6 * fabs(v_trial))
+                    return 1;
+                if (roundoff_stop &&
+                    fabs(rsfeq) <= 16.0 * 2.220446049250313e-16 *
+                                       (fabs(v_trial) + Tc * (0.5 * fabs(taoc_new) + fabs(trial[3]))))
+                    return 2;
+                const double nv = v_trial - rsfeq / drsfeqdv;
+                if (nv <= 0.0)
+                    v_trial = v_trial / 2.0;
+                else
+                    v_trial = nv;
             }
-            if (fabs(rsfeq / drsfeqdv) < 1.0e-14 * fabs(v_trial) && fabs(rsfeq) < 1.0e-6 * fabs(v_trial)) break;
-            const double nv = v_trial - rsfeq / drsfeqdv;
-            if (nv <= 0.0)
-                v_trial = v_trial / 2.0;
-            else
-                v_trial = nv;
+            return 0;
+        };
+        // inputs of the learned guess (and of a harvested training row)
+        const double v_plain = v_trial;
+        float feat[kMlpIn];
+        feat[0] = (float)trial[3];
+        feat[1] = (float)T[0];
+        feat[2] = (float)(C.friclaw == 3 ? log(statetmp) : statetmp);
+        feat[3] = (float)P.A;
+        feat[4] = (float)P.B;
+        feat[5] = (float)log(v_plain > 1.0e-30 ? v_plain : 1.0e-30);
+        int its = 0, its2 = 0, fb = 0, res;
+        if (Fs.k_fallback > 0) {
+            // iteration policy of the faster mode: start from the proposal (the
+            // network's where it predicts a change, else the plain guess), stop
+            // at convergence or at the roundoff floor, and fall back to the
+            // reference's solve from the plain guess after k_fallback iterations
+            double v0 = v_plain;
+            if constexpr (kLearn) {
+                const int w = (int)(threadIdx.x >> 6);
+                const float y = mlp_wave_forward(Fs.net, feat, mlp_lds0 + w * 64 * kLd0, mlp_lds + w * 64 * kLd);
+                if (!active) return;
+                const double r = (double)(y * Fs.net.norm[2 * kMlpIn + 1] + Fs.net.norm[2 * kMlpIn]);
+                const double vp = v_plain > 1.0e-30 ? v_plain : 1.0e-30;
+                if (fabs(r) >= (double)Fs.gate && r < 60.0) v0 = vp * exp(r);
+            }
+            res = newton(v0, Fs.k_fallback, its, true);
+            if (res == 0) {
+                fb = 1;
+                res = newton(v_plain, 20, its2, false);
+            }
+        } else {
+            res = newton(v_plain, 20, its, false);
+        }
+        if constexpr (kLearn) {
+            if (!active) return;
+        }
+        Fs.nstat[i] = Fs.nstat[i] + 1;
+        Fs.nstat[ld + i] = Fs.nstat[ld + i] + its + its2;
+        Fs.nstat[2 * ld + i] = Fs.nstat[2 * ld + i] + fb;
+        Fs.nstat[3 * ld + i] = Fs.nstat[3 * ld + i] + (res == 0 ? 1 : 0);
+        Fs.nstat[4 * ld + i] = Fs.nstat[4 * ld + i] + (res == 2 ? 1 : 0);
+        if (Fs.harvest) {
+            const int64_t row = Fs.harvest_base + i;
+            if (row < Fs.harvest_cap) {
+                float *hr = Fs.harvest + row * 7;
+                for (int k = 0; k < kMlpIn; k++) hr[k] = feat[k];
+                hr[6] = (float)log(v_trial > 1.0e-30 ? v_trial : 1.0e-30);
+            }
         }
         if (C.TPV == 105 && v_trial < FR(46)) v_trial = FR(46);
         FR(20) = stateTmp;
@@ -908,6 +973,9 @@ __global__ void __launch_bounds__(128) fault_kernel(FaultState Fs, NodeState N, 
             FR(31 + j) = q[j][1][1] + (xyzAcc[j] + xyzR[j] / ms) * dt;
             FR(34 + j) = q[j][0][1] + (-xyzAcc[j] + xyzR[j] / mm) * dt;
         }
+    }
+    if constexpr (kLearn) {
+        if (!active) return;
     }
     // storeOnFaultStationQuantSCEC, src/faulting.f90:518-541
     const int sta = Fs.station[i];
@@ -1051,10 +1119,16 @@ hipError_t launch_update_from_acc(const NodeState &N, const int32_t *list3, int6
 
 template <typename R>
 hipError_t launch_fault(const FaultState &F, const NodeState &N, const StepConsts &C, const R *ef, int64_t nreg,
-                        const R *efp, int64_t npe, int nt, double time, int mode, hipStream_t s)
+                        const R *efp, int64_t npe, int nt, double time, int mode, bool learn, hipStream_t s)
 {
     if (F.nf == 0) return hipSuccess;
-    fault_kernel<R><<<grid_for(F.nf, 128), 128, 0, s>>>(F, N, C, ef, nreg, efp, npe, nt, time, mode);
+    // learned mode keeps every lane of the grid alive: fric and nstat rows
+    // must cover the whole grid
+    if (learn && F.ld < (int64_t)grid_for(F.nf, 128) * 128) return hipErrorInvalidValue;
+    if (learn)
+        fault_kernel<R, true><<<grid_for(F.nf, 128), 128, 0, s>>>(F, N, C, ef, nreg, efp, npe, nt, time, mode);
+    else
+        fault_kernel<R, false><<<grid_for(F.nf, 128), 128, 0, s>>>(F, N, C, ef, nreg, efp, npe, nt, time, mode);
     return hipGetLastError();
 }
 
@@ -1075,7 +1149,7 @@ hipError_t launch_station(const OffStations &S, const NodeState &N, int nt, doub
     template hipError_t launch_node_pml<R>(const NodeState &, const PmlNodes &, const R *, int64_t, double, int, int,  \
                                            hipStream_t);                                                             \
     template hipError_t launch_fault<R>(const FaultState &, const NodeState &, const StepConsts &, const R *, int64_t, \
-                                        const R *, int64_t, int, double, int, hipStream_t);
+                                        const R *, int64_t, int, double, int, bool, hipStream_t);
 EQD_INSTANTIATE(double)
 EQD_INSTANTIATE(float)
 #undef EQD_INSTANTIATE

@@ -18,7 +18,11 @@ LIB_PATH = os.environ.get("EQD_LIB", os.path.join(HERE, "libeqd.so"))
 
 EXPORTS = ["eqd_create", "eqd_set", "eqd_commit", "eqd_run", "eqd_get", "eqd_clock", "eqd_nan_info",
            "eqd_kernel_time", "eqd_set_timing", "eqd_device_bytes", "eqd_layout_info", "eqd_last_error",
-           "eqd_destroy"]
+           "eqd_destroy", "eqd_set_guess", "eqd_harvest", "eqd_harvest_get", "eqd_newton_stats", "eqd_mlp_forward",
+           "eqd_mlp_grad", "eqd_mlp_train"]
+
+MLP_NTRAIN = 8833
+MLP_NPARAMS = 8847
 
 FIELDS = {
     "meshCoor": (1, "f8"), "eqNumIndexArr": (2, "i4"), "eqNumStartIndexLoc": (3, "i4"),
@@ -54,6 +58,11 @@ class Scalars(C.Structure):
                 "precision", "device")]
 
 
+class TrainOpts(C.Structure):
+    _fields_ = [("epochs", C.c_int32), ("batch", C.c_int32), ("lr", C.c_float), ("beta1", C.c_float),
+                ("beta2", C.c_float), ("eps", C.c_float), ("seed", C.c_uint64), ("device", C.c_int32)]
+
+
 class EqdError(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__(f"eqdyna_amd error {code}: {msg}")
@@ -85,6 +94,14 @@ def lib():
         L.eqd_last_error.argtypes = [C.c_void_p]
         L.eqd_last_error.restype = C.c_char_p
         L.eqd_destroy.argtypes = [C.c_void_p]
+        fp = C.POINTER(C.c_float)
+        L.eqd_set_guess.argtypes = [C.c_void_p, fp, C.c_int64, C.c_int32, C.c_float]
+        L.eqd_harvest.argtypes = [C.c_void_p, C.c_int32, C.c_int64]
+        L.eqd_harvest_get.argtypes = [C.c_void_p, fp, C.c_int64, C.POINTER(C.c_int64)]
+        L.eqd_newton_stats.argtypes = [C.c_void_p, C.POINTER(C.c_int64)]
+        L.eqd_mlp_forward.argtypes = [fp, fp, C.c_int64, fp, C.c_int32]
+        L.eqd_mlp_grad.argtypes = [fp, fp, fp, C.c_int64, fp, fp, C.c_int32]
+        L.eqd_mlp_train.argtypes = [fp, fp, C.c_int64, C.POINTER(TrainOpts), fp, fp]
         _lib = L
     return _lib
 
@@ -194,6 +211,36 @@ class Solver:
             out[k] = v.value
         return out
 
+    def set_guess(self, params, k_fallback: int = 6, gate: float = 0.5) -> None:
+        """Faster rate-and-state mode: Newton from the network's proposal where
+        it predicts |ln(V/V_prev)| >= gate (params: the float32 parameter
+        vector), stop at convergence or at the roundoff floor, fall back to the
+        reference's solve after k_fallback iterations.  params None with
+        k_fallback > 0: the same policy from the plain guess; params None with
+        k_fallback 0: the reference's solve."""
+        if params is None:
+            self._check(self.L.eqd_set_guess(self.h, None, 0, int(k_fallback), 0.0))
+            return
+        p = _f32(params, MLP_NPARAMS)
+        self._check(self.L.eqd_set_guess(self.h, _fptr(p), p.size, int(k_fallback), float(gate)))
+
+    def harvest(self, every: int, capacity: int) -> None:
+        self._check(self.L.eqd_harvest(self.h, int(every), int(capacity)))
+
+    def harvest_get(self) -> np.ndarray:
+        """Recorded rows [n, 7]: the six network inputs and the converged ln V."""
+        n = C.c_int64()
+        self._check(self.L.eqd_harvest_get(self.h, None, 0, C.byref(n)))
+        out = np.zeros((n.value, 7), dtype=np.float32)
+        if n.value:
+            self._check(self.L.eqd_harvest_get(self.h, _fptr(out), n.value, C.byref(n)))
+        return out
+
+    def newton_stats(self) -> Dict[str, int]:
+        v = (C.c_int64 * 5)()
+        self._check(self.L.eqd_newton_stats(self.h, v))
+        return {"solves": v[0], "iterations": v[1], "fallbacks": v[2], "unconverged": v[3], "roundoff_stops": v[4]}
+
     def device_bytes(self) -> int:
         return int(self.L.eqd_device_bytes(self.h))
 
@@ -207,3 +254,47 @@ class Solver:
             self.close()
         except Exception:
             pass
+
+
+def _f32(a, size=None) -> np.ndarray:
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    if size is not None and a.size != size:
+        raise ValueError(f"expected {size} float32 values, got {a.size}")
+    return a
+
+
+def _fptr(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def mlp_forward(params, X, device: int = 0) -> np.ndarray:
+    """The network's r = ln(V / V_prev) for raw inputs X[n, 6] (GPU)."""
+    p, X = _f32(params, MLP_NPARAMS), _f32(X)
+    out = np.zeros(len(X), dtype=np.float32)
+    rc = lib().eqd_mlp_forward(_fptr(p), _fptr(X), len(X), _fptr(out), int(device))
+    if rc != 0:
+        raise EqdError(rc, "eqd_mlp_forward failed")
+    return out
+
+
+def mlp_grad(params, X, r, device: int = 0):
+    """(loss, gradient[MLP_NTRAIN]) of the mean squared error against targets r over all samples (GPU)."""
+    p, X, y = _f32(params, MLP_NPARAMS), _f32(X), _f32(r)
+    g = np.zeros(MLP_NTRAIN, dtype=np.float32)
+    loss = C.c_float()
+    rc = lib().eqd_mlp_grad(_fptr(p), _fptr(X), _fptr(y), len(X), _fptr(g), C.byref(loss), int(device))
+    if rc != 0:
+        raise EqdError(rc, "eqd_mlp_grad failed")
+    return float(loss.value), g
+
+
+def mlp_train(params, X, r, epochs: int, batch: int = 4096, lr: float = 1e-3, seed: int = 0,
+              beta1: float = 0.9, beta2: float = 0.999, eps: float = 1e-8, device: int = 0):
+    """Train on the GPU against targets r; returns (trained parameters, mean loss per epoch)."""
+    p, X, y = _f32(params, MLP_NPARAMS).copy(), _f32(X), _f32(r)
+    o = TrainOpts(int(epochs), int(batch), float(lr), float(beta1), float(beta2), float(eps), int(seed), int(device))
+    losses = np.zeros(int(epochs), dtype=np.float32)
+    rc = lib().eqd_mlp_train(_fptr(X), _fptr(y), len(X), C.byref(o), _fptr(p), _fptr(losses))
+    if rc != 0:
+        raise EqdError(rc, "eqd_mlp_train failed")
+    return p, losses

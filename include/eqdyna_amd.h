@@ -146,6 +146,72 @@ int eqd_layout_info(eqd_handle *h, const char *key, int64_t *value);
 const char *eqd_last_error(eqd_handle *h);
 int eqd_destroy(eqd_handle *h);
 
+/* ---------------------------------------------------------------------------
+ * Learned initial guess of the rate-and-state Newton solve.  No counterpart in
+ * the reference, whose NewtonRaphson (src/faulting.f90:459-516) starts from
+ * the previous slip rate; the solve itself and its convergence test are
+ * unchanged, the network only proposes the starting point.
+ *
+ * Network: 6 -> 64 -> 64 -> 64 -> 1, GELU; bfloat16 weights and activations
+ * on the matrix cores, float32 accumulation.  Inputs per fault node (from
+ * solveRSF, src/faulting.f90:190-327): trial shear traction (Pa), effective
+ * normal stress (Pa), state variable (psi of the slip law, ln theta of the
+ * ageing law), a, b, ln |V_prev| (m/s); output r = ln(V / V_prev), the
+ * proposal being V_prev * exp(r).
+ * Parameter vector (float32, EQD_MLP_NPARAMS values): W1[64][6] b1[64]
+ * W2[64][64] b2[64] W3[64][64] b3[64] w4[64] b4 in_shift[6] in_scale[6]
+ * out_shift out_scale; normalised input (x - in_shift) * in_scale; r =
+ * y * out_scale + out_shift.  The first EQD_MLP_NTRAIN values are trained. */
+#define EQD_MLP_NTRAIN 8833
+#define EQD_MLP_NPARAMS 8847
+
+/* Iteration policy of the faster rate-and-state mode (k_fallback > 0): the
+ * solve starts from a proposal, stops when the reference's convergence test
+ * is met or when the residual is within 16 ulps of the size of its terms
+ * (the roundoff floor, where the reference keeps iterating without
+ * improving), and after k_fallback iterations without either restarts from
+ * the plain guess with the reference's solve (20 iterations).  The proposal
+ * is the network's V_prev * exp(r) where it predicts |r| >= gate, else the
+ * plain guess; params = NULL (nparams = 0) uses the plain guess throughout.
+ * k_fallback = 0 with params = NULL restores the reference's solve.  Takes
+ * effect at the next eqd_run. */
+int eqd_set_guess(eqd_handle *h, const float *params, int64_t nparams, int32_t k_fallback, float gate);
+
+/* Record (the six network inputs, converged ln V) of every rate-and-state
+ * node every `every` steps of the following eqd_run calls into a device
+ * buffer of `capacity` rows (7 floats each); capacity = 0 stops and frees it. */
+int eqd_harvest(eqd_handle *h, int32_t every, int64_t capacity);
+/* Copy up to max_rows recorded rows (row-major [rows][7]); *rows = recorded. */
+int eqd_harvest_get(eqd_handle *h, float *out, int64_t max_rows, int64_t *rows);
+
+/* Newton counters accumulated since eqd_commit: stats[0] node solves,
+ * stats[1] iterations, stats[2] fallbacks to the plain guess, stats[3] solves
+ * that ended without meeting the convergence test, stats[4] solves stopped
+ * at the roundoff floor. */
+int eqd_newton_stats(eqd_handle *h, int64_t stats[5]);
+
+/* Standalone use of the network on the GPU (host arrays, device `device`):
+ * X[n][6] raw inputs, r[n] outputs. */
+int eqd_mlp_forward(const float *params, const float *X, int64_t n, float *r_out, int32_t device);
+/* Mean squared error of the normalised output against the normalised
+ * targets r[n], and its gradient with respect to the first EQD_MLP_NTRAIN
+ * parameters, over all n samples. */
+int eqd_mlp_grad(const float *params, const float *X, const float *r, int64_t n, float *grad_out,
+                 float *loss_out, int32_t device);
+
+typedef struct {
+    int32_t epochs, batch;
+    float lr, beta1, beta2, eps;
+    uint64_t seed; /* shuffling */
+    int32_t device;
+} eqd_mlp_train_opts;
+
+/* Adam on the mean squared error, batches drawn from a seeded permutation
+ * each epoch.  params holds the initial parameters (with the normalisation)
+ * and receives the trained ones; epoch_loss[epochs] the mean loss per epoch. */
+int eqd_mlp_train(const float *X, const float *r, int64_t n, const eqd_mlp_train_opts *opts, float *params,
+                  float *epoch_loss);
+
 #ifdef __cplusplus
 }
 #endif

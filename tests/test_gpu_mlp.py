@@ -1,0 +1,145 @@
+"""The learned rate-and-state Newton guess on the GPU: the network's forward
+pass and gradient against the CPU oracle (oracle/mlp_oracle.py, which the
+CPU tests pin against finite differences), deterministic training, and the
+solver with the guess against the reference's FP64 run.
+
+Tolerances: bfloat16 operands make the GPU and the oracle round at the same
+points, but float32 accumulation order differs and a value next to a
+bfloat16 rounding tie can round the other way, so outputs agree to 2e-2 of
+the output scale (median 1e-3) and each gradient tensor to 2e-2 relative L2.
+With the guess, Newton still decides the slip rate, so the run must stay
+within the stated error bars of the faster modes: rupture time within 0.5 dt
+on average and peak slip rate within 1 %."""
+import numpy as np
+import pytest
+
+import mlp_oracle as M
+from eqdyna_amd import solver
+from eqdyna_amd.solver import Solver
+from fixtures import channel_errors, channels, load_golden
+
+
+def _data(n, seed):
+    rng = np.random.default_rng(seed)
+    X = np.stack([rng.uniform(1e7, 3e7, n), rng.uniform(-4e7, -1e7, n), rng.uniform(0.3, 0.9, n),
+                  rng.uniform(0.008, 0.016, n), np.full(n, 0.012), rng.uniform(-28, 1, n)], axis=1)
+    # a smooth target the network can represent
+    y = 0.8 * X[:, 5] + 3.0 * np.tanh((X[:, 0] + 0.6 * X[:, 1]) / 5e6) + 40.0 * (X[:, 3] - 0.012)
+    return X.astype(np.float32), y.astype(np.float32)
+
+
+def _params(X, y, seed):
+    p = M.init_params(X, y, seed)
+    rng = np.random.default_rng(seed + 7)
+    p[:M.NTRAIN] += rng.normal(0, 0.05, M.NTRAIN).astype(np.float32)  # non-zero biases too
+    return p
+
+
+@pytest.mark.gpu
+def test_mlp_forward_matches_oracle():
+    X, y = _data(1000, 1)  # not a multiple of 64
+    p = _params(X, y, 2)
+    got = solver.mlp_forward(p, X)
+    ref = M.predict(p, X, bf16_points=True)
+    err = np.abs(got - ref) / float(p[-1])
+    print(f"forward: max {err.max():.2e} median {np.median(err):.2e} (in output-scale units)", flush=True)
+    assert err.max() <= 2e-2 and np.median(err) <= 1e-3
+
+
+@pytest.mark.gpu
+def test_mlp_gradient_matches_oracle():
+    X, y = _data(700, 3)
+    p = _params(X, y, 4)
+    loss, g = solver.mlp_grad(p, X, y)
+    loss_ref, g_ref = M.loss_and_grad(p, X, y, bf16_points=True)
+    assert abs(loss - loss_ref) <= 1e-3 * loss_ref
+    o = 0
+    for name, shape in M.SIZES:
+        n = int(np.prod(shape))
+        a, b = g[o:o + n], g_ref[o:o + n]
+        rel = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
+        print(f"grad {name}: rel L2 {rel:.2e}", flush=True)
+        assert rel <= 2e-2, name
+        o += n
+
+
+@pytest.mark.gpu
+def test_mlp_training_converges_and_is_deterministic():
+    X, y = _data(65536, 5)
+    p0 = M.init_params(X, y, 6)
+    l0, _ = solver.mlp_grad(p0, X[:4096], y[:4096])
+    p1, losses = solver.mlp_train(p0, X, y, epochs=8, batch=2048, lr=2e-3, seed=11)
+    p2, _ = solver.mlp_train(p0, X, y, epochs=8, batch=2048, lr=2e-3, seed=11)
+    l1, _ = solver.mlp_grad(p1, X[:4096], y[:4096])
+    print(f"training loss {l0:.3e} -> {l1:.3e}; per epoch {np.round(losses, 5)}", flush=True)
+    assert np.array_equal(p1, p2)
+    assert np.all(p1[M.NTRAIN:] == p0[M.NTRAIN:])  # normalisation is not trained
+    assert l1 < 0.05 * l0
+
+
+def _harvest_and_train(bundle, sc, precision="fp64", epochs=60):
+    s = Solver(bundle, sc, precision)
+    nstep = int(sc["nstep"])
+    s.harvest(1, int(sc["nftnd"]) * nstep)
+    s.run(nstep)
+    rows = s.harvest_get()
+    plain = s.newton_stats()
+    s.close()
+    X, r = rows[:, :6], rows[:, 6] - rows[:, 5]  # target ln(V / V_prev)
+    p0 = M.init_params(X, r, 0)
+    p, _ = solver.mlp_train(p0, X, r, epochs=epochs, batch=1024, lr=2e-3, seed=1)
+    return rows, plain, p
+
+
+def _run(bundle, sc, params=None, k=0, gate=0.5):
+    s = Solver(bundle, sc, "fp64")
+    if params is not None or k > 0:
+        s.set_guess(params, k_fallback=k, gate=gate)
+    s.run(int(sc["nstep"]))
+    out = s.outputs()
+    st = s.newton_stats()
+    s.close()
+    return out, st
+
+
+def _errors(out, ref, dt):
+    ch = channel_errors(channels(out), channels(ref))
+    worst = max(ch, key=ch.get)
+    rt_g, rt_r = out["fnft"], ref["fnft"]
+    both = (rt_g < 5000) & (rt_r < 5000)
+    drt = np.abs(rt_g[both] - rt_r[both]) / dt if both.any() else np.zeros(1)
+    pk = np.max(np.abs(out["fric"][:, 75] - ref["fric"][:, 75])) / np.max(np.abs(ref["fric"][:, 75]))
+    return worst, ch[worst], float(drt.mean()), float(pk)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["box_rsf4", "box_rsf3", "box_rough"])
+def test_learned_guess_vs_reference(name):
+    """Three arms on one case: the reference's solve, the faster mode's
+    iteration policy from the plain guess, and the same policy from the
+    network's proposal (trained on this case's own harvest).  Both faster
+    arms must stay within the stated error bars and save iterations over the
+    reference's solve; what the network adds over the plain-guess policy is
+    printed (on a case this small it need not be positive)."""
+    print(f"case {name}", flush=True)
+    bundle, sc, ref = load_golden(name)
+    nf, nstep = int(sc["nftnd"]), int(sc["nstep"])
+    rows, plain, p = _harvest_and_train(bundle, sc)
+    assert rows.shape == (nf * nstep, 7) and np.all(np.isfinite(rows))
+    assert plain["solves"] == nf * nstep and plain["fallbacks"] == 0 and plain["roundoff_stops"] == 0
+    arms = {"reference": (None, 0), "policy_plain": (None, 6), "policy_learned": (p, 6)}
+    res = {}
+    for arm, (params, k) in arms.items():
+        out, st = _run(bundle, sc, params, k)
+        res[arm] = (st, _errors(out, ref, sc["dt"]))
+        print(f"[{name}] {arm}: {st}; iterations/solve {st['iterations'] / st['solves']:.2f}; "
+              f"worst channel {res[arm][1][0]} {res[arm][1][1]:.2e}; rupture time mean {res[arm][1][2]:.3f} dt; "
+              f"peak slip rate {res[arm][1][3]:.2e}", flush=True)
+    assert res["reference"][0]["iterations"] == plain["iterations"]
+    for arm in ("policy_plain", "policy_learned"):
+        st, (_, _, drt, pk) = res[arm]
+        assert st["solves"] == nf * nstep
+        assert drt <= 0.5 and pk <= 0.01, arm
+        assert st["iterations"] < plain["iterations"], arm
+    gain = 1.0 - res["policy_learned"][0]["iterations"] / res["policy_plain"][0]["iterations"]
+    print(f"[{name}] network gain in iterations over the plain-guess policy: {100 * gain:.1f} %", flush=True)
