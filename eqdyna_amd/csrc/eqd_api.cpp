@@ -579,6 +579,7 @@ int eqd_commit(eqd_handle *h)
                 if ((r2 = dupload(h, &pc, cls)) || (r2 = dupload(h, &pt, tab))) return r2;
                 E.opc = pc;
                 E.optab = pt;
+                E.n_opc = (int32_t)(tab.size() / kOpW);
                 h->op_classes_reg = (int64_t)(tab.size() / kOpW);
             } else if ((r2 = up(&E.shp, r_shp)) || (r2 = up(&E.det, r_det)) || (r2 = up(&E.lam, r_lam)) ||
                        (r2 = up(&E.mu, r_mu)) || (r2 = up(&E.vp, r_vp)) || (r2 = up(&E.rho, r_rho)) ||
@@ -593,6 +594,7 @@ int eqd_commit(eqd_handle *h)
                 if ((r2 = dupload(h, &pc, cls)) || (r2 = dupload(h, &pt, tab))) return r2;
                 E.phic = pc;
                 E.phitab = pt;
+                E.n_phic = (int32_t)(tab.size() / 32);
                 h->phi_classes_reg = (int64_t)(tab.size() / 32);
             } else if ((r2 = up(&E.phi, r_phi))) {
                 return r2;

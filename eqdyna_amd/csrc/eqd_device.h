@@ -65,6 +65,7 @@ struct RegElems {
     const R *optab;
     const uint32_t *phic;
     const R *phitab;
+    int32_t n_opc, n_phic;  // rows of optab and phitab
 };
 
 // layout of one operator-class row

@@ -35,38 +35,85 @@ __device__ __forceinline__ R sqr(R x) { return x * x; }
 // node's force.  Every accumulation keeps the reference's order (nodes in
 // ascending order, modes in ascending order), and the register footprint is
 // a few dozen values instead of four 24-vectors.
-// Node-loop unrolling of the regular element kernel: 1 keeps the register
-// footprint (and so occupancy) best; larger values trade it for more loads in
-// flight per wave.
+// Nodes whose v and u are in flight ahead of the arithmetic in the element
+// kernels' gather pass: more hides more memory latency per wave, fewer keeps
+// the register footprint (and so the number of waves per SIMD) down.
+#ifndef EQD_PREFETCH
+#define EQD_PREFETCH 1
+#endif
+// Unrolling of the element kernels' force pass.
 #ifndef EQD_ELEM_UNROLL
 #define EQD_ELEM_UNROLL 1
 #endif
 
-template <typename R, bool kQ, bool kDP, int kHG, bool kMass>
+//
+// kLds: both class tables are staged in LDS (dynamic shared memory of
+// (n_opc * kOpW + n_phic * 32) values) at block start, so the operator reads
+// of the element loop are LDS reads instead of vector-memory loads.
+template <typename R, bool kQ, bool kDP, int kHG, bool kMass, bool kLds>
 __global__ void __launch_bounds__(256) elem_regular_kernel(RegElems<R> E, NodeState N, StepConsts C)
 {
+    extern __shared__ __attribute__((aligned(16))) unsigned char eqd_smem[];
+    typedef __attribute__((address_space(3))) R lds_R;
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= E.n) return;
+    const R *op = nullptr, *pht = nullptr;
+    const lds_R *lrow = nullptr, *lphr = nullptr;
+    if constexpr (kLds) {
+        lds_R *lop = (lds_R *)eqd_smem, *lph = lop + E.n_opc * kOpW;
+        for (int j = threadIdx.x; j < E.n_opc * kOpW; j += blockDim.x) lop[j] = E.optab[j];
+        for (int j = threadIdx.x; j < E.n_phic * 32; j += blockDim.x) lph[j] = E.phitab[j];
+        __syncthreads();
+        if (t >= E.n) return;
+        lrow = lop + (int64_t)E.opc[t] * kOpW;
+        lphr = lph + (int64_t)E.phic[t] * 32;
+    } else {
+        if (t >= E.n) return;
+        // operators: one class-table row, or the element's own arrays
+        op = E.optab ? E.optab + (int64_t)E.opc[t] * kOpW : nullptr;
+        pht = E.phitab ? E.phitab + (int64_t)E.phic[t] * 32 : nullptr;
+    }
     const int64_t n = E.n, nn = N.nn;
     constexpr bool kF32 = sizeof(R) != sizeof(double);
-    // operators: one class-table row, or the element's own arrays
-    const R *op = E.optab ? E.optab + (int64_t)E.opc[t] * kOpW : nullptr;
-    const R *pht = E.phitab ? E.phitab + (int64_t)E.phic[t] * 32 : nullptr;
-    auto SHP = [&](int i) -> R { return op ? op[kOpShp + i] : E.shp[i * n + t]; };
-    auto PHI = [&](int m, int a) -> R { return pht ? pht[m * 8 + a] : E.phi[(m * 8 + a) * n + t]; };
+    auto SHP = [&](int i) -> R {
+        if constexpr (kLds) return lrow[kOpShp + i];
+        else return op ? op[kOpShp + i] : E.shp[i * n + t];
+    };
+    auto PHI = [&](int m, int a) -> R {
+        if constexpr (kLds) return lphr[m * 8 + a];
+        else return pht ? pht[m * 8 + a] : E.phi[(m * 8 + a) * n + t];
+    };
+    // a scalar operator k of the row, or from its per-element array
+    auto OPK = [&](int k, const R *arr) -> R {
+        if constexpr (kLds) return lrow[k];
+        else return op ? op[k] : arr[t];
+    };
     const R rdk = (R)C.rdampk;
+    // all eight node ids first (independent loads), then the nodes' v and u
+    // one node ahead of the arithmetic, so a gather is always in flight
+    int nds[8];
+#pragma unroll
+    for (int a = 0; a < 8; a++) nds[a] = E.ien[a * n + t];
+    // EQD_PREFETCH nodes in flight: slot a % EQD_PREFETCH holds node a's v and u
+    constexpr int P = EQD_PREFETCH;
+    double vq[P][3], uq[P][3];
+#pragma unroll
+    for (int b = 0; b < P; b++)
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            vq[b][k] = N.vel[k * nn + nds[b]];
+            uq[b][k] = N.disp[k * nn + nds[b]];
+        }
     // FP32 bulk: velocities and displacements relative to the first node
     // (formed in FP64).  Strain, strain rate and the hourglass modes are
     // blind to a common translation (sum_a dN_a/dx = 0, Phi orthogonal to
     // constants), so only the rounding changes, and it is now relative to the
     // deformation rather than to the absolute motion.
     double v0[3] = {0, 0, 0}, u0[3] = {0, 0, 0};
-    const int nd0 = E.ien[t];
     if constexpr (kF32) {
 #pragma unroll
         for (int k = 0; k < 3; k++) {
-            v0[k] = N.vel[k * nn + nd0];
-            u0[k] = N.disp[k * nn + nd0];
+            v0[k] = vq[0][k];
+            u0[k] = uq[0][k];
         }
     }
     // pass 1: strain rate, strain (Q), hourglass mode amplitudes
@@ -75,18 +122,21 @@ __global__ void __launch_bounds__(256) elem_regular_kernel(RegElems<R> E, NodeSt
     R qv[3][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0, 0}};    // kHG 2: sum_a v_a Gamma_ja
     constexpr int fi[4][8] = {{1, 1, -1, -1, -1, -1, 1, 1}, {1, -1, -1, 1, -1, 1, 1, -1},
                               {1, -1, 1, -1, 1, -1, 1, -1}, {1, -1, 1, -1, -1, 1, -1, 1}};
-#pragma unroll EQD_ELEM_UNROLL
+#pragma unroll EQD_PREFETCH
     for (int a = 0; a < 8; a++) {
-        const int nd = a == 0 ? nd0 : E.ien[a * n + t];
+        const int b = a % P;
         R vl[3], dl[3];
 #pragma unroll
         for (int k = 0; k < 3; k++) {
-            if constexpr (kF32) {
-                vl[k] = (R)(N.vel[k * nn + nd] - v0[k]);
-                dl[k] = (R)(N.disp[k * nn + nd] - u0[k]);
-            } else {
-                vl[k] = N.vel[k * nn + nd];
-                dl[k] = N.disp[k * nn + nd];
+            vl[k] = (R)(vq[b][k] - v0[k]);
+            dl[k] = (R)(uq[b][k] - u0[k]);
+        }
+        if (a + P < 8) {
+            const int nx = nds[a + P];
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                vq[b][k] = N.vel[k * nn + nx];
+                uq[b][k] = N.disp[k * nn + nx];
             }
         }
         const R n1 = SHP(a * 3), n2 = SHP(a * 3 + 1), n3 = SHP(a * 3 + 2);
@@ -121,8 +171,8 @@ __global__ void __launch_bounds__(256) elem_regular_kernel(RegElems<R> E, NodeSt
                 for (int j = 0; j < 4; j++) qv[i][j] = qv[i][j] + vl[i] * (R)fi[j][a];
         }
     }
-    const R lam = op ? op[kOpLam] : E.lam[t], miu = op ? op[kOpMu] : E.mu[t];
-    const R det = op ? op[kOpDet] : E.det[t];
+    const R lam = OPK(kOpLam, E.lam), miu = OPK(kOpMu, E.mu);
+    const R det = OPK(kOpDet, E.det);
     const R cdiag = lam + (R)2 * miu;
     R srt[6];
 #pragma unroll
@@ -211,7 +261,7 @@ __global__ void __launch_bounds__(256) elem_regular_kernel(RegElems<R> E, NodeSt
     if (kHG == 1) {
         R ss[6];
 #pragma unroll
-        for (int i = 0; i < 6; i++) ss[i] = op ? op[kOpSs + i] : E.ss[i * n + t];
+        for (int i = 0; i < 6; i++) ss[i] = kLds ? (R)lrow[kOpSs + i] : (op ? op[kOpSs + i] : E.ss[i * n + t]);
 #pragma unroll
         for (int m = 0; m < 4; m++) {
             g[m][0] = ss[0] * pd[m][0] + ss[1] * pd[m][1] + ss[2] * pd[m][2];
@@ -219,7 +269,7 @@ __global__ void __launch_bounds__(256) elem_regular_kernel(RegElems<R> E, NodeSt
             g[m][2] = ss[2] * pd[m][0] + ss[4] * pd[m][1] + ss[5] * pd[m][2];
         }
     } else if (kHG == 2) {
-        const R rho = op ? op[kOpRho] : E.rho[t], vp = op ? op[kOpVp] : E.vp[t];
+        const R rho = OPK(kOpRho, E.rho), vp = OPK(kOpVp, E.vp);
         coef = (R)0.25 * (R)C.kapa_hg * rho * vp * (R)pow((double)det * C.w, 2.0 / 3.0);
     }
     // pass 2: one node's force at a time
@@ -227,7 +277,7 @@ __global__ void __launch_bounds__(256) elem_regular_kernel(RegElems<R> E, NodeSt
     for (int a = 0; a < 8; a++) {
         R f[3] = {0, 0, 0};
         if (kMass) {
-            const int nd = E.ien[a * n + t];
+            const int nd = nds[a];
 #pragma unroll
             for (int k = 0; k < 3; k++) {
                 // absolute velocity here: the Rayleigh mass term is not translation invariant
@@ -281,14 +331,26 @@ __global__ void __launch_bounds__(256) elem_pml_kernel(PmlElems<R> E, NodeState 
     const R *pht = E.phitab ? E.phitab + (int64_t)E.phic[p] * 32 : nullptr;
     auto SHP = [&](int i) -> R { return op ? op[kOpShp + i] : E.shp[i * n + p]; };
     auto PHI = [&](int m, int a) -> R { return pht ? pht[m * 8 + a] : E.phi[(m * 8 + a) * n + p]; };
+    // node ids first, then each node's v and u one node ahead of the arithmetic
+    int nds[8];
+#pragma unroll
+    for (int a = 0; a < 8; a++) nds[a] = E.ien[a * n + p];
+    constexpr int P = EQD_PREFETCH;
+    double vq[P][3], uq[P][3];
+#pragma unroll
+    for (int b = 0; b < P; b++)
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            vq[b][k] = N.vel[k * nn + nds[b]];
+            uq[b][k] = N.disp[k * nn + nds[b]];
+        }
     // velocities (relative to node 1 in the FP32 variant, see elem_regular_kernel)
     double v0[3] = {0.0, 0.0, 0.0}, u0[3] = {0.0, 0.0, 0.0};
-    const int nd0 = E.ien[p];
     if (!kF64) {
 #pragma unroll
         for (int k = 0; k < 3; k++) {
-            v0[k] = N.vel[k * nn + nd0];
-            u0[k] = N.disp[k * nn + nd0];
+            v0[k] = vq[0][k];
+            u0[k] = uq[0][k];
         }
     }
     constexpr int fi[4][8] = {{1, 1, -1, -1, -1, -1, 1, 1}, {1, -1, -1, 1, -1, 1, 1, -1},
@@ -297,12 +359,30 @@ __global__ void __launch_bounds__(256) elem_pml_kernel(PmlElems<R> E, NodeState 
     R Dx_vx = 0, Dy_vy = 0, Dz_vz = 0, Dx_vy = 0, Dy_vx = 0, Dx_vz = 0, Dz_vx = 0, Dy_vz = 0, Dz_vy = 0;
     R pd[4][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
     R qv[3][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0, 0}};
-#pragma unroll EQD_ELEM_UNROLL
+#pragma unroll EQD_PREFETCH
     for (int a = 0; a < 8; a++) {
-        const int nd = a == 0 ? nd0 : E.ien[a * n + p];
+        const int b = a % P;
         R va[3];
+        R dlh[3];
 #pragma unroll
-        for (int k = 0; k < 3; k++) va[k] = (R)(N.vel[k * nn + nd] - v0[k]);
+        for (int k = 0; k < 3; k++) va[k] = (R)(vq[b][k] - v0[k]);
+        if (kHG == 1) {
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                if (kF64)
+                    dlh[k] = (R)(uq[b][k] + C.rdampk * (double)va[k]);
+                else
+                    dlh[k] = (R)((uq[b][k] - u0[k]) + C.rdampk * (vq[b][k] - v0[k]));
+            }
+        }
+        if (a + P < 8) {
+            const int nx = nds[a + P];
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                vq[b][k] = N.vel[k * nn + nx];
+                if (kHG == 1) uq[b][k] = N.disp[k * nn + nx];
+            }
+        }
         const R n1 = SHP(a * 3), n2 = SHP(a * 3 + 1), n3 = SHP(a * 3 + 2);
         sr[0] = sr[0] + n1 * va[0];
         sr[1] = sr[1] + n2 * va[1];
@@ -320,14 +400,6 @@ __global__ void __launch_bounds__(256) elem_pml_kernel(PmlElems<R> E, NodeState 
         Dy_vz = Dy_vz + n2 * va[2];
         Dz_vy = Dz_vy + n3 * va[1];
         if (kHG == 1) {
-            R dlh[3];
-#pragma unroll
-            for (int k = 0; k < 3; k++) {
-                if (kF64)
-                    dlh[k] = (R)(N.disp[k * nn + nd] + C.rdampk * (double)va[k]);
-                else
-                    dlh[k] = (R)((N.disp[k * nn + nd] - u0[k]) + C.rdampk * (N.vel[k * nn + nd] - v0[k]));
-            }
 #pragma unroll
             for (int m = 0; m < 4; m++) {
                 const R ph = PHI(m, a);
@@ -423,7 +495,7 @@ __global__ void __launch_bounds__(256) elem_pml_kernel(PmlElems<R> E, NodeState 
         const R g1 = SHP(a * 3), g2 = SHP(a * 3 + 1), g3 = SHP(a * 3 + 2);
         R f[12];
         if (kMass) {
-            const int nd = E.ien[a * n + p];
+            const int nd = nds[a];
 #pragma unroll
             for (int k = 0; k < 9; k++) f[k] = 0;
 #pragma unroll
@@ -1048,7 +1120,16 @@ hipError_t launch_elem_regular(const RegElems<R> &E, const NodeState &N, const S
 {
     if (E.n == 0) return hipSuccess;
     const unsigned g = grid_for(E.n, 256);
-#define EQD_LAUNCH(Q, DP, HG, MASS) elem_regular_kernel<R, Q, DP, HG, MASS><<<g, 256, 0, s>>>(E, N, C)
+    // class tables into LDS when both exist and together take at most 16 KiB
+    const size_t lds = (E.optab && E.phitab) ? sizeof(R) * ((size_t)E.n_opc * kOpW + (size_t)E.n_phic * 32) : 0;
+    const bool use_lds = lds > 0 && lds <= 16384;
+#define EQD_LAUNCH(Q, DP, HG, MASS)                                                                   \
+    do {                                                                                              \
+        if (use_lds)                                                                                  \
+            elem_regular_kernel<R, Q, DP, HG, MASS, true><<<g, 256, lds, s>>>(E, N, C);               \
+        else                                                                                          \
+            elem_regular_kernel<R, Q, DP, HG, MASS, false><<<g, 256, 0, s>>>(E, N, C);                \
+    } while (0)
     if (q && mass) {
         if (hg == 1) EQD_LAUNCH(true, false, 1, true);
         else if (hg == 2) EQD_LAUNCH(true, false, 2, true);
