@@ -678,7 +678,11 @@ int eqd_commit(eqd_handle *h)
             std::vector<uint32_t> cls;
             std::vector<R> tab;
             const int64_t maxc = no_classes ? 0 : std::max<int64_t>(1, np / 16);
-            if (classify_rows<R>(np, kOpWPml, [&](int64_t p, R *row) {
+            // the damping varies element by element across the layer: it stays a
+            // per-element array, so the operator rows (as for regular elements)
+            // collapse to a few classes
+            if ((r2 = up(&P.damp, p_damp))) return r2;
+            if (classify_rows<R>(np, kOpW, [&](int64_t p, R *row) {
                     for (int i = 0; i < 24; i++) row[kOpShp + i] = (R)p_shp[i * np + p];
                     row[kOpDet] = (R)p_det[p];
                     row[kOpLam] = (R)p_lam[p];
@@ -686,17 +690,17 @@ int eqd_commit(eqd_handle *h)
                     row[kOpVp] = (R)p_vp[p];
                     row[kOpRho] = (R)p_rho[p];
                     for (int i = 0; i < 6; i++) row[kOpSs + i] = (R)p_ss[i * np + p];
-                    for (int i = 0; i < 3; i++) row[kOpDamp + i] = (R)p_damp[i * np + p];
                 }, maxc, cls, tab)) {
                 uint32_t *pc;
                 R *pt;
                 if ((r2 = dupload(h, &pc, cls)) || (r2 = dupload(h, &pt, tab))) return r2;
                 P.opc = pc;
                 P.optab = pt;
-                h->op_classes_pml = (int64_t)(tab.size() / kOpWPml);
+                P.n_opc = (int32_t)(tab.size() / kOpW);
+                h->op_classes_pml = (int64_t)(tab.size() / kOpW);
             } else if ((r2 = up(&P.shp, p_shp)) || (r2 = up(&P.det, p_det)) || (r2 = up(&P.lam, p_lam)) ||
                        (r2 = up(&P.mu, p_mu)) || (r2 = up(&P.vp, p_vp)) || (r2 = up(&P.rho, p_rho)) ||
-                       (r2 = up(&P.ss, p_ss)) || (r2 = up(&P.damp, p_damp))) {
+                       (r2 = up(&P.ss, p_ss))) {
                 return r2;
             }
             if (classify_rows<R>(np, 32, [&](int64_t p, R *row) {
@@ -707,6 +711,7 @@ int eqd_commit(eqd_handle *h)
                 if ((r2 = dupload(h, &pc, cls)) || (r2 = dupload(h, &pt, tab))) return r2;
                 P.phic = pc;
                 P.phitab = pt;
+                P.n_phic = (int32_t)(tab.size() / 32);
                 h->phi_classes_pml = (int64_t)(tab.size() / 32);
             } else if ((r2 = up(&P.phi, p_phi))) {
                 return r2;

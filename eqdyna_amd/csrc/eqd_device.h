@@ -69,8 +69,7 @@ struct RegElems {
 };
 
 // layout of one operator-class row
-constexpr int kOpShp = 0, kOpDet = 24, kOpLam = 25, kOpMu = 26, kOpVp = 27, kOpRho = 28, kOpSs = 29,
-              kOpDamp = 35, kOpW = 35, kOpWPml = 38;
+constexpr int kOpShp = 0, kOpDet = 24, kOpLam = 25, kOpMu = 26, kOpVp = 27, kOpRho = 28, kOpSs = 29, kOpW = 35;
 
 template <typename R>
 struct PmlElems {
@@ -80,14 +79,15 @@ struct PmlElems {
     const R *det, *lam, *mu, *vp, *rho;
     R *stress;             // [21][n]: 15 split + 6 initial
     const R *ss, *phi, *emass;
-    const R *damp;         // [3][n] element-centre damping d_x, d_y, d_z
+    const R *damp;         // [3][n] element-centre damping d_x, d_y, d_z (always per element)
     const uint8_t *mask12; // [n] bit a: local node a carries 12 components
     R *ef;                 // [96][n] slot (a*12+k); 3-component nodes use k=0..2
-    // operator classes as in RegElems; a PML row also holds the damping (kOpWPml values)
+    // operator classes as in RegElems (kOpW values per row)
     const uint32_t *opc;
     const R *optab;
     const uint32_t *phic;
     const R *phitab;
+    int32_t n_opc, n_phic;
 };
 
 struct PmlNodes {

@@ -320,17 +320,42 @@ __global__ void __launch_bounds__(256) elem_regular_kernel(RegElems<R> E, NodeSt
 // accumulates the velocity gradients and hourglass mode amplitudes; pass 2
 // forms each node's 12 (or collapsed 3) force slots, hourglass included, and
 // writes every slot once.
-template <typename R, int kHG, bool kMass>
+template <typename R, int kHG, bool kMass, bool kLds>
 __global__ void __launch_bounds__(256) elem_pml_kernel(PmlElems<R> E, NodeState N, StepConsts C)
 {
+    extern __shared__ __attribute__((aligned(16))) unsigned char eqd_smem[];
+    typedef __attribute__((address_space(3))) R lds_R;
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= E.n) return;
+    const R *op = nullptr, *pht = nullptr;
+    const lds_R *lrow = nullptr, *lphr = nullptr;
+    if constexpr (kLds) {
+        // class tables staged in LDS, as in elem_regular_kernel
+        lds_R *lop = (lds_R *)eqd_smem, *lph = lop + E.n_opc * kOpW;
+        for (int j = threadIdx.x; j < E.n_opc * kOpW; j += blockDim.x) lop[j] = E.optab[j];
+        for (int j = threadIdx.x; j < E.n_phic * 32; j += blockDim.x) lph[j] = E.phitab[j];
+        __syncthreads();
+        if (p >= E.n) return;
+        lrow = lop + (int64_t)E.opc[p] * kOpW;
+        lphr = lph + (int64_t)E.phic[p] * 32;
+    } else {
+        if (p >= E.n) return;
+        op = E.optab ? E.optab + (int64_t)E.opc[p] * kOpW : nullptr;
+        pht = E.phitab ? E.phitab + (int64_t)E.phic[p] * 32 : nullptr;
+    }
     const int64_t n = E.n, nn = N.nn;
     constexpr bool kF64 = sizeof(R) == sizeof(double);
-    const R *op = E.optab ? E.optab + (int64_t)E.opc[p] * kOpWPml : nullptr;
-    const R *pht = E.phitab ? E.phitab + (int64_t)E.phic[p] * 32 : nullptr;
-    auto SHP = [&](int i) -> R { return op ? op[kOpShp + i] : E.shp[i * n + p]; };
-    auto PHI = [&](int m, int a) -> R { return pht ? pht[m * 8 + a] : E.phi[(m * 8 + a) * n + p]; };
+    auto SHP = [&](int i) -> R {
+        if constexpr (kLds) return lrow[kOpShp + i];
+        else return op ? op[kOpShp + i] : E.shp[i * n + p];
+    };
+    auto PHI = [&](int m, int a) -> R {
+        if constexpr (kLds) return lphr[m * 8 + a];
+        else return pht ? pht[m * 8 + a] : E.phi[(m * 8 + a) * n + p];
+    };
+    auto OPK = [&](int k, const R *arr) -> R {
+        if constexpr (kLds) return lrow[k];
+        else return op ? op[k] : arr[p];
+    };
     // node ids first, then each node's v and u one node ahead of the arithmetic
     int nds[8];
 #pragma unroll
@@ -413,7 +438,7 @@ __global__ void __launch_bounds__(256) elem_pml_kernel(PmlElems<R> E, NodeState 
                 for (int j = 0; j < 4; j++) qv[i][j] = qv[i][j] + va[i] * (R)fi[j][a];
         }
     }
-    const R lam = op ? op[kOpLam] : E.lam[p], miu = op ? op[kOpMu] : E.mu[p], det = op ? op[kOpDet] : E.det[p];
+    const R lam = OPK(kOpLam, E.lam), miu = OPK(kOpMu, E.mu), det = OPK(kOpDet, E.det);
     const R cdiag = lam + (R)2.0 * miu;
     R srt[6];
 #pragma unroll
@@ -427,8 +452,7 @@ __global__ void __launch_bounds__(256) elem_pml_kernel(PmlElems<R> E, NodeState 
     for (int i = 3; i < 6; i++) srt[i] = miu * sr[i];
 
     const R dt = (R)C.dt;
-    const R d0 = op ? op[kOpDamp] : E.damp[p], d1 = op ? op[kOpDamp + 1] : E.damp[n + p],
-            d2 = op ? op[kOpDamp + 2] : E.damp[2 * n + p];
+    const R d0 = E.damp[p], d1 = E.damp[n + p], d2 = E.damp[2 * n + p];
     const R l2m = lam + (R)2 * miu;
     const R coef[15] = {l2m, lam, lam, lam, l2m, lam, lam, lam, l2m, miu, miu, miu, miu, miu, miu};
     const R grad[15] = {Dx_vx, Dy_vy, Dz_vz, Dx_vx, Dy_vy, Dz_vz, Dx_vx, Dy_vy, Dz_vz,
@@ -460,7 +484,7 @@ __global__ void __launch_bounds__(256) elem_pml_kernel(PmlElems<R> E, NodeState 
     if (kHG == 1) {
         R ss[6];
 #pragma unroll
-        for (int i = 0; i < 6; i++) ss[i] = op ? op[kOpSs + i] : E.ss[i * n + p];
+        for (int i = 0; i < 6; i++) ss[i] = kLds ? (R)lrow[kOpSs + i] : (op ? op[kOpSs + i] : E.ss[i * n + p]);
 #pragma unroll
         for (int m = 0; m < 4; m++) {
             g[m][0] = ss[0] * pd[m][0] + ss[1] * pd[m][1] + ss[2] * pd[m][2];
@@ -468,7 +492,7 @@ __global__ void __launch_bounds__(256) elem_pml_kernel(PmlElems<R> E, NodeState 
             g[m][2] = ss[2] * pd[m][0] + ss[4] * pd[m][1] + ss[5] * pd[m][2];
         }
     } else if (kHG == 2) {
-        const R rho = op ? op[kOpRho] : E.rho[p], vp = op ? op[kOpVp] : E.vp[p];
+        const R rho = OPK(kOpRho, E.rho), vp = OPK(kOpVp, E.vp);
         coef2 = (R)0.25 * (R)C.kapa_hg * rho * vp * (R)pow((double)det * C.w, 2.0 / 3.0);
     }
 
@@ -542,20 +566,35 @@ template <typename R>
 __device__ __forceinline__ void gather3(const NodeState &N, int node, const R *ef, int64_t nreg,
                                         const R *efp, int64_t npe, double F[3])
 {
+    // entries in batches of eight: all entry loads, then all force loads,
+    // then the sums in entry order (so at most two memory latencies per batch)
     F[0] = 0.0; F[1] = 0.0; F[2] = 0.0;
     const int beg = N.rowptr[node], end = N.rowptr[node + 1];
-    for (int j = beg; j < end; j++) {
-        const uint32_t e = N.ent[j];
-        const int64_t idx = e & kEntIdxMask;
-        const int a = (e >> kEntLocalShift) & 7;
-        if (e & kEntPml) {
-            F[0] = F[0] + (double)efp[(a * 12 + 0) * npe + idx];
-            F[1] = F[1] + (double)efp[(a * 12 + 1) * npe + idx];
-            F[2] = F[2] + (double)efp[(a * 12 + 2) * npe + idx];
-        } else {
-            F[0] = F[0] + (double)ef[(a * 3 + 0) * nreg + idx];
-            F[1] = F[1] + (double)ef[(a * 3 + 1) * nreg + idx];
-            F[2] = F[2] + (double)ef[(a * 3 + 2) * nreg + idx];
+    for (int j0 = beg; j0 < end; j0 += 8) {
+        uint32_t e[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) e[k] = j0 + k < end ? N.ent[j0 + k] : 0u;
+        R f[8][3];
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            if (j0 + k < end) {
+                const int64_t idx = e[k] & kEntIdxMask;
+                const int a = (e[k] >> kEntLocalShift) & 7;
+                const bool pml = (e[k] & kEntPml) != 0;
+                const R *b = pml ? efp + (int64_t)(a * 12) * npe + idx : ef + (int64_t)(a * 3) * nreg + idx;
+                const int64_t st = pml ? npe : nreg;
+                f[k][0] = b[0];
+                f[k][1] = b[st];
+                f[k][2] = b[2 * st];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            if (j0 + k < end) {
+                F[0] = F[0] + (double)f[k][0];
+                F[1] = F[1] + (double)f[k][1];
+                F[2] = F[2] + (double)f[k][2];
+            }
         }
     }
 }
@@ -616,13 +655,29 @@ __global__ void __launch_bounds__(256) node_pml_kernel(NodeState N, PmlNodes P, 
     double F[12];
 #pragma unroll
     for (int k = 0; k < 12; k++) F[k] = 0.0;
+    // entries in batches of four: entry loads, force loads, then the sums in order
     const int beg = N.rowptr[node], end = N.rowptr[node + 1];
-    for (int j = beg; j < end; j++) {
-        const uint32_t e = N.ent[j];
-        const int64_t idx = e & kEntIdxMask;
-        const int a = (e >> kEntLocalShift) & 7;
+    for (int j0 = beg; j0 < end; j0 += 4) {
+        uint32_t e[4];
 #pragma unroll
-        for (int k = 0; k < 12; k++) F[k] = F[k] + (double)efp[(a * 12 + k) * npe + idx];
+        for (int k = 0; k < 4; k++) e[k] = j0 + k < end ? N.ent[j0 + k] : 0u;
+        R f[4][12];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            if (j0 + k < end) {
+                const int64_t idx = e[k] & kEntIdxMask;
+                const int a = (e[k] >> kEntLocalShift) & 7;
+#pragma unroll
+                for (int c = 0; c < 12; c++) f[k][c] = efp[(a * 12 + c) * npe + idx];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            if (j0 + k < end) {
+#pragma unroll
+                for (int c = 0; c < 12; c++) F[c] = F[c] + (double)f[k][c];
+            }
+        }
     }
     const double m = N.mass[node];
     if (mode == 1) {
@@ -1160,15 +1215,25 @@ hipError_t launch_elem_pml(const PmlElems<R> &E, const NodeState &N, const StepC
 {
     if (E.n == 0) return hipSuccess;
     const unsigned g = grid_for(E.n, 256);
+    const size_t lds = (E.optab && E.phitab) ? sizeof(R) * ((size_t)E.n_opc * kOpW + (size_t)E.n_phic * 32) : 0;
+    const bool use_lds = lds > 0 && lds <= 16384;
+#define EQD_LAUNCH(HG, MASS)                                                                          \
+    do {                                                                                              \
+        if (use_lds)                                                                                  \
+            elem_pml_kernel<R, HG, MASS, true><<<g, 256, lds, s>>>(E, N, C);                          \
+        else                                                                                          \
+            elem_pml_kernel<R, HG, MASS, false><<<g, 256, 0, s>>>(E, N, C);                           \
+    } while (0)
     if (mass) {
-        if (hg == 1) elem_pml_kernel<R, 1, true><<<g, 256, 0, s>>>(E, N, C);
-        else if (hg == 2) elem_pml_kernel<R, 2, true><<<g, 256, 0, s>>>(E, N, C);
-        else elem_pml_kernel<R, 0, true><<<g, 256, 0, s>>>(E, N, C);
+        if (hg == 1) EQD_LAUNCH(1, true);
+        else if (hg == 2) EQD_LAUNCH(2, true);
+        else EQD_LAUNCH(0, true);
     } else {
-        if (hg == 1) elem_pml_kernel<R, 1, false><<<g, 256, 0, s>>>(E, N, C);
-        else if (hg == 2) elem_pml_kernel<R, 2, false><<<g, 256, 0, s>>>(E, N, C);
-        else elem_pml_kernel<R, 0, false><<<g, 256, 0, s>>>(E, N, C);
+        if (hg == 1) EQD_LAUNCH(1, false);
+        else if (hg == 2) EQD_LAUNCH(2, false);
+        else EQD_LAUNCH(0, false);
     }
+#undef EQD_LAUNCH
     return hipGetLastError();
 }
 
