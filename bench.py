@@ -1,21 +1,27 @@
 #!/usr/bin/env python3
 """Throughput of the EQdyna explicit time step on one MI355X.
 
-Workload (default): BASELINE config 2 at full size -- 480 x 240 x 240 =
-27,648,000 hexahedra at 50 m with a 10-element PML, a seeded five-layer medium,
-the reference's Q model and a slip-weakening split-node fault -- built by
-eqdyna_amd.mesh and stepped by libeqd.so.  A "step" is one explicit time step
-(src/driver.f90:9-34) over the whole mesh, with the mesh resident in HBM.
+Default workload (c4): BASELINE config 4 -- config 3 (240 x 120 x 120 =
+3,456,000 hexahedra at 100 m with a 10-element PML, a seeded fractal rough
+fault, slip-law rate-and-state friction with strong velocity weakening,
+dt = 0.005 s, 3000 steps) plus the learned initial guess of the
+rate-and-state Newton solve -- run twice on the GPU: the plain FP64 mode
+(the reference's arithmetic and Newton solve) and the FP32-bulk mode with
+the learned guess (eqdyna_amd/rsf_guess_c3.npz).  `value` is the FP32+learned
+throughput; the FP64 throughput, their ratio and the FP32+learned run's
+rupture-time and peak-slip-rate errors against the FP64 run are in `config`.
+Other workloads (c1, c2, c3) time one run in the --precision given.
 
-Prints one JSON line:
+A "step" is one explicit time step (src/driver.f90:9-34) over the whole mesh,
+with the mesh resident in HBM.  Prints one JSON line:
   value       Mdof-steps/s = 3 * N_nodes * steps / t / 1e6 (N_nodes = grid nodes +
               split master nodes; PML split components are not counted)
-  roofline    the element-force kernel: algorithmic bytes per launch (stored-
-              operator model, per regular hexahedron: IEN 32 + dN/dx 192 +
-              det 8 + lambda,mu 16 + stress R+W 96 + SS 48 + Phi 256 = 648 B in
-              FP64, 340 B in FP32; + 96/48 B of Q memory variables) divided by
-              its average launch time, measured with HIP events on the library's
-              stream over the timed steps; peak = 8 TB/s HBM3E.
+  roofline    the run's dominant kernel (largest total time): algorithmic bytes
+              per launch (stored-operator model, per unit: regular hexahedron
+              648 B FP64 / 340 B FP32 (+96/48 B Q memory variables), PML
+              hexahedron 912 / 470 B, 3-component node 200 / 152 B, PML node
+              530 / 300 B) divided by its average launch time, measured with HIP
+              events on the library's stream over the timed steps; peak = 8 TB/s.
   cpu_baseline the reference Fortran (oracle/_ref/eqd_ref) on MPI ranks on the
               host's cores, on a bounded sample of the same workload.
 """
@@ -24,8 +30,6 @@ from __future__ import annotations
 import argparse
 import json
 import os
-import re
-import subprocess
 import sys
 import tempfile
 import time
@@ -35,8 +39,17 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Mdof-steps/s: plain FP64 vs FP32+learned RSF guess at equal rupture-time error"
 HBM_PEAK_GBS = 8000.0
-ELEM_BYTES = {"fp64": 648, "fp32": 340}
-ELEM_BYTES_Q = {"fp64": 96, "fp32": 48}
+# algorithmic bytes per unit of each kernel (stored-operator model)
+UNIT_BYTES = {
+    "element": {"fp64": 648, "fp32": 340},
+    "element_pml": {"fp64": 912, "fp32": 470},
+    "node": {"fp64": 200, "fp32": 152},
+    "node_pml": {"fp64": 530, "fp32": 300},
+}
+Q_BYTES = {"fp64": 96, "fp32": 48}
+KERNEL_NAMES = {"element": "elem_regular_kernel", "element_pml": "elem_pml_kernel", "node": "node3_kernel",
+                "node_pml": "node_pml_kernel"}
+GUESS = {"k_fallback": 6, "gate": 0.5}
 
 
 def log(msg: str) -> None:
@@ -52,10 +65,13 @@ def build_workload(name: str, steps_total: int):
     elif name == "c1":
         c = case.c1(steps=steps_total)
         desc = "BASELINE config 1: 120x60x60 = 432,000 hexahedra at 200 m incl. 10-element PML, homogeneous, SW fault"
-    elif name == "c3":
+    elif name in ("c3", "c4"):
         c = case.c3(steps=steps_total)
         desc = ("BASELINE config 3: 240x120x120 = 3,456,000 hexahedra at 100 m incl. 10-element PML, seeded fractal "
                 "rough fault (rms/length 0.005, H=1), slip-law RSF with strong velocity weakening, seeded a field")
+        if name == "c4":
+            desc = ("BASELINE config 4: config 3 plus the learned RSF Newton guess (6-64-64-64-1 GELU, bf16 MFMA), "
+                    "FP64 reference mode vs FP32 bulk + learned guess; " + desc[len("BASELINE config 3: "):])
     else:
         raise SystemExit(f"unknown workload {name}")
     t = time.perf_counter()
@@ -65,28 +81,81 @@ def build_workload(name: str, steps_total: int):
     return c, bundle, sc, desc
 
 
-def load_traffic(workload: str, precision: str):
-    p = os.path.join(ROOT, "profiles", "element_traffic.json")
+def load_traffic(workload: str, precision: str, kernel: str):
+    p = os.path.join(ROOT, "profiles", "kernel_traffic.json")
     if not os.path.exists(p):
         return None
     try:
-        d = json.load(open(p))
-        e = d.get(f"{workload}_{precision}")
-        if not e:
-            return None
-        return float(e["bytes_per_launch"])
-    except Exception:
+        e = json.load(open(p)).get(f"{workload}_{precision}_{kernel}")
+        return float(e["bytes_per_launch"]) if e else None
+    except Exception:  # noqa: BLE001
         return None
 
 
-def cpu_baseline_reference(ranks: int, steps: int):
-    """Reference Fortran on `ranks` MPI ranks, C2 physics on a bounded sub-box."""
+def run_one(bundle, sc, precision: str, warmup: int, steps: int, guess=None):
+    """One timed run: warm-up steps, then `steps` timed with per-kernel events."""
+    from eqdyna_amd.solver import Solver
+    t = time.perf_counter()
+    s = Solver(bundle, sc, precision, device=0, nstep=warmup + steps)
+    if guess is not None:
+        s.set_guess(guess, GUESS["k_fallback"], GUESS["gate"])
+    layout = s.layout_info()
+    log(f"{precision}{'+learned' if guess is not None else ''}: uploaded ({s.device_bytes() / 1e9:.1f} GB on the "
+        f"device) in {time.perf_counter() - t:.1f} s; layout {layout}")
+    s.run(warmup)
+    s.set_timing(True)
+    t0 = time.perf_counter()
+    s.run(steps)  # returns after the stream has drained
+    wall = time.perf_counter() - t0
+    ktimes = {k: s.kernel_time(k) for k in ("element", "element_pml", "fault", "node", "node_pml", "station")}
+    r = {"wall": wall, "ktimes": ktimes, "layout": layout, "nt": s.clock()[0], "newton": s.newton_stats(),
+         "fnft": s.get("fnft"), "fric": s.get("fric")}
+    s.close()
+    for k, (ms, n) in ktimes.items():
+        log(f"  kernel {k}: {ms:.4f} ms avg over {n} launches")
+    return r
+
+
+def roofline(run, counts, precision: str, q: bool, workload: str):
+    """Roofline of the run's dominant kernel (largest total time)."""
+    tot = {k: ms * n for k, (ms, n) in run["ktimes"].items() if k in UNIT_BYTES}
+    k = max(tot, key=tot.get)
+    ms = run["ktimes"][k][0]
+    per = UNIT_BYTES[k][precision] + (Q_BYTES[precision] if (q and k == "element") else 0)
+    alg = per * counts[k]
+    achieved = alg / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": load_traffic(workload, precision, k),
+            "kernel": KERNEL_NAMES[k], "algorithmic_bytes_per_unit": per, "units_per_launch": int(counts[k]),
+            "algorithmic_bytes_per_launch": int(alg), "avg_launch_ms": round(ms, 5)}
+
+
+def errors_vs(run, ref, dt):
+    import numpy as np
+    a, b = run["fnft"], ref["fnft"]
+    both = (a < 5000) & (b < 5000)
+    d = np.abs(a[both] - b[both]) / dt if both.any() else np.zeros(1)
+    pk_a, pk_b = run["fric"][:, 75], ref["fric"][:, 75]
+    return {"rupture_time_mean_dt": float(d.mean()), "rupture_time_max_dt": float(d.max()),
+            "ruptured_nodes": int((b < 5000).sum()), "rupture_flips": int(((a < 5000) != (b < 5000)).sum()),
+            "peak_slip_rate_rel": float(np.max(np.abs(pk_a - pk_b)) / max(float(np.max(pk_b)), 1e-300))}
+
+
+def cpu_baseline_reference(workload: str, ranks: int, steps: int):
+    """Reference Fortran on `ranks` MPI ranks on a bounded sample of the workload."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import refrun
     from eqdyna_amd import case, mesh
     if not refrun.available() or not os.path.exists(refrun.MPIEXEC):
         return None
-    c = case.c2_sample(steps=steps)
+    if workload in ("c3", "c4"):
+        c = case.c3(steps=steps)
+        sample = (f"config 3 at full size (3,456,000 hexahedra, rough fault, slip-law RSF), first {steps} of 3000 "
+                  f"steps, plain Newton guess")
+    else:
+        c = case.c2_sample(steps=steps)
+        sample = (f"C2 physics (50 m, 5-layer medium, Q, SW fault) on a 240x120x120 = 3,456,000-hexahedron "
+                  f"sub-box, {steps} steps")
     nn = mesh.mesh_sizes(c).nn  # one-rank node count: grid nodes + split masters
     fac = {1: (1, 1, 1), 2: (2, 1, 1), 4: (2, 2, 1), 8: (2, 2, 2), 16: (4, 2, 2), 32: (4, 4, 2)}
     if ranks not in fac:
@@ -96,11 +165,8 @@ def cpu_baseline_reference(ranks: int, steps: int):
         t = time.perf_counter()
         _, _, _, t_loop = refrun.run_reference(c, d, ranks, dump=False, timeout=900)
         log(f"reference on {ranks} ranks: t_loop {t_loop:.2f} s ({time.perf_counter() - t:.1f} s with meshing)")
-    return {
-        "value": 3.0 * nn * steps / t_loop / 1e6, "unit": "Mdof-steps/s", "cores": ranks, "kind": "reference",
-        "sample": f"C2 physics (50 m, 5-layer medium, Q, SW fault) on a 240x120x120 = 3,456,000-hexahedron "
-                  f"sub-box, {steps} steps, reference Fortran (amdflang -O2) on {ranks} MPI ranks, time loop only",
-    }
+    return {"value": 3.0 * nn * steps / t_loop / 1e6, "unit": "Mdof-steps/s", "cores": ranks, "kind": "reference",
+            "sample": sample + f", reference Fortran (amdflang -O2) on {ranks} MPI ranks, time loop only"}
 
 
 def cpu_baseline_port(steps: int):
@@ -135,10 +201,10 @@ def calibrate_copy(nbytes: int) -> int:
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (c4: 2990, others: 200)")
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--workload", default="c2", choices=["c2", "c1", "c3"])
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
+    ap.add_argument("--workload", default="c4", choices=["c4", "c3", "c2", "c1"])
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"], help="c1-c3 only")
     ap.add_argument("--cpu-baseline", default="auto", choices=["auto", "reference", "port", "none"])
     ap.add_argument("--cpu-ranks", type=int, default=16)
     ap.add_argument("--cpu-steps", type=int, default=60)
@@ -151,57 +217,67 @@ def main() -> int:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}). "
               "The halo-exchange design for several GPUs is described in DESIGN.md.", file=sys.stderr)
         return 2
-
-    from eqdyna_amd.solver import Solver
-    total = args.warmup + args.steps
+    steps = args.steps if args.steps is not None else (2990 if args.workload == "c4" else 200)
+    total = args.warmup + steps
     c, bundle, sc, desc = build_workload(args.workload, total)
-    n_reg = int((bundle["elemTypeArr"] != 2).sum())
+    et = bundle["elemTypeArr"]
+    ndof = bundle["numOfDofPerNodeArr"]
+    counts = {"element": int((et != 2).sum()), "element_pml": int((et == 2).sum()),
+              "node_pml": int((ndof == 12).sum()),
+              # split pairs are updated by the fault kernel, not the node kernel
+              "node": int((ndof == 3).sum()) - 2 * int(sc["nftnd"])}
     nn = int(sc["totalNumOfNodes"])
-    t = time.perf_counter()
-    s = Solver(bundle, sc, args.precision, device=0, nstep=total)
-    layout = s.layout_info()
-    log(f"uploaded ({s.device_bytes() / 1e9:.1f} GB on the device) in {time.perf_counter() - t:.1f} s; layout {layout}")
+    extra = {}
+    if args.workload == "c4":
+        from eqdyna_amd import learned
+        params, meta = learned.load()
+        ref = run_one(bundle, sc, "fp64", args.warmup, steps)
+        run = run_one(bundle, sc, "fp32", args.warmup, steps, guess=params)
+        precision = "fp32"
+        v64 = 3.0 * nn * steps / ref["wall"] / 1e6
+        n64, n32 = ref["newton"], run["newton"]
+        extra = {
+            "fp64_value": round(v64, 2), "fp64_ms_per_step": round(ref["wall"] / steps * 1e3, 4),
+            "ratio_fp32_learned_vs_fp64": None,
+            "errors_vs_fp64": errors_vs(run, ref, sc["dt"]),
+            "newton_fp64": {**n64, "iterations_per_solve": round(n64["iterations"] / max(n64["solves"], 1), 3)},
+            "newton_fp32_learned": {**n32, "iterations_per_solve": round(n32["iterations"] / max(n32["solves"], 1), 3),
+                                    "fallback_rate": round(n32["fallbacks"] / max(n32["solves"], 1), 5)},
+            "guess": {**GUESS, "weights": "eqdyna_amd/rsf_guess_c3.npz", "training_rows": meta.get("rows")},
+        }
+    else:
+        precision = args.precision
+        run = run_one(bundle, sc, precision, args.warmup, steps)
     del bundle
-    s.run(args.warmup)
-    s.set_timing(True)
-    t0 = time.perf_counter()
-    s.run(args.steps)  # returns after the stream has drained
-    t1 = time.perf_counter()
-    wall = t1 - t0
-    ktimes = {k: s.kernel_time(k) for k in ("element", "element_pml", "fault", "node", "node_pml", "station")}
-    nt, _ = s.clock()
-    s.close()
-    for k, (ms, n) in ktimes.items():
-        log(f"kernel {k}: {ms:.4f} ms avg over {n} launches")
-    value = 3.0 * nn * args.steps / wall / 1e6
-    elem_ms = ktimes["element"][0]
-    per_elem = ELEM_BYTES[args.precision] + (ELEM_BYTES_Q[args.precision] if c.C_Q == 1 else 0)
-    alg_bytes = per_elem * n_reg
-    achieved = alg_bytes / (elem_ms * 1e-3) / 1e9 if elem_ms > 0 else 0.0
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": load_traffic(args.workload, args.precision),
-                "kernel": "elem_regular_kernel", "algorithmic_bytes_per_launch": alg_bytes,
-                "avg_launch_ms": round(elem_ms, 5)}
+    value = 3.0 * nn * steps / run["wall"] / 1e6
+    if extra:
+        extra["ratio_fp32_learned_vs_fp64"] = round(value / extra["fp64_value"], 4)
+    roof = roofline(run, counts, precision, c.C_Q == 1, args.workload)
 
     cpu = None
     if args.cpu_baseline in ("auto", "reference"):
         try:
-            cpu = cpu_baseline_reference(args.cpu_ranks, args.cpu_steps)
+            cpu = cpu_baseline_reference(args.workload, args.cpu_ranks, args.cpu_steps)
         except Exception as e:  # noqa: BLE001
             log(f"reference CPU baseline failed: {e}")
             cpu = None
     if cpu is None and args.cpu_baseline in ("auto", "port"):
         cpu = cpu_baseline_port(20)
 
+    if precision == "fp64":
+        dtype = "f64"
+    else:
+        dtype = "f32 bulk / f64 fault" + (" / bf16 guess" if args.workload == "c4" else "")
     out = {
-        "metric": METRIC, "value": round(value, 2), "unit": "Mdof-steps/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": round(wall / args.steps * 1e3, 4), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f64" if args.precision == "fp64" else "f32 bulk / f64 fault",
+        "metric": METRIC, "value": round(value, 2), "unit": "Mdof-steps/s", "n_gpus": 1, "steps": steps,
+        "warmup": args.warmup, "ms_per_step": round(run["wall"] / steps * 1e3, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": dtype,
         "data": "synthetic (seeded case generator; no external data)",
         "config": {"workload": desc, "elements": int(sc["totalNumOfElements"]), "nodes": nn,
-                   "regular_elements": n_reg, "fault_pairs": int(sc["nftnd"]), "precision": args.precision,
-                   "steps_run": nt, "operator_classes": {k: v for k, v in layout.items() if "classes" in k}},
-        "roofline": roofline, "cpu_baseline": cpu,
+                   "regular_elements": counts["element"], "fault_pairs": int(sc["nftnd"]), "precision": precision,
+                   "steps_run": run["nt"],
+                   "operator_classes": {k: v for k, v in run["layout"].items() if "classes" in k}, **extra},
+        "roofline": roof, "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
     return 0

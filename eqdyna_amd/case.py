@@ -649,17 +649,26 @@ def rough_fractal(c: Case, alpha: float, hurst: float, seed: int) -> None:
     c.rough = np.ascontiguousarray(rows)
 
 
-def c3(steps: int = 3000, seed: int = 3) -> Case:
+def c3(steps: int = 3000, seed: int = 3, tau0: float = 10.5e6, vs_top: float = 2000.0, sn: float = -30.0e6,
+       nucdtau0: float = 18.0e6) -> Case:
     """BASELINE config 3 at 100 m: 240 x 120 x 120 = 3,456,000 hexahedra
     including a 10-element PML; homogeneous rock; a seeded fractal rough
     fault (rms height / length 0.005, H = 1) 16 km along strike and 9.8 km
     down dip; slip law with strong velocity weakening (friclaw 4): a seeded
-    in [0.008, 0.016] (0.02 in a 1 km velocity-strengthening rim on the
-    sides and bottom), b = 0.012, L = 0.02 m, f0 = 0.6, V0 = 1e-6 m/s,
-    fw = 0.1, Vw = 0.1 m/s; sigma_n = -30 MPa (inside the rough-fault cap of
-    [-40, -10] MPa), tau_0 = 18.75 MPa with the state that makes the initial
-    slip rate 1e-12 m/s, TPV104 smooth nucleation (3 km, 1 s, 15 MPa);
-    dt = 0.005 s."""
+    in [0.008, 0.016] (0.02 in a velocity-strengthening rim: 1 km on the
+    sides and bottom, the top vs_top metres), b = 0.012, L = 0.02 m,
+    f0 = 0.6, V0 = 1e-6 m/s, fw = 0.1, Vw = 0.1 m/s; sigma_n = -30 MPa
+    (inside the rough-fault cap of [-40, -10] MPa), tau_0 = 10.5 MPa with
+    the state that makes the initial slip rate 1e-12 m/s, TPV104 smooth
+    nucleation (3 km, 1 s, 18 MPa); dt = 0.005 s.
+
+    Stability: the reference updates the normal-stress state theta_pc
+    explicitly (src/faulting.f90:35-52), which diverges where V*dt/L > 2
+    (V > 8 m/s here) while the normal stress moves, as it does on a rough
+    fault.  With tau_0 = 18.75 MPa and no surface layer the reference's own
+    run (and this one, identically) ends in NaN at step 369; the stress
+    level and the 2 km velocity-strengthening top above keep the whole
+    3000 steps finite."""
     c = _strike_slip_box(100.0, 10, "c3")
     c.xmin, c.xmax = -11.0e3, 11.0e3
     c.ymin, c.ymax = -5.0e3, 5.0e3
@@ -674,17 +683,16 @@ def c3(steps: int = 3000, seed: int = 3) -> Case:
     c.C_nuclea = 1
     c.insertFaultType = 1
     c.xsource, c.ysource, c.zsource = 0.0, 0.0, -6.0e3
-    c.nucR, c.nucT, c.nucdtau0 = 3.0e3, 1.0, 15.0e6
+    c.nucR, c.nucT, c.nucdtau0 = 3.0e3, 1.0, nucdtau0
     rough_fractal(c, 0.005, 1.0, seed)
     rng = np.random.default_rng(seed + 1000)
     o = c.new_ofv()
     b, L, f0, v0, fw, vw, vini = 0.012, 0.02, 0.6, 1.0e-6, 0.1, 0.1, 1.0e-12
-    sn, tau0 = -30.0e6, 18.75e6
     a_field = 0.008 + 0.008 * rng.random((c.nfz, c.nfx))
     for ix, x in enumerate(c.fx):
         for iz, z in enumerate(c.fz):
             a = a_field[iz, ix]
-            if abs(x) > c.fxmax - 1.0e3 - 0.01 or z < c.fzmin + 1.0e3 + 0.01:
+            if abs(x) > c.fxmax - 1.0e3 - 0.01 or z < c.fzmin + 1.0e3 + 0.01 or z > -vs_top - 0.01:
                 a = 0.02
             o[1, iz, ix], o[2, iz, ix], o[3, iz, ix] = 0.6, 0.45, 0.3
             o[7, iz, ix] = sn

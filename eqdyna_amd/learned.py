@@ -30,7 +30,7 @@ import numpy as np
 
 from . import case as cases
 from . import mesh
-from .solver import MLP_NPARAMS, MLP_NTRAIN, Solver, mlp_train
+from .solver import MLP_NPARAMS, EqdError, Solver, mlp_train
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 DEFAULT_WEIGHTS = os.path.join(HERE, "rsf_guess_c3.npz")
@@ -67,11 +67,11 @@ def init_params(X: np.ndarray, r: np.ndarray, seed: int) -> np.ndarray:
 
 def training_case(seed: int, steps: int) -> cases.Case:
     """A seeded variant of config 3: its own fractal surface and a field, a
-    uniform initial shear in [17, 20.5] MPa and a hypocentre along strike in
+    uniform initial shear in [9.5, 10.5] MPa and a hypocentre along strike in
     [-3, 3] km."""
     c = cases.c3(steps=steps, seed=seed)
     rng = np.random.default_rng(10_000 + seed)
-    tau0 = float(rng.uniform(17.0e6, 20.5e6))
+    tau0 = float(rng.uniform(9.5e6, 10.5e6))
     c.xsource = float(np.round(rng.uniform(-3.0e3, 3.0e3), -2))
     o = c.ofv
     for ix in range(c.nfx):
@@ -83,18 +83,31 @@ def training_case(seed: int, steps: int) -> cases.Case:
     return c
 
 
-def harvest(case_list: List[cases.Case], every: int, precision: str = "fp32") -> Tuple[np.ndarray, np.ndarray]:
-    """Run each case with the plain guess and collect (inputs, converged ln V)."""
+def harvest(case_list: List[cases.Case], every: int, precision: str = "fp32",
+            want: int | None = None) -> Tuple[np.ndarray, np.ndarray]:
+    """Run each case with the plain guess and collect (inputs, converged ln V).
+    A case whose run ends in a NaN velocity (the reference's explicit
+    normal-stress state can diverge at high slip rate, see case.c3) is
+    dropped; with want, cases are used until that many have run through."""
     Xs, ys = [], []
+    used = 0
     for c in case_list:
+        if want is not None and used >= want:
+            break
         t = time.perf_counter()
         b, sc = mesh.build_case(c)
         s = Solver(b, sc, precision)
         nstep = int(sc["nstep"])
         s.harvest(every, int(sc["nftnd"]) * (nstep // every))
-        s.run(nstep)
+        try:
+            s.run(nstep)
+        except EqdError as e:
+            log(f"{c.name}: dropped ({e})")
+            s.close()
+            continue
         rows = s.harvest_get()
         s.close()
+        used += 1
         ok = np.all(np.isfinite(rows), axis=1)
         Xs.append(rows[ok, :NIN])
         ys.append(rows[ok, NIN])
@@ -192,12 +205,12 @@ def main(argv=None) -> int:
     ap.add_argument("--k-fallback", type=int, default=6)
     a = ap.parse_args(argv)
     t = time.perf_counter()
-    X, lnv = harvest([training_case(100 + k, a.steps) for k in range(a.seeds)], a.every)
+    X, lnv = harvest([training_case(100 + k, a.steps) for k in range(3 * a.seeds)], a.every, want=a.seeds)
     log(f"harvested {len(X)} rows in {time.perf_counter() - t:.1f} s")
     t = time.perf_counter()
     p, losses = train(X, lnv, a.epochs, a.batch, a.lr, seed=7)
     log(f"trained {a.epochs} epochs in {time.perf_counter() - t:.1f} s; loss per epoch {np.round(losses, 5).tolist()}")
-    meta = {"rows": int(len(X)), "seeds": list(range(100, 100 + a.seeds)), "steps": a.steps, "every": a.every,
+    meta = {"rows": int(len(X)), "cases": a.seeds, "first_seed": 100, "steps": a.steps, "every": a.every,
             "epochs": a.epochs, "batch": a.batch, "lr": a.lr, "loss": [float(x) for x in losses]}
     ab = ab_test(cases.c3(steps=a.eval_steps), p, a.k_fallback, a.gate)
     meta["ab_test_c3_seed3"] = ab
