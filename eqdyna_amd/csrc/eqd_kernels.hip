@@ -122,6 +122,20 @@ __global__ void __launch_bounds__(256) elem_regular_kernel(RegElems<R> E, NodeSt
     R qv[3][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0, 0}};    // kHG 2: sum_a v_a Gamma_ja
     constexpr int fi[4][8] = {{1, 1, -1, -1, -1, -1, 1, 1}, {1, -1, -1, 1, -1, 1, 1, -1},
                               {1, -1, 1, -1, 1, -1, 1, -1}, {1, -1, 1, -1, -1, 1, -1, 1}};
+    // a node's operator values (dN/dx, and Phi of the four modes), loaded one
+    // node ahead like its v and u
+    constexpr int kNo = kHG == 1 ? 7 : 3;
+    auto load_op = [&](int a, R o[kNo]) {
+#pragma unroll
+        for (int k = 0; k < 3; k++) o[k] = SHP(a * 3 + k);
+        if constexpr (kHG == 1) {
+#pragma unroll
+            for (int m = 0; m < 4; m++) o[3 + m] = PHI(m, a);
+        }
+    };
+    // (from LDS the values are read where they are used)
+    R opc_[kNo], opn_[kNo];
+    if constexpr (!kLds) load_op(0, opc_);
 #pragma unroll EQD_PREFETCH
     for (int a = 0; a < 8; a++) {
         const int b = a % P;
@@ -139,7 +153,12 @@ __global__ void __launch_bounds__(256) elem_regular_kernel(RegElems<R> E, NodeSt
                 uq[b][k] = N.disp[k * nn + nx];
             }
         }
-        const R n1 = SHP(a * 3), n2 = SHP(a * 3 + 1), n3 = SHP(a * 3 + 2);
+        if constexpr (kLds) {
+            load_op(a, opc_);
+        } else {
+            if (a < 7) load_op(a + 1, opn_);
+        }
+        const R n1 = opc_[0], n2 = opc_[1], n3 = opc_[2];
         sr[0] = sr[0] + n1 * vl[0];
         sr[1] = sr[1] + n2 * vl[1];
         sr[2] = sr[2] + n3 * vl[2];
@@ -160,7 +179,7 @@ __global__ void __launch_bounds__(256) elem_regular_kernel(RegElems<R> E, NodeSt
             for (int k = 0; k < 3; k++) ut[k] = dl[k] + rdk * vl[k];
 #pragma unroll
             for (int m = 0; m < 4; m++) {
-                const R ph = PHI(m, a);
+                const R ph = opc_[3 + m];
 #pragma unroll
                 for (int k = 0; k < 3; k++) pd[m][k] = pd[m][k] + ph * ut[k];
             }
@@ -169,6 +188,10 @@ __global__ void __launch_bounds__(256) elem_regular_kernel(RegElems<R> E, NodeSt
             for (int i = 0; i < 3; i++)
 #pragma unroll
                 for (int j = 0; j < 4; j++) qv[i][j] = qv[i][j] + vl[i] * (R)fi[j][a];
+        }
+        if constexpr (!kLds) {
+#pragma unroll
+            for (int k = 0; k < kNo; k++) opc_[k] = opn_[k];
         }
     }
     const R lam = OPK(kOpLam, E.lam), miu = OPK(kOpMu, E.mu);
@@ -272,9 +295,15 @@ __global__ void __launch_bounds__(256) elem_regular_kernel(RegElems<R> E, NodeSt
         const R rho = OPK(kOpRho, E.rho), vp = OPK(kOpVp, E.vp);
         coef = (R)0.25 * (R)C.kapa_hg * rho * vp * (R)pow((double)det * C.w, 2.0 / 3.0);
     }
-    // pass 2: one node's force at a time
+    // pass 2: one node's force at a time (operators again one node ahead)
+    if constexpr (!kLds) load_op(0, opc_);
 #pragma unroll EQD_ELEM_UNROLL
     for (int a = 0; a < 8; a++) {
+        if constexpr (kLds) {
+            load_op(a, opc_);
+        } else {
+            if (a < 7) load_op(a + 1, opn_);
+        }
         R f[3] = {0, 0, 0};
         if (kMass) {
             const int nd = nds[a];
@@ -286,7 +315,7 @@ __global__ void __launch_bounds__(256) elem_regular_kernel(RegElems<R> E, NodeSt
                 f[k] = (R)0 - al * E.emass[(a * 3 + k) * n + t];
             }
         }
-        const R n1 = SHP(a * 3), n2 = SHP(a * 3 + 1), n3 = SHP(a * 3 + 2);
+        const R n1 = opc_[0], n2 = opc_[1], n3 = opc_[2];
         f[0] = f[0] + (n1 * st[0] + n3 * st[4] + n2 * st[5]);
         f[1] = f[1] + (n2 * st[1] + n3 * st[3] + n1 * st[5]);
         f[2] = f[2] + (n3 * st[2] + n2 * st[3] + n1 * st[4]);
@@ -294,7 +323,7 @@ __global__ void __launch_bounds__(256) elem_regular_kernel(RegElems<R> E, NodeSt
             // Kosloff-Frazier stiffness hourglass, src/hrglss.f90:20-55
 #pragma unroll
             for (int m = 0; m < 4; m++) {
-                const R ph = PHI(m, a);
+                const R ph = opc_[3 + m];
 #pragma unroll
                 for (int k = 0; k < 3; k++) f[k] = f[k] - ph * g[m][k];
             }
@@ -310,6 +339,10 @@ __global__ void __launch_bounds__(256) elem_regular_kernel(RegElems<R> E, NodeSt
         }
 #pragma unroll
         for (int k = 0; k < 3; k++) E.ef[(a * 3 + k) * n + t] = f[k];
+        if constexpr (!kLds) {
+#pragma unroll
+            for (int k = 0; k < kNo; k++) opc_[k] = opn_[k];
+        }
     }
 }
 
@@ -384,9 +417,26 @@ __global__ void __launch_bounds__(256) elem_pml_kernel(PmlElems<R> E, NodeState 
     R Dx_vx = 0, Dy_vy = 0, Dz_vz = 0, Dx_vy = 0, Dy_vx = 0, Dx_vz = 0, Dz_vx = 0, Dy_vz = 0, Dz_vy = 0;
     R pd[4][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
     R qv[3][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0, 0}};
+    // a node's operator values, one node ahead when they come from memory
+    constexpr int kNo = kHG == 1 ? 7 : 3;
+    auto load_op = [&](int a, R o[kNo]) {
+#pragma unroll
+        for (int k = 0; k < 3; k++) o[k] = SHP(a * 3 + k);
+        if constexpr (kHG == 1) {
+#pragma unroll
+            for (int m = 0; m < 4; m++) o[3 + m] = PHI(m, a);
+        }
+    };
+    R opc_[kNo], opn_[kNo];
+    if constexpr (!kLds) load_op(0, opc_);
 #pragma unroll EQD_PREFETCH
     for (int a = 0; a < 8; a++) {
         const int b = a % P;
+        if constexpr (kLds) {
+            load_op(a, opc_);
+        } else {
+            if (a < 7) load_op(a + 1, opn_);
+        }
         R va[3];
         R dlh[3];
 #pragma unroll
@@ -408,7 +458,7 @@ __global__ void __launch_bounds__(256) elem_pml_kernel(PmlElems<R> E, NodeState 
                 if (kHG == 1) uq[b][k] = N.disp[k * nn + nx];
             }
         }
-        const R n1 = SHP(a * 3), n2 = SHP(a * 3 + 1), n3 = SHP(a * 3 + 2);
+        const R n1 = opc_[0], n2 = opc_[1], n3 = opc_[2];
         sr[0] = sr[0] + n1 * va[0];
         sr[1] = sr[1] + n2 * va[1];
         sr[2] = sr[2] + n3 * va[2];
@@ -427,7 +477,7 @@ __global__ void __launch_bounds__(256) elem_pml_kernel(PmlElems<R> E, NodeState 
         if (kHG == 1) {
 #pragma unroll
             for (int m = 0; m < 4; m++) {
-                const R ph = PHI(m, a);
+                const R ph = opc_[3 + m];
 #pragma unroll
                 for (int k = 0; k < 3; k++) pd[m][k] = pd[m][k] + ph * dlh[k];
             }
@@ -436,6 +486,10 @@ __global__ void __launch_bounds__(256) elem_pml_kernel(PmlElems<R> E, NodeState 
             for (int i = 0; i < 3; i++)
 #pragma unroll
                 for (int j = 0; j < 4; j++) qv[i][j] = qv[i][j] + va[i] * (R)fi[j][a];
+        }
+        if constexpr (!kLds) {
+#pragma unroll
+            for (int k = 0; k < kNo; k++) opc_[k] = opn_[k];
         }
     }
     const R lam = OPK(kOpLam, E.lam), miu = OPK(kOpMu, E.mu), det = OPK(kOpDet, E.det);
@@ -496,14 +550,20 @@ __global__ void __launch_bounds__(256) elem_pml_kernel(PmlElems<R> E, NodeState 
         coef2 = (R)0.25 * (R)C.kapa_hg * rho * vp * (R)pow((double)det * C.w, 2.0 / 3.0);
     }
 
+    if constexpr (!kLds) load_op(0, opc_);
 #pragma unroll EQD_ELEM_UNROLL
     for (int a = 0; a < 8; a++) {
+        if constexpr (kLds) {
+            load_op(a, opc_);
+        } else {
+            if (a < 7) load_op(a + 1, opn_);
+        }
         // hourglass force on this node, subtracted mode by mode
         R hg[3] = {0, 0, 0};
         if (kHG == 1) {
 #pragma unroll
             for (int m = 0; m < 4; m++) {
-                const R ph = PHI(m, a);
+                const R ph = opc_[3 + m];
 #pragma unroll
                 for (int k = 0; k < 3; k++) hg[k] = hg[k] - ph * g[m][k];
             }
@@ -516,7 +576,7 @@ __global__ void __launch_bounds__(256) elem_pml_kernel(PmlElems<R> E, NodeState 
                 hg[i] = fv;
             }
         }
-        const R g1 = SHP(a * 3), g2 = SHP(a * 3 + 1), g3 = SHP(a * 3 + 2);
+        const R g1 = opc_[0], g2 = opc_[1], g3 = opc_[2];
         R f[12];
         if (kMass) {
             const int nd = nds[a];
@@ -555,6 +615,10 @@ __global__ void __launch_bounds__(256) elem_pml_kernel(PmlElems<R> E, NodeState 
             E.ef[(a * 12 + 0) * n + p] = f[0] + f[1] + f[2] + f[9] + hg[0];
             E.ef[(a * 12 + 1) * n + p] = f[3] + f[4] + f[5] + f[10] + hg[1];
             E.ef[(a * 12 + 2) * n + p] = f[6] + f[7] + f[8] + f[11] + hg[2];
+        }
+        if constexpr (!kLds) {
+#pragma unroll
+            for (int k = 0; k < kNo; k++) opc_[k] = opn_[k];
         }
     }
 }
