@@ -293,6 +293,56 @@ static bool classify_rows(int64_t n, int width, RowFn row, int64_t max_classes, 
     return true;
 }
 
+// The element operators of one class (regular or PML) on the device: either
+// lossless classes (both the operator rows -- kOpW values -- and the
+// hourglass Phi rows -- 32 values -- collapse to at most n/16 distinct rows)
+// or, when they do not, one AoSoA array of kOpB values per element in blocks
+// of 64 elements: value k of element t at ((t / 64) * kOpB + k) * 64 + t % 64.
+template <typename R, typename E, typename OpRow, typename PhiRow>
+static int upload_operators(eqd_handle *h, E &X, int64_t n, bool no_classes, OpRow oprow, PhiRow phirow,
+                            int64_t *n_op_classes, int64_t *n_phi_classes)
+{
+    std::vector<uint32_t> ocls, pcls;
+    std::vector<R> otab, ptab;
+    const int64_t maxc = no_classes ? 0 : std::max<int64_t>(1, n / 16);
+    const bool ok = classify_rows<R>(n, kOpW, oprow, maxc, ocls, otab) &&
+                    classify_rows<R>(n, 32, phirow, maxc, pcls, ptab);
+    int rc;
+    if (ok) {
+        uint32_t *pc;
+        R *pt;
+        if ((rc = dupload(h, &pc, ocls)) || (rc = dupload(h, &pt, otab))) return rc;
+        X.opc = pc;
+        X.optab = pt;
+        X.n_opc = (int32_t)(otab.size() / kOpW);
+        if ((rc = dupload(h, &pc, pcls)) || (rc = dupload(h, &pt, ptab))) return rc;
+        X.phic = pc;
+        X.phitab = pt;
+        X.n_phic = (int32_t)(ptab.size() / 32);
+        *n_op_classes = X.n_opc;
+        *n_phi_classes = X.n_phic;
+        return 0;
+    }
+    const int64_t nb = (n + 63) / 64;
+    std::vector<R> b((size_t)(nb * 64 * kOpB), (R)0);
+#pragma omp parallel for schedule(static)
+    for (int64_t t = 0; t < n; t++) {
+        R row[kOpB];
+        oprow(t, row);
+        phirow(t, row + kOpW);
+        R *dst = b.data() + (t / 64) * kOpB * 64 + (t % 64);
+        for (int k = 0; k < kOpB; k++) dst[k * 64] = row[k];
+    }
+    R *pb;
+    if ((rc = dupload(h, &pb, b))) return rc;
+    X.opb = pb;
+    X.optab = nullptr;
+    X.phitab = nullptr;
+    *n_op_classes = 0;
+    *n_phi_classes = 0;
+    return 0;
+}
+
 template <typename T>
 static void keep_copy(eqd_handle *h, int f, std::map<int, std::vector<T>> &dst)
 {
@@ -560,12 +610,9 @@ int eqd_commit(eqd_handle *h)
             int r2;
             E.n = nr;
             E.ien = d_ien;
-            // operator classes (bit-identical rows share a table row)
-            std::vector<uint32_t> cls;
-            std::vector<R> tab;
-            // EQD_NO_OPERATOR_CLASSES=1 forces per-element arrays (used to test that both give equal results)
-            const int64_t maxc = no_classes ? 0 : std::max<int64_t>(1, nr / 16);
-            if (classify_rows<R>(nr, kOpW, [&](int64_t t, R *row) {
+            // operators: lossless classes, or per element in AoSoA blocks
+            // (EQD_NO_OPERATOR_CLASSES=1 forces the latter; used to test that both give equal results)
+            if ((r2 = upload_operators<R>(h, E, nr, no_classes, [&](int64_t t, R *row) {
                     for (int i = 0; i < 24; i++) row[kOpShp + i] = (R)r_shp[i * nr + t];
                     row[kOpDet] = (R)r_det[t];
                     row[kOpLam] = (R)r_lam[t];
@@ -573,32 +620,10 @@ int eqd_commit(eqd_handle *h)
                     row[kOpVp] = (R)r_vp[t];
                     row[kOpRho] = (R)r_rho[t];
                     for (int i = 0; i < 6; i++) row[kOpSs + i] = (R)r_ss[i * nr + t];
-                }, maxc, cls, tab)) {
-                uint32_t *pc;
-                R *pt;
-                if ((r2 = dupload(h, &pc, cls)) || (r2 = dupload(h, &pt, tab))) return r2;
-                E.opc = pc;
-                E.optab = pt;
-                E.n_opc = (int32_t)(tab.size() / kOpW);
-                h->op_classes_reg = (int64_t)(tab.size() / kOpW);
-            } else if ((r2 = up(&E.shp, r_shp)) || (r2 = up(&E.det, r_det)) || (r2 = up(&E.lam, r_lam)) ||
-                       (r2 = up(&E.mu, r_mu)) || (r2 = up(&E.vp, r_vp)) || (r2 = up(&E.rho, r_rho)) ||
-                       (r2 = up(&E.ss, r_ss))) {
-                return r2;
-            }
-            if (classify_rows<R>(nr, 32, [&](int64_t t, R *row) {
+                }, [&](int64_t t, R *row) {
                     for (int i = 0; i < 32; i++) row[i] = (R)r_phi[i * nr + t];
-                }, maxc, cls, tab)) {
-                uint32_t *pc;
-                R *pt;
-                if ((r2 = dupload(h, &pc, cls)) || (r2 = dupload(h, &pt, tab))) return r2;
-                E.phic = pc;
-                E.phitab = pt;
-                E.n_phic = (int32_t)(tab.size() / 32);
-                h->phi_classes_reg = (int64_t)(tab.size() / 32);
-            } else if ((r2 = up(&E.phi, r_phi))) {
+                }, &h->op_classes_reg, &h->phi_classes_reg)))
                 return r2;
-            }
             if (s.rdampm != 0.0 || s.C_elastic == 0) {
                 if ((r2 = up(&E.emass, r_em))) return r2;
             }
@@ -682,7 +707,7 @@ int eqd_commit(eqd_handle *h)
             // per-element array, so the operator rows (as for regular elements)
             // collapse to a few classes
             if ((r2 = up(&P.damp, p_damp))) return r2;
-            if (classify_rows<R>(np, kOpW, [&](int64_t p, R *row) {
+            if ((r2 = upload_operators<R>(h, P, np, no_classes, [&](int64_t p, R *row) {
                     for (int i = 0; i < 24; i++) row[kOpShp + i] = (R)p_shp[i * np + p];
                     row[kOpDet] = (R)p_det[p];
                     row[kOpLam] = (R)p_lam[p];
@@ -690,32 +715,10 @@ int eqd_commit(eqd_handle *h)
                     row[kOpVp] = (R)p_vp[p];
                     row[kOpRho] = (R)p_rho[p];
                     for (int i = 0; i < 6; i++) row[kOpSs + i] = (R)p_ss[i * np + p];
-                }, maxc, cls, tab)) {
-                uint32_t *pc;
-                R *pt;
-                if ((r2 = dupload(h, &pc, cls)) || (r2 = dupload(h, &pt, tab))) return r2;
-                P.opc = pc;
-                P.optab = pt;
-                P.n_opc = (int32_t)(tab.size() / kOpW);
-                h->op_classes_pml = (int64_t)(tab.size() / kOpW);
-            } else if ((r2 = up(&P.shp, p_shp)) || (r2 = up(&P.det, p_det)) || (r2 = up(&P.lam, p_lam)) ||
-                       (r2 = up(&P.mu, p_mu)) || (r2 = up(&P.vp, p_vp)) || (r2 = up(&P.rho, p_rho)) ||
-                       (r2 = up(&P.ss, p_ss))) {
-                return r2;
-            }
-            if (classify_rows<R>(np, 32, [&](int64_t p, R *row) {
+                }, [&](int64_t p, R *row) {
                     for (int i = 0; i < 32; i++) row[i] = (R)p_phi[i * np + p];
-                }, maxc, cls, tab)) {
-                uint32_t *pc;
-                R *pt;
-                if ((r2 = dupload(h, &pc, cls)) || (r2 = dupload(h, &pt, tab))) return r2;
-                P.phic = pc;
-                P.phitab = pt;
-                P.n_phic = (int32_t)(tab.size() / 32);
-                h->phi_classes_pml = (int64_t)(tab.size() / 32);
-            } else if ((r2 = up(&P.phi, p_phi))) {
+                }, &h->op_classes_pml, &h->phi_classes_pml)))
                 return r2;
-            }
             if (s.rdampm != 0.0 || s.C_elastic == 0) {
                 if ((r2 = up(&P.emass, p_em))) return r2;
             }

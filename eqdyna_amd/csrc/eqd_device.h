@@ -40,54 +40,58 @@ struct NodeState {
     int32_t *nanflag;      // [2] node+1, step
 };
 
+// layout of one operator row: dN_a/dx_k at kOpShp + a*3+k, det, lambda, mu,
+// Vp, rho (viscous hourglass only), SS (6); kOpB-wide per-element rows add
+// Phi_a of mode m at kOpW + m*8+a
+constexpr int kOpShp = 0, kOpDet = 24, kOpLam = 25, kOpMu = 26, kOpVp = 27, kOpRho = 28, kOpSs = 29, kOpW = 35,
+              kOpB = 67;
+
+// value k of element t's per-element operator row (AoSoA, blocks of 64 elements)
+template <typename R>
+__host__ __device__ __forceinline__ R opb_at(const R *opb, int64_t t, int k)
+{
+    return opb[((t >> 6) * kOpB + k) * 64 + (t & 63)];
+}
+
 template <typename R>
 struct RegElems {
     int64_t n;
     const int32_t *ien;   // [8][n]   0-based node ids
-    const R *shp;         // [24][n]  dN_a/dx_k at (a*3+k)
-    const R *det;         // [n]      eledet
-    const R *lam, *mu;    // [n]
-    const R *vp, *rho;    // [n]      (viscous hourglass only)
     R *stress;            // [6 or 12][n]
-    const R *ss;          // [6][n]
-    const R *phi;         // [32][n]  Phi_a of mode m at (m*8+a)
     const R *emass;       // [24][n]  (inertia/gravity term only)
     const double *porep;  // [n]      (plastic only)
     double *pstrain;      // [n]      (plastic only)
     const uint8_t *qidx;  // [n]      (Q only) row of qtab
     const double *qtab;   // [16][6]  exp(-dt/tau), 1-exp(-dt/tau), 8*w_p, 8*w_s, c_p, c_s
     R *ef;                // [24][n]  force on local node a, component c at (a*3+c)
-    // Operator classes: elements whose operators are bit-identical share one
-    // table row.  When optab is set, dN/dx, det, lambda, mu, Vp, rho and SS come
-    // from optab[opc[t]] (kOpW values per row, kOp* offsets) instead of the
-    // per-element arrays; likewise Phi from phitab[phic[t]] (32 per row).
+    // Operators: with classes (optab set), elements whose operators are
+    // bit-identical share one row of optab (kOpW values, at opc[t]) and of
+    // phitab (32 values, at phic[t]); otherwise each element has its own
+    // kOpB-value row in opb (opb_at).
     const uint32_t *opc;
     const R *optab;
     const uint32_t *phic;
     const R *phitab;
     int32_t n_opc, n_phic;  // rows of optab and phitab
+    const R *opb;
 };
-
-// layout of one operator-class row
-constexpr int kOpShp = 0, kOpDet = 24, kOpLam = 25, kOpMu = 26, kOpVp = 27, kOpRho = 28, kOpSs = 29, kOpW = 35;
 
 template <typename R>
 struct PmlElems {
     int64_t n;
     const int32_t *ien;    // [8][n]
-    const R *shp;          // [24][n]
-    const R *det, *lam, *mu, *vp, *rho;
     R *stress;             // [21][n]: 15 split + 6 initial
-    const R *ss, *phi, *emass;
+    const R *emass;
     const R *damp;         // [3][n] element-centre damping d_x, d_y, d_z (always per element)
     const uint8_t *mask12; // [n] bit a: local node a carries 12 components
     R *ef;                 // [96][n] slot (a*12+k); 3-component nodes use k=0..2
-    // operator classes as in RegElems (kOpW values per row)
+    // operators as in RegElems
     const uint32_t *opc;
     const R *optab;
     const uint32_t *phic;
     const R *phitab;
     int32_t n_opc, n_phic;
+    const R *opb;
 };
 
 struct PmlNodes {

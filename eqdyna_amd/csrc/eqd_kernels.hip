@@ -76,16 +76,16 @@ __global__ void __launch_bounds__(256) elem_regular_kernel(RegElems<R> E, NodeSt
     constexpr bool kF32 = sizeof(R) != sizeof(double);
     auto SHP = [&](int i) -> R {
         if constexpr (kLds) return lrow[kOpShp + i];
-        else return op ? op[kOpShp + i] : E.shp[i * n + t];
+        else return op ? op[kOpShp + i] : opb_at(E.opb, t, kOpShp + i);
     };
     auto PHI = [&](int m, int a) -> R {
         if constexpr (kLds) return lphr[m * 8 + a];
-        else return pht ? pht[m * 8 + a] : E.phi[(m * 8 + a) * n + t];
+        else return pht ? pht[m * 8 + a] : opb_at(E.opb, t, kOpW + m * 8 + a);
     };
-    // a scalar operator k of the row, or from its per-element array
-    auto OPK = [&](int k, const R *arr) -> R {
+    // scalar operator k of the element's row
+    auto OPK = [&](int k) -> R {
         if constexpr (kLds) return lrow[k];
-        else return op ? op[k] : arr[t];
+        else return op ? op[k] : opb_at(E.opb, t, k);
     };
     const R rdk = (R)C.rdampk;
     // all eight node ids first (independent loads), then the nodes' v and u
@@ -194,8 +194,8 @@ __global__ void __launch_bounds__(256) elem_regular_kernel(RegElems<R> E, NodeSt
             for (int k = 0; k < kNo; k++) opc_[k] = opn_[k];
         }
     }
-    const R lam = OPK(kOpLam, E.lam), miu = OPK(kOpMu, E.mu);
-    const R det = OPK(kOpDet, E.det);
+    const R lam = OPK(kOpLam), miu = OPK(kOpMu);
+    const R det = OPK(kOpDet);
     const R cdiag = lam + (R)2 * miu;
     R srt[6];
 #pragma unroll
@@ -284,7 +284,7 @@ __global__ void __launch_bounds__(256) elem_regular_kernel(RegElems<R> E, NodeSt
     if (kHG == 1) {
         R ss[6];
 #pragma unroll
-        for (int i = 0; i < 6; i++) ss[i] = kLds ? (R)lrow[kOpSs + i] : (op ? op[kOpSs + i] : E.ss[i * n + t]);
+        for (int i = 0; i < 6; i++) ss[i] = OPK(kOpSs + i);
 #pragma unroll
         for (int m = 0; m < 4; m++) {
             g[m][0] = ss[0] * pd[m][0] + ss[1] * pd[m][1] + ss[2] * pd[m][2];
@@ -292,7 +292,7 @@ __global__ void __launch_bounds__(256) elem_regular_kernel(RegElems<R> E, NodeSt
             g[m][2] = ss[2] * pd[m][0] + ss[4] * pd[m][1] + ss[5] * pd[m][2];
         }
     } else if (kHG == 2) {
-        const R rho = OPK(kOpRho, E.rho), vp = OPK(kOpVp, E.vp);
+        const R rho = OPK(kOpRho), vp = OPK(kOpVp);
         coef = (R)0.25 * (R)C.kapa_hg * rho * vp * (R)pow((double)det * C.w, 2.0 / 3.0);
     }
     // pass 2: one node's force at a time (operators again one node ahead)
@@ -379,15 +379,15 @@ __global__ void __launch_bounds__(256) elem_pml_kernel(PmlElems<R> E, NodeState 
     constexpr bool kF64 = sizeof(R) == sizeof(double);
     auto SHP = [&](int i) -> R {
         if constexpr (kLds) return lrow[kOpShp + i];
-        else return op ? op[kOpShp + i] : E.shp[i * n + p];
+        else return op ? op[kOpShp + i] : opb_at(E.opb, p, kOpShp + i);
     };
     auto PHI = [&](int m, int a) -> R {
         if constexpr (kLds) return lphr[m * 8 + a];
-        else return pht ? pht[m * 8 + a] : E.phi[(m * 8 + a) * n + p];
+        else return pht ? pht[m * 8 + a] : opb_at(E.opb, p, kOpW + m * 8 + a);
     };
-    auto OPK = [&](int k, const R *arr) -> R {
+    auto OPK = [&](int k) -> R {
         if constexpr (kLds) return lrow[k];
-        else return op ? op[k] : arr[p];
+        else return op ? op[k] : opb_at(E.opb, p, k);
     };
     // node ids first, then each node's v and u one node ahead of the arithmetic
     int nds[8];
@@ -492,7 +492,7 @@ __global__ void __launch_bounds__(256) elem_pml_kernel(PmlElems<R> E, NodeState 
             for (int k = 0; k < kNo; k++) opc_[k] = opn_[k];
         }
     }
-    const R lam = OPK(kOpLam, E.lam), miu = OPK(kOpMu, E.mu), det = OPK(kOpDet, E.det);
+    const R lam = OPK(kOpLam), miu = OPK(kOpMu), det = OPK(kOpDet);
     const R cdiag = lam + (R)2.0 * miu;
     R srt[6];
 #pragma unroll
@@ -538,7 +538,7 @@ __global__ void __launch_bounds__(256) elem_pml_kernel(PmlElems<R> E, NodeState 
     if (kHG == 1) {
         R ss[6];
 #pragma unroll
-        for (int i = 0; i < 6; i++) ss[i] = kLds ? (R)lrow[kOpSs + i] : (op ? op[kOpSs + i] : E.ss[i * n + p]);
+        for (int i = 0; i < 6; i++) ss[i] = OPK(kOpSs + i);
 #pragma unroll
         for (int m = 0; m < 4; m++) {
             g[m][0] = ss[0] * pd[m][0] + ss[1] * pd[m][1] + ss[2] * pd[m][2];
@@ -546,7 +546,7 @@ __global__ void __launch_bounds__(256) elem_pml_kernel(PmlElems<R> E, NodeState 
             g[m][2] = ss[2] * pd[m][0] + ss[4] * pd[m][1] + ss[5] * pd[m][2];
         }
     } else if (kHG == 2) {
-        const R rho = OPK(kOpRho, E.rho), vp = OPK(kOpVp, E.vp);
+        const R rho = OPK(kOpRho), vp = OPK(kOpVp);
         coef2 = (R)0.25 * (R)C.kapa_hg * rho * vp * (R)pow((double)det * C.w, 2.0 / 3.0);
     }
 
