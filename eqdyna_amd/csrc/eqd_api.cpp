@@ -66,7 +66,8 @@ struct eqd_handle {
     int32_t nt = 0;
     double time = 0.0;
     bool pending = true;  // an update from acc is due at the start of the next step
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr, stream2 = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     std::vector<void *> allocs;
     int64_t bytes = 0;
     // host-side copies of the layout needed for downloads
@@ -406,6 +407,11 @@ int eqd_commit(eqd_handle *h)
         return fail(h, EQD_EINVAL, "friclaw %d is not supported (1-4; thermal pressurization is not built)", s.friclaw);
     HIPCHK(h, hipSetDevice(s.device));
     HIPCHK(h, hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+    // a second stream (default priority) for the fault pairs, which overlap the
+    // node kernels within a step; events join the two every step
+    HIPCHK(h, hipStreamCreateWithFlags(&h->stream2, hipStreamNonBlocking));
+    HIPCHK(h, hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming));
+    HIPCHK(h, hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming));
 
     const int64_t nn = s.nn, ne = s.ne;
     const double *coor = F64(h, EQD_MESHCOOR);
@@ -901,7 +907,7 @@ int eqd_commit(eqd_handle *h)
 
 }  // extern "C"
 
-static void timed_begin(eqd_handle *h, const char *name, hipEvent_t *stop)
+static void timed_begin(eqd_handle *h, const char *name, hipEvent_t *stop, hipStream_t st = nullptr)
 {
     *stop = nullptr;
     if (!h->timing) return;
@@ -912,14 +918,14 @@ static void timed_begin(eqd_handle *h, const char *name, hipEvent_t *stop)
         (void)hipEventCreate(&b);
         t.ev.push_back({a, b});
     }
-    (void)hipEventRecord(t.ev[t.used].first, h->stream);
+    (void)hipEventRecord(t.ev[t.used].first, st ? st : h->stream);
     *stop = t.ev[t.used].second;
     t.used++;
 }
 
-static void timed_end(eqd_handle *h, hipEvent_t stop)
+static void timed_end(eqd_handle *h, hipEvent_t stop, hipStream_t st = nullptr)
 {
-    if (stop) (void)hipEventRecord(stop, h->stream);
+    if (stop) (void)hipEventRecord(stop, st ? st : h->stream);
 }
 
 template <typename R>
@@ -949,7 +955,10 @@ static int run_steps(eqd_handle *h, const RegElems<R> &E, const PmlElems<R> &P, 
         timed_begin(h, "element_pml", &ev);
         HIPCHK(h, launch_elem_pml<R>(P, h->N, h->C, s.C_hg, mass, st));
         timed_end(h, ev);
-        timed_begin(h, "fault", &ev);
+        // fork: the fault pairs on the second stream, beside the node kernels
+        HIPCHK(h, hipEventRecord(h->ev_fork, st));
+        HIPCHK(h, hipStreamWaitEvent(h->stream2, h->ev_fork, 0));
+        timed_begin(h, "fault", &ev, h->stream2);
         FaultState F = h->F;
         F.harvest = nullptr;
         if (h->harvest && h->harvest_every > 0 && nt % h->harvest_every == 0 && h->harvest_rows < h->harvest_cap) {
@@ -958,8 +967,9 @@ static int run_steps(eqd_handle *h, const RegElems<R> &E, const PmlElems<R> &P, 
             F.harvest_cap = h->harvest_cap;
             h->harvest_rows = std::min(h->harvest_cap, h->harvest_rows + F.nf);
         }
-        HIPCHK(h, launch_fault<R>(F, h->N, h->C, E.ef, E.n, P.ef, P.n, nt, h->time, mode, h->learn, st));
-        timed_end(h, ev);
+        HIPCHK(h, launch_fault<R>(F, h->N, h->C, E.ef, E.n, P.ef, P.n, nt, h->time, mode, h->learn, h->stream2));
+        timed_end(h, ev, h->stream2);
+        HIPCHK(h, hipEventRecord(h->ev_join, h->stream2));
         timed_begin(h, "node", &ev);
         HIPCHK(h, launch_node3<R>(h->N, h->d_node3_free, (int64_t)h->node3_free.size(), E.ef, E.n, P.ef, P.n, dt,
                                   mode, nt + 1, st));
@@ -967,6 +977,8 @@ static int run_steps(eqd_handle *h, const RegElems<R> &E, const PmlElems<R> &P, 
         timed_begin(h, "node_pml", &ev);
         HIPCHK(h, launch_node_pml<R>(h->N, h->PN, P.ef, P.n, dt, mode, nt + 1, st));
         timed_end(h, ev);
+        // join: the next step (and anything after the run) sees the fault pairs' update
+        HIPCHK(h, hipStreamWaitEvent(st, h->ev_join, 0));
     }
     h->pending = true;
     return EQD_OK;
@@ -1285,6 +1297,7 @@ int eqd_destroy(eqd_handle *h)
     if (!h) return EQD_OK;
     (void)hipSetDevice(h->s.device);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
+    if (h->stream2) (void)hipStreamSynchronize(h->stream2);
     for (void *p : h->allocs) (void)hipFree(p);
     if (h->harvest) (void)hipFree(h->harvest);
     for (auto &kv : h->timers)
@@ -1292,6 +1305,9 @@ int eqd_destroy(eqd_handle *h)
             (void)hipEventDestroy(e.first);
             (void)hipEventDestroy(e.second);
         }
+    if (h->ev_fork) (void)hipEventDestroy(h->ev_fork);
+    if (h->ev_join) (void)hipEventDestroy(h->ev_join);
+    if (h->stream2) (void)hipStreamDestroy(h->stream2);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
     return EQD_OK;
