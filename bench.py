@@ -1,7 +1,7 @@
 #!/usr/bin/env python3
 """Throughput of the EQdyna explicit time step on one MI355X.
 
-Default workload (c4): BASELINE config 4 -- config 3 (240 x 120 x 120 =
+Default workload (c4): the rough-fault rate-and-state box c3 (240 x 120 x 120 =
 3,456,000 hexahedra at 100 m with a 10-element PML, a seeded fractal rough
 fault, slip-law rate-and-state friction with strong velocity weakening,
 dt = 0.005 s, 3000 steps) plus the learned initial guess of the
@@ -60,18 +60,18 @@ def build_workload(name: str, steps_total: int):
     from eqdyna_amd import case, mesh
     if name == "c2":
         c = case.c2(steps=steps_total)
-        desc = ("BASELINE config 2: 480x240x240 = 27,648,000 hexahedra at 50 m incl. 10-element PML, "
+        desc = ("c2: 480x240x240 = 27,648,000 hexahedra at 50 m incl. 10-element PML, "
                 "seeded 5-layer medium (Vs 2000->3464 m/s), Q (reference law), slip-weakening split-node fault")
     elif name == "c1":
         c = case.c1(steps=steps_total)
-        desc = "BASELINE config 1: 120x60x60 = 432,000 hexahedra at 200 m incl. 10-element PML, homogeneous, SW fault"
+        desc = "c1: 120x60x60 = 432,000 hexahedra at 200 m incl. 10-element PML, homogeneous, SW fault"
     elif name in ("c3", "c4"):
         c = case.c3(steps=steps_total)
-        desc = ("BASELINE config 3: 240x120x120 = 3,456,000 hexahedra at 100 m incl. 10-element PML, seeded fractal "
+        desc = ("c3: 240x120x120 = 3,456,000 hexahedra at 100 m incl. 10-element PML, seeded fractal "
                 "rough fault (rms/length 0.005, H=1), slip-law RSF with strong velocity weakening, seeded a field")
         if name == "c4":
-            desc = ("BASELINE config 4: config 3 plus the learned RSF Newton guess (6-64-64-64-1 GELU, bf16 MFMA), "
-                    "FP64 reference mode vs FP32 bulk + learned guess; " + desc[len("BASELINE config 3: "):])
+            desc = ("c4: c3 plus the learned RSF Newton guess (6-64-64-64-1 GELU, bf16 MFMA), "
+                    "FP64 reference mode vs FP32 bulk + learned guess; " + desc[len("c3: "):])
     else:
         raise SystemExit(f"unknown workload {name}")
     t = time.perf_counter()
@@ -150,7 +150,7 @@ def cpu_baseline_reference(workload: str, ranks: int, steps: int):
         return None
     if workload in ("c3", "c4"):
         c = case.c3(steps=steps)
-        sample = (f"config 3 at full size (3,456,000 hexahedra, rough fault, slip-law RSF), first {steps} of 3000 "
+        sample = (f"c3 at full size (3,456,000 hexahedra, rough fault, slip-law RSF), first {steps} of 3000 "
                   f"steps, plain Newton guess")
     else:
         c = case.c2_sample(steps=steps)
@@ -215,7 +215,7 @@ def main() -> int:
         return calibrate_copy(args.calibrate_copy)
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}). "
-              "The halo-exchange design for several GPUs is described in DESIGN.md.", file=sys.stderr)
+              "Several GPUs would each take a slab of the mesh with a halo exchange; that path is not built.", file=sys.stderr)
         return 2
     steps = args.steps if args.steps is not None else (2990 if args.workload == "c4" else 200)
     total = args.warmup + steps

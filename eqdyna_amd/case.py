@@ -505,7 +505,7 @@ def layered_medium(seed: int, n_layers: int = 5, vs_top: float = 2000.0, vs_bott
 
 
 def _strike_slip_box(d: float, npml: int, name: str) -> Case:
-    """Shared set-up of the BASELINE boxes: a vertical strike-slip split-node
+    """Shared set-up of the benchmark boxes (c1, c2): a vertical strike-slip split-node
     fault 16 km along strike, rupturable from 2 to 9.8 km depth (the fault
     plane must reach the free surface in the reference, src/meshgen.f90:476-531,
     so the top 2 km are locked with mu_s = 1e4 as TPV8 does), linear slip
@@ -544,7 +544,7 @@ def _strike_slip_box(d: float, npml: int, name: str) -> Case:
 
 
 def c1(steps: int = 600) -> Case:
-    """homogeneous elastic box at 200 m, 120 x 60 x 60 =
+    """c1: homogeneous elastic box at 200 m, 120 x 60 x 60 =
     432,000 hexahedra including a 10-element PML on five faces, rho = 2670,
     Vp = 6000, Vs = 3464, dt = 0.01 s."""
     c = _strike_slip_box(200.0, 10, "c1")
@@ -560,7 +560,7 @@ def c1(steps: int = 600) -> Case:
 
 
 def c2(steps: int = 4000, seed: int = 2, q: bool = True) -> Case:
-    """BASELINE config 2 at 50 m: 480 x 240 x 240 = 27,648,000 hexahedra
+    """c2, the large elastic box at 50 m: 480 x 240 x 240 = 27,648,000 hexahedra
     including a 10-element PML, a seeded five-layer medium (Vs 2000 -> 3464
     m/s with depth, Vp/Vs = 1.732, Gardner density), the reference's
     frequency-independent Q (C_Q = 1, src/calcElemKU.f90:77-132) and
@@ -651,7 +651,7 @@ def rough_fractal(c: Case, alpha: float, hurst: float, seed: int) -> None:
 
 def c3(steps: int = 3000, seed: int = 3, tau0: float = 10.5e6, vs_top: float = 2000.0, sn: float = -30.0e6,
        nucdtau0: float = 18.0e6) -> Case:
-    """BASELINE config 3 at 100 m: 240 x 120 x 120 = 3,456,000 hexahedra
+    """c3, the rough-fault rate-and-state box at 100 m: 240 x 120 x 120 = 3,456,000 hexahedra
     including a 10-element PML; homogeneous rock; a seeded fractal rough
     fault (rms height / length 0.005, H = 1) 16 km along strike and 9.8 km
     down dip; slip law with strong velocity weakening (friclaw 4): a seeded

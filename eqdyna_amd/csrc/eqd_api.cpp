@@ -68,6 +68,7 @@ struct eqd_handle {
     bool pending = true;  // an update from acc is due at the start of the next step
     hipStream_t stream = nullptr, stream2 = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    bool single_stream = false;
     std::vector<void *> allocs;
     int64_t bytes = 0;
     // host-side copies of the layout needed for downloads
@@ -410,6 +411,10 @@ int eqd_commit(eqd_handle *h)
     // a second stream (default priority) for the fault pairs, which overlap the
     // node kernels within a step; events join the two every step
     HIPCHK(h, hipStreamCreateWithFlags(&h->stream2, hipStreamNonBlocking));
+    {
+        const char *e = std::getenv("EQD_SINGLE_STREAM");
+        h->single_stream = e && e[0] == '1';
+    }
     HIPCHK(h, hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming));
     HIPCHK(h, hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming));
 
@@ -928,6 +933,26 @@ static void timed_end(eqd_handle *h, hipEvent_t stop, hipStream_t st = nullptr)
     if (stop) (void)hipEventRecord(stop, st ? st : h->stream);
 }
 
+// Add the recorded launches' durations to the timers and recycle their
+// events (a bounded pool: at most kTimerPool launches per kernel in flight).
+constexpr size_t kTimerPool = 256;
+static int drain_timers(eqd_handle *h)
+{
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream2));
+    for (auto &kv : h->timers) {
+        KTimer &t = kv.second;
+        for (size_t i = 0; i < t.used; i++) {
+            float ms = 0;
+            (void)hipEventElapsedTime(&ms, t.ev[i].first, t.ev[i].second);
+            t.total_ms += ms;
+        }
+        t.launches += (int64_t)t.used;
+        t.used = 0;
+    }
+    return EQD_OK;
+}
+
 template <typename R>
 static int run_steps(eqd_handle *h, const RegElems<R> &E, const PmlElems<R> &P, int32_t nsteps)
 {
@@ -956,9 +981,13 @@ static int run_steps(eqd_handle *h, const RegElems<R> &E, const PmlElems<R> &P, 
         HIPCHK(h, launch_elem_pml<R>(P, h->N, h->C, s.C_hg, mass, st));
         timed_end(h, ev);
         // fork: the fault pairs on the second stream, beside the node kernels
-        HIPCHK(h, hipEventRecord(h->ev_fork, st));
-        HIPCHK(h, hipStreamWaitEvent(h->stream2, h->ev_fork, 0));
-        timed_begin(h, "fault", &ev, h->stream2);
+        // (EQD_SINGLE_STREAM=1 keeps them on the main stream, in step order)
+        hipStream_t fst = h->single_stream ? st : h->stream2;
+        if (!h->single_stream) {
+            HIPCHK(h, hipEventRecord(h->ev_fork, st));
+            HIPCHK(h, hipStreamWaitEvent(h->stream2, h->ev_fork, 0));
+        }
+        timed_begin(h, "fault", &ev, fst);
         FaultState F = h->F;
         F.harvest = nullptr;
         if (h->harvest && h->harvest_every > 0 && nt % h->harvest_every == 0 && h->harvest_rows < h->harvest_cap) {
@@ -967,9 +996,9 @@ static int run_steps(eqd_handle *h, const RegElems<R> &E, const PmlElems<R> &P, 
             F.harvest_cap = h->harvest_cap;
             h->harvest_rows = std::min(h->harvest_cap, h->harvest_rows + F.nf);
         }
-        HIPCHK(h, launch_fault<R>(F, h->N, h->C, E.ef, E.n, P.ef, P.n, nt, h->time, mode, h->learn, h->stream2));
-        timed_end(h, ev, h->stream2);
-        HIPCHK(h, hipEventRecord(h->ev_join, h->stream2));
+        HIPCHK(h, launch_fault<R>(F, h->N, h->C, E.ef, E.n, P.ef, P.n, nt, h->time, mode, h->learn, fst));
+        timed_end(h, ev, fst);
+        if (!h->single_stream) HIPCHK(h, hipEventRecord(h->ev_join, h->stream2));
         timed_begin(h, "node", &ev);
         HIPCHK(h, launch_node3<R>(h->N, h->d_node3_free, (int64_t)h->node3_free.size(), E.ef, E.n, P.ef, P.n, dt,
                                   mode, nt + 1, st));
@@ -978,7 +1007,15 @@ static int run_steps(eqd_handle *h, const RegElems<R> &E, const PmlElems<R> &P, 
         HIPCHK(h, launch_node_pml<R>(h->N, h->PN, P.ef, P.n, dt, mode, nt + 1, st));
         timed_end(h, ev);
         // join: the next step (and anything after the run) sees the fault pairs' update
-        HIPCHK(h, hipStreamWaitEvent(st, h->ev_join, 0));
+        if (!h->single_stream) HIPCHK(h, hipStreamWaitEvent(st, h->ev_join, 0));
+        if (h->timing) {
+            size_t most = 0;
+            for (auto &kv : h->timers) most = std::max(most, kv.second.used);
+            if (most >= kTimerPool) {
+                int rc = drain_timers(h);
+                if (rc) return rc;
+            }
+        }
     }
     h->pending = true;
     return EQD_OK;
@@ -999,17 +1036,7 @@ extern "C" int eqd_run(eqd_handle *h, int32_t nsteps)
     int rc = h->s.precision == EQD_PREC_FP32 ? run_steps(h, h->E32, h->P32, nsteps) : run_steps(h, h->E64, h->P64, nsteps);
     if (rc) return rc;
     HIPCHK(h, hipStreamSynchronize(h->stream));
-    if (h->timing) {
-        for (auto &kv : h->timers) {
-            KTimer &t = kv.second;
-            for (size_t i = 0; i < t.used; i++) {
-                float ms = 0;
-                (void)hipEventElapsedTime(&ms, t.ev[i].first, t.ev[i].second);
-                t.total_ms += ms;
-            }
-            t.launches = (int64_t)t.used;
-        }
-    }
+    if (h->timing && (rc = drain_timers(h))) return rc;
     int32_t flag[2];
     HIPCHK(h, hipMemcpy(flag, h->N.nanflag, sizeof(flag), hipMemcpyDeviceToHost));
     if (flag[0] != 0) return fail(h, EQD_ENAN, "Velocity NaN at node %d at time step %d", flag[0], flag[1]);

@@ -11,7 +11,7 @@ iterations before it restarts from the plain guess; Newton's convergence
 test decides the slip rate either way.
 
 Training data: every `every` steps of seeded variants of the rough-fault
-rate-and-state workload the solver records, per node, the
+rate-and-state workload (case.c3) the solver records, per node, the
 six inputs and the converged ln V (eqd_harvest).
 
     python -m eqdyna_amd.learned --out eqdyna_amd/rsf_guess_c3.npz
@@ -66,7 +66,7 @@ def init_params(X: np.ndarray, r: np.ndarray, seed: int) -> np.ndarray:
 
 
 def training_case(seed: int, steps: int) -> cases.Case:
-    """A seeded variant of config 3: its own fractal surface and a field, a
+    """A seeded variant of case.c3: its own fractal surface and a field, a
     uniform initial shear in [9.5, 10.5] MPa and a hypocentre along strike in
     [-3, 3] km."""
     c = cases.c3(steps=steps, seed=seed)

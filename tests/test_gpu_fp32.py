@@ -1,7 +1,7 @@
 """FP32 bulk mode (element force + hourglass in FP32; fault traction, Newton
 solve, state and nodal update in FP64) against the reference's FP64 run.
 
-Stated error bars (north_star): rupture time within 0.5*dt on average over the
+Error bars of the FP32 mode: rupture time within 0.5*dt on average over the
 ruptured nodes, peak slip rate within 1 % of the FP64 value."""
 import numpy as np
 import pytest
