@@ -14,29 +14,7 @@
 #include <unordered_map>
 #include <vector>
 
-#include "../../include/eqdyna_amd.h"
-#include "eqd_device.h"
-
-namespace eqd {
-template <typename R>
-hipError_t launch_elem_regular(const RegElems<R> &, const NodeState &, const StepConsts &, bool, bool, int, bool,
-                               hipStream_t);
-template <typename R>
-hipError_t launch_elem_pml(const PmlElems<R> &, const NodeState &, const StepConsts &, int, bool, hipStream_t);
-template <typename R>
-hipError_t launch_node3(const NodeState &, const int32_t *, int64_t, const R *, int64_t, const R *, int64_t,
-                        double, int, int, hipStream_t);
-template <typename R>
-hipError_t launch_node_pml(const NodeState &, const PmlNodes &, const R *, int64_t, double, int, int,
-                           hipStream_t);
-hipError_t launch_update_from_acc(const NodeState &, const int32_t *, int64_t, const PmlNodes &, double, int,
-                                  hipStream_t);
-template <typename R>
-hipError_t launch_fault(const FaultState &, const NodeState &, const StepConsts &, const R *, int64_t,
-                        const R *, int64_t, int, double, int, bool, hipStream_t);
-hipError_t launch_mlp_prepare(const float *p, __bf16 *wb, float *w4r, hipStream_t s);
-hipError_t launch_station(const OffStations &, const NodeState &, int, double, hipStream_t);
-}  // namespace eqd
+#include "eqd_internal.h"
 
 using namespace eqd;
 
@@ -47,76 +25,7 @@ struct FieldSpec {
     int64_t count;
 };
 
-struct KTimer {
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
-    size_t used = 0;
-    double total_ms = 0;
-    int64_t launches = 0;
-};
-
 }  // namespace
-
-struct eqd_handle {
-    eqd_scalars s{};
-    std::string err;
-    std::map<int, const void *> src;         // caller's arrays, read during eqd_commit
-    std::map<int, std::vector<double>> fd;   // f64 fields kept after commit
-    std::map<int, std::vector<int32_t>> fi;  // i32 fields kept after commit
-    bool committed = false;
-    int32_t nt = 0;
-    double time = 0.0;
-    bool pending = true;  // an update from acc is due at the start of the next step
-    hipStream_t stream = nullptr, stream2 = nullptr;
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
-    bool single_stream = false;
-    std::vector<void *> allocs;
-    int64_t bytes = 0;
-    // host-side copies of the layout needed for downloads
-    std::vector<int32_t> reg_ids, pml_ids;     // element ids (0-based) per class
-    std::vector<int32_t> node3_all, node3_free, pml_node_ids;
-    std::vector<int32_t> pml_node_index;       // node -> PML node index or -1
-    int32_t n_reg = 0, n_pml = 0, nreg_stress = 6;
-    // device objects
-    NodeState N{};
-    RegElems<double> E64{};
-    RegElems<float> E32{};
-    PmlElems<double> P64{};
-    PmlElems<float> P32{};
-    int64_t n_pml_elems = 0;
-    int64_t op_classes_reg = 0, phi_classes_reg = 0, op_classes_pml = 0, phi_classes_pml = 0;
-    PmlNodes PN{};
-    FaultState F{};
-    OffStations S{};
-    StepConsts C{};
-    int32_t *d_node3_free = nullptr, *d_node3_all = nullptr;
-    // learned Newton guess: float32 parameters and their bfloat16 operands
-    bool learn = false;
-    float *mlp_p = nullptr, *mlp_w4r = nullptr;
-    __bf16 *mlp_wb = nullptr;
-    // harvest of training rows
-    int32_t harvest_every = 0;
-    float *harvest = nullptr;
-    int64_t harvest_cap = 0, harvest_rows = 0;
-    bool timing = false;
-    std::map<std::string, KTimer> timers;
-};
-
-static int fail(eqd_handle *h, int code, const char *fmt, ...)
-{
-    char buf[1024];
-    va_list ap;
-    va_start(ap, fmt);
-    vsnprintf(buf, sizeof(buf), fmt, ap);
-    va_end(ap);
-    if (h) h->err = buf;
-    return code;
-}
-
-#define HIPCHK(h, x)                                                                              \
-    do {                                                                                          \
-        hipError_t e_ = (x);                                                                      \
-        if (e_ != hipSuccess) return fail(h, EQD_EHIP, "%s: %s", #x, hipGetErrorString(e_));      \
-    } while (0)
 
 static bool field_spec(const eqd_scalars &s, int f, FieldSpec &o)
 {
@@ -407,16 +316,17 @@ int eqd_commit(eqd_handle *h)
     if (s.friclaw < 1 || s.friclaw > 4)
         return fail(h, EQD_EINVAL, "friclaw %d is not supported (1-4; thermal pressurization is not built)", s.friclaw);
     HIPCHK(h, hipSetDevice(s.device));
-    HIPCHK(h, hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
-    // a second stream (default priority) for the fault pairs, which overlap the
-    // node kernels within a step; events join the two every step
-    HIPCHK(h, hipStreamCreateWithFlags(&h->stream2, hipStreamNonBlocking));
-    {
-        const char *e = std::getenv("EQD_SINGLE_STREAM");
-        h->single_stream = e && e[0] == '1';
+    // the handle's own engine (streams, launch tables); built at the first run
+    if (!h->eng) {
+        h->eng = new eqd_engine();
+        int rc = engine_init(h->eng, {h});
+        if (rc) {
+            h->err = h->eng->err;
+            return rc;
+        }
+        h->stream = h->eng->st;
+        h->eng->timing = h->timing;
     }
-    HIPCHK(h, hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming));
-    HIPCHK(h, hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming));
 
     const int64_t nn = s.nn, ne = s.ne;
     const double *coor = F64(h, EQD_MESHCOOR);
@@ -692,7 +602,6 @@ int eqd_commit(eqd_handle *h)
                 for (int a = 0; a < 8; a++) p_phi[(mm * 8 + a) * np + p] = phi[e * 32 + mm * 8 + a];
             for (int i = 0; i < 21; i++) p_st[i * np + p] = stress[(int64_t)sci[e] + i];
         }
-        h->n_pml_elems = np;
         int rc;
         int32_t *pi;
         uint8_t *pm;
@@ -899,6 +808,11 @@ int eqd_commit(eqd_handle *h)
     C.nucR = s.nucR; C.nucRuptVel = s.nucRuptVel; C.nucdtau0 = s.nucdtau0; C.nucT = s.nucT;
     C.C_elastic = s.C_elastic; C.C_Q = s.C_Q; C.C_hg = s.C_hg; C.C_nuclea = s.C_nuclea; C.friclaw = s.friclaw;
     C.TPV = s.TPV; C.insertFaultType = s.insertFaultType; C.nucfault = s.nucfault;
+    {
+        int rc = dalloc(h, &h->d_clock, 1);
+        if (rc) return rc;
+        HIPCHK(h, hipMemset(h->d_clock, 0, sizeof(Clock)));
+    }
     HIPCHK(h, hipDeviceSynchronize());
     // keep what eqd_get needs to rebuild reference-layout arrays; the caller's
     // buffers are not referenced after this point
@@ -906,141 +820,22 @@ int eqd_commit(eqd_handle *h)
     for (int f : {EQD_EQNUMINDEXARR, EQD_EQNUMSTARTINDEXLOC, EQD_STRESSCOMPINDEXARR}) keep_copy(h, f, h->fi);
     h->src.clear();
     h->committed = true;
-    h->pending = true;
+    h->version++;
     return EQD_OK;
 }
 
 }  // extern "C"
 
-static void timed_begin(eqd_handle *h, const char *name, hipEvent_t *stop, hipStream_t st = nullptr)
-{
-    *stop = nullptr;
-    if (!h->timing) return;
-    KTimer &t = h->timers[name];
-    if (t.used == t.ev.size()) {
-        hipEvent_t a, b;
-        (void)hipEventCreate(&a);
-        (void)hipEventCreate(&b);
-        t.ev.push_back({a, b});
-    }
-    (void)hipEventRecord(t.ev[t.used].first, st ? st : h->stream);
-    *stop = t.ev[t.used].second;
-    t.used++;
-}
-
-static void timed_end(eqd_handle *h, hipEvent_t stop, hipStream_t st = nullptr)
-{
-    if (stop) (void)hipEventRecord(stop, st ? st : h->stream);
-}
-
-// Add the recorded launches' durations to the timers and recycle their
-// events (a bounded pool: at most kTimerPool launches per kernel in flight).
-constexpr size_t kTimerPool = 256;
-static int drain_timers(eqd_handle *h)
-{
-    HIPCHK(h, hipStreamSynchronize(h->stream));
-    HIPCHK(h, hipStreamSynchronize(h->stream2));
-    for (auto &kv : h->timers) {
-        KTimer &t = kv.second;
-        for (size_t i = 0; i < t.used; i++) {
-            float ms = 0;
-            (void)hipEventElapsedTime(&ms, t.ev[i].first, t.ev[i].second);
-            t.total_ms += ms;
-        }
-        t.launches += (int64_t)t.used;
-        t.used = 0;
-    }
-    return EQD_OK;
-}
-
-template <typename R>
-static int run_steps(eqd_handle *h, const RegElems<R> &E, const PmlElems<R> &P, int32_t nsteps)
-{
-    const eqd_scalars &s = h->s;
-    const double dt = s.dt;
-    const bool q = s.C_Q == 1, dp = s.C_elastic == 0, mass = s.rdampm != 0.0 || s.C_elastic == 0;
-    hipStream_t st = h->stream;
-    hipEvent_t ev;
-    if (h->pending) {
-        HIPCHK(h, launch_update_from_acc(h->N, h->d_node3_all, (int64_t)h->node3_all.size(), h->PN, dt, h->nt + 1, st));
-        h->pending = false;
-    }
-    for (int32_t k = 0; k < nsteps; k++) {
-        h->nt += 1;
-        h->time = h->time + dt;
-        const int nt = h->nt;
-        const int mode = (k == nsteps - 1) ? 1 : 0;
-        if (nt > s.nstep) return fail(h, EQD_EINVAL, "step %d exceeds nstep %d (station history length)", nt, s.nstep);
-        timed_begin(h, "station", &ev);
-        HIPCHK(h, launch_station(h->S, h->N, nt, h->time, st));
-        timed_end(h, ev);
-        timed_begin(h, "element", &ev);
-        HIPCHK(h, launch_elem_regular<R>(E, h->N, h->C, q, dp, s.C_hg, mass, st));
-        timed_end(h, ev);
-        timed_begin(h, "element_pml", &ev);
-        HIPCHK(h, launch_elem_pml<R>(P, h->N, h->C, s.C_hg, mass, st));
-        timed_end(h, ev);
-        // fork: the fault pairs on the second stream, beside the node kernels
-        // (EQD_SINGLE_STREAM=1 keeps them on the main stream, in step order)
-        hipStream_t fst = h->single_stream ? st : h->stream2;
-        if (!h->single_stream) {
-            HIPCHK(h, hipEventRecord(h->ev_fork, st));
-            HIPCHK(h, hipStreamWaitEvent(h->stream2, h->ev_fork, 0));
-        }
-        timed_begin(h, "fault", &ev, fst);
-        FaultState F = h->F;
-        F.harvest = nullptr;
-        if (h->harvest && h->harvest_every > 0 && nt % h->harvest_every == 0 && h->harvest_rows < h->harvest_cap) {
-            F.harvest = h->harvest;
-            F.harvest_base = h->harvest_rows;
-            F.harvest_cap = h->harvest_cap;
-            h->harvest_rows = std::min(h->harvest_cap, h->harvest_rows + F.nf);
-        }
-        HIPCHK(h, launch_fault<R>(F, h->N, h->C, E.ef, E.n, P.ef, P.n, nt, h->time, mode, h->learn, fst));
-        timed_end(h, ev, fst);
-        if (!h->single_stream) HIPCHK(h, hipEventRecord(h->ev_join, h->stream2));
-        timed_begin(h, "node", &ev);
-        HIPCHK(h, launch_node3<R>(h->N, h->d_node3_free, (int64_t)h->node3_free.size(), E.ef, E.n, P.ef, P.n, dt,
-                                  mode, nt + 1, st));
-        timed_end(h, ev);
-        timed_begin(h, "node_pml", &ev);
-        HIPCHK(h, launch_node_pml<R>(h->N, h->PN, P.ef, P.n, dt, mode, nt + 1, st));
-        timed_end(h, ev);
-        // join: the next step (and anything after the run) sees the fault pairs' update
-        if (!h->single_stream) HIPCHK(h, hipStreamWaitEvent(st, h->ev_join, 0));
-        if (h->timing) {
-            size_t most = 0;
-            for (auto &kv : h->timers) most = std::max(most, kv.second.used);
-            if (most >= kTimerPool) {
-                int rc = drain_timers(h);
-                if (rc) return rc;
-            }
-        }
-    }
-    h->pending = true;
-    return EQD_OK;
-}
-
 extern "C" int eqd_run(eqd_handle *h, int32_t nsteps)
 {
     if (!h) return EQD_EINVAL;
     if (!h->committed) return fail(h, EQD_ESTATE, "eqd_run before eqd_commit");
+    if (h->ensemble) return fail(h, EQD_ESTATE, "the handle belongs to an ensemble: run the ensemble");
     if (nsteps < 0) return fail(h, EQD_EINVAL, "nsteps < 0");
     if (nsteps == 0) return EQD_OK;
-    HIPCHK(h, hipSetDevice(h->s.device));
-    for (auto &kv : h->timers) {
-        kv.second.used = 0;
-        kv.second.total_ms = 0;
-        kv.second.launches = 0;
-    }
-    int rc = h->s.precision == EQD_PREC_FP32 ? run_steps(h, h->E32, h->P32, nsteps) : run_steps(h, h->E64, h->P64, nsteps);
-    if (rc) return rc;
-    HIPCHK(h, hipStreamSynchronize(h->stream));
-    if (h->timing && (rc = drain_timers(h))) return rc;
-    int32_t flag[2];
-    HIPCHK(h, hipMemcpy(flag, h->N.nanflag, sizeof(flag), hipMemcpyDeviceToHost));
-    if (flag[0] != 0) return fail(h, EQD_ENAN, "Velocity NaN at node %d at time step %d", flag[0], flag[1]);
-    return EQD_OK;
+    const int rc = engine_run(h->eng, nsteps);
+    if (rc) h->err = h->eng->err;
+    return rc;
 }
 
 template <typename T>
@@ -1182,8 +977,9 @@ int eqd_nan_info(eqd_handle *h, int32_t *node, int32_t *nt)
 int eqd_kernel_time(eqd_handle *h, const char *name, double *avg_ms, int64_t *launches)
 {
     if (!h || !name) return EQD_EINVAL;
-    auto it = h->timers.find(name);
-    if (it == h->timers.end() || it->second.launches == 0) {
+    if (!h->eng) return fail(h, EQD_ESTATE, "eqd_kernel_time before eqd_commit");
+    auto it = h->eng->timers.find(name);
+    if (it == h->eng->timers.end() || it->second.launches == 0) {
         if (avg_ms) *avg_ms = 0.0;
         if (launches) *launches = 0;
         return EQD_OK;
@@ -1197,6 +993,7 @@ int eqd_set_timing(eqd_handle *h, int32_t on)
 {
     if (!h) return EQD_EINVAL;
     h->timing = on != 0;
+    if (h->eng) h->eng->timing = h->timing;
     return EQD_OK;
 }
 
@@ -1212,6 +1009,7 @@ int eqd_set_guess(eqd_handle *h, const float *params, int64_t nparams, int32_t k
         // mode's iteration policy without the network (k_fallback > 0)
         h->learn = false;
         h->F.k_fallback = k_fallback;
+        h->version++;
         return EQD_OK;
     }
     if (nparams != EQD_MLP_NPARAMS)
@@ -1247,6 +1045,7 @@ int eqd_set_guess(eqd_handle *h, const float *params, int64_t nparams, int32_t k
     h->F.k_fallback = k_fallback;
     h->F.gate = gate;
     h->learn = true;
+    h->version++;
     return EQD_OK;
 }
 
@@ -1261,16 +1060,19 @@ int eqd_harvest(eqd_handle *h, int32_t every, int64_t capacity)
         (void)hipFree(h->harvest);
         h->harvest = nullptr;
     }
-    h->harvest_cap = 0;
-    h->harvest_rows = 0;
-    h->harvest_every = 0;
+    h->F.harvest = nullptr;
+    h->F.harvest_cap = 0;
+    h->F.harvest_every = 0;
+    h->F.harvest_t0 = h->nt;
+    h->version++;
     if (capacity == 0) return EQD_OK;
     if (hipMalloc(&h->harvest, sizeof(float) * 7 * (size_t)capacity) != hipSuccess) {
         h->harvest = nullptr;
         return fail(h, EQD_EOOM, "harvest buffer of %lld rows", (long long)capacity);
     }
-    h->harvest_cap = capacity;
-    h->harvest_every = every;
+    h->F.harvest = h->harvest;
+    h->F.harvest_cap = capacity;
+    h->F.harvest_every = every;
     return EQD_OK;
 }
 
@@ -1280,8 +1082,13 @@ int eqd_harvest_get(eqd_handle *h, float *out, int64_t max_rows, int64_t *rows)
     if (!h->committed) return fail(h, EQD_ESTATE, "eqd_harvest_get before eqd_commit");
     HIPCHK(h, hipSetDevice(h->s.device));
     HIPCHK(h, hipStreamSynchronize(h->stream));
-    *rows = h->harvest_rows;
-    const int64_t n = std::min(max_rows, h->harvest_rows);
+    // the fault kernel writes nf rows at every harvest step since harvest_t0
+    const FaultState &F = h->F;
+    int64_t have = 0;
+    if (F.harvest && F.harvest_every > 0)
+        have = std::min(F.harvest_cap, F.nf * (int64_t)(h->nt / F.harvest_every - F.harvest_t0 / F.harvest_every));
+    *rows = have;
+    const int64_t n = std::min(max_rows, have);
     if (n > 0) HIPCHK(h, hipMemcpy(out, h->harvest, sizeof(float) * 7 * n, hipMemcpyDeviceToHost));
     return EQD_OK;
 }
@@ -1322,20 +1129,14 @@ const char *eqd_last_error(eqd_handle *h) { return h ? h->err.c_str() : "null ha
 int eqd_destroy(eqd_handle *h)
 {
     if (!h) return EQD_OK;
+    if (h->ensemble) return fail(h, EQD_ESTATE, "the handle belongs to an ensemble: destroy the ensemble first");
     (void)hipSetDevice(h->s.device);
-    if (h->stream) (void)hipStreamSynchronize(h->stream);
-    if (h->stream2) (void)hipStreamSynchronize(h->stream2);
+    if (h->eng) {
+        engine_free(h->eng);
+        delete h->eng;
+    }
     for (void *p : h->allocs) (void)hipFree(p);
     if (h->harvest) (void)hipFree(h->harvest);
-    for (auto &kv : h->timers)
-        for (auto &e : kv.second.ev) {
-            (void)hipEventDestroy(e.first);
-            (void)hipEventDestroy(e.second);
-        }
-    if (h->ev_fork) (void)hipEventDestroy(h->ev_fork);
-    if (h->ev_join) (void)hipEventDestroy(h->ev_join);
-    if (h->stream2) (void)hipStreamDestroy(h->stream2);
-    if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
     return EQD_OK;
 }

@@ -111,10 +111,12 @@ struct FaultState {
     MlpNet net;
     int32_t k_fallback;     // 0: the reference's solve; > 0: the faster mode's iteration policy
     float gate;             // predicted |ln(V/V_prev)| below which the plain guess is kept
-    // harvest of (inputs, converged ln V): rows [harvest_cap][7]; this step's
-    // rows start at harvest_base (no harvest when harvest is null)
+    // harvest of (inputs, converged ln V): rows [harvest_cap][7], nf rows at
+    // every step nt > harvest_t0 with nt % harvest_every == 0, in step order
+    // (no harvest when harvest is null)
     float *harvest;
-    int64_t harvest_base, harvest_cap;
+    int64_t harvest_cap;
+    int32_t harvest_every, harvest_t0;
     const int32_t *nsmp;    // [2][nf] slave, master (0-based node ids)
     const double *un, *us, *ud; // [3][nf]
     const double *arn;      // [nf]
@@ -132,5 +134,75 @@ struct OffStations {
     const int32_t *idh;    // [3][nrow]: node (0-based), component (0-based), kind (1 disp, 2 vel)
     double *hist;          // [nstep][nrow+1]
 };
+
+// The step clock of one scenario, in device memory.  The clock kernel
+// advances it at the start of every step (nt += 1, time += dt, the same FP64
+// additions as the reference's timeElapsed), so a captured step replays
+// unchanged; every kernel of the step reads nt, time and mode from here.
+struct Clock {
+    double time;      // timeElapsed after this step's advance
+    int32_t nt;       // current step
+    int32_t mode;     // 1 on the last step of a run: store F/m instead of the fused update
+    int32_t run_end;  // the step at which the current run ends
+    int32_t pad;
+};
+
+// launch kinds of one step
+enum LaunchKind : int {
+    kKElem = 0, kKPml, kKNode3, kKNodePml, kKFault, kKStation, kKClock, kKPend3, kKPendPml, kKinds
+};
+
+// Everything one scenario's step reads: every kernel takes an array of these
+// (one per scenario of a batched launch) and finds its scenario from the
+// launch's block-to-scenario table; blk0[kind] is the scenario's first block
+// in that kind's launch.
+template <typename R>
+struct Member {
+    RegElems<R> E;
+    PmlElems<R> P;
+    NodeState N;
+    PmlNodes PN;
+    FaultState F;
+    OffStations S;
+    StepConsts C;
+    const int32_t *node3_free;  // 3-component nodes that are not split nodes
+    int64_t n_node3_free;
+    const int32_t *node3_all;   // all 3-component nodes
+    int64_t n_node3_all;
+    Clock *clk;
+    int32_t blk0[kKinds];
+};
+
+// one launch of one kind: grid and block size, the block-to-scenario table
+// (device memory, null for a single scenario), XCD remapping, dynamic LDS
+struct LaunchGrid {
+    unsigned grid;
+    int block;
+    int remap;
+    const uint16_t *bmem;
+    size_t lds;
+};
+
+#ifdef __HIPCC__
+// Block b of a launch runs on XCD b % 8 (blocks are dealt out round-robin).
+// With remap the blocks are renumbered so that each XCD takes one contiguous
+// range of the launch's work: neighbouring elements, which share nodes, are
+// then served by the same L2.  A bijection of [0, gridDim.x).
+__device__ __forceinline__ unsigned launch_block(int remap)
+{
+    const unsigned b = blockIdx.x, g = gridDim.x;
+    if (!remap) return b;
+    const unsigned x = b & 7u, i = b >> 3, q = g >> 3, r = g & 7u;
+    // XCD x runs q + 1 blocks when x < r, else q
+    return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
+}
+
+// scenario of logical block gb (a batched launch has a table; a single
+// scenario's launch has none)
+__device__ __forceinline__ int member_of(const uint16_t *bmem, unsigned gb)
+{
+    return bmem ? __builtin_amdgcn_readfirstlane((int)bmem[gb]) : 0;
+}
+#endif
 
 }  // namespace eqd

@@ -51,11 +51,17 @@ __device__ __forceinline__ R sqr(R x) { return x * x; }
 // (n_opc * kOpW + n_phic * 32) values) at block start, so the operator reads
 // of the element loop are LDS reads instead of vector-memory loads.
 template <typename R, bool kQ, bool kDP, int kHG, bool kMass, bool kLds>
-__global__ void __launch_bounds__(256) elem_regular_kernel(RegElems<R> E, NodeState N, StepConsts C)
+__global__ void __launch_bounds__(256) elem_regular_kernel(const Member<R> *__restrict__ Ms,
+                                                           const uint16_t *__restrict__ bmem, int remap)
 {
     extern __shared__ __attribute__((aligned(16))) unsigned char eqd_smem[];
     typedef __attribute__((address_space(3))) R lds_R;
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned gb = launch_block(remap);
+    const Member<R> &M = Ms[member_of(bmem, gb)];
+    const RegElems<R> E = M.E;
+    const NodeState N = M.N;
+    const StepConsts C = M.C;
+    const int64_t t = (int64_t)(gb - (unsigned)M.blk0[kKElem]) * blockDim.x + threadIdx.x;
     const R *op = nullptr, *pht = nullptr;
     const lds_R *lrow = nullptr, *lphr = nullptr;
     if constexpr (kLds) {
@@ -354,11 +360,17 @@ __global__ void __launch_bounds__(256) elem_regular_kernel(RegElems<R> E, NodeSt
 // forms each node's 12 (or collapsed 3) force slots, hourglass included, and
 // writes every slot once.
 template <typename R, int kHG, bool kMass, bool kLds>
-__global__ void __launch_bounds__(256) elem_pml_kernel(PmlElems<R> E, NodeState N, StepConsts C)
+__global__ void __launch_bounds__(256) elem_pml_kernel(const Member<R> *__restrict__ Ms,
+                                                       const uint16_t *__restrict__ bmem, int remap)
 {
     extern __shared__ __attribute__((aligned(16))) unsigned char eqd_smem[];
     typedef __attribute__((address_space(3))) R lds_R;
-    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned gb = launch_block(remap);
+    const Member<R> &M = Ms[member_of(bmem, gb)];
+    const PmlElems<R> E = M.P;
+    const NodeState N = M.N;
+    const StepConsts C = M.C;
+    const int64_t p = (int64_t)(gb - (unsigned)M.blk0[kKPml]) * blockDim.x + threadIdx.x;
     const R *op = nullptr, *pht = nullptr;
     const lds_R *lrow = nullptr, *lphr = nullptr;
     if constexpr (kLds) {
@@ -677,13 +689,20 @@ __device__ __forceinline__ void flag_nan(const NodeState &N, int node, int step,
 // mode 0: gather, a = F/m, then the update of the next step (fused);
 // mode 1: gather, store a = F/m (the last step of a run).
 template <typename R>
-__global__ void __launch_bounds__(256) node3_kernel(NodeState N, const int32_t *list, int64_t nlist, const R *ef,
-                                                    int64_t nreg, const R *efp, int64_t npe, double dt,
-                                                    int mode, int next_step)
+__global__ void __launch_bounds__(256) node3_kernel(const Member<R> *__restrict__ Ms, const uint16_t *__restrict__ bmem,
+                                                    int remap)
 {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nlist) return;
-    const int node = list[i];
+    const unsigned gb = launch_block(remap);
+    const Member<R> &M = Ms[member_of(bmem, gb)];
+    const int64_t i = (int64_t)(gb - (unsigned)M.blk0[kKNode3]) * blockDim.x + threadIdx.x;
+    if (i >= M.n_node3_free) return;
+    const NodeState N = M.N;
+    const R *ef = M.E.ef, *efp = M.P.ef;
+    const int64_t nreg = M.E.n, npe = M.P.n;
+    const double dt = M.C.dt;
+    const Clock ck = *M.clk;
+    const int mode = ck.mode, next_step = ck.nt + 1;
+    const int node = M.node3_free[i];
     const int64_t nn = N.nn;
     double F[3];
     gather3<R>(N, node, ef, nreg, efp, npe, F);
@@ -709,11 +728,20 @@ __global__ void __launch_bounds__(256) node3_kernel(NodeState N, const int32_t *
 // regular elements never touch a PML node (checked on the host).
 // src/driver.f90:107-141 for the damped split update.
 template <typename R>
-__global__ void __launch_bounds__(256) node_pml_kernel(NodeState N, PmlNodes P, const R *efp, int64_t npe,
-                                                       double dt, int mode, int next_step)
+__global__ void __launch_bounds__(256) node_pml_kernel(const Member<R> *__restrict__ Ms,
+                                                       const uint16_t *__restrict__ bmem, int remap)
 {
-    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= P.n) return;
+    const unsigned gb = launch_block(remap);
+    const Member<R> &M = Ms[member_of(bmem, gb)];
+    const int64_t q = (int64_t)(gb - (unsigned)M.blk0[kKNodePml]) * blockDim.x + threadIdx.x;
+    if (q >= M.PN.n) return;
+    const NodeState N = M.N;
+    const PmlNodes P = M.PN;
+    const R *efp = M.P.ef;
+    const int64_t npe = M.P.n;
+    const double dt = M.C.dt;
+    const Clock ck = *M.clk;
+    const int mode = ck.mode, next_step = ck.nt + 1;
     const int node = P.node[q];
     const int64_t nn = N.nn, np = P.n;
     double F[12];
@@ -772,12 +800,18 @@ __global__ void __launch_bounds__(256) node_pml_kernel(NodeState N, PmlNodes P, 
 
 // Pending update at the start of a run: v += a*dt from the stored F/m
 // (velDispUpdate with the previous step's nodalForceArr).
-__global__ void __launch_bounds__(256) update3_from_acc_kernel(NodeState N, const int32_t *list, int64_t nlist,
-                                                               double dt, int next_step)
+template <typename R>
+__global__ void __launch_bounds__(256) update3_from_acc_kernel(const Member<R> *__restrict__ Ms,
+                                                               const uint16_t *__restrict__ bmem)
 {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nlist) return;
-    const int node = list[i];
+    const unsigned gb = blockIdx.x;
+    const Member<R> &M = Ms[member_of(bmem, gb)];
+    const int64_t i = (int64_t)(gb - (unsigned)M.blk0[kKPend3]) * blockDim.x + threadIdx.x;
+    if (i >= M.n_node3_all) return;
+    const NodeState N = M.N;
+    const double dt = M.C.dt;
+    const int next_step = M.clk->nt + 1;
+    const int node = M.node3_all[i];
     const int64_t nn = N.nn;
     double vn[3];
 #pragma unroll
@@ -790,10 +824,18 @@ __global__ void __launch_bounds__(256) update3_from_acc_kernel(NodeState N, cons
     flag_nan(N, node, next_step, vn[0], vn[1], vn[2]);
 }
 
-__global__ void __launch_bounds__(256) update_pml_from_acc_kernel(NodeState N, PmlNodes P, double dt, int next_step)
+template <typename R>
+__global__ void __launch_bounds__(256) update_pml_from_acc_kernel(const Member<R> *__restrict__ Ms,
+                                                                  const uint16_t *__restrict__ bmem)
 {
-    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= P.n) return;
+    const unsigned gb = blockIdx.x;
+    const Member<R> &M = Ms[member_of(bmem, gb)];
+    const int64_t q = (int64_t)(gb - (unsigned)M.blk0[kKPendPml]) * blockDim.x + threadIdx.x;
+    if (q >= M.PN.n) return;
+    const NodeState N = M.N;
+    const PmlNodes P = M.PN;
+    const double dt = M.C.dt;
+    const int next_step = M.clk->nt + 1;
     const int node = P.node[q];
     const int64_t nn = N.nn, np = P.n;
     const double d[3] = {P.damp[q], P.damp[np + q], P.damp[2 * np + q]};
@@ -884,13 +926,21 @@ __device__ __forceinline__ void slip_law(double V2, double &psi, const RsfParams
 // then: lanes past the last pair read pair nf-1 and write only their own
 // padding rows of fric, then leave).
 template <typename R, bool kLearn>
-__global__ void __launch_bounds__(128) fault_kernel(FaultState Fs, NodeState N, StepConsts C, const R *ef,
-                                                    int64_t nreg, const R *efp, int64_t npe, int nt,
-                                                    double time, int mode)
+__global__ void __launch_bounds__(128) fault_kernel(const Member<R> *__restrict__ Ms, const uint16_t *__restrict__ bmem)
 {
     __shared__ __attribute__((aligned(16))) __bf16 mlp_lds0[kLearn ? 2 * 64 * kLd0 : 8];
     __shared__ __attribute__((aligned(16))) __bf16 mlp_lds[kLearn ? 2 * 64 * kLd : 8];
-    const int64_t i0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned gb = blockIdx.x;
+    const Member<R> &M = Ms[member_of(bmem, gb)];
+    const FaultState Fs = M.F;
+    const NodeState N = M.N;
+    const StepConsts C = M.C;
+    const R *ef = M.E.ef, *efp = M.P.ef;
+    const int64_t nreg = M.E.n, npe = M.P.n;
+    const Clock ck = *M.clk;
+    const int nt = ck.nt, mode = ck.mode;
+    const double time = ck.time;
+    const int64_t i0 = (int64_t)(gb - (unsigned)M.blk0[kKFault]) * blockDim.x + threadIdx.x;
     if (!kLearn && i0 >= Fs.nf) return;
     const bool active = i0 < Fs.nf;
     const int64_t i = active ? i0 : Fs.nf - 1;
@@ -1132,8 +1182,10 @@ __global__ void __launch_bounds__(128) fault_kernel(FaultState Fs, NodeState N, 
         Fs.nstat[2 * ld + i] = Fs.nstat[2 * ld + i] + fb;
         Fs.nstat[3 * ld + i] = Fs.nstat[3 * ld + i] + (res == 0 ? 1 : 0);
         Fs.nstat[4 * ld + i] = Fs.nstat[4 * ld + i] + (res == 2 ? 1 : 0);
-        if (Fs.harvest) {
-            const int64_t row = Fs.harvest_base + i;
+        if (Fs.harvest && Fs.harvest_every > 0 && nt % Fs.harvest_every == 0 && nt > Fs.harvest_t0) {
+            // this is harvest step k (0-based) since harvest_t0
+            const int64_t k = (nt - 1) / Fs.harvest_every - Fs.harvest_t0 / Fs.harvest_every;
+            const int64_t row = k * nf + i;
             if (row < Fs.harvest_cap) {
                 float *hr = Fs.harvest + row * 7;
                 for (int k = 0; k < kMlpIn; k++) hr[k] = feat[k];
@@ -1211,10 +1263,18 @@ __global__ void __launch_bounds__(128) fault_kernel(FaultState Fs, NodeState N, 
 }
 
 // storeOffFaultStData, src/driver.f90:157-180
-__global__ void station_kernel(OffStations S, NodeState N, int nt, double time)
+template <typename R>
+__global__ void __launch_bounds__(64) station_kernel(const Member<R> *__restrict__ Ms, const uint16_t *__restrict__ bmem)
 {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned gb = blockIdx.x;
+    const Member<R> &M = Ms[member_of(bmem, gb)];
+    const OffStations S = M.S;
+    const NodeState N = M.N;
+    const int r = (int)(gb - (unsigned)M.blk0[kKStation]) * blockDim.x + threadIdx.x;
     if (r > S.nrow) return;
+    const Clock ck = *M.clk;
+    const int nt = ck.nt;
+    const double time = ck.time;
     double *col = S.hist + (int64_t)(nt - 1) * (S.nrow + 1);
     if (r == 0) {
         col[0] = time;
@@ -1228,65 +1288,67 @@ __global__ void station_kernel(OffStations S, NodeState N, int nt, double time)
         col[r] = N.vel[k * N.nn + node];
 }
 
+// The step clock: nt += 1 and time += dt for every scenario of the launch,
+// and the last-step flag of the run (src/driver.f90:9-11).
+template <typename R>
+__global__ void __launch_bounds__(64) clock_kernel(const Member<R> *__restrict__ Ms, int nmem)
+{
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= nmem) return;
+    Clock *c = Ms[m].clk;
+    const int nt = c->nt + 1;
+    c->nt = nt;
+    c->time = c->time + Ms[m].C.dt;
+    c->mode = nt == c->run_end ? 1 : 0;
+}
+
 // ------------------------------------------------------------------------
 // launchers (host side)
 static inline unsigned grid_for(int64_t n, int block) { return (unsigned)((n + block - 1) / block); }
 
-
 template <typename R>
-hipError_t launch_elem_regular(const RegElems<R> &E, const NodeState &N, const StepConsts &C, bool q, bool dp,
-                               int hg, bool mass, hipStream_t s)
+hipError_t launch_elem(const Member<R> *M, const LaunchGrid &g, int q, int dp, int hg, int mass, int lds_on,
+                       hipStream_t s)
 {
-    if (E.n == 0) return hipSuccess;
-    const unsigned g = grid_for(E.n, 256);
-    // class tables into LDS when both exist and together take at most 16 KiB
-    const size_t lds = (E.optab && E.phitab) ? sizeof(R) * ((size_t)E.n_opc * kOpW + (size_t)E.n_phic * 32) : 0;
-    const bool use_lds = lds > 0 && lds <= 16384;
-#define EQD_LAUNCH(Q, DP, HG, MASS)                                                                   \
-    do {                                                                                              \
-        if (use_lds)                                                                                  \
-            elem_regular_kernel<R, Q, DP, HG, MASS, true><<<g, 256, lds, s>>>(E, N, C);               \
-        else                                                                                          \
-            elem_regular_kernel<R, Q, DP, HG, MASS, false><<<g, 256, 0, s>>>(E, N, C);                \
+    if (g.grid == 0) return hipSuccess;
+    if (g.block < 64 || g.block > 256 || g.block % 64) return hipErrorInvalidValue;
+    // class tables in LDS (at most 16 KiB) or operators from memory
+    const size_t lds = lds_on ? g.lds : 0;
+#define EQD_LAUNCH(Q, DP, HG, MASS)                                                                        \
+    do {                                                                                                   \
+        if (lds_on)                                                                                        \
+            elem_regular_kernel<R, Q, DP, HG, MASS, true><<<g.grid, g.block, lds, s>>>(M, g.bmem, g.remap); \
+        else                                                                                               \
+            elem_regular_kernel<R, Q, DP, HG, MASS, false><<<g.grid, g.block, 0, s>>>(M, g.bmem, g.remap); \
     } while (0)
-    if (q && mass) {
-        if (hg == 1) EQD_LAUNCH(true, false, 1, true);
-        else if (hg == 2) EQD_LAUNCH(true, false, 2, true);
-        else EQD_LAUNCH(true, false, 0, true);
-    } else if (q) {
-        if (hg == 1) EQD_LAUNCH(true, false, 1, false);
-        else if (hg == 2) EQD_LAUNCH(true, false, 2, false);
-        else EQD_LAUNCH(true, false, 0, false);
-    } else if (dp) {
-        if (hg == 1) EQD_LAUNCH(false, true, 1, true);
-        else if (hg == 2) EQD_LAUNCH(false, true, 2, true);
-        else EQD_LAUNCH(false, true, 0, true);
-    } else if (mass) {
-        if (hg == 1) EQD_LAUNCH(false, false, 1, true);
-        else if (hg == 2) EQD_LAUNCH(false, false, 2, true);
-        else EQD_LAUNCH(false, false, 0, true);
-    } else {
-        if (hg == 1) EQD_LAUNCH(false, false, 1, false);
-        else if (hg == 2) EQD_LAUNCH(false, false, 2, false);
-        else EQD_LAUNCH(false, false, 0, false);
-    }
+#define EQD_HG(Q, DP, MASS)                         \
+    do {                                            \
+        if (hg == 1) EQD_LAUNCH(Q, DP, 1, MASS);    \
+        else if (hg == 2) EQD_LAUNCH(Q, DP, 2, MASS); \
+        else EQD_LAUNCH(Q, DP, 0, MASS);            \
+    } while (0)
+    if (q && mass) EQD_HG(true, false, true);
+    else if (q) EQD_HG(true, false, false);
+    else if (dp) EQD_HG(false, true, true);
+    else if (mass) EQD_HG(false, false, true);
+    else EQD_HG(false, false, false);
+#undef EQD_HG
 #undef EQD_LAUNCH
     return hipGetLastError();
 }
 
 template <typename R>
-hipError_t launch_elem_pml(const PmlElems<R> &E, const NodeState &N, const StepConsts &C, int hg, bool mass, hipStream_t s)
+hipError_t launch_pml(const Member<R> *M, const LaunchGrid &g, int hg, int mass, int lds_on, hipStream_t s)
 {
-    if (E.n == 0) return hipSuccess;
-    const unsigned g = grid_for(E.n, 256);
-    const size_t lds = (E.optab && E.phitab) ? sizeof(R) * ((size_t)E.n_opc * kOpW + (size_t)E.n_phic * 32) : 0;
-    const bool use_lds = lds > 0 && lds <= 16384;
+    if (g.grid == 0) return hipSuccess;
+    if (g.block < 64 || g.block > 256 || g.block % 64) return hipErrorInvalidValue;
+    const size_t lds = lds_on ? g.lds : 0;
 #define EQD_LAUNCH(HG, MASS)                                                                          \
     do {                                                                                              \
-        if (use_lds)                                                                                  \
-            elem_pml_kernel<R, HG, MASS, true><<<g, 256, lds, s>>>(E, N, C);                          \
+        if (lds_on)                                                                                   \
+            elem_pml_kernel<R, HG, MASS, true><<<g.grid, g.block, lds, s>>>(M, g.bmem, g.remap);     \
         else                                                                                          \
-            elem_pml_kernel<R, HG, MASS, false><<<g, 256, 0, s>>>(E, N, C);                           \
+            elem_pml_kernel<R, HG, MASS, false><<<g.grid, g.block, 0, s>>>(M, g.bmem, g.remap);      \
     } while (0)
     if (mass) {
         if (hg == 1) EQD_LAUNCH(1, true);
@@ -1302,64 +1364,70 @@ hipError_t launch_elem_pml(const PmlElems<R> &E, const NodeState &N, const StepC
 }
 
 template <typename R>
-hipError_t launch_node3(const NodeState &N, const int32_t *list, int64_t nlist, const R *ef, int64_t nreg,
-                        const R *efp, int64_t npe, double dt, int mode, int next_step, hipStream_t s)
+hipError_t launch_node3(const Member<R> *M, const LaunchGrid &g, hipStream_t s)
 {
-    if (nlist == 0) return hipSuccess;
-    node3_kernel<R><<<grid_for(nlist, 256), 256, 0, s>>>(N, list, nlist, ef, nreg, efp, npe, dt, mode, next_step);
+    if (g.grid == 0) return hipSuccess;
+    if (g.block < 64 || g.block > 256 || g.block % 64) return hipErrorInvalidValue;
+    node3_kernel<R><<<g.grid, g.block, 0, s>>>(M, g.bmem, g.remap);
     return hipGetLastError();
 }
 
 template <typename R>
-hipError_t launch_node_pml(const NodeState &N, const PmlNodes &P, const R *efp, int64_t npe, double dt, int mode,
-                           int next_step, hipStream_t s)
+hipError_t launch_node_pml(const Member<R> *M, const LaunchGrid &g, hipStream_t s)
 {
-    if (P.n == 0) return hipSuccess;
-    node_pml_kernel<R><<<grid_for(P.n, 256), 256, 0, s>>>(N, P, efp, npe, dt, mode, next_step);
-    return hipGetLastError();
-}
-
-hipError_t launch_update_from_acc(const NodeState &N, const int32_t *list3, int64_t n3, const PmlNodes &P, double dt,
-                                  int next_step, hipStream_t s)
-{
-    if (n3 > 0) update3_from_acc_kernel<<<grid_for(n3, 256), 256, 0, s>>>(N, list3, n3, dt, next_step);
-    if (P.n > 0) update_pml_from_acc_kernel<<<grid_for(P.n, 256), 256, 0, s>>>(N, P, dt, next_step);
+    if (g.grid == 0) return hipSuccess;
+    if (g.block < 64 || g.block > 256 || g.block % 64) return hipErrorInvalidValue;
+    node_pml_kernel<R><<<g.grid, g.block, 0, s>>>(M, g.bmem, g.remap);
     return hipGetLastError();
 }
 
 template <typename R>
-hipError_t launch_fault(const FaultState &F, const NodeState &N, const StepConsts &C, const R *ef, int64_t nreg,
-                        const R *efp, int64_t npe, int nt, double time, int mode, bool learn, hipStream_t s)
+hipError_t launch_pending(const Member<R> *M, const LaunchGrid &g3, const LaunchGrid &gp, hipStream_t s)
 {
-    if (F.nf == 0) return hipSuccess;
-    // learned mode keeps every lane of the grid alive: fric and nstat rows
-    // must cover the whole grid
-    if (learn && F.ld < (int64_t)grid_for(F.nf, 128) * 128) return hipErrorInvalidValue;
+    if (g3.grid > 0) update3_from_acc_kernel<R><<<g3.grid, g3.block, 0, s>>>(M, g3.bmem);
+    if (gp.grid > 0) update_pml_from_acc_kernel<R><<<gp.grid, gp.block, 0, s>>>(M, gp.bmem);
+    return hipGetLastError();
+}
+
+template <typename R>
+hipError_t launch_fault(const Member<R> *M, const LaunchGrid &g, int learn, hipStream_t s)
+{
+    if (g.grid == 0) return hipSuccess;
+    // the learned variant evaluates the network one wavefront at a time
+    if (g.block != 64 && g.block != 128) return hipErrorInvalidValue;
     if (learn)
-        fault_kernel<R, true><<<grid_for(F.nf, 128), 128, 0, s>>>(F, N, C, ef, nreg, efp, npe, nt, time, mode);
+        fault_kernel<R, true><<<g.grid, g.block, 0, s>>>(M, g.bmem);
     else
-        fault_kernel<R, false><<<grid_for(F.nf, 128), 128, 0, s>>>(F, N, C, ef, nreg, efp, npe, nt, time, mode);
+        fault_kernel<R, false><<<g.grid, g.block, 0, s>>>(M, g.bmem);
     return hipGetLastError();
 }
 
-hipError_t launch_station(const OffStations &S, const NodeState &N, int nt, double time, hipStream_t s)
+template <typename R>
+hipError_t launch_station(const Member<R> *M, const LaunchGrid &g, hipStream_t s)
 {
-    if (S.nrow == 0) return hipSuccess;
-    station_kernel<<<grid_for(S.nrow + 1, 64), 64, 0, s>>>(S, N, nt, time);
+    if (g.grid == 0) return hipSuccess;
+    if (g.block != 64) return hipErrorInvalidValue;
+    station_kernel<R><<<g.grid, 64, 0, s>>>(M, g.bmem);
     return hipGetLastError();
 }
 
-#define EQD_INSTANTIATE(R)                                                                                         \
-    template hipError_t launch_elem_regular<R>(const RegElems<R> &, const NodeState &, const StepConsts &, bool, bool, \
-                                               int, bool, hipStream_t);                                              \
-    template hipError_t launch_elem_pml<R>(const PmlElems<R> &, const NodeState &, const StepConsts &, int, bool,     \
-                                           hipStream_t);                                                             \
-    template hipError_t launch_node3<R>(const NodeState &, const int32_t *, int64_t, const R *, int64_t, const R *,    \
-                                        int64_t, double, int, int, hipStream_t);                                     \
-    template hipError_t launch_node_pml<R>(const NodeState &, const PmlNodes &, const R *, int64_t, double, int, int,  \
-                                           hipStream_t);                                                             \
-    template hipError_t launch_fault<R>(const FaultState &, const NodeState &, const StepConsts &, const R *, int64_t, \
-                                        const R *, int64_t, int, double, int, bool, hipStream_t);
+template <typename R>
+hipError_t launch_clock(const Member<R> *M, int nmem, hipStream_t s)
+{
+    if (nmem <= 0) return hipSuccess;
+    clock_kernel<R><<<grid_for(nmem, 64), 64, 0, s>>>(M, nmem);
+    return hipGetLastError();
+}
+
+#define EQD_INSTANTIATE(R)                                                                                     \
+    template hipError_t launch_elem<R>(const Member<R> *, const LaunchGrid &, int, int, int, int, int, hipStream_t); \
+    template hipError_t launch_pml<R>(const Member<R> *, const LaunchGrid &, int, int, int, hipStream_t);           \
+    template hipError_t launch_node3<R>(const Member<R> *, const LaunchGrid &, hipStream_t);                        \
+    template hipError_t launch_node_pml<R>(const Member<R> *, const LaunchGrid &, hipStream_t);                     \
+    template hipError_t launch_pending<R>(const Member<R> *, const LaunchGrid &, const LaunchGrid &, hipStream_t);  \
+    template hipError_t launch_fault<R>(const Member<R> *, const LaunchGrid &, int, hipStream_t);                   \
+    template hipError_t launch_station<R>(const Member<R> *, const LaunchGrid &, hipStream_t);                      \
+    template hipError_t launch_clock<R>(const Member<R> *, int, hipStream_t);
 EQD_INSTANTIATE(double)
 EQD_INSTANTIATE(float)
 #undef EQD_INSTANTIATE

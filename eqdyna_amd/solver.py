@@ -8,6 +8,7 @@ There is no CPU fallback: without the built library or a GPU this raises.
 from __future__ import annotations
 
 import ctypes as C
+import json
 import os
 from typing import Dict
 
@@ -19,7 +20,13 @@ LIB_PATH = os.environ.get("EQD_LIB", os.path.join(HERE, "libeqd.so"))
 EXPORTS = ["eqd_create", "eqd_set", "eqd_commit", "eqd_run", "eqd_get", "eqd_clock", "eqd_nan_info",
            "eqd_kernel_time", "eqd_set_timing", "eqd_device_bytes", "eqd_layout_info", "eqd_last_error",
            "eqd_destroy", "eqd_set_guess", "eqd_harvest", "eqd_harvest_get", "eqd_newton_stats", "eqd_mlp_forward",
-           "eqd_mlp_grad", "eqd_mlp_train"]
+           "eqd_mlp_grad", "eqd_mlp_train", "eqd_set_launch", "eqd_get_launch", "eqd_tune", "eqd_set_graph",
+           "eqd_ensemble_create", "eqd_ensemble_run", "eqd_ensemble_tune", "eqd_ensemble_set_launch",
+           "eqd_ensemble_set_graph", "eqd_ensemble_set_timing", "eqd_ensemble_kernel_time",
+           "eqd_ensemble_last_error", "eqd_ensemble_destroy"]
+
+# launch kinds (EQD_KIND_*)
+KINDS = {"element": 0, "element_pml": 1, "node": 2, "node_pml": 3, "fault": 4}
 
 MLP_NTRAIN = 8833
 MLP_NPARAMS = 8847
@@ -102,6 +109,21 @@ def lib():
         L.eqd_mlp_forward.argtypes = [fp, fp, C.c_int64, fp, C.c_int32]
         L.eqd_mlp_grad.argtypes = [fp, fp, fp, C.c_int64, fp, fp, C.c_int32]
         L.eqd_mlp_train.argtypes = [fp, fp, C.c_int64, C.POINTER(TrainOpts), fp, fp]
+        i32p = C.POINTER(C.c_int32)
+        L.eqd_set_launch.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32]
+        L.eqd_get_launch.argtypes = [C.c_void_p, C.c_int32, i32p, i32p]
+        L.eqd_tune.argtypes = [C.c_void_p, C.c_int32, C.c_char_p, C.c_int64]
+        L.eqd_set_graph.argtypes = [C.c_void_p, C.c_int32]
+        L.eqd_ensemble_create.argtypes = [C.POINTER(C.c_void_p), C.c_int32, C.POINTER(C.c_void_p)]
+        L.eqd_ensemble_run.argtypes = [C.c_void_p, C.c_int32]
+        L.eqd_ensemble_tune.argtypes = [C.c_void_p, C.c_int32, C.c_char_p, C.c_int64]
+        L.eqd_ensemble_set_launch.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32]
+        L.eqd_ensemble_set_graph.argtypes = [C.c_void_p, C.c_int32]
+        L.eqd_ensemble_set_timing.argtypes = [C.c_void_p, C.c_int32]
+        L.eqd_ensemble_kernel_time.argtypes = [C.c_void_p, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
+        L.eqd_ensemble_last_error.argtypes = [C.c_void_p]
+        L.eqd_ensemble_last_error.restype = C.c_char_p
+        L.eqd_ensemble_destroy.argtypes = [C.c_void_p]
         _lib = L
     return _lib
 
@@ -244,6 +266,29 @@ class Solver:
     def device_bytes(self) -> int:
         return int(self.L.eqd_device_bytes(self.h))
 
+    def set_launch(self, kind: str, block: int, remap: int = 0) -> None:
+        """Launch shape of one kind (KINDS): block size and XCD remapping."""
+        self._check(self.L.eqd_set_launch(self.h, KINDS[kind], int(block), int(remap)))
+
+    def get_launch(self) -> Dict[str, tuple]:
+        out = {}
+        for k, v in KINDS.items():
+            b, r = C.c_int32(), C.c_int32()
+            self._check(self.L.eqd_get_launch(self.h, v, C.byref(b), C.byref(r)))
+            out[k] = (b.value, r.value)
+        return out
+
+    def tune(self, steps: int = 5) -> dict:
+        """Autotuner: times every launch shape on this scenario (its state is
+        restored afterwards) and keeps the fastest; returns the report."""
+        buf = C.create_string_buffer(1 << 16)
+        self._check(self.L.eqd_tune(self.h, int(steps), buf, len(buf)))
+        return json.loads(buf.value.decode())
+
+    def set_graph(self, on: bool) -> None:
+        """Capture one step as a HIP graph and replay it in run()."""
+        self._check(self.L.eqd_set_graph(self.h, 1 if on else 0))
+
     def close(self) -> None:
         if getattr(self, "h", None):
             self.L.eqd_destroy(self.h)
@@ -298,3 +343,58 @@ def mlp_train(params, X, r, epochs: int, batch: int = 4096, lr: float = 1e-3, se
     if rc != 0:
         raise EqdError(rc, "eqd_mlp_train failed")
     return p, losses
+
+
+class Ensemble:
+    """Scenarios advanced together: one launch per kind per step over all of
+    them, the step captured once as a HIP graph and replayed.  The members stay
+    Solver objects for reading results; they must not be run on their own or
+    closed while the ensemble exists."""
+
+    def __init__(self, members):
+        self.L = lib()
+        self.members = list(members)
+        arr = (C.c_void_p * len(self.members))(*[m.h for m in self.members])
+        e = C.c_void_p()
+        rc = self.L.eqd_ensemble_create(arr, len(self.members), C.byref(e))
+        if rc != 0:
+            raise EqdError(rc, "eqd_ensemble_create rejected the members")
+        self.e = e
+
+    def _check(self, rc: int) -> None:
+        if rc != 0:
+            raise EqdError(rc, self.L.eqd_ensemble_last_error(self.e).decode())
+
+    def run(self, nsteps: int) -> None:
+        self._check(self.L.eqd_ensemble_run(self.e, int(nsteps)))
+
+    def tune(self, steps: int = 5) -> dict:
+        buf = C.create_string_buffer(1 << 16)
+        self._check(self.L.eqd_ensemble_tune(self.e, int(steps), buf, len(buf)))
+        return json.loads(buf.value.decode())
+
+    def set_launch(self, kind: str, block: int, remap: int = 0) -> None:
+        self._check(self.L.eqd_ensemble_set_launch(self.e, KINDS[kind], int(block), int(remap)))
+
+    def set_graph(self, on: bool) -> None:
+        self._check(self.L.eqd_ensemble_set_graph(self.e, 1 if on else 0))
+
+    def set_timing(self, on: bool) -> None:
+        self._check(self.L.eqd_ensemble_set_timing(self.e, 1 if on else 0))
+
+    def kernel_time(self, name: str):
+        ms = C.c_double()
+        n = C.c_int64()
+        self._check(self.L.eqd_ensemble_kernel_time(self.e, name.encode(), C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+    def close(self) -> None:
+        if getattr(self, "e", None):
+            self.L.eqd_ensemble_destroy(self.e)
+            self.e = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

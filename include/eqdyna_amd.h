@@ -212,6 +212,52 @@ typedef struct {
 int eqd_mlp_train(const float *X, const float *r, int64_t n, const eqd_mlp_train_opts *opts, float *params,
                   float *epoch_loss);
 
+/* ---------------------------------------------------------------------------
+ * Launch shapes, captured steps and ensembles.  No counterpart in the
+ * reference (one MPI rank per subdomain, one scenario per run); they change
+ * how the step is launched, never what it computes: every launch shape, the
+ * captured graph and the ensemble give bit-identical results.
+ *
+ * Launch kinds: */
+#define EQD_KIND_ELEMENT 0     /* regular elements */
+#define EQD_KIND_ELEMENT_PML 1 /* PML elements */
+#define EQD_KIND_NODE 2        /* 3-component nodes */
+#define EQD_KIND_NODE_PML 3    /* PML nodes */
+#define EQD_KIND_FAULT 4       /* split-node pairs */
+
+/* Block size (64, 128 or 256; fault pairs 64 or 128) and XCD remapping (1:
+ * each of the 8 XCDs takes one contiguous range of the launch) of one kind. */
+int eqd_set_launch(eqd_handle *h, int32_t kind, int32_t block, int32_t remap);
+int eqd_get_launch(eqd_handle *h, int32_t kind, int32_t *block, int32_t *remap);
+
+/* Autotuner: times every launch shape of each kind on the handle's own
+ * scenario (1 + steps steps per candidate, state saved before and restored
+ * after, so the scenario does not advance; needs nt + 1 + steps <= nstep),
+ * keeps the fastest, and writes a JSON report (per kind: the choice, its
+ * milliseconds per launch and all candidates) into report[report_cap]. */
+int eqd_tune(eqd_handle *h, int32_t steps, char *report, int64_t report_cap);
+
+/* on = 1: eqd_run captures one step as a HIP graph (once) and replays it;
+ * per-kernel timing (eqd_set_timing) runs steps uncaptured. */
+int eqd_set_graph(eqd_handle *h, int32_t on);
+
+/* An ensemble advances committed scenarios of one precision on one device
+ * together: each step is one launch per kind over all of them, captured as a
+ * HIP graph and replayed.  Members keep their own state (eqd_get, eqd_clock,
+ * eqd_newton_stats work as usual) but cannot be run or destroyed on their own
+ * until the ensemble is destroyed. */
+typedef struct eqd_ensemble eqd_ensemble;
+int eqd_ensemble_create(eqd_handle *const *members, int32_t n, eqd_ensemble **out);
+/* nsteps steps of every member (each needs nt + nsteps <= its nstep). */
+int eqd_ensemble_run(eqd_ensemble *e, int32_t nsteps);
+int eqd_ensemble_tune(eqd_ensemble *e, int32_t steps, char *report, int64_t report_cap);
+int eqd_ensemble_set_launch(eqd_ensemble *e, int32_t kind, int32_t block, int32_t remap);
+int eqd_ensemble_set_graph(eqd_ensemble *e, int32_t on);
+int eqd_ensemble_set_timing(eqd_ensemble *e, int32_t on);
+int eqd_ensemble_kernel_time(eqd_ensemble *e, const char *name, double *avg_ms, int64_t *launches);
+const char *eqd_ensemble_last_error(eqd_ensemble *e);
+int eqd_ensemble_destroy(eqd_ensemble *e);
+
 #ifdef __cplusplus
 }
 #endif

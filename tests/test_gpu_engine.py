@@ -1,0 +1,147 @@
+"""How a step is launched never changes what it computes: HIP graph replay,
+every launch shape, the autotuner and batched ensembles must all reproduce
+the plain eager run of each scenario bit for bit."""
+import numpy as np
+import pytest
+
+from eqdyna_amd import learned
+from eqdyna_amd.solver import Ensemble, Solver
+from fixtures import load_golden
+
+KEYS = ["fric", "fnft", "velArr", "dispArr", "v1", "stressArr", "onFaultQuantHistSCECForm"]
+
+
+def outputs(s):
+    return {k: s.get(k) for k in KEYS}
+
+
+def assert_same(a, b, tag):
+    for k in KEYS:
+        x, y = a[k], b[k]
+        same = np.array_equal(x, y, equal_nan=True)
+        if not same:
+            d = np.nanmax(np.abs(x.astype(np.float64) - y.astype(np.float64)))
+            pytest.fail(f"{tag}: {k} differs (max abs {d:.3e})")
+
+
+def eager_run(name, precision, chunks, guess=None):
+    bundle, sc, _ = load_golden(name)
+    s = Solver(bundle, sc, precision)
+    if guess is not None:
+        s.set_guess(*guess)
+    for n in chunks:
+        s.run(n)
+    out = outputs(s)
+    s.close()
+    return out
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,precision", [("box_rsf4", "fp64"), ("box_sw", "fp32"), ("box_q", "fp64")])
+def test_graph_replay_matches_eager(name, precision):
+    print(f"case {name} {precision}", flush=True)
+    bundle, sc, _ = load_golden(name)
+    nstep = int(sc["nstep"])
+    chunks = [1, 7, nstep - 8 - 5, 5]
+    ref = eager_run(name, precision, chunks)
+    s = Solver(bundle, sc, precision)
+    s.set_graph(True)
+    for n in chunks:
+        s.run(n)  # every call replays the captured step n times, then synchronizes
+    assert s.clock()[0] == nstep
+    assert_same(outputs(s), ref, f"{name} graph")
+    s.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["box_rsf4", "box_layered"])
+def test_launch_shapes_do_not_change_results(name):
+    print(f"case {name}", flush=True)
+    bundle, sc, _ = load_golden(name)
+    nstep = int(sc["nstep"])
+    ref = eager_run(name, "fp64", [nstep])
+    for block, remap in ((64, 1), (128, 0), (256, 1)):
+        s = Solver(bundle, sc, "fp64")
+        for kind in ("element", "element_pml", "node", "node_pml"):
+            s.set_launch(kind, block, remap)
+        s.set_launch("fault", 64 if block == 64 else 128, 0)
+        s.run(nstep)
+        assert_same(outputs(s), ref, f"{name} block {block} remap {remap}")
+        s.close()
+
+
+@pytest.mark.gpu
+def test_tuner_restores_state_and_keeps_results():
+    name = "box_rsf3"
+    bundle, sc, _ = load_golden(name)
+    nstep = int(sc["nstep"])
+    ref = eager_run(name, "fp64", [10, nstep - 10])
+    s = Solver(bundle, sc, "fp64")
+    s.run(10)
+    before = outputs(s)
+    rep = s.tune(steps=3)
+    print(f"tuner report: {rep}", flush=True)
+    assert set(rep) == {"element", "element_pml", "node", "node_pml", "fault"}
+    for k, v in rep.items():
+        assert v["ms"] > 0 and len(v["candidates"]) == (2 if k == "fault" else 6)
+        assert (v["block"], v["remap"]) == s.get_launch()[k]
+    assert s.clock()[0] == 10
+    assert_same(outputs(s), before, "state after tuning")
+    s.run(nstep - 10)
+    assert_same(outputs(s), ref, "run after tuning")
+    s.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_ensemble_matches_individual_runs(precision):
+    # scenarios of different sizes and physics: slip and time weakening, both
+    # rate-and-state laws (one with the learned guess), Q, a rough fault
+    names = ["box_sw", "box_tw", "box_rsf3", "box_rsf4", "box_q", "box_rough", "box_layered"]
+    params, _ = learned.load()
+    guesses = {"box_rsf4": (params, 6, 0.5), "box_rsf3": (None, 4, 0.5)}
+    steps = {}
+    refs = {}
+    for n in names:
+        _, sc, _ = load_golden(n)
+        steps[n] = int(sc["nstep"])
+        print(f"individual {n} {precision}", flush=True)
+        refs[n] = eager_run(n, precision, [steps[n]], guesses.get(n))
+    common = min(steps.values())
+    members = []
+    for n in names:
+        bundle, sc, _ = load_golden(n)
+        s = Solver(bundle, sc, precision)
+        if n in guesses:
+            s.set_guess(*guesses[n])
+        members.append(s)
+    ens = Ensemble(members)
+    print(f"ensemble of {len(members)} scenarios, {common} common steps", flush=True)
+    ens.run(3)
+    ens.run(common - 3)
+    ens.close()
+    for n, s in zip(names, members):
+        assert s.clock()[0] == common
+        if steps[n] > common:
+            s.run(steps[n] - common)  # back on its own engine after the ensemble
+        assert_same(outputs(s), refs[n], f"ensemble member {n}")
+        s.close()
+
+
+@pytest.mark.gpu
+def test_ensemble_tuner():
+    names = ["box_sw", "box_rsf4", "box_layered"]
+    members = []
+    for n in names:
+        bundle, sc, _ = load_golden(n)
+        members.append(Solver(bundle, sc, "fp32"))
+    ens = Ensemble(members)
+    rep = ens.tune(steps=2)
+    print(f"ensemble tuner report: {rep}", flush=True)
+    assert "element" in rep and "fault" in rep
+    assert all(m.clock()[0] == 0 for m in members)
+    ens.run(20)
+    ens.close()
+    for m in members:
+        assert m.clock()[0] == 20
+        m.close()
