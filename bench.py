@@ -10,7 +10,11 @@ rate-and-state Newton solve -- run twice on the GPU: the plain FP64 mode
 the learned guess (eqdyna_amd/rsf_guess_c3.npz).  `value` is the FP32+learned
 throughput; the FP64 throughput, their ratio and the FP32+learned run's
 rupture-time and peak-slip-rate errors against the FP64 run are in `config`.
-Other workloads (c1, c2, c3) time one run in the --precision given.
+Other workloads (c1, c2, c3) time one run in the --precision given; c5 is
+the batched ensemble (64 seeded scenarios of uneven size, FP32 bulk / FP64
+fault, advanced together as one graph-captured step per time step).
+Before the timed steps the launch-shape tuner (eqd_tune) picks block sizes
+and XCD remapping per kernel on the workload itself (--tune 0 skips it).
 
 A "step" is one explicit time step (src/driver.f90:9-34) over the whole mesh,
 with the mesh resident in HBM.  Prints one JSON line:
@@ -92,8 +96,9 @@ def load_traffic(workload: str, precision: str, kernel: str):
         return None
 
 
-def run_one(bundle, sc, precision: str, warmup: int, steps: int, guess=None):
-    """One timed run: warm-up steps, then `steps` timed with per-kernel events."""
+def run_one(bundle, sc, precision: str, warmup: int, steps: int, guess=None, tune: int = 0):
+    """One timed run: warm-up steps, the tuner (state restored), then `steps`
+    timed with per-kernel events."""
     from eqdyna_amd.solver import Solver
     t = time.perf_counter()
     s = Solver(bundle, sc, precision, device=0, nstep=warmup + steps)
@@ -103,13 +108,18 @@ def run_one(bundle, sc, precision: str, warmup: int, steps: int, guess=None):
     log(f"{precision}{'+learned' if guess is not None else ''}: uploaded ({s.device_bytes() / 1e9:.1f} GB on the "
         f"device) in {time.perf_counter() - t:.1f} s; layout {layout}")
     s.run(warmup)
+    tuned = None
+    if tune > 0 and steps > tune + 1:
+        tuned = s.tune(tune)
+        log(f"  tuner: " + ", ".join(f"{k} block {v['block']} remap {v['remap']} ({v['ms']:.4f} ms)"
+                                     for k, v in tuned.items()))
     s.set_timing(True)
     t0 = time.perf_counter()
     s.run(steps)  # returns after the stream has drained
     wall = time.perf_counter() - t0
     ktimes = {k: s.kernel_time(k) for k in ("element", "element_pml", "fault", "node", "node_pml", "station")}
     r = {"wall": wall, "ktimes": ktimes, "layout": layout, "nt": s.clock()[0], "newton": s.newton_stats(),
-         "fnft": s.get("fnft"), "fric": s.get("fric")}
+         "fnft": s.get("fnft"), "fric": s.get("fric"), "launch": s.get_launch()}
     s.close()
     for k, (ms, n) in ktimes.items():
         log(f"  kernel {k}: {ms:.4f} ms avg over {n} launches")
@@ -152,6 +162,9 @@ def cpu_baseline_reference(workload: str, ranks: int, steps: int):
         c = case.c3(steps=steps)
         sample = (f"c3 at full size (3,456,000 hexahedra, rough fault, slip-law RSF), first {steps} of 3000 "
                   f"steps, plain Newton guess")
+    elif workload == "c5":
+        c = case.c5(steps=steps)[0]
+        sample = f"scenario 0 of the c5 ensemble ({c.name}, {'slip-law RSF' if c.friclaw == 4 else 'slip weakening'}), {steps} steps"
     else:
         c = case.c2_sample(steps=steps)
         sample = (f"C2 physics (50 m, 5-layer medium, Q, SW fault) on a 240x120x120 = 3,456,000-hexahedron "
@@ -203,7 +216,9 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None, help="timed steps (c4: 2990, others: 200)")
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--workload", default="c4", choices=["c4", "c3", "c2", "c1"])
+    ap.add_argument("--workload", default="c4", choices=["c4", "c3", "c2", "c1", "c5"])
+    ap.add_argument("--tune", type=int, default=4, help="steps per candidate of the launch-shape tuner (0: off)")
+    ap.add_argument("--scenarios", type=int, default=64, help="c5 only: ensemble size")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"], help="c1-c3 only")
     ap.add_argument("--cpu-baseline", default="auto", choices=["auto", "reference", "port", "none"])
     ap.add_argument("--cpu-ranks", type=int, default=16)
@@ -217,6 +232,8 @@ def main() -> int:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}). "
               "Several GPUs would each take a slab of the mesh with a halo exchange; that path is not built.", file=sys.stderr)
         return 2
+    if args.workload == "c5":
+        return bench_c5(args)
     steps = args.steps if args.steps is not None else (2990 if args.workload == "c4" else 200)
     total = args.warmup + steps
     c, bundle, sc, desc = build_workload(args.workload, total)
@@ -231,8 +248,8 @@ def main() -> int:
     if args.workload == "c4":
         from eqdyna_amd import learned
         params, meta = learned.load()
-        ref = run_one(bundle, sc, "fp64", args.warmup, steps)
-        run = run_one(bundle, sc, "fp32", args.warmup, steps, guess=params)
+        ref = run_one(bundle, sc, "fp64", args.warmup, steps, tune=args.tune)
+        run = run_one(bundle, sc, "fp32", args.warmup, steps, guess=params, tune=args.tune)
         precision = "fp32"
         v64 = 3.0 * nn * steps / ref["wall"] / 1e6
         n64, n32 = ref["newton"], run["newton"]
@@ -247,7 +264,7 @@ def main() -> int:
         }
     else:
         precision = args.precision
-        run = run_one(bundle, sc, precision, args.warmup, steps)
+        run = run_one(bundle, sc, precision, args.warmup, steps, tune=args.tune)
     del bundle
     value = 3.0 * nn * steps / run["wall"] / 1e6
     if extra:
@@ -276,7 +293,90 @@ def main() -> int:
         "config": {"workload": desc, "elements": int(sc["totalNumOfElements"]), "nodes": nn,
                    "regular_elements": counts["element"], "fault_pairs": int(sc["nftnd"]), "precision": precision,
                    "steps_run": run["nt"],
-                   "operator_classes": {k: v for k, v in run["layout"].items() if "classes" in k}, **extra},
+                   "operator_classes": {k: v for k, v in run["layout"].items() if "classes" in k},
+                   "launch_shapes": {k: {"block": b, "remap": r} for k, (b, r) in run["launch"].items()}, **extra},
+        "roofline": roof, "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def bench_c5(args) -> int:
+    """The batched ensemble: every scenario built and uploaded, advanced
+    together (one launch per kind per step over all scenarios, the step
+    captured as a HIP graph and replayed); value = sum over scenarios of
+    3 * N_nodes * steps / wall time.  Per-kernel times for the roofline come
+    from an uncaptured pass of --kt steps after the timed ones (events cannot
+    sit inside a captured step)."""
+    from eqdyna_amd import case, mesh
+    from eqdyna_amd.solver import Ensemble, Solver
+    steps = args.steps if args.steps is not None else 200
+    kt = 20
+    total = args.warmup + steps + kt
+    cases = case.c5(n=args.scenarios, steps=total)
+    members = []
+    counts = {"element": 0, "element_pml": 0, "node": 0, "node_pml": 0}
+    nn = ne = nf = 0
+    sizes = []
+    t = time.perf_counter()
+    for c in cases:
+        bundle, sc = mesh.build_case(c, nthreads=min(16, os.cpu_count() or 1))
+        et, ndof = bundle["elemTypeArr"], bundle["numOfDofPerNodeArr"]
+        counts["element"] += int((et != 2).sum())
+        counts["element_pml"] += int((et == 2).sum())
+        counts["node_pml"] += int((ndof == 12).sum())
+        counts["node"] += int((ndof == 3).sum()) - 2 * int(sc["nftnd"])
+        nn += int(sc["totalNumOfNodes"])
+        ne += int(sc["totalNumOfElements"])
+        nf += int(sc["nftnd"])
+        sizes.append(int(sc["totalNumOfElements"]))
+        members.append(Solver(bundle, sc, "fp32", device=0, nstep=total))
+        del bundle
+    dev_gb = sum(m.device_bytes() for m in members) / 1e9
+    log(f"c5: {len(members)} scenarios, {ne} elements, {nn} nodes, {nf} fault pairs, {dev_gb:.1f} GB on the device, "
+        f"built and uploaded in {time.perf_counter() - t:.1f} s")
+    ens = Ensemble(members)
+    ens.run(args.warmup)
+    tuned = None
+    if args.tune > 0:
+        t = time.perf_counter()
+        tuned = ens.tune(args.tune)
+        log(f"  tuner ({time.perf_counter() - t:.1f} s): " +
+            ", ".join(f"{k} block {v['block']} remap {v['remap']} ({v['ms']:.4f} ms)" for k, v in tuned.items()))
+    t0 = time.perf_counter()
+    ens.run(steps)  # graph replay; returns after the stream has drained
+    wall = time.perf_counter() - t0
+    ens.set_graph(False)
+    ens.set_timing(True)
+    ens.run(kt)
+    ktimes = {k: ens.kernel_time(k) for k in ("element", "element_pml", "fault", "node", "node_pml", "station")}
+    for k, (ms, n) in ktimes.items():
+        log(f"  kernel {k}: {ms:.4f} ms avg over {n} launches")
+    ens.close()
+    fric = [m.s.friclaw for m in members]
+    for m in members:
+        m.close()
+    value = 3.0 * nn * steps / wall / 1e6
+    roof = roofline({"ktimes": ktimes}, counts, "fp32", False, "c5")
+    roof["kernel_times_from"] = f"uncaptured pass of {kt} steps after the timed steps"
+    cpu = None
+    if args.cpu_baseline in ("auto", "reference"):
+        try:
+            cpu = cpu_baseline_reference("c5", args.cpu_ranks, args.cpu_steps)
+        except Exception as e:  # noqa: BLE001
+            log(f"reference CPU baseline failed: {e}")
+    out = {
+        "metric": "Mdof-steps/s (aggregate over the ensemble)", "value": round(value, 2), "unit": "Mdof-steps/s",
+        "n_gpus": 1, "steps": steps, "warmup": args.warmup, "ms_per_step": round(wall / steps * 1e3, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32 bulk / f64 fault",
+        "data": "synthetic (seeded case generator; no external data)",
+        "config": {"workload": f"c5: batched ensemble of {len(members)} seeded scenarios at 100 m (8-24 x 6-12 x 6-12 km "
+                               "incl. a 10-element PML, 5-layer media, von Karman tau_0, SW or slip-law RSF), "
+                               "one HIP-graph-captured step over all scenarios",
+                   "scenarios": len(members), "elements": ne, "nodes": nn, "fault_pairs": nf,
+                   "elements_min": min(sizes), "elements_max": max(sizes),
+                   "slip_weakening": sum(1 for f in fric if f == 1), "rate_and_state": sum(1 for f in fric if f >= 3),
+                   "device_gb": round(dev_gb, 1), "tuner": tuned},
         "roofline": roof, "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -702,3 +702,117 @@ def c3(steps: int = 3000, seed: int = 3, tau0: float = 10.5e6, vs_top: float = 2
             o[46, iz, ix] = vini
             o[20, iz, ix] = state_steady_state(a, b, L, v0, f0, tau0, sn, vini, 4)
     return c
+
+
+def von_karman_field(nx: int, nz: int, spacing: float, corr_len: float, hurst: float, seed: int) -> np.ndarray:
+    """Seeded 2-D von Karman random field on an (nz, nx) grid: a complex
+    Gaussian spectrum shaped to the von Karman power spectrum
+    P(k) ~ (1 + k^2 a^2)^-(H+1) (correlation length a, Hurst exponent H),
+    the mean removed, transformed back on a grid padded to twice the size (so
+    the field does not wrap around), normalised to zero mean and unit standard
+    deviation."""
+    rng = np.random.default_rng(seed)
+    px = 1 << int(math.ceil(math.log2(2 * nx)))
+    pz = 1 << int(math.ceil(math.log2(2 * nz)))
+    kx = 2.0 * math.pi * np.fft.fftfreq(px, d=spacing)
+    kz = 2.0 * math.pi * np.fft.fftfreq(pz, d=spacing)
+    ksq = kz[:, None] ** 2 + kx[None, :] ** 2
+    power = (1.0 + ksq * corr_len ** 2) ** -(hurst + 1.0)
+    power[0, 0] = 0.0
+    spec = (rng.standard_normal((pz, px)) + 1j * rng.standard_normal((pz, px))) * np.sqrt(0.5 * power)
+    w = np.real(np.fft.ifft2(spec))[:nz, :nx]
+    return (w - w.mean()) / w.std()
+
+
+def c5_scenario(seed: int, steps: int = 200, index: int = 0) -> Case:
+    """One scenario of the c5 ensemble, drawn from `seed` at 100 m with a
+    10-element PML: a box 8-24 km along strike, 6-12 km wide and 6-12 km deep
+    (80-240 x 60-120 x 60-120 hexahedra including the PML), a seeded
+    five-layer medium (Vs 2000 -> 3464 m/s), a vertical strike-slip fault
+    reaching the surface with 0.5 km of rock beyond its ends and bottom, and
+    either slip weakening (mu_s 0.677, mu_d 0.45, Dc 0.3 m, sigma_n -100 MPa,
+    mean tau_0 62 MPa, a 3 x 3 km patch at 75 MPa) or slip-law
+    rate-and-state friction (a in [0.008, 0.016] with a 0.5 km
+    velocity-strengthening rim, b 0.012, L 0.02 m, sigma_n -30 MPa, mean tau_0 10.5 MPa, TPV104
+    nucleation).  tau_0 carries a von Karman field (correlation length 2 km,
+    H 0.75) of 10 % standard deviation, capped below the static strength
+    outside the nucleation zone; the hypocentre is drawn on the fault;
+    dt 0.005 s."""
+    rng = np.random.default_rng(seed)
+    d, npml = 100.0, 10
+    nxe = 2 * int(rng.integers(40, 121))   # 80..240 elements along strike
+    nye = 2 * int(rng.integers(30, 61))    # 60..120 across
+    nze = int(rng.integers(60, 121))       # 60..120 in depth
+    rsf = bool(rng.random() < 0.5)
+    c = Case(name=f"c5_{index}")
+    c.dx = c.dy = c.dz = d
+    c.nPML, c.R = npml, 0.001
+    c.enlarging_ratio = 1.0
+    c.xmax = (nxe // 2 - npml) * d
+    c.xmin = -c.xmax
+    half_y = nye // 2 - npml - 1          # uniform cells per side (one growth cell follows)
+    c.nuni_y_minus = c.nuni_y_plus = half_y
+    c.ymax = (half_y + 1) * d
+    c.ymin = -c.ymax
+    c.zmin, c.zmax = -(nze - npml) * d, 0.0
+    c.fxmax = c.xmax - 0.5e3
+    c.fxmin = -c.fxmax
+    c.fymin = c.fymax = 0.0
+    c.fzmin, c.fzmax = c.zmin + 0.5e3, 0.0
+    c.nmat, c.n2mat = 5, 4
+    c.mat = layered_medium(seed + 7, max_depth=-c.zmin)
+    c.vp, c.vs, c.rou = c.mat[-1, 1], c.mat[-1, 2], c.mat[-1, 3]
+    c.roumax, c.vmaxPML = float(c.mat[:, 3].max()), float(c.mat[:, 1].max())
+    c.dt = 0.005
+    c.term = steps * c.dt
+    c.nucR = 1.5e3
+    # hypocentre on the fault, the nucleation zone inside the rupturable part
+    c.xsource = float(np.round(rng.uniform(c.fxmin + 1.5e3, c.fxmax - 1.5e3), -2))
+    zlo = c.fzmin + 1.5e3
+    c.zsource = float(np.round(rng.uniform(zlo, max(zlo, -2.5e3)), -2))
+    c.ysource = 0.0
+    field = von_karman_field(c.nfx, c.nfz, d, 2.0e3, 0.75, seed + 13)
+    o = c.new_ofv()
+    if rsf:
+        c.friclaw, c.tpv, c.C_nuclea = 4, 104, 1
+        c.nucR, c.nucT, c.nucdtau0 = 1.5e3, 1.0, 18.0e6
+        b, L, f0, v0, fw, vw, vini, sn, tau_mean = 0.012, 0.02, 0.6, 1.0e-6, 0.1, 0.1, 1.0e-12, -30.0e6, 10.5e6
+        a_field = 0.008 + 0.008 * rng.random((c.nfz, c.nfx))
+        for ix, x in enumerate(c.fx):
+            for iz, z in enumerate(c.fz):
+                a = a_field[iz, ix]
+                if abs(x) > c.fxmax - 0.5e3 - 0.01 or z < c.fzmin + 0.5e3 + 0.01 or z > -0.5e3 - 0.01:
+                    a = 0.02
+                tau = tau_mean * (1.0 + 0.1 * field[iz, ix])
+                o[1, iz, ix], o[2, iz, ix], o[3, iz, ix] = 0.6, 0.45, 0.3
+                o[7, iz, ix], o[8, iz, ix] = sn, tau
+                o[9, iz, ix], o[10, iz, ix], o[11, iz, ix] = a, b, L
+                o[12, iz, ix], o[13, iz, ix], o[14, iz, ix], o[15, iz, ix] = v0, f0, fw, vw
+                o[46, iz, ix] = vini
+                o[20, iz, ix] = state_steady_state(a, b, L, v0, f0, tau, sn, vini, 4)
+    else:
+        c.friclaw, c.tpv, c.C_nuclea = 1, 8, 0
+        mus, mud, sn, tau_mean = 0.677, 0.45, -100.0e6, 62.0e6
+        for ix, x in enumerate(c.fx):
+            for iz, z in enumerate(c.fz):
+                o[1, iz, ix] = mus
+                if abs(abs(x) - c.fxmax) < 0.01 or abs(z - c.fzmin) < 0.01 or z > -0.5e3 + 0.01:
+                    o[1, iz, ix] = 1.0e4
+                o[2, iz, ix], o[3, iz, ix], o[7, iz, ix] = mud, 0.3, sn
+                tau = min(tau_mean * (1.0 + 0.1 * field[iz, ix]), 0.98 * mus * -sn)
+                if abs(x - c.xsource) <= 1.5e3 and abs(z - c.zsource) <= 1.5e3:
+                    tau = 75.0e6
+                o[8, iz, ix] = tau
+    fx_km = [round(v / 1.0e3, 1) for v in (c.fxmin / 2.0, c.fxmax / 2.0)]
+    fz_km = [round(v / 1.0e3, 1) for v in (c.fzmin / 2.0, c.fzmin / 4.0)]
+    c.st_on_fault = [[x, z] for x in fx_km for z in fz_km]
+    yk = round(c.ymax / 2.0e3, 1)
+    c.st_off_fault = [[x, y, 0.0] for x in fx_km for y in (-yk, yk)]
+    return c
+
+
+def c5(seed: int = 5, n: int = 64, steps: int = 200) -> List[Case]:
+    """c5, the batched ensemble: n scenarios of uneven size and physics
+    (c5_scenario) drawn from one seed."""
+    rng = np.random.default_rng(seed)
+    return [c5_scenario(int(s), steps, i) for i, s in enumerate(rng.integers(0, 2**31 - 1, size=n))]

@@ -1,0 +1,57 @@
+"""Seeded workload generators (CPU): the c5 ensemble draws scenarios in the
+stated ranges, deterministically, and its random fields have the stated
+statistics."""
+import numpy as np
+
+from eqdyna_amd import case
+
+
+def elements(c):
+    nx = round((c.xmax - c.xmin) / c.dx) + 2 * c.nPML
+    ny = 2 * (c.nuni_y_minus + 1 + c.nPML)
+    nz = round(-c.zmin / c.dz) + c.nPML
+    return nx, ny, nz
+
+
+def test_c5_scenarios_in_range_and_deterministic():
+    a = case.c5(seed=5, n=64, steps=10)
+    b = case.c5(seed=5, n=64, steps=10)
+    assert len(a) == 64
+    laws = set()
+    for ca, cb in zip(a, b):
+        nx, ny, nz = elements(ca)
+        assert 80 <= nx <= 240 and 60 <= ny <= 120 and 60 <= nz <= 120, (ca.name, nx, ny, nz)
+        assert ca.dx == ca.dy == ca.dz == 100.0 and ca.nPML == 10
+        assert ca.fxmin < ca.xsource < ca.fxmax and ca.fzmin < ca.zsource < 0.0
+        assert np.array_equal(ca.ofv, cb.ofv) and elements(ca) == elements(cb)
+        # layered medium: Vs increasing with depth from 2000 to 3464 m/s
+        assert np.all(np.diff(ca.mat[:, 2]) > 0) and ca.mat[0, 2] == 2000.0 and ca.mat[-1, 2] == 3464.0
+        laws.add(ca.friclaw)
+    assert laws == {1, 4}
+    assert not np.array_equal(a[0].ofv, a[1].ofv)
+
+
+def test_c5_tau0_heterogeneity():
+    for c in case.c5(seed=5, n=8, steps=10):
+        tau = c.ofv[8]
+        if c.friclaw == 4:
+            inner = tau  # no cap or patch on rate-and-state faults
+            rel = inner.std() / inner.mean()
+            assert 0.05 < rel < 0.15, rel
+        else:
+            strength = c.ofv[1] * -c.ofv[7]
+            patch = tau == 75.0e6
+            assert np.all(tau[~patch] < strength[~patch])
+
+
+def test_von_karman_field_statistics():
+    f = case.von_karman_field(128, 96, 100.0, 2.0e3, 0.75, seed=1)
+    assert f.shape == (96, 128)
+    assert abs(f.mean()) < 1e-12 and abs(f.std() - 1.0) < 1e-12
+    # correlation: neighbours 100 m apart are strongly correlated, points
+    # 6 km apart (three correlation lengths) nearly independent
+    near = np.corrcoef(f[:, :-1].ravel(), f[:, 1:].ravel())[0, 1]
+    far = np.corrcoef(f[:, :-60].ravel(), f[:, 60:].ravel())[0, 1]
+    assert near > 0.8 and abs(far) < 0.3, (near, far)
+    g = case.von_karman_field(128, 96, 100.0, 2.0e3, 0.75, seed=1)
+    assert np.array_equal(f, g)
