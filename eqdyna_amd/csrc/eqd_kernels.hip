@@ -58,6 +58,12 @@ __device__ __forceinline__ R sqr(R x) { return x * x; }
 #ifndef EQD_KEEP_OPS_PML
 #define EQD_KEEP_OPS_PML 0
 #endif
+// PML kernel with operators from memory: 1 stages each thread's dN/dx (24
+// values) in LDS between the passes (its own slots, no barrier), so the force
+// pass re-reads only Phi from memory.
+#ifndef EQD_STAGE_SHP_PML
+#define EQD_STAGE_SHP_PML 1
+#endif
 
 //
 // kLds: both class tables are staged in LDS (dynamic shared memory of
@@ -465,6 +471,22 @@ __global__ void __launch_bounds__(256) elem_pml_kernel(const Member<R> *__restri
         }
     };
     constexpr bool kKeep = !kLds && (EQD_KEEP_OPS_PML == 2 || (EQD_KEEP_OPS_PML == 1 && sizeof(R) == sizeof(float)));
+    constexpr bool kStage = !kLds && !kKeep && EQD_STAGE_SHP_PML;
+    lds_R *stage = (lds_R *)eqd_smem;
+    const unsigned bd = blockDim.x, tx = threadIdx.x;
+    // the force pass's operator values: dN/dx from the stage, Phi from memory
+    auto load_op2 = [&](int a, R o[kNo]) {
+        if constexpr (kStage) {
+#pragma unroll
+            for (int k = 0; k < 3; k++) o[k] = stage[(a * 3 + k) * bd + tx];
+            if constexpr (kHG == 1) {
+#pragma unroll
+                for (int m = 0; m < 4; m++) o[3 + m] = PHI(m, a);
+            }
+        } else {
+            load_op(a, o);
+        }
+    };
     R opc_[kNo], opn_[kNo];
     R keep[kKeep ? 8 : 1][kNo];
     if constexpr (!kLds) load_op(0, opc_);
@@ -478,6 +500,10 @@ constexpr int kU1 = kKeep ? 8 : EQD_PREFETCH;
             if constexpr (kKeep) {
 #pragma unroll
                 for (int k = 0; k < kNo; k++) keep[a][k] = opc_[k];
+            }
+            if constexpr (kStage) {
+#pragma unroll
+                for (int k = 0; k < 3; k++) stage[(a * 3 + k) * bd + tx] = opc_[k];
             }
             if (a < 7) load_op(a + 1, opn_);
         }
@@ -594,7 +620,7 @@ constexpr int kU1 = kKeep ? 8 : EQD_PREFETCH;
         coef2 = (R)0.25 * (R)C.kapa_hg * rho * vp * (R)pow((double)det * C.w, 2.0 / 3.0);
     }
 
-    if constexpr (!kLds && !kKeep) load_op(0, opc_);
+    if constexpr (!kLds && !kKeep) load_op2(0, opc_);
 constexpr int kU2 = kKeep ? 8 : EQD_ELEM_UNROLL;
 #pragma unroll kU2
     for (int a = 0; a < 8; a++) {
@@ -604,7 +630,7 @@ constexpr int kU2 = kKeep ? 8 : EQD_ELEM_UNROLL;
 #pragma unroll
             for (int k = 0; k < kNo; k++) opc_[k] = keep[a][k];
         } else {
-            if (a < 7) load_op(a + 1, opn_);
+            if (a < 7) load_op2(a + 1, opn_);
         }
         // hourglass force on this node, subtracted mode by mode
         R hg[3] = {0, 0, 0};
@@ -1391,13 +1417,14 @@ hipError_t launch_pml(const Member<R> *M, const LaunchGrid &g, int hg, int mass,
 {
     if (g.grid == 0) return hipSuccess;
     if (g.block < 64 || g.block > 256 || g.block % 64) return hipErrorInvalidValue;
-    const size_t lds = lds_on ? g.lds : 0;
+    // class tables, or each thread's staged dN/dx (24 values)
+    const size_t lds = lds_on ? g.lds : (EQD_STAGE_SHP_PML ? (size_t)g.block * 24 * sizeof(R) : 0);
 #define EQD_LAUNCH(HG, MASS)                                                                          \
     do {                                                                                              \
         if (lds_on)                                                                                   \
             elem_pml_kernel<R, HG, MASS, true><<<g.grid, g.block, lds, s>>>(M, g.bmem, g.remap);     \
         else                                                                                          \
-            elem_pml_kernel<R, HG, MASS, false><<<g.grid, g.block, 0, s>>>(M, g.bmem, g.remap);      \
+            elem_pml_kernel<R, HG, MASS, false><<<g.grid, g.block, lds, s>>>(M, g.bmem, g.remap);    \
     } while (0)
     if (mass) {
         if (hg == 1) EQD_LAUNCH(1, true);
