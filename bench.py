@@ -219,6 +219,8 @@ def main() -> int:
     ap.add_argument("--workload", default="c4", choices=["c4", "c3", "c2", "c1", "c5"])
     ap.add_argument("--tune", type=int, default=4, help="steps per candidate of the launch-shape tuner (0: off)")
     ap.add_argument("--scenarios", type=int, default=64, help="c5 only: ensemble size")
+    ap.add_argument("--precompute", type=int, default=1,
+                    help="also time the device-side element precompute (eqd_precompute) on the workload's mesh")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"], help="c1-c3 only")
     ap.add_argument("--cpu-baseline", default="auto", choices=["auto", "reference", "port", "none"])
     ap.add_argument("--cpu-ranks", type=int, default=16)
@@ -245,6 +247,8 @@ def main() -> int:
               "node": int((ndof == 3).sum()) - 2 * int(sc["nftnd"])}
     nn = int(sc["totalNumOfNodes"])
     extra = {}
+    if args.precompute:
+        extra["device_precompute"] = bench_precompute(bundle)
     if args.workload == "c4":
         from eqdyna_amd import learned
         params, meta = learned.load()
@@ -267,7 +271,7 @@ def main() -> int:
         run = run_one(bundle, sc, precision, args.warmup, steps, tune=args.tune)
     del bundle
     value = 3.0 * nn * steps / run["wall"] / 1e6
-    if extra:
+    if "fp64_value" in extra:
         extra["ratio_fp32_learned_vs_fp64"] = round(value / extra["fp64_value"], 4)
     roof = roofline(run, counts, precision, c.C_Q == 1, args.workload)
 
@@ -299,6 +303,21 @@ def main() -> int:
     }
     print(json.dumps(out), flush=True)
     return 0
+
+
+def bench_precompute(bundle) -> dict:
+    """Device-side element precompute and nodal masses of the workload's
+    mesh (the reference host's assembleGlobalMass element loop), checked bit
+    for bit against the host restatement that built the bundle."""
+    import numpy as np
+    from eqdyna_amd import solver
+    t = time.perf_counter()
+    out, ms = solver.precompute(bundle)
+    wall = time.perf_counter() - t
+    same = all(np.array_equal(out[k].ravel(), np.asarray(bundle[k], dtype=np.float64).ravel())
+               for k in ("eleshp", "eledet", "elemass", "ss", "phi", "nodalMassArr", "fnms"))
+    log(f"device precompute: kernels {ms:.2f} ms, {wall:.2f} s with copies and checks; bitwise equal: {same}")
+    return {"kernel_ms": round(ms, 3), "wall_s_with_copies": round(wall, 3), "bitwise_equal_to_host": bool(same)}
 
 
 def bench_c5(args) -> int:

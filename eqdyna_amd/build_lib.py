@@ -16,7 +16,7 @@ LIB = os.path.join(HERE, "libeqd.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("EQD_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["eqd_kernels.hip", "eqd_mlp.hip", "eqd_api.cpp", "eqd_engine.cpp"]
+SOURCES = ["eqd_kernels.hip", "eqd_mlp.hip", "eqd_precompute.hip", "eqd_api.cpp", "eqd_engine.cpp"]
 FLAGS = ["-O3", "-std=c++17", f"--offload-arch={ARCH}", "-fPIC", "-ffp-contract=off",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
          "-fopenmp", "-Wno-unused-command-line-argument"]

@@ -23,7 +23,7 @@ EXPORTS = ["eqd_create", "eqd_set", "eqd_commit", "eqd_run", "eqd_get", "eqd_clo
            "eqd_mlp_grad", "eqd_mlp_train", "eqd_set_launch", "eqd_get_launch", "eqd_tune", "eqd_set_graph",
            "eqd_ensemble_create", "eqd_ensemble_run", "eqd_ensemble_tune", "eqd_ensemble_set_launch",
            "eqd_ensemble_set_graph", "eqd_ensemble_set_timing", "eqd_ensemble_kernel_time",
-           "eqd_ensemble_last_error", "eqd_ensemble_destroy"]
+           "eqd_ensemble_last_error", "eqd_ensemble_destroy", "eqd_precompute"]
 
 # launch kinds (EQD_KIND_*)
 KINDS = {"element": 0, "element_pml": 1, "node": 2, "node_pml": 3, "fault": 4}
@@ -68,6 +68,13 @@ class Scalars(C.Structure):
 class TrainOpts(C.Structure):
     _fields_ = [("epochs", C.c_int32), ("batch", C.c_int32), ("lr", C.c_float), ("beta1", C.c_float),
                 ("beta2", C.c_float), ("eps", C.c_float), ("seed", C.c_uint64), ("device", C.c_int32)]
+
+
+class PrecomputeArgs(C.Structure):
+    _fields_ = [(n, C.c_int64) for n in ("nn", "ne", "neq", "size_eqidx")] + \
+        [(n, C.c_void_p) for n in ("meshCoor", "nodeElemIdRelation", "elemTypeArr", "mat", "numOfDofPerNodeArr",
+                                   "eqNumStartIndexLoc", "eqNumIndexArr", "eleshp", "eledet", "elemass", "ss",
+                                   "phi", "nodalMassArr", "fnms")] + [("device", C.c_int32)]
 
 
 class EqdError(RuntimeError):
@@ -124,6 +131,7 @@ def lib():
         L.eqd_ensemble_last_error.argtypes = [C.c_void_p]
         L.eqd_ensemble_last_error.restype = C.c_char_p
         L.eqd_ensemble_destroy.argtypes = [C.c_void_p]
+        L.eqd_precompute.argtypes = [C.POINTER(PrecomputeArgs), C.POINTER(C.c_int32), C.POINTER(C.c_double)]
         _lib = L
     return _lib
 
@@ -398,3 +406,31 @@ class Ensemble:
             self.close()
         except Exception:
             pass
+
+
+def precompute(bundle: Dict[str, np.ndarray], device: int = 0):
+    """Device-side element precompute and nodal masses (eqd_precompute) from a
+    bundle's mesh arrays; returns (arrays, kernel milliseconds).  nodalMassArr
+    and fnms start from zero, as in the reference host."""
+    L = lib()
+    coor = np.ascontiguousarray(bundle["meshCoor"], dtype="<f8")
+    ien = np.ascontiguousarray(bundle["nodeElemIdRelation"], dtype="<i4")
+    et = np.ascontiguousarray(bundle["elemTypeArr"], dtype="<i4")
+    mat = np.ascontiguousarray(bundle["mat"], dtype="<f8")
+    ndof = np.ascontiguousarray(bundle["numOfDofPerNodeArr"], dtype="<i4")
+    esl = np.ascontiguousarray(bundle["eqNumStartIndexLoc"], dtype="<i4")
+    eqi = np.ascontiguousarray(bundle["eqNumIndexArr"], dtype="<i4")
+    nn, ne = ndof.size, et.size
+    neq = int(np.asarray(bundle["nodalMassArr"]).size)
+    out = {"eleshp": np.zeros(24 * ne), "eledet": np.zeros(ne), "elemass": np.zeros(24 * ne), "ss": np.zeros(6 * ne),
+           "phi": np.zeros(32 * ne), "nodalMassArr": np.zeros(neq), "fnms": np.zeros(nn)}
+    a = PrecomputeArgs(nn, ne, neq, eqi.size, coor.ctypes.data, ien.ctypes.data, et.ctypes.data, mat.ctypes.data,
+                       ndof.ctypes.data, esl.ctypes.data, eqi.ctypes.data, out["eleshp"].ctypes.data,
+                       out["eledet"].ctypes.data, out["elemass"].ctypes.data, out["ss"].ctypes.data,
+                       out["phi"].ctypes.data, out["nodalMassArr"].ctypes.data, out["fnms"].ctypes.data, device)
+    bad = C.c_int32()
+    ms = C.c_double()
+    rc = L.eqd_precompute(C.byref(a), C.byref(bad), C.byref(ms))
+    if rc != 0:
+        raise EqdError(rc, f"eqd_precompute failed (element {bad.value})")
+    return out, ms.value

@@ -213,6 +213,31 @@ int eqd_mlp_train(const float *X, const float *r, int64_t n, const eqd_mlp_train
                   float *epoch_loss);
 
 /* ---------------------------------------------------------------------------
+ * Device-side one-time precompute: replaces the element loop of
+ * assembleGlobalMass (src/assembleGlobalMass.f90:283-406, with
+ * calcGlobalShapeFunc, src/calcGlobalShapeFunc.f90:1-75, contm,
+ * calcSSPhi4Hrgls and vlm, src/library.f90:60-93): per element dN/dx (eleshp),
+ * Jacobian (eledet), lumped mass (elemass), SS and Phi, and each element's
+ * lumped mass added to its nodes' equations (nodalMassArr) and to fnms, in
+ * the reference's element order (bit-identical sums).  Host arrays in the
+ * reference's layouts; nodalMassArr and fnms are added to, as the reference
+ * adds to its zeroed arrays.  Returns EQD_EINVAL with *bad_elem (1-based) for
+ * a non-positive Jacobian or a node id out of range; *device_ms (may be NULL)
+ * receives the kernels' time without the copies. */
+typedef struct {
+    int64_t nn, ne, neq, size_eqidx;
+    const double *meshCoor;            /* (3, nn) */
+    const int32_t *nodeElemIdRelation; /* (8, ne), 1-based */
+    const int32_t *elemTypeArr;        /* (ne): 11/12 are wedges */
+    const double *mat;                 /* (ne, 5): vp, vs, rho, lambda, mu */
+    const int32_t *numOfDofPerNodeArr, *eqNumStartIndexLoc, *eqNumIndexArr;
+    double *eleshp, *eledet, *elemass, *ss, *phi; /* out: (3,8,ne) (ne) (24,ne) (6,ne) (8,4,ne) */
+    double *nodalMassArr, *fnms;                  /* in/out: (neq) (nn) */
+    int32_t device;
+} eqd_precompute_args;
+int eqd_precompute(const eqd_precompute_args *a, int32_t *bad_elem, double *device_ms);
+
+/* ---------------------------------------------------------------------------
  * Launch shapes, captured steps and ensembles.  No counterpart in the
  * reference (one MPI rank per subdomain, one scenario per run); they change
  * how the step is launched, never what it computes: every launch shape, the
