@@ -177,9 +177,11 @@ def cpu_baseline_reference(workload: str, ranks: int, steps: int):
     with tempfile.TemporaryDirectory() as d:
         t = time.perf_counter()
         _, _, _, t_loop = refrun.run_reference(c, d, ranks, dump=False, timeout=900)
-        log(f"reference on {ranks} ranks: t_loop {t_loop:.2f} s ({time.perf_counter() - t:.1f} s with meshing)")
+        log(f"reference on {ranks} ranks: t_loop {t_loop:.2f} s ({time.perf_counter() - t:.1f} s with meshing); "
+            f"start-up phases {refrun.LAST_PHASES}")
     return {"value": 3.0 * nn * steps / t_loop / 1e6, "unit": "Mdof-steps/s", "cores": ranks, "kind": "reference",
-            "sample": sample + f", reference Fortran (amdflang -O2) on {ranks} MPI ranks, time loop only"}
+            "sample": sample + f", reference Fortran (amdflang -O2) on {ranks} MPI ranks, time loop only",
+            "startup_s": dict(refrun.LAST_PHASES)}
 
 
 def cpu_baseline_port(steps: int):
@@ -219,6 +221,9 @@ def main() -> int:
     ap.add_argument("--workload", default="c4", choices=["c4", "c3", "c2", "c1", "c5"])
     ap.add_argument("--tune", type=int, default=4, help="steps per candidate of the launch-shape tuner (0: off)")
     ap.add_argument("--scenarios", type=int, default=64, help="c5 only: ensemble size")
+    ap.add_argument("--host-startup", type=int, default=0, metavar="RANKS",
+                    help="only time the reference host's start-up (meshgen, assembleGlobalMass) on the workload's "
+                         "full mesh on RANKS MPI ranks, and the device precompute of the same mesh, then exit")
     ap.add_argument("--precompute", type=int, default=1,
                     help="also time the device-side element precompute (eqd_precompute) on the workload's mesh")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"], help="c1-c3 only")
@@ -234,6 +239,8 @@ def main() -> int:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}). "
               "Several GPUs would each take a slab of the mesh with a halo exchange; that path is not built.", file=sys.stderr)
         return 2
+    if args.host_startup:
+        return host_startup(args.workload, args.host_startup)
     if args.workload == "c5":
         return bench_c5(args)
     steps = args.steps if args.steps is not None else (2990 if args.workload == "c4" else 200)
@@ -302,6 +309,32 @@ def main() -> int:
         "roofline": roof, "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+    return 0
+
+
+def host_startup(workload: str, ranks: int) -> int:
+    """The reference host's start-up phases on the workload's full mesh (one
+    step), beside the device precompute of the same mesh."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import refrun
+    from eqdyna_amd import case, mesh
+    c = {"c1": case.c1, "c2": case.c2, "c3": case.c3, "c4": case.c3}[workload](steps=1)
+    fac = {1: (1, 1, 1), 2: (2, 1, 1), 4: (2, 2, 1), 8: (2, 2, 2), 16: (4, 2, 2), 32: (4, 4, 2)}
+    c.nx, c.ny, c.nz = fac[ranks]
+    with tempfile.TemporaryDirectory() as d:
+        t = time.perf_counter()
+        refrun.run_reference(c, d, ranks, dump=False, timeout=1800)
+        wall = time.perf_counter() - t
+    phases = dict(refrun.LAST_PHASES)
+    log(f"reference start-up on {ranks} ranks: {phases} ({wall:.1f} s for the whole run)")
+    c.nx = c.ny = c.nz = 1
+    t = time.perf_counter()
+    bundle, sc = mesh.build_case(c, nthreads=min(16, os.cpu_count() or 1))
+    build = time.perf_counter() - t
+    dev = bench_precompute(bundle)
+    print(json.dumps({"workload": workload, "elements": int(sc["totalNumOfElements"]), "reference_ranks": ranks,
+                      "reference_startup_s": phases, "reference_run_wall_s": round(wall, 2),
+                      "host_builder_s": round(build, 2), "device_precompute": dev}), flush=True)
     return 0
 
 

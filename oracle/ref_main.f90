@@ -24,7 +24,7 @@ program eqd_ref_main
     implicit none
     include 'mpif.h'
     integer (kind = 4) :: ierr, dumpBundle
-    real (kind = dp) :: tLoop0, tLoop
+    real (kind = dp) :: tLoop0, tLoop, tMesh, tMass
 
     call MPI_Init(ierr)
     call mpi_comm_rank(MPI_COMM_WORLD, me, ierr)
@@ -50,11 +50,19 @@ program eqd_ref_main
     call calcLocalShapeFunc
     call mesh4num
     call ref_alloc_init
+    call MPI_Barrier(MPI_COMM_WORLD, ierr)
+    tMesh = MPI_WTIME()
     call meshgen
+    tMesh = MPI_WTIME() - tMesh
     call ref_read_on_fault_raw
     call ref_alloc_after_meshgen
+    call MPI_Barrier(MPI_COMM_WORLD, ierr)
+    tMass = MPI_WTIME()
     call assembleGlobalMass
+    tMass = MPI_WTIME() - tMass
     call ref_init_vel
+    call MPI_Allreduce(MPI_IN_PLACE, tMesh, 1, MPI_DOUBLE_PRECISION, MPI_MAX, MPI_COMM_WORLD, ierr)
+    call MPI_Allreduce(MPI_IN_PLACE, tMass, 1, MPI_DOUBLE_PRECISION, MPI_MAX, MPI_COMM_WORLD, ierr)
 
     if (dumpBundle == 1) call ref_dump_bundle('bundle_in')
 
@@ -67,8 +75,9 @@ program eqd_ref_main
     call output_frt
     if (dumpBundle == 1) call ref_dump_outputs('ref_out')
     if (me == 0) then
-        write(*,'(A,I12,A,I12,A,I8,A,ES16.8)') 'EQD_REF nodes ', totalNumOfNodes, ' elems ', totalNumOfElements, &
-            ' steps ', nstep, ' t_loop_s ', tLoop
+        write(*,'(A,I12,A,I12,A,I8,A,ES16.8,A,ES16.8,A,ES16.8)') 'EQD_REF nodes ', totalNumOfNodes, &
+            ' elems ', totalNumOfElements, ' steps ', nstep, ' t_loop_s ', tLoop, ' t_meshgen_s ', tMesh, &
+            ' t_mass_s ', tMass
     endif
     call MPI_Finalize(ierr)
 end program eqd_ref_main

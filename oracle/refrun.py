@@ -19,6 +19,9 @@ def available() -> bool:
     return os.path.exists(REF_BIN)
 
 
+LAST_PHASES = {}
+
+
 def run_reference(case, workdir: str, nranks: int = 1, dump: bool = True, timeout: int = 3600):
     """Write the case inputs into ``workdir``, run the reference on ``nranks``
     MPI ranks (ranks = case.nx*case.ny*case.nz must hold), and return
@@ -40,6 +43,12 @@ def run_reference(case, workdir: str, nranks: int = 1, dump: bool = True, timeou
         raise RuntimeError(f"reference run failed ({p.returncode}):\n{p.stdout[-3000:]}\n{p.stderr[-3000:]}")
     m = re.search(r"t_loop_s\s+(\S+)", p.stdout)
     t_loop = float(m.group(1)) if m else float("nan")
+    # start-up phases of the last run (seconds, max over ranks)
+    LAST_PHASES.clear()
+    for k in ("t_meshgen_s", "t_mass_s"):
+        m = re.search(k + r"\s+(\S+)", p.stdout)
+        if m:
+            LAST_PHASES[k] = float(m.group(1))
     if not dump:
         return None, None, None, t_loop
     b = load_arrays(os.path.join(workdir, "bundle_in"))
