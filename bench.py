@@ -264,7 +264,7 @@ def main() -> int:
         precision = "fp32"
         v64 = 3.0 * nn * steps / ref["wall"] / 1e6
         n64, n32 = ref["newton"], run["newton"]
-        extra = {
+        extra.update({
             "fp64_value": round(v64, 2), "fp64_ms_per_step": round(ref["wall"] / steps * 1e3, 4),
             "ratio_fp32_learned_vs_fp64": None,
             "errors_vs_fp64": errors_vs(run, ref, sc["dt"]),
@@ -272,7 +272,7 @@ def main() -> int:
             "newton_fp32_learned": {**n32, "iterations_per_solve": round(n32["iterations"] / max(n32["solves"], 1), 3),
                                     "fallback_rate": round(n32["fallbacks"] / max(n32["solves"], 1), 5)},
             "guess": {**GUESS, "weights": "eqdyna_amd/rsf_guess_c3.npz", "training_rows": meta.get("rows")},
-        }
+        })
     else:
         precision = args.precision
         run = run_one(bundle, sc, precision, args.warmup, steps, tune=args.tune)
