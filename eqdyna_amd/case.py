@@ -369,6 +369,30 @@ def box_rsf(friclaw: int = 4, nx_half: int = 6, nz: int = 6, ny_half: int = 3, d
     return c
 
 
+def box_tp(steps: int = 60, seed: int = 5, name: str = "box_tp") -> Case:
+    """box_rsf with thermal pressurization (friclaw 5: slip law with strong
+    velocity weakening plus the pore pressure and temperature of
+    src/thermop.f90) and TPV105 nucleation and creep floor: hydraulic
+    diffusivity seeded in [4e-4, 4e-3] m^2/s, thermal diffusivity 1e-6 m^2/s,
+    rho*c 2.7e6 J/(m^3 K), Lambda 0.1 MPa/K, T_ini 483.15 K, p_ini 80 MPa
+    (the parameter values of scripts/defaultParameters.py:139-146)."""
+    c = box_rsf(friclaw=4, steps=steps, name=name)
+    c.friclaw = 5
+    c.tpv = 105
+    rng = np.random.default_rng(seed)
+    o = c.ofv
+    for ix in range(c.nfx):
+        for iz in range(c.nfz):
+            o[16, iz, ix] = 4.0e-4 + 3.6e-3 * rng.random()
+            o[17, iz, ix] = 1.0e-6
+            o[18, iz, ix] = 2.7e6
+            o[19, iz, ix] = 0.1e6
+            o[40, iz, ix] = 0.02
+            o[41, iz, ix] = 483.15
+            o[42, iz, ix] = 80.0e6
+    return c
+
+
 def box_plastic(steps: int = 40, friclaw: int = 1, name: str = "box_plastic") -> Case:
     """Small box with Drucker-Prager viscoplasticity (C_elastic = 0): gravity,
     lithostatic initial stress (src/meshgen.f90:976-996) and no on-fault

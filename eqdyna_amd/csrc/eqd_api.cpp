@@ -313,8 +313,7 @@ int eqd_commit(eqd_handle *h)
     if (s.non > 0 && !h->src.count(EQD_ANONFS)) return fail(h, EQD_EINVAL, "anonfs was not set");
     if (s.noff > 0 && !h->src.count(EQD_IDHIST)) return fail(h, EQD_EINVAL, "idhist was not set");
     if (s.C_Q == 1 && s.C_elastic == 0) return fail(h, EQD_EINVAL, "Q model can only work with elastic code");
-    if (s.friclaw < 1 || s.friclaw > 4)
-        return fail(h, EQD_EINVAL, "friclaw %d is not supported (1-4; thermal pressurization is not built)", s.friclaw);
+    if (s.friclaw < 1 || s.friclaw > 5) return fail(h, EQD_EINVAL, "friclaw %d is not supported (1-5)", s.friclaw);
     HIPCHK(h, hipSetDevice(s.device));
     // the handle's own engine (streams, launch tables); built at the first run
     if (!h->eng) {
@@ -766,6 +765,12 @@ int eqd_commit(eqd_handle *h)
         if ((rc = dalloc(h, &d, non * (int64_t)s.nstep * 12))) return rc;
         HIPCHK(h, hipMemset(d, 0, sizeof(double) * non * s.nstep * 12));
         F.hist = d;
+        F.tph = nullptr;
+        if (s.friclaw == 5) {
+            if ((rc = dalloc(h, &d, 2 * nf * (int64_t)s.nstep))) return rc;
+            HIPCHK(h, hipMemset(d, 0, sizeof(double) * 2 * nf * s.nstep));
+            F.tph = d;
+        }
     }
     // ---------------- off-fault stations
     {
@@ -803,6 +808,8 @@ int eqd_commit(eqd_handle *h)
     C.sinphi = s.sinphi;
     C.exp_dt_tv = s.C_elastic == 0 ? std::exp(-s.dt / s.tv) : 0.0;
     C.kapa_hg = s.kapa_hg;
+    C.fric_tp_h = s.fric_tp_h;
+    C.pi = s.pi;
     C.slipRateThres = s.slipRateThres;
     C.xsource = s.xsource; C.ysource = s.ysource; C.zsource = s.zsource;
     C.nucR = s.nucR; C.nucRuptVel = s.nucRuptVel; C.nucdtau0 = s.nucdtau0; C.nucT = s.nucT;

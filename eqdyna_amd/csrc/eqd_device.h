@@ -24,7 +24,7 @@ constexpr uint32_t kEntIdxMask = 0x0FFFFFFFu;
 constexpr int kEntLocalShift = 28;
 
 struct StepConsts {
-    double dt, w, rdampk, rdampm, gterm, ccosphi, sinphi, exp_dt_tv, kapa_hg;
+    double dt, w, rdampk, rdampm, gterm, ccosphi, sinphi, exp_dt_tv, kapa_hg, fric_tp_h, pi;
     double slipRateThres, xsource, ysource, zsource, nucR, nucRuptVel, nucdtau0, nucT;
     int32_t C_elastic, C_Q, C_hg, C_nuclea, friclaw, TPV, insertFaultType, nucfault;
 };
@@ -127,6 +127,9 @@ struct FaultState {
     const int32_t *station; // [nf] on-fault station (0-based) or -1
     double *hist;           // [non][nstep][12]
     int32_t nstep;
+    // thermal pressurization (friclaw 5): slip rate and shear traction of
+    // every pair at every step, [2][nf][nstep] (onFaultTPHist), else null
+    double *tph;
 };
 
 struct OffStations {
@@ -149,7 +152,7 @@ struct Clock {
 
 // launch kinds of one step
 enum LaunchKind : int {
-    kKElem = 0, kKPml, kKNode3, kKNodePml, kKFault, kKStation, kKClock, kKPend3, kKPendPml, kKinds
+    kKElem = 0, kKPml, kKNode3, kKNodePml, kKFault, kKStation, kKClock, kKPend3, kKPendPml, kKTp, kKinds
 };
 
 // Everything one scenario's step reads: every kernel takes an array of these

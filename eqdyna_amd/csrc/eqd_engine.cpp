@@ -22,7 +22,7 @@ using namespace eqd;
 namespace {
 
 const char *const kKindName[kKinds] = {"element", "element_pml", "node", "node_pml", "fault",
-                                       "station", "clock", "pending", "pending_pml"};
+                                       "station", "clock", "pending", "pending_pml", "thermop"};
 
 template <typename R>
 const RegElems<R> &elems(const eqd_handle *h);
@@ -68,6 +68,7 @@ int64_t count_of(const eqd_handle *h, int kind)
     case kKStation: return h->S.nrow > 0 ? (int64_t)h->S.nrow + 1 : 0;
     case kKPend3: return (int64_t)h->node3_all.size();
     case kKPendPml: return h->PN.n;
+    case kKTp: return h->F.tph ? h->F.nf * 64 : 0;  // one wavefront per pair
     default: return 0;
     }
 }
@@ -257,6 +258,10 @@ int enqueue_step(eqd_engine *e)
         HIPCHK(e, hipEventRecord(e->ev_fork, st));
         HIPCHK(e, hipStreamWaitEvent(e->st2, e->ev_fork, 0));
     }
+    // thermal pressurization (friclaw 5) before the fault pairs, src/driver.f90:27
+    timed_begin(e, kKTp, &ev, fst);
+    for (const Group &g : e->groups[kKTp]) HIPCHK(e, launch_thermop<R>(dM, g.g, fst));
+    timed_end(ev, fst);
     timed_begin(e, kKFault, &ev, fst);
     for (const Group &g : e->groups[kKFault]) HIPCHK(e, launch_fault<R>(dM, g.g, g.sig, fst));
     timed_end(ev, fst);
@@ -386,6 +391,7 @@ std::vector<Region> mutable_regions(const eqd_engine *e)
             add(h->F.hist, (size_t)(h->s.non > 0 ? h->s.non : 1) * h->s.nstep * 12 * sizeof(double));
         }
         if (h->S.nrow > 0) add(h->S.hist, (size_t)(h->S.nrow + 1) * h->s.nstep * sizeof(double));
+        if (h->F.tph) add(h->F.tph, 2 * (size_t)h->F.nf * h->s.nstep * sizeof(double));
         add(h->d_clock, sizeof(Clock));
     }
     return r;

@@ -31,6 +31,8 @@ template <typename R>
 hipError_t launch_station(const Member<R> *, const LaunchGrid &, hipStream_t);
 template <typename R>
 hipError_t launch_clock(const Member<R> *, int nmem, hipStream_t);
+template <typename R>
+hipError_t launch_thermop(const Member<R> *, const LaunchGrid &, hipStream_t);
 hipError_t launch_mlp_prepare(const float *p, __bf16 *wb, float *w4r, hipStream_t s);
 
 struct KTimer {

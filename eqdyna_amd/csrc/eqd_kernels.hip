@@ -1275,6 +1275,10 @@ __global__ void __launch_bounds__(128) fault_kernel(const Member<R> *__restrict_
         FR(80) = T[2];
         FR(47) = v_trial;
         FR(48) = sqrt(T[1] * T[1] + T[2] * T[2]);
+        if (Fs.tph) {  // onFaultTPHist, src/faulting.f90:301-302
+            Fs.tph[i * Fs.nstep + (nt - 1)] = FR(47);
+            Fs.tph[(nf + i) * Fs.nstep + (nt - 1)] = FR(48);
+        }
         double acc[3], xyzAcc[3], xyzR[3];
         acc[0] = -rate[0] / dt - slip[0] / dt / dt;
         acc[1] = (v_trial * (trial[1] / trial[3]) - rate[1]) / dt;
@@ -1361,6 +1365,55 @@ __global__ void __launch_bounds__(64) station_kernel(const Member<R> *__restrict
         col[r] = N.disp[k * N.nn + node];
     else if (kind == 2)
         col[r] = N.vel[k * N.nn + node];
+}
+
+// thermop, src/thermop.f90:1-40: pore pressure change fric(51) and
+// temperature fric(52) of every pair from its history of slip rate and shear
+// traction up to the previous step.  One wavefront per pair: lanes take
+// every 64th past step (the same terms as the reference, each in its
+// operation order), then the lane sums are added in a fixed butterfly order,
+// so the result is deterministic and differs from the reference's sequential
+// sum only by rounding.
+template <typename R>
+__global__ void __launch_bounds__(256) thermop_kernel(const Member<R> *__restrict__ Ms,
+                                                      const uint16_t *__restrict__ bmem)
+{
+    const unsigned gb = blockIdx.x;
+    const Member<R> &M = Ms[member_of(bmem, gb)];
+    const int64_t tid = (int64_t)(gb - (unsigned)M.blk0[kKTp]) * blockDim.x + threadIdx.x;
+    const int64_t i = tid >> 6;  // whole wavefronts share i (blocks are multiples of 64)
+    const int lane = (int)(threadIdx.x & 63);
+    const FaultState Fs = M.F;
+    if (i >= Fs.nf) return;
+    const StepConsts C = M.C;
+    const int nt = M.clk->nt;
+    const int64_t ld = Fs.ld, nf = Fs.nf;
+    double *fr = Fs.fric;
+#define FR(k) fr[(int64_t)((k) - 1) * ld + i]
+    const double dt = C.dt, hh = C.fric_tp_h * C.fric_tp_h;
+    const double omega = FR(16), kapa = FR(17);
+    const double *hv = Fs.tph + i * Fs.nstep, *ht = Fs.tph + (nf + i) * Fs.nstep;
+    double sp = 0.0, st = 0.0;
+    for (int j = 1 + lane; j <= nt - 1; j += 64) {
+        const double w = fabs(ht[j - 1]) * hv[j - 1];
+        const double m = (double)(nt - j);
+        double ker = -kapa / (omega - kapa) / sqrt(4.0 * kapa * m * dt + 2.0 * hh);
+        ker = ker + omega / (omega - kapa) / sqrt(4.0 * omega * m * dt + 2.0 * hh);
+        sp = sp + w * ker * dt;
+        const double kt = 1.0 / sqrt(4.0 * kapa * m * dt + 2.0 * hh);
+        st = st + w * kt * dt;
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        sp = sp + __shfl_xor(sp, off, 64);
+        st = st + __shfl_xor(st, off, 64);
+    }
+    if (lane == 0) {
+        const double gama = FR(19) / FR(18);
+        FR(51) = sp * gama / sqrt(C.pi);
+        FR(52) = st / FR(18) / sqrt(C.pi) + FR(41);
+    }
+#undef FR
 }
 
 // The step clock: nt += 1 and time += dt for every scenario of the launch,
@@ -1479,6 +1532,15 @@ hipError_t launch_fault(const Member<R> *M, const LaunchGrid &g, int learn, hipS
 }
 
 template <typename R>
+hipError_t launch_thermop(const Member<R> *M, const LaunchGrid &g, hipStream_t s)
+{
+    if (g.grid == 0) return hipSuccess;
+    if (g.block < 64 || g.block > 256 || g.block % 64) return hipErrorInvalidValue;
+    thermop_kernel<R><<<g.grid, g.block, 0, s>>>(M, g.bmem);
+    return hipGetLastError();
+}
+
+template <typename R>
 hipError_t launch_station(const Member<R> *M, const LaunchGrid &g, hipStream_t s)
 {
     if (g.grid == 0) return hipSuccess;
@@ -1503,6 +1565,7 @@ hipError_t launch_clock(const Member<R> *M, int nmem, hipStream_t s)
     template hipError_t launch_pending<R>(const Member<R> *, const LaunchGrid &, const LaunchGrid &, hipStream_t);  \
     template hipError_t launch_fault<R>(const Member<R> *, const LaunchGrid &, int, hipStream_t);                   \
     template hipError_t launch_station<R>(const Member<R> *, const LaunchGrid &, hipStream_t);                      \
+    template hipError_t launch_thermop<R>(const Member<R> *, const LaunchGrid &, hipStream_t);                      \
     template hipError_t launch_clock<R>(const Member<R> *, int, hipStream_t);
 EQD_INSTANTIATE(double)
 EQD_INSTANTIATE(float)

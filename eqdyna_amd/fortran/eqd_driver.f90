@@ -26,6 +26,7 @@ module eqd_abi
         integer(c_int32_t) :: nstep, nn, ne, neq, size_eqidx, size_stress, nf, non, noff, nPML
         integer(c_int32_t) :: C_elastic, C_Q, C_hg, C_nuclea, friclaw, TPV, insertFaultType, nucfault
         integer(c_int32_t) :: precision, device
+        real(c_double) :: fric_tp_h
     end type eqd_scalars
 
     ! field ids of enum eqd_field
@@ -138,6 +139,7 @@ subroutine driver
     s%C_elastic = C_elastic; s%C_Q = C_Q; s%C_hg = C_hg; s%C_nuclea = C_nuclea
     s%friclaw = friclaw; s%TPV = TPV; s%insertFaultType = insertFaultType; s%nucfault = nucfault
     s%precision = 0; s%device = 0
+    s%fric_tp_h = fric_tp_h
     if (ntotft /= 1) stop 'eqdyna_amd: one fault (ntotft = 1) is supported'
     if (totalNumOfMPIProcs /= 1) stop 'eqdyna_amd: one MPI rank per GPU (npx = npy = npz = 1)'
 

@@ -62,7 +62,7 @@ class Scalars(C.Structure):
               [(n, C.c_int32) for n in
                ("nstep", "nn", "ne", "neq", "size_eqidx", "size_stress", "nf", "non", "noff", "nPML",
                 "C_elastic", "C_Q", "C_hg", "C_nuclea", "friclaw", "TPV", "insertFaultType", "nucfault",
-                "precision", "device")]
+                "precision", "device")] + [("fric_tp_h", C.c_double)]
 
 
 class TrainOpts(C.Structure):
@@ -159,6 +159,8 @@ def make_scalars(sc: Dict[str, float], precision: str = "fp64", device: int = 0,
         setattr(s, n, int(sc[n]))
     s.precision = PREC[precision]
     s.device = int(device)
+    # the reference never assigns its module variable fric_tp_h (src/globalvar.f90:42): zero
+    s.fric_tp_h = float(sc.get("fric_tp_h", 0.0))
     return s
 
 

@@ -54,6 +54,7 @@ typedef struct {
     int32_t C_elastic, C_Q, C_hg, C_nuclea, friclaw, TPV, insertFaultType, nucfault;
     int32_t precision;                /* EQD_PREC_* */
     int32_t device;                   /* HIP device ordinal */
+    double fric_tp_h;                 /* thermal pressurization half-width (src/thermop.f90) */
 } eqd_scalars;
 
 /* Array ids for eqd_set / eqd_get.  Shapes are Fortran (column-major). */

@@ -751,6 +751,10 @@ static void faulting(const eqdo_scalars *s, eqdo_state *st)
             for (int j = 0; j < 3; j++) FRIC(78 + j, i) = T[j];
             FRIC(47, i) = v_trial;
             FRIC(48, i) = sqrt(sq(T[1]) + sq(T[2]));
+            if (st->tpHist) {  /* src/faulting.f90:301-302 */
+                st->tpHist[((size_t)(st->nt - 1) * s->nf + i - 1) * 2 + 0] = FRIC(47, i);
+                st->tpHist[((size_t)(st->nt - 1) * s->nf + i - 1) * 2 + 1] = FRIC(48, i);
+            }
             acc[0] = -rate[0] / dt - slip[0] / dt / dt;
             acc[1] = (v_trial * (trial[1] / trial[3]) - rate[1]) / dt;
             acc[2] = (v_trial * (trial[2] / trial[3]) - rate[2]) / dt;
@@ -788,6 +792,34 @@ static void faulting(const eqdo_scalars *s, eqdo_state *st)
     }
 }
 
+/* thermop, src/thermop.f90:1-40: pore pressure change and temperature of
+ * every pair from the history of slip rate and shear traction up to the
+ * previous step (onFaultTPHist(1:2, i, 1:nt-1)). */
+#define TPH(k, i, j) st->tpHist[((size_t)((j) - 1) * s->nf + (i) - 1) * 2 + (k) - 1]
+static void thermop(const eqdo_scalars *s, eqdo_state *st)
+{
+    const double dt = s->dt, h = s->fric_tp_h, nt = st->nt;
+    for (int i = 1; i <= s->nf; i++) {
+        const double gama = FRIC(19, i) / FRIC(18, i);
+        const double omega = FRIC(16, i), kapa = FRIC(17, i);
+        double tmp = 0.0;
+        for (int j = 1; j <= nt - 1; j++) {
+            double ker = -kapa / (omega - kapa) / sqrt(4.0 * kapa * (double)(nt - j) * dt + 2.0 * (h * h));
+            ker = ker + omega / (omega - kapa) / sqrt(4.0 * omega * (double)(nt - j) * dt + 2.0 * (h * h));
+            tmp = tmp + fabs(TPH(2, i, j)) * TPH(1, i, j) * ker * dt;
+        }
+        const double patnode = tmp * gama / sqrt(s->pi);
+        tmp = 0.0;
+        for (int j = 1; j <= nt - 1; j++) {
+            const double ker = 1.0 / sqrt(4.0 * kapa * (double)(nt - j) * dt + 2.0 * (h * h));
+            tmp = tmp + fabs(TPH(2, i, j)) * TPH(1, i, j) * ker * dt;
+        }
+        const double tatnode = tmp / FRIC(18, i) / sqrt(s->pi);
+        FRIC(51, i) = patnode;
+        FRIC(52, i) = tatnode + FRIC(41, i);
+    }
+}
+
 /* ------------------------------------------------------------------- driver
  * src/driver.f90:3-36 (one rank: MPI4NodalQuant is the identity). */
 int eqdo_run(const eqdo_scalars *s, eqdo_state *st, int32_t nsteps)
@@ -800,6 +832,7 @@ int eqdo_run(const eqdo_scalars *s, eqdo_state *st, int32_t nsteps)
         memset(st->nodalForceArr, 0, sizeof(double) * (size_t)s->neq);
         assemble_global_ku(s, st);
         hourglass(s, st);
+        if (s->friclaw == 5) thermop(s, st);
         faulting(s, st);
         for (int i = 0; i < s->neq; i++) st->nodalForceArr[i] = st->nodalForceArr[i] / st->nodalMassArr[i];
     }

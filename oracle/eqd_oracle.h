@@ -21,6 +21,7 @@ typedef struct {
     double kapa_hg, dx, R, vmaxPML, PMLb[8], pi;
     int32_t nstep, nn, ne, neq, nf, non, noff, nPML;
     int32_t C_elastic, C_Q, C_hg, C_nuclea, friclaw, TPV, insertFaultType, nucfault;
+    double fric_tp_h;                /* thermal pressurization half-width (module variable) */
 } eqdo_scalars;
 
 typedef struct {
@@ -56,6 +57,9 @@ typedef struct {
     double timeElapsed;
     /* error report */
     int32_t err_node, err_nt;
+    /* thermal pressurization history onFaultTPHist (2, nf, nstep): slip rate
+       and shear traction of every pair at every step (friclaw 5), or NULL */
+    double *tpHist;
 } eqdo_state;
 
 /* Advance nsteps time steps.  Returns 0, or 1 when a NaN velocity appears

@@ -46,7 +46,7 @@ class Scalars(C.Structure):
               [(n, C.c_int32) for n in
                ("nstep", "nn", "ne", "neq", "nf", "non", "noff", "nPML",
                 "C_elastic", "C_Q", "C_hg", "C_nuclea", "friclaw", "TPV", "insertFaultType",
-                "nucfault")]
+                "nucfault")] + [("fric_tp_h", C.c_double)]
 
 
 _PTR_FIELDS = ["meshCoor", "eqNumIndexArr", "eqNumStartIndexLoc", "numOfDofPerNodeArr", "ien",
@@ -58,7 +58,8 @@ _PTR_FIELDS = ["meshCoor", "eqNumIndexArr", "eqNumStartIndexLoc", "numOfDofPerNo
 
 class State(C.Structure):
     _fields_ = [(n, C.c_void_p) for n in _PTR_FIELDS] + [
-        ("nt", C.c_int32), ("timeElapsed", C.c_double), ("err_node", C.c_int32), ("err_nt", C.c_int32)]
+        ("nt", C.c_int32), ("timeElapsed", C.c_double), ("err_node", C.c_int32), ("err_nt", C.c_int32),
+        ("tpHist", C.c_void_p)]
 
 
 def make_scalars(sc: Dict[str, float], nsteps_total: int | None = None) -> Scalars:
@@ -79,6 +80,8 @@ def make_scalars(sc: Dict[str, float], nsteps_total: int | None = None) -> Scala
     s.nPML = int(sc["nPML"])
     for n in ("C_elastic", "C_Q", "C_hg", "C_nuclea", "friclaw", "TPV", "insertFaultType", "nucfault"):
         setattr(s, n, int(sc[n]))
+    # the reference never assigns its module variable fric_tp_h (src/globalvar.f90:42), so it is zero
+    s.fric_tp_h = float(sc.get("fric_tp_h", 0.0))
     return s
 
 
@@ -135,6 +138,9 @@ def run_oracle(bundle: Dict[str, np.ndarray], sc: Dict[str, float], nsteps: int 
         setattr(st, k, out[k].ctypes.data)
     st.onFault = out["onFaultQuantHistSCECForm"].ctypes.data
     st.offFault = out["OffFaultStGramSCEC"].ctypes.data if s.noff > 0 else None
+    if s.friclaw == 5:
+        out["tpHist"] = np.zeros((nstep, max(s.nf, 1), 2))
+        st.tpHist = out["tpHist"].ctypes.data
     st.nt = 0
     st.timeElapsed = 0.0
     rc = L.eqdo_run(C.byref(s), C.byref(st), nstep)

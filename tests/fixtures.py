@@ -6,7 +6,7 @@ import numpy as np
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 CASES = ["box_sw", "box_tw", "box_layered", "box_rsf3", "box_rsf4", "box_q", "box_plastic", "box_hg2",
-         "box_dip", "box_rough"]
+         "box_dip", "box_rough", "box_tp"]
 
 # fric slots the time step writes (1-based, src/faulting.f90)
 FRIC_SLOTS = [20, 23, 24, 31, 32, 33, 34, 35, 36, 47, 48, 71, 72, 73, 74, 75, 76, 77, 78, 79, 80]
@@ -45,20 +45,36 @@ GROUPS = {
     "onfault_time": "time", "onfault_sliprate_s": "st_rate", "onfault_sliprate_d": "st_rate",
     "onfault_state": "state", "onfault_slip_s": "st_slip", "onfault_slip_d": "st_slip", "onfault_slip_n": "st_slip",
     "onfault_tau_s": "st_traction", "onfault_tau_d": "st_traction", "onfault_sigma_n": "st_traction",
+    "onfault_pore_pressure": "st_pressure", "onfault_temperature": "st_temperature",
     "station_series": "station", "velArr": "velArr", "dispArr": "dispArr", "stressArr": "stressArr",
     "pstrain": "pstrain",
 }
 
 
-def channels(out):
-    """Named time series / state channels used for parity reports."""
+# fric slots the reference leaves undefined: with thermal pressurization
+# (friclaw 5) its NewtonRaphson returns the local thetaPcTmp, which it assigns
+# only for friclaw < 5 (src/faulting.f90:459-516), into fric(23), and fric(24)
+# is computed from that value the next step
+UNDEFINED_SLOTS = {5: (23, 24)}
+
+
+def undefined_slots(sc):
+    return UNDEFINED_SLOTS.get(int(sc["friclaw"]), ())
+
+
+def channels(out, skip=()):
+    """Named time series / state channels used for parity reports (fric
+    slots in ``skip`` left out)."""
     ch = {}
     fr = out["fric"]
     for k in FRIC_SLOTS:
+        if k in skip:
+            continue
         ch[f"fric{k}"] = fr[:, k - 1]
     ch["fnft"] = np.where(out["fnft"] > 5000.0, 0.0, out["fnft"])
     h = out["onFaultQuantHistSCECForm"]
-    names = ["time", "sliprate_s", "sliprate_d", "state", "slip_s", "slip_d", "slip_n", "tau_s", "tau_d", "sigma_n"]
+    names = ["time", "sliprate_s", "sliprate_d", "state", "slip_s", "slip_d", "slip_n", "tau_s", "tau_d", "sigma_n",
+             "pore_pressure", "temperature"]
     for j, nm in enumerate(names):
         ch["onfault_" + nm] = h[:, :, j]
     if "OffFaultStGramSCEC" in out:

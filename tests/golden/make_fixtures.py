@@ -42,6 +42,7 @@ def cases():
         hg2,
         case.box_dip(name="box_dip"),
         case.box_rough(name="box_rough"),
+        case.box_tp(name="box_tp"),
     ]
 
 

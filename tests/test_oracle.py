@@ -4,15 +4,19 @@ import numpy as np
 import pytest
 
 import oracle
-from fixtures import CASES, load_golden
+from fixtures import CASES, load_golden, undefined_slots
 
 
 @pytest.mark.parametrize("name", CASES)
 def test_oracle_bitwise_vs_reference(name):
     bundle, sc, ref = load_golden(name)
     out = oracle.run_oracle(bundle, sc)
+    skip = [k - 1 for k in undefined_slots(sc)]
     for k, v in ref.items():
-        assert np.array_equal(out[k], v), f"{name}: {k} differs from the reference"
+        a = out[k]
+        if k == "fric" and skip:
+            a, v = np.delete(a, skip, axis=1), np.delete(v, skip, axis=1)
+        assert np.array_equal(a, v), f"{name}: {k} differs from the reference"
 
 
 def test_oracle_nan_stop():
