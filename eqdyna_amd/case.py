@@ -231,7 +231,63 @@ def boxcar_b1(x, ww, w):
     return 0.0
 
 
+def boxcar_b2(y, ww, w):
+    """Smooth boxcar B2 of TPV105 in depth y >= 0 (scripts/lib.py:36-48)."""
+    if y < w:
+        return 0.5 * (1.0 + math.tanh(w / (w - y) - w / (y + 1e-30)))
+    if y <= ww:
+        return 1.0
+    if y < ww + w:
+        return 0.5 * (1.0 + math.tanh(w / (y - ww - w) + w / (y - ww)))
+    return 0.0
+
+
+def boxcar_b3(y, ww, w):
+    """Smooth boxcar B3 of TPV105 in depth y >= 0 (scripts/lib.py:49-58)."""
+    if y <= ww:
+        return 1.0
+    if y < ww + w:
+        return 0.5 * (1.0 + math.tanh(w / (y - ww - w) + w / (y - ww)))
+    return 0.0
+
+
 # ---------------------------------------------------------------- presets
+def tpv1053d() -> Case:
+    """SCEC TPV105-3D with thermal pressurization as configured by the
+    reference's default parameters (scripts/defaultParameters.py, which
+    case_input/test.tpv1053d/user_defined_params.py uses with term = 5 s): the
+    class defaults plus the depth-dependent normal stress, the smooth-boxcar
+    fields of a, Vw and the hydraulic diffusivity, and 2 x 2 x 1 ranks (the
+    case behind test.reference.results/test.tpv1053d)."""
+    c = Case(name="tpv1053d")
+    c.term = 5.0
+    c.nx, c.ny, c.nz = 2, 2, 1
+    grav = 9.8
+    o = c.new_ofv()
+    for ix, x in enumerate(c.fx):
+        for iz, z in enumerate(c.fz):
+            o[1, iz, ix], o[2, iz, ix], o[3, iz, ix], o[4, iz, ix] = 0.18, 0.12, 0.3, 0.0
+            o[7, iz, ix] = -max(min(grav * 1670.0 * abs(z), 45.0e6), grav * 1670.0 * c.dx / 2.0)
+            o[8, iz, ix] = -0.41 * o[7, iz, ix]
+            t1 = boxcar_b1(x, 15.0e3, 3.0e3)
+            t2 = boxcar_b2(-z, 15.0e3, 3.0e3)
+            o[9, iz, ix] = 0.01 + (1.0 - t1 * t2) * 0.01
+            o[10, iz, ix], o[11, iz, ix], o[12, iz, ix], o[13, iz, ix] = 0.014, 0.4, 1e-6, 0.6
+            o[14, iz, ix] = 0.2
+            o[15, iz, ix] = 0.1 + 0.9 * (1.0 - t1 * t2)
+            t4 = boxcar_b3(-z, 15.0e3, 3.0e3)
+            o[16, iz, ix] = 4.0e-4 + 1.0 * (1.0 - t1 * t4)
+            o[17, iz, ix], o[18, iz, ix], o[19, iz, ix] = 1.0e-6, 2.7e6, 0.1e6
+            o[40, iz, ix], o[41, iz, ix], o[42, iz, ix] = 0.02, 483.15, 80.0e6
+            o[46, iz, ix] = 1e-16
+            o[20, iz, ix] = state_steady_state(o[9, iz, ix], o[10, iz, ix], o[11, iz, ix], o[12, iz, ix],
+                                               o[13, iz, ix], o[8, iz, ix], o[7, iz, ix], o[46, iz, ix], 5)
+    c.st_on_fault = [[0.0, -3.0], [0.0, -7.5], [0.0, -12.0], [9.0, -7.5], [12.0, -3.0], [12.0, -12.0], [15.0, -7.5],
+                     [18.0, -7.5], [-9.0, -7.5], [-12.0, -3.0], [-12.0, -12.0], [-15.0, -7.5], [-18.0, -7.5]]
+    c.st_off_fault = [[0, 9, 0], [0, -9, 0], [12, 6, 0], [12, -6, 0], [-12, 6, 0], [-12, -6, 0]]
+    return c
+
+
 def tpv8() -> Case:
     """SCEC TPV8 as configured by case_input/test.tpv8/user_defined_params.py
     (the case behind test.reference.results/test.tpv8).  roumax and vmaxPML keep
