@@ -126,6 +126,24 @@ def run_one(bundle, sc, precision: str, warmup: int, steps: int, guess=None, tun
     return r
 
 
+def run_graph(bundle, sc, precision: str, warmup: int, steps: int, guess=None, launch=None) -> float:
+    """The same run with every step replayed from one captured HIP graph (no
+    per-kernel events); returns its wall time."""
+    from eqdyna_amd.solver import Solver
+    s = Solver(bundle, sc, precision, device=0, nstep=warmup + steps)
+    if guess is not None:
+        s.set_guess(guess, GUESS["k_fallback"], GUESS["gate"])
+    for k, (b, r) in (launch or {}).items():
+        s.set_launch(k, b, r)
+    s.run(warmup)
+    s.set_graph(True)
+    t0 = time.perf_counter()
+    s.run(steps)
+    wall = time.perf_counter() - t0
+    s.close()
+    return wall
+
+
 def roofline(run, counts, precision: str, q: bool, workload: str):
     """Roofline of the run's dominant kernel (largest total time)."""
     tot = {k: ms * n for k, (ms, n) in run["ktimes"].items() if k in UNIT_BYTES}
@@ -224,6 +242,7 @@ def main() -> int:
     ap.add_argument("--host-startup", type=int, default=0, metavar="RANKS",
                     help="only time the reference host's start-up (meshgen, assembleGlobalMass) on the workload's "
                          "full mesh on RANKS MPI ranks, and the device precompute of the same mesh, then exit")
+    ap.add_argument("--graph", type=int, default=1, help="c4: also time both modes replayed from a captured step")
     ap.add_argument("--precompute", type=int, default=1,
                     help="also time the device-side element precompute (eqd_precompute) on the workload's mesh")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"], help="c1-c3 only")
@@ -276,11 +295,19 @@ def main() -> int:
     else:
         precision = args.precision
         run = run_one(bundle, sc, precision, args.warmup, steps, tune=args.tune)
-    del bundle
     value = 3.0 * nn * steps / run["wall"] / 1e6
     if "fp64_value" in extra:
         extra["ratio_fp32_learned_vs_fp64"] = round(value / extra["fp64_value"], 4)
+    if args.graph and args.workload == "c4":
+        w64 = run_graph(bundle, sc, "fp64", args.warmup, steps, launch=ref["launch"])
+        w32 = run_graph(bundle, sc, "fp32", args.warmup, steps, guess=params, launch=run["launch"])
+        g64, g32 = 3.0 * nn * steps / w64 / 1e6, 3.0 * nn * steps / w32 / 1e6
+        log(f"graph replay: fp64 {g64:.1f}, fp32+learned {g32:.1f} Mdof-steps/s")
+        extra["graph_replay"] = {"fp64_value": round(g64, 2), "fp32_learned_value": round(g32, 2),
+                                 "ratio": round(g32 / g64, 4),
+                                 "note": "same steps replayed from one captured step; no per-kernel events"}
     roof = roofline(run, counts, precision, c.C_Q == 1, args.workload)
+    del bundle
 
     cpu = None
     if args.cpu_baseline in ("auto", "reference"):
