@@ -1100,6 +1100,78 @@ int eqd_harvest_get(eqd_handle *h, float *out, int64_t max_rows, int64_t *rows)
     return EQD_OK;
 }
 
+// record index of the next record step after step nt
+static int32_t next_record(int32_t nt, int32_t every) { return nt <= 0 ? 0 : (nt - 1) / every + 1; }
+
+int eqd_record(eqd_handle *h, const int32_t *node_ids, int32_t n_nodes, int32_t every, int32_t capacity)
+{
+    if (!h) return EQD_EINVAL;
+    if (!h->committed) return fail(h, EQD_ESTATE, "eqd_record before eqd_commit");
+    if (n_nodes < 0 || (n_nodes > 0 && !node_ids) || capacity < 0 || (capacity > 0 && every < 1))
+        return fail(h, EQD_EINVAL, "bad record nodes/every/capacity");
+    for (int32_t i = 0; i < n_nodes; i++)
+        if (node_ids[i] < 1 || node_ids[i] > h->s.nn) return fail(h, EQD_EINVAL, "record node %d out of range", i + 1);
+    HIPCHK(h, hipSetDevice(h->s.device));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    for (void *&p : h->rec_bufs) {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+    }
+    h->d_rec = nullptr;
+    h->rec = Recorder{};
+    h->version++;
+    if (capacity == 0) return EQD_OK;
+    std::vector<int32_t> ids(node_ids, node_ids + n_nodes);
+    for (int32_t &v : ids) v -= 1;
+    const size_t nv = sizeof(double) * 3 * (size_t)capacity * (size_t)std::max(n_nodes, 1);
+    const size_t ns = sizeof(double) * (size_t)capacity * (size_t)std::max<int64_t>(h->F.nf, 1);
+    if (hipMalloc(&h->rec_bufs[0], sizeof(Recorder) + sizeof(int32_t) * std::max(n_nodes, 1)) != hipSuccess ||
+        hipMalloc(&h->rec_bufs[1], nv) != hipSuccess || hipMalloc(&h->rec_bufs[2], ns) != hipSuccess)
+        return fail(h, EQD_EOOM, "record buffers of %d records", capacity);
+    Recorder &R = h->rec;
+    R.every = every;
+    R.capacity = capacity;
+    R.n_nodes = n_nodes;
+    R.base = next_record(h->nt, every);  // the first record is the next record step
+    R.nodes = (const int32_t *)((char *)h->rec_bufs[0] + sizeof(Recorder));
+    R.vel = (double *)h->rec_bufs[1];
+    R.slip_rate = h->F.nf > 0 ? (double *)h->rec_bufs[2] : nullptr;
+    if (n_nodes > 0)
+        HIPCHK(h, hipMemcpy((void *)R.nodes, ids.data(), sizeof(int32_t) * n_nodes, hipMemcpyHostToDevice));
+    h->d_rec = (Recorder *)h->rec_bufs[0];
+    HIPCHK(h, hipMemcpy(h->d_rec, &R, sizeof(Recorder), hipMemcpyHostToDevice));
+    return EQD_OK;
+}
+
+int eqd_record_get(eqd_handle *h, double *vel, double *slip_rate, int32_t max_records, int32_t *count)
+{
+    if (!h || !count || max_records < 0) return EQD_EINVAL;
+    if (!h->committed) return fail(h, EQD_ESTATE, "eqd_record_get before eqd_commit");
+    *count = 0;
+    if (!h->d_rec) return EQD_OK;
+    HIPCHK(h, hipSetDevice(h->s.device));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    Recorder &R = h->rec;
+    const int32_t have = next_record(h->nt, R.every) - R.base;  // record steps since the last drain
+    if (std::min(have, R.capacity) > max_records)
+        return fail(h, EQD_EINVAL, "%d records waiting, room for %d", std::min(have, R.capacity), max_records);
+    const int32_t n = std::min(have, R.capacity);
+    if (n > 0) {
+        if (vel && R.n_nodes > 0)
+            HIPCHK(h, hipMemcpy(vel, R.vel, sizeof(double) * 3 * R.n_nodes * (size_t)n, hipMemcpyDeviceToHost));
+        if (slip_rate && R.slip_rate)
+            HIPCHK(h, hipMemcpy(slip_rate, R.slip_rate, sizeof(double) * h->F.nf * (size_t)n, hipMemcpyDeviceToHost));
+    }
+    *count = n;
+    // the buffer starts over at the next record step
+    R.base = next_record(h->nt, R.every);
+    HIPCHK(h, hipMemcpy(h->d_rec, &R, sizeof(Recorder), hipMemcpyHostToDevice));
+    if (have > n)
+        return fail(h, EQD_EINVAL, "%d record steps since the last eqd_record_get, capacity %d: the last %d dropped",
+                    have, R.capacity, have - n);
+    return EQD_OK;
+}
+
 int eqd_newton_stats(eqd_handle *h, int64_t stats[5])
 {
     if (!h || !stats) return EQD_EINVAL;
@@ -1144,6 +1216,8 @@ int eqd_destroy(eqd_handle *h)
     }
     for (void *p : h->allocs) (void)hipFree(p);
     if (h->harvest) (void)hipFree(h->harvest);
+    for (void *p : h->rec_bufs)
+        if (p) (void)hipFree(p);
     delete h;
     return EQD_OK;
 }

@@ -152,7 +152,18 @@ struct Clock {
 
 // launch kinds of one step
 enum LaunchKind : int {
-    kKElem = 0, kKPml, kKNode3, kKNodePml, kKFault, kKStation, kKClock, kKPend3, kKPendPml, kKTp, kKinds
+    kKElem = 0, kKPml, kKNode3, kKNodePml, kKFault, kKStation, kKClock, kKPend3, kKPendPml, kKTp, kKRecord, kKinds
+};
+
+// Ground-motion and source-evolution records (the data of output_gm and
+// output_src_evol, src/library_output.f90:267-312): at every step nt with
+// (nt - 1) % every == 0 the velocities of the listed nodes and fric(47) of
+// every pair go to record slot (nt - 1) / every - base (dropped past capacity).
+struct Recorder {
+    int32_t every, capacity, n_nodes, base;
+    const int32_t *nodes;  // [n_nodes] 0-based node ids
+    double *vel;           // [capacity][n_nodes][3]
+    double *slip_rate;     // [capacity][nf]
 };
 
 // Everything one scenario's step reads: every kernel takes an array of these
@@ -173,6 +184,7 @@ struct Member {
     const int32_t *node3_all;   // all 3-component nodes
     int64_t n_node3_all;
     Clock *clk;
+    const Recorder *rec;  // null when nothing is recorded
     int32_t blk0[kKinds];
 };
 

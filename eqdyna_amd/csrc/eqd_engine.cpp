@@ -22,7 +22,7 @@ using namespace eqd;
 namespace {
 
 const char *const kKindName[kKinds] = {"element", "element_pml", "node", "node_pml", "fault",
-                                       "station", "clock", "pending", "pending_pml", "thermop"};
+                                       "station", "clock", "pending", "pending_pml", "thermop", "record"};
 
 template <typename R>
 const RegElems<R> &elems(const eqd_handle *h);
@@ -54,6 +54,7 @@ Member<R> make_member(const eqd_engine *e, const eqd_handle *h)
     M.node3_all = h->d_node3_all;
     M.n_node3_all = (int64_t)h->node3_all.size();
     M.clk = h->d_clock;
+    M.rec = e->no_harvest ? nullptr : h->d_rec;  // the tuner's trial steps record nothing
     return M;
 }
 
@@ -69,6 +70,7 @@ int64_t count_of(const eqd_handle *h, int kind)
     case kKPend3: return (int64_t)h->node3_all.size();
     case kKPendPml: return h->PN.n;
     case kKTp: return h->F.tph ? h->F.nf * 64 : 0;  // one wavefront per pair
+    case kKRecord: return h->d_rec ? h->rec.n_nodes : 0;
     default: return 0;
     }
 }
@@ -241,6 +243,7 @@ int enqueue_step(eqd_engine *e)
     HIPCHK(e, launch_clock<R>(dM, (int)e->hs.size(), st));
     timed_begin(e, kKStation, &ev, st);
     for (const Group &g : e->groups[kKStation]) HIPCHK(e, launch_station<R>(dM, g.g, st));
+    for (const Group &g : e->groups[kKRecord]) HIPCHK(e, launch_record<R>(dM, g.g, st));
     timed_end(ev, st);
     timed_begin(e, kKElem, &ev, st);
     for (const Group &g : e->groups[kKElem])

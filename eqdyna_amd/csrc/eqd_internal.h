@@ -33,6 +33,8 @@ template <typename R>
 hipError_t launch_clock(const Member<R> *, int nmem, hipStream_t);
 template <typename R>
 hipError_t launch_thermop(const Member<R> *, const LaunchGrid &, hipStream_t);
+template <typename R>
+hipError_t launch_record(const Member<R> *, const LaunchGrid &, hipStream_t);
 hipError_t launch_mlp_prepare(const float *p, __bf16 *wb, float *w4r, hipStream_t s);
 
 struct KTimer {
@@ -121,6 +123,10 @@ struct eqd_handle {
     __bf16 *mlp_wb = nullptr;
     // harvest buffer of training rows (its schedule is in F)
     float *harvest = nullptr;
+    // ground-motion / source-evolution records: host copy and device objects
+    eqd::Recorder rec{};
+    eqd::Recorder *d_rec = nullptr;
+    void *rec_bufs[3] = {nullptr, nullptr, nullptr};
     bool timing = false;  // per-kernel event timing of the handle's own runs
     // bumped by every change the engine's launches depend on
     int64_t version = 0;

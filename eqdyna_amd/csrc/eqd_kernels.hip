@@ -1316,6 +1316,14 @@ __global__ void __launch_bounds__(128) fault_kernel(const Member<R> *__restrict_
     }
     // storeRuptureTime, src/faulting.f90:329-341
     if (Fs.fnft[i] > 5000.0 && rate[3] >= C.slipRateThres) Fs.fnft[i] = time;
+    // source-evolution record (output_src_evol, src/library_output.f90:297-312)
+    if (M.rec) {
+        const Recorder rc = *M.rec;
+        if ((nt - 1) % rc.every == 0) {
+            const int64_t slot = (nt - 1) / rc.every - rc.base;
+            if (slot >= 0 && slot < rc.capacity) rc.slip_rate[slot * nf + i] = FR(47);
+        }
+    }
 #undef FR
     // F/m (src/driver.f90:29) and the pair's update for the next step
     const double msl = N.mass[ns], mma = N.mass[nm];
@@ -1414,6 +1422,27 @@ __global__ void __launch_bounds__(256) thermop_kernel(const Member<R> *__restric
         FR(52) = st / FR(18) / sqrt(C.pi) + FR(41);
     }
 #undef FR
+}
+
+// Ground-motion record (output_gm, src/library_output.f90:267-279): the
+// listed nodes' velocities at a record step, taken where the reference's
+// velArr holds them (after this step's update, before its forces).
+template <typename R>
+__global__ void __launch_bounds__(256) record_kernel(const Member<R> *__restrict__ Ms, const uint16_t *__restrict__ bmem)
+{
+    const unsigned gb = blockIdx.x;
+    const Member<R> &M = Ms[member_of(bmem, gb)];
+    const int64_t t = (int64_t)(gb - (unsigned)M.blk0[kKRecord]) * blockDim.x + threadIdx.x;
+    const Recorder rc = *M.rec;
+    if (t >= rc.n_nodes) return;
+    const int nt = M.clk->nt;
+    if ((nt - 1) % rc.every != 0) return;
+    const int64_t slot = (nt - 1) / rc.every - rc.base;
+    if (slot < 0 || slot >= rc.capacity) return;
+    const int node = rc.nodes[t];
+    const int64_t nn = M.N.nn;
+#pragma unroll
+    for (int k = 0; k < 3; k++) rc.vel[(slot * rc.n_nodes + t) * 3 + k] = M.N.vel[k * nn + node];
 }
 
 // The step clock: nt += 1 and time += dt for every scenario of the launch,
@@ -1532,6 +1561,15 @@ hipError_t launch_fault(const Member<R> *M, const LaunchGrid &g, int learn, hipS
 }
 
 template <typename R>
+hipError_t launch_record(const Member<R> *M, const LaunchGrid &g, hipStream_t s)
+{
+    if (g.grid == 0) return hipSuccess;
+    if (g.block < 64 || g.block > 256 || g.block % 64) return hipErrorInvalidValue;
+    record_kernel<R><<<g.grid, g.block, 0, s>>>(M, g.bmem);
+    return hipGetLastError();
+}
+
+template <typename R>
 hipError_t launch_thermop(const Member<R> *M, const LaunchGrid &g, hipStream_t s)
 {
     if (g.grid == 0) return hipSuccess;
@@ -1566,6 +1604,7 @@ hipError_t launch_clock(const Member<R> *M, int nmem, hipStream_t s)
     template hipError_t launch_fault<R>(const Member<R> *, const LaunchGrid &, int, hipStream_t);                   \
     template hipError_t launch_station<R>(const Member<R> *, const LaunchGrid &, hipStream_t);                      \
     template hipError_t launch_thermop<R>(const Member<R> *, const LaunchGrid &, hipStream_t);                      \
+    template hipError_t launch_record<R>(const Member<R> *, const LaunchGrid &, hipStream_t);                       \
     template hipError_t launch_clock<R>(const Member<R> *, int, hipStream_t);
 EQD_INSTANTIATE(double)
 EQD_INSTANTIATE(float)

@@ -68,6 +68,19 @@ module eqd_abi
             type(*), dimension(*) :: host
             integer(c_int64_t), value :: count
         end function eqd_get
+        integer(c_int) function eqd_record(h, node_ids, n_nodes, every, capacity) bind(C, name='eqd_record')
+            import :: c_ptr, c_int, c_int32_t
+            type(c_ptr), value :: h
+            integer(c_int32_t), dimension(*), intent(in) :: node_ids
+            integer(c_int32_t), value :: n_nodes, every, capacity
+        end function eqd_record
+        integer(c_int) function eqd_record_get(h, vel, slip_rate, max_records, cnt) bind(C, name='eqd_record_get')
+            import :: c_ptr, c_int, c_int32_t, c_double
+            type(c_ptr), value :: h
+            real(c_double), dimension(*) :: vel, slip_rate
+            integer(c_int32_t), value :: max_records
+            integer(c_int32_t), intent(out) :: cnt
+        end function eqd_record_get
         integer(c_int) function eqd_clock(h, nt, t) bind(C, name='eqd_clock')
             import :: c_ptr, c_int, c_int32_t, c_double
             type(c_ptr), value :: h
@@ -119,8 +132,10 @@ subroutine driver
     type(eqd_scalars) :: s
     type(c_ptr) :: h
     integer(c_int) :: rc
-    integer(c_int32_t) :: ntc, badnode, badnt, chunk, done
+    integer(c_int32_t) :: ntc, badnode, badnt, chunk, done, nrec, irec, inode
+    integer(c_int32_t), parameter :: maxrec = 16
     real(c_double) :: tc
+    real(c_double), allocatable :: recv(:, :, :), recs(:, :)
     integer(c_int64_t) :: nf, non, noff
     integer :: k
 
@@ -178,15 +193,25 @@ subroutine driver
     call eqd_check(h, eqd_set(h, EQD_FNFT, fnft, nf), 'set fnft')
     call eqd_check(h, eqd_commit(h), 'commit')
 
+    ! Ground motion and source evolution (src/driver.f90:30-33): the library
+    ! records the surface nodes' velocities and fric(47) on the device at
+    ! every step with mod(nt,10) == 1; after each chunk the records are
+    ! appended to the files output_gm and output_src_evol write
+    ! (src/library_output.f90:267-312: stream binary, per record the three
+    ! velocity components of each surface node, and fric(47) of each pair).
+    ! Those routines re-open their still-connected units at every call, which
+    ! this compiler's runtime rejects, so the files are written here, one OPEN
+    ! per chunk, with the same content.
+    if (outputGroundMotion == 1) then
+        allocate(recv(3, max(surface_nnode, 1), maxrec), recs(max(nftnd(1), 1), maxrec))
+        call eqd_check(h, eqd_record(h, surfaceNodeIdArr, int(surface_nnode, c_int32_t), 10_c_int32_t, maxrec), &
+                       'record')
+    endif
     ! Advance in chunks that end where the reference prints progress
-    ! (src/driver.f90:13-17) or writes ground motion (src/driver.f90:30-33).
+    ! (src/driver.f90:13-17), at most 100 steps (10 records).
     done = 0
     do while (done < nstep)
-        chunk = nstep - done
-        if (outputGroundMotion == 1) then
-            ! stop after the next step nt with mod(nt,10) == 1
-            chunk = min(chunk, int(mod(1 - mod(done + 1, 10) + 10, 10) + 1, c_int32_t))
-        endif
+        chunk = min(nstep - done, 100_c_int32_t)
         if (me == masterProcsId) then
             k = done + 1
             if (mod(k, 100) == 1) then
@@ -207,11 +232,26 @@ subroutine driver
         rc = eqd_clock(h, ntc, tc)
         nt = ntc
         timeElapsed = tc
-        if (outputGroundMotion == 1 .and. mod(nt, 10) == 1) then
-            call eqd_check(h, eqd_get(h, EQD_VELARR, velArr, int(size(velArr), c_int64_t)), 'get velArr')
-            call eqd_check(h, eqd_get(h, EQD_FRIC, fric, 100 * nf), 'get fric')
-            call output_gm
-            call output_src_evol
+        if (outputGroundMotion == 1) then
+            call eqd_check(h, eqd_record_get(h, recv, recs, maxrec, nrec), 'record_get')
+            if (nrec > 0 .and. surface_nnode > 0) then
+                open(unit=10009+me, file='gm'//mm, status='unknown', position='append', access='stream')
+                do irec = 1, nrec
+                    do inode = 1, surface_nnode
+                        write(10009+me) recv(1, inode, irec), recv(2, inode, irec), recv(3, inode, irec)
+                    enddo
+                enddo
+                close(10009+me)
+            endif
+            if (nrec > 0 .and. nftnd(1) > 0) then
+                open(unit=30009+me, file='src_evol'//mm, position='append', access='stream')
+                do irec = 1, nrec
+                    do inode = 1, nftnd(1)
+                        write(30009+me) recs(inode, irec)
+                    enddo
+                enddo
+                close(30009+me)
+            endif
         endif
     enddo
 

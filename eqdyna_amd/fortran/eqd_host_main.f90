@@ -39,6 +39,7 @@ program eqdyna_gpu
     call meshgen
     call checkMeshMaterial
     call host_read_on_fault
+    if (outputGroundMotion == 1 .or. outputFinalSurfDisp == 1) call host_find_surface_nodes
     call host_alloc_after_meshgen
     call assembleGlobalMass
     call host_init_vel
@@ -49,6 +50,7 @@ program eqdyna_gpu
     call output_offfault_st
     call output_frt
     if (output_plastic == 1) call output_plastic_strain
+    if (outputFinalSurfDisp == 1) call output_finalSurfDisp
     call MPI_Finalize(ierr)
 end program eqdyna_gpu
 
@@ -179,3 +181,30 @@ subroutine host_read_on_fault
     enddo
     deallocate(v)
 end subroutine host_read_on_fault
+
+! find_surfaceNodeIdArr (src/library_output.f90:247-264): the surface nodes
+! within 20 km of the fault's extent, in node order, and their coordinates in
+! surface_coor.txt<rank>.  Restated because the reference's routine re-opens
+! its output unit, still connected, for every node, which this compiler's
+! runtime rejects; the selection and the file are the same.
+subroutine host_find_surface_nodes
+    use globalvar
+    implicit none
+    integer (kind = 4) :: i
+    logical :: opened
+    opened = .false.
+    do i = 1, totalNumOfNodes
+        if ((meshCoor(1,i) < fltxyz(2,1,1) + 20.0d3) .and. (meshCoor(1,i) > fltxyz(1,1,1) - 20.0d3) &
+                .and. (meshCoor(2,i) < fltxyz(2,2,1) + 20.0d3) .and. (meshCoor(2,i) > fltxyz(1,2,1) - 20.0d3) &
+                .and. (abs(meshCoor(3,i)) < dx/1000)) then
+            surface_nnode = surface_nnode + 1
+            surfaceNodeIdArr(surface_nnode) = i
+            if (.not. opened) then
+                open(unit=10008+me, file='surface_coor.txt'//mm, status='unknown', position='append')
+                opened = .true.
+            endif
+            write(10008+me,'(1x,3e18.7e4)') meshCoor(1,i), meshCoor(2,i), meshCoor(3,i)
+        endif
+    enddo
+    if (opened) close(10008+me)
+end subroutine host_find_surface_nodes

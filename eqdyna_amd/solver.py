@@ -23,7 +23,7 @@ EXPORTS = ["eqd_create", "eqd_set", "eqd_commit", "eqd_run", "eqd_get", "eqd_clo
            "eqd_mlp_grad", "eqd_mlp_train", "eqd_set_launch", "eqd_get_launch", "eqd_tune", "eqd_set_graph",
            "eqd_ensemble_create", "eqd_ensemble_run", "eqd_ensemble_tune", "eqd_ensemble_set_launch",
            "eqd_ensemble_set_graph", "eqd_ensemble_set_timing", "eqd_ensemble_kernel_time",
-           "eqd_ensemble_last_error", "eqd_ensemble_destroy", "eqd_precompute"]
+           "eqd_ensemble_last_error", "eqd_ensemble_destroy", "eqd_precompute", "eqd_record", "eqd_record_get"]
 
 # launch kinds (EQD_KIND_*)
 KINDS = {"element": 0, "element_pml": 1, "node": 2, "node_pml": 3, "fault": 4}
@@ -132,6 +132,8 @@ def lib():
         L.eqd_ensemble_last_error.restype = C.c_char_p
         L.eqd_ensemble_destroy.argtypes = [C.c_void_p]
         L.eqd_precompute.argtypes = [C.POINTER(PrecomputeArgs), C.POINTER(C.c_int32), C.POINTER(C.c_double)]
+        L.eqd_record.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_int32]
+        L.eqd_record_get.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.POINTER(C.c_int32)]
         _lib = L
     return _lib
 
@@ -276,6 +278,22 @@ class Solver:
     def device_bytes(self) -> int:
         return int(self.L.eqd_device_bytes(self.h))
 
+    def record(self, node_ids, every: int = 10, capacity: int = 64) -> None:
+        """Record the velocities of node_ids (1-based) and fric(47) at every
+        step nt with (nt - 1) % every == 0 (output_gm / output_src_evol)."""
+        ids = np.ascontiguousarray(node_ids, dtype="<i4")
+        self._rec_nodes = ids.size
+        self._check(self.L.eqd_record(self.h, ids.ctypes.data, ids.size, int(every), int(capacity)))
+
+    def record_get(self, max_records: int = 1024):
+        """(vel [count, n_nodes, 3], slip_rate [count, nf]) recorded since the last call."""
+        n = getattr(self, "_rec_nodes", 0)
+        vel = np.zeros((max_records, max(n, 1), 3))
+        sr = np.zeros((max_records, max(self.s.nf, 1)))
+        cnt = C.c_int32()
+        self._check(self.L.eqd_record_get(self.h, vel.ctypes.data, sr.ctypes.data, int(max_records), C.byref(cnt)))
+        return vel[:cnt.value, :n], sr[:cnt.value, :self.s.nf]
+
     def set_launch(self, kind: str, block: int, remap: int = 0) -> None:
         """Launch shape of one kind (KINDS): block size and XCD remapping."""
         self._check(self.L.eqd_set_launch(self.h, KINDS[kind], int(block), int(remap)))
@@ -382,6 +400,22 @@ class Ensemble:
         buf = C.create_string_buffer(1 << 16)
         self._check(self.L.eqd_ensemble_tune(self.e, int(steps), buf, len(buf)))
         return json.loads(buf.value.decode())
+
+    def record(self, node_ids, every: int = 10, capacity: int = 64) -> None:
+        """Record the velocities of node_ids (1-based) and fric(47) at every
+        step nt with (nt - 1) % every == 0 (output_gm / output_src_evol)."""
+        ids = np.ascontiguousarray(node_ids, dtype="<i4")
+        self._rec_nodes = ids.size
+        self._check(self.L.eqd_record(self.h, ids.ctypes.data, ids.size, int(every), int(capacity)))
+
+    def record_get(self, max_records: int = 1024):
+        """(vel [count, n_nodes, 3], slip_rate [count, nf]) recorded since the last call."""
+        n = getattr(self, "_rec_nodes", 0)
+        vel = np.zeros((max_records, max(n, 1), 3))
+        sr = np.zeros((max_records, max(self.s.nf, 1)))
+        cnt = C.c_int32()
+        self._check(self.L.eqd_record_get(self.h, vel.ctypes.data, sr.ctypes.data, int(max_records), C.byref(cnt)))
+        return vel[:cnt.value, :n], sr[:cnt.value, :self.s.nf]
 
     def set_launch(self, kind: str, block: int, remap: int = 0) -> None:
         self._check(self.L.eqd_ensemble_set_launch(self.e, KINDS[kind], int(block), int(remap)))

@@ -214,6 +214,21 @@ int eqd_mlp_train(const float *X, const float *r, int64_t n, const eqd_mlp_train
                   float *epoch_loss);
 
 /* ---------------------------------------------------------------------------
+ * Ground-motion and source-evolution records: the data output_gm and
+ * output_src_evol write every 10 steps (src/driver.f90:30-33,
+ * src/library_output.f90:267-312) recorded on the device, so the host no
+ * longer copies all of velArr and fric back to write a few nodes.  At every
+ * step nt with (nt - 1) % every == 0 the velocities of node_ids (1-based, e.g.
+ * surfaceNodeIdArr) and fric(47) of every pair go to the next of `capacity`
+ * records; capacity 0 stops recording.  eqd_record_get copies the records
+ * since the last call (vel [count][n_nodes][3], slip_rate [count][nf], either
+ * may be NULL) and empties the buffer.  EQD_EINVAL if more records wait than
+ * max_records (nothing is copied), or if more record steps passed than the
+ * capacity holds (the first `capacity` are copied, the rest were dropped). */
+int eqd_record(eqd_handle *h, const int32_t *node_ids, int32_t n_nodes, int32_t every, int32_t capacity);
+int eqd_record_get(eqd_handle *h, double *vel, double *slip_rate, int32_t max_records, int32_t *count);
+
+/* ---------------------------------------------------------------------------
  * Device-side one-time precompute: replaces the element loop of
  * assembleGlobalMass (src/assembleGlobalMass.f90:283-406, with
  * calcGlobalShapeFunc, src/calcGlobalShapeFunc.f90:1-75, contm,

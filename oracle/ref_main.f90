@@ -55,6 +55,7 @@ program eqd_ref_main
     call meshgen
     tMesh = MPI_WTIME() - tMesh
     call ref_read_on_fault_raw
+    if (outputGroundMotion == 1 .or. outputFinalSurfDisp == 1) call ref_find_surface_nodes
     call ref_alloc_after_meshgen
     call MPI_Barrier(MPI_COMM_WORLD, ierr)
     tMass = MPI_WTIME()
@@ -73,6 +74,7 @@ program eqd_ref_main
     call MPI_Allreduce(MPI_IN_PLACE, tLoop, 1, MPI_DOUBLE_PRECISION, MPI_MAX, MPI_COMM_WORLD, ierr)
 
     call output_frt
+    if (outputFinalSurfDisp == 1) call output_finalSurfDisp
     if (dumpBundle == 1) call ref_dump_outputs('ref_out')
     if (me == 0) then
         write(*,'(A,I12,A,I12,A,I8,A,ES16.8,A,ES16.8,A,ES16.8)') 'EQD_REF nodes ', totalNumOfNodes, &
@@ -453,3 +455,30 @@ subroutine ref_dump_outputs(dir)
     if (noff > 0) call dump_r8(dir, 'OffFaultStGramSCEC', OffFaultStGramSCEC, (6*noff+1)*nstep, 2, 6*noff+1, nstep, 1)
     close(81)
 end subroutine ref_dump_outputs
+
+! find_surfaceNodeIdArr (src/library_output.f90:247-264): the surface nodes
+! within 20 km of the fault's extent, in node order, and their coordinates in
+! surface_coor.txt<rank>.  Restated because the reference's routine re-opens
+! its output unit, still connected, for every node, which this compiler's
+! runtime rejects; the selection and the file are the same.
+subroutine ref_find_surface_nodes
+    use globalvar
+    implicit none
+    integer (kind = 4) :: i
+    logical :: opened
+    opened = .false.
+    do i = 1, totalNumOfNodes
+        if ((meshCoor(1,i) < fltxyz(2,1,1) + 20.0d3) .and. (meshCoor(1,i) > fltxyz(1,1,1) - 20.0d3) &
+                .and. (meshCoor(2,i) < fltxyz(2,2,1) + 20.0d3) .and. (meshCoor(2,i) > fltxyz(1,2,1) - 20.0d3) &
+                .and. (abs(meshCoor(3,i)) < dx/1000)) then
+            surface_nnode = surface_nnode + 1
+            surfaceNodeIdArr(surface_nnode) = i
+            if (.not. opened) then
+                open(unit=10008+me, file='surface_coor.txt'//mm, status='unknown', position='append')
+                opened = .true.
+            endif
+            write(10008+me,'(1x,3e18.7e4)') meshCoor(1,i), meshCoor(2,i), meshCoor(3,i)
+        endif
+    enddo
+    if (opened) close(10008+me)
+end subroutine ref_find_surface_nodes

@@ -48,3 +48,42 @@ def test_reference_host_with_gpu_driver_matches_reference():
     assert err.max() <= 2e-6
     # the reference's station writer ran on the GPU results (one file per station)
     assert len(gs) == len(c.st_on_fault)
+
+
+@pytest.mark.gpu
+def test_ground_motion_output_matches_library_records():
+    # outputGroundMotion = 1: the library records the surface velocities and
+    # fric(47) every 10 steps on the device and the host writes them in the
+    # format of output_gm / output_src_evol (src/library_output.f90:267-312;
+    # the reference's own routines cannot run under this compiler: they
+    # re-open a connected unit).  The files must hold, bit for bit, what the
+    # same GPU step records through the Python front end on the same mesh.
+    if not os.path.exists(GPU_HOST):
+        pytest.skip("the Fortran host build needs the reference sources (built where they are present)")
+    from eqdyna_amd import mesh
+    from eqdyna_amd.solver import Solver
+    c = case.tpv8()
+    c.nx = c.ny = c.nz = 1
+    c.outputGroundMotion = 1
+    with tempfile.TemporaryDirectory() as dg:
+        c.write_inputs(dg)
+        print("running the Fortran host with the GPU driver on TPV8 with ground-motion output", flush=True)
+        p = subprocess.run([GPU_HOST], cwd=dg, capture_output=True, text=True, timeout=300)
+        assert p.returncode == 0 and "EQD_GPU" in p.stdout, p.stdout[-2000:] + p.stderr[-2000:]
+        gm = np.fromfile(os.path.join(dg, "gm0"), dtype="<f8")
+        se = np.fromfile(os.path.join(dg, "src_evol0"), dtype="<f8")
+    bundle, sc = mesh.build_case(c)
+    xyz = np.asarray(bundle["meshCoor"]).reshape(-1, 3)
+    # surface nodes within 20 km of the fault's extent (find_surfaceNodeIdArr, src/library_output.f90:247-264)
+    sel = ((xyz[:, 0] < c.fxmax + 20.0e3) & (xyz[:, 0] > c.fxmin - 20.0e3) & (xyz[:, 1] < c.fymax + 20.0e3) &
+           (xyz[:, 1] > c.fymin - 20.0e3) & (np.abs(xyz[:, 2]) < c.dx / 1000))
+    ids = (np.nonzero(sel)[0] + 1).astype(np.int32)
+    nstep = int(sc["nstep"])
+    s = Solver(bundle, sc)
+    s.record(ids, every=10, capacity=nstep // 10 + 1)
+    s.run(nstep)
+    v, r = s.record_get(nstep // 10 + 1)
+    s.close()
+    print(f"{len(v)} records of {len(ids)} surface nodes and {r.shape[1]} pairs", flush=True)
+    assert gm.size == v.size and se.size == r.size
+    assert np.array_equal(gm, v.ravel()) and np.array_equal(se, r.ravel())
