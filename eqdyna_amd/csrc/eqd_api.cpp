@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <map>
 #include <string>
+#include <tuple>
 #include <type_traits>
 #include <unordered_map>
 #include <vector>
@@ -254,6 +255,64 @@ static int upload_operators(eqd_handle *h, E &X, int64_t n, bool no_classes, OpR
     return 0;
 }
 
+// Element order within a class: bricks of 4 (x) x 8 (z) x 32 (y) elements,
+// y fastest inside a brick and bricks in y, then z, then x order, so that the
+// elements that share a node run close together in time and the node's data
+// is still in L2 when its next element needs it.  Grid indices come from the
+// element centroids: x and z as ranks of the distinct centroid coordinates
+// (to the millimetre), y as the rank within each (x, z) column, which holds
+// for columns shifted by a rough fault surface.  Results do not depend on the
+// order: every node sums its elements in ascending reference id.  Selected by
+// EQD_ELEMENT_ORDER=1; measured no faster than the reference order on the
+// 50 m and rough-fault boxes (the element kernels' L2 misses are mostly their
+// compulsory streams), so the reference order is the default.
+static void brick_order(std::vector<int32_t> &ids, const double *coor, const int32_t *ien)
+{
+    const int64_t n = (int64_t)ids.size();
+    if (n == 0) return;
+    std::vector<int64_t> qx(n), qz(n);
+    std::vector<double> cy(n);
+    for (int64_t t = 0; t < n; t++) {
+        const int64_t e = ids[t];
+        double c[3] = {0.0, 0.0, 0.0};
+        for (int a = 0; a < 8; a++)
+            for (int k = 0; k < 3; k++) c[k] += coor[(int64_t)(ien[e * 8 + a] - 1) * 3 + k];
+        qx[t] = std::llround(c[0] / 8.0 * 1000.0);
+        qz[t] = std::llround(c[2] / 8.0 * 1000.0);
+        cy[t] = c[1] / 8.0;
+    }
+    auto ranks = [n](const std::vector<int64_t> &q) {
+        std::vector<int64_t> u(q);
+        std::sort(u.begin(), u.end());
+        u.erase(std::unique(u.begin(), u.end()), u.end());
+        std::vector<int32_t> r(n);
+        for (int64_t t = 0; t < n; t++) r[t] = (int32_t)(std::lower_bound(u.begin(), u.end(), q[t]) - u.begin());
+        return r;
+    };
+    const std::vector<int32_t> xi = ranks(qx), zi = ranks(qz);
+    std::vector<int64_t> ord(n);
+    for (int64_t t = 0; t < n; t++) ord[t] = t;
+    std::sort(ord.begin(), ord.end(), [&](int64_t a, int64_t b) {
+        if (xi[a] != xi[b]) return xi[a] < xi[b];
+        if (zi[a] != zi[b]) return zi[a] < zi[b];
+        if (cy[a] != cy[b]) return cy[a] < cy[b];
+        return ids[a] < ids[b];
+    });
+    std::vector<int32_t> yi(n);
+    for (int64_t k = 0; k < n; k++) {
+        const int64_t t = ord[k];
+        const bool same = k > 0 && xi[ord[k - 1]] == xi[t] && zi[ord[k - 1]] == zi[t];
+        yi[t] = same ? yi[ord[k - 1]] + 1 : 0;
+    }
+    auto key = [&](int64_t t) {
+        return std::make_tuple(xi[t] / 4, zi[t] / 8, yi[t] / 32, xi[t] % 4, zi[t] % 8, yi[t] % 32, ids[t]);
+    };
+    std::sort(ord.begin(), ord.end(), [&](int64_t a, int64_t b) { return key(a) < key(b); });
+    std::vector<int32_t> out(n);
+    for (int64_t k = 0; k < n; k++) out[k] = ids[ord[k]];
+    ids.swap(out);
+}
+
 template <typename T>
 static void keep_copy(eqd_handle *h, int f, std::map<int, std::vector<T>> &dst)
 {
@@ -435,6 +494,13 @@ int eqd_commit(eqd_handle *h)
         const int need = t == 2 ? 21 : 12;
         if (sci[e] < 0 || (int64_t)sci[e] + need > s.size_stress + (t == 2 ? 0 : 6))
             return fail(h, EQD_EINVAL, "element %lld: stress index out of range", (long long)e + 1);
+    }
+    {
+        const char *o = std::getenv("EQD_ELEMENT_ORDER");
+        if (o && o[0] == '1') {
+            brick_order(h->reg_ids, coor, ien);
+            brick_order(h->pml_ids, coor, ien);
+        }
     }
     const int64_t nr = (int64_t)h->reg_ids.size(), np = (int64_t)h->pml_ids.size();
     if (nr > (int64_t)kEntIdxMask || np > (int64_t)kEntIdxMask) return fail(h, EQD_EINVAL, "too many elements in one class");

@@ -145,3 +145,18 @@ def test_ensemble_tuner():
     for m in members:
         assert m.clock()[0] == 20
         m.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["box_rsf4", "box_dip", "box_rough", "box_tp"])
+def test_element_order_does_not_change_results(name, monkeypatch):
+    # elements in bricks (EQD_ELEMENT_ORDER=1) or in reference order: every
+    # node still sums its elements in ascending reference id
+    print(f"case {name}", flush=True)
+    bundle, sc, _ = load_golden(name)
+    nstep = int(sc["nstep"])
+    monkeypatch.setenv("EQD_ELEMENT_ORDER", "0")
+    ref = eager_run(name, "fp64", [nstep])
+    monkeypatch.setenv("EQD_ELEMENT_ORDER", "1")
+    got = eager_run(name, "fp64", [nstep])
+    assert_same(got, ref, f"{name} brick order")
