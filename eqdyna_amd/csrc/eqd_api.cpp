@@ -205,22 +205,120 @@ static bool classify_rows(int64_t n, int width, RowFn row, int64_t max_classes, 
     return true;
 }
 
+// Compact hourglass operators of one element (phi_compact): for each mode
+// the least-squares fit of (1/s_m, h_m/s_m) to the element's eight Phi values
+// given dN/dx, in long double on column-scaled normal equations.  Returns
+// false when the rebuilt values miss the given ones by more than 1e-11 of
+// max(1, |Phi|): Phi that is not of the reference's form keeps its table.
+static bool fit_phi_compact(const double shp[24], const double phi[32], double c[16])
+{
+    for (int m = 0; m < 4; m++) {
+        long double A[8][4], b[8];
+        for (int a = 0; a < 8; a++) {
+            A[a][0] = ((kGammaNeg[m] >> a) & 1u) ? -1.0L : 1.0L;
+            for (int k = 0; k < 3; k++) A[a][1 + k] = -(long double)shp[a * 3 + k];
+            b[a] = phi[m * 8 + a];
+        }
+        long double sc[4];
+        for (int j = 0; j < 4; j++) {
+            long double q = 0;
+            for (int a = 0; a < 8; a++) q += A[a][j] * A[a][j];
+            sc[j] = q > 0 ? 1.0L / std::sqrt(q) : 1.0L;
+            for (int a = 0; a < 8; a++) A[a][j] *= sc[j];
+        }
+        long double N[4][5];
+        for (int i = 0; i < 4; i++) {
+            for (int j = 0; j < 4; j++) {
+                long double q = 0;
+                for (int a = 0; a < 8; a++) q += A[a][i] * A[a][j];
+                N[i][j] = q;
+            }
+            long double q = 0;
+            for (int a = 0; a < 8; a++) q += A[a][i] * b[a];
+            N[i][4] = q;
+        }
+        for (int i = 0; i < 4; i++) {
+            int piv = i;
+            for (int r = i + 1; r < 4; r++)
+                if (std::fabs(N[r][i]) > std::fabs(N[piv][i])) piv = r;
+            if (!(std::fabs(N[piv][i]) > 1e-30L)) return false;
+            if (piv != i)
+                for (int j = 0; j < 5; j++) std::swap(N[i][j], N[piv][j]);
+            for (int r = i + 1; r < 4; r++) {
+                const long double f = N[r][i] / N[i][i];
+                for (int j = i; j < 5; j++) N[r][j] -= f * N[i][j];
+            }
+        }
+        long double x[4];
+        for (int i = 3; i >= 0; i--) {
+            long double q = N[i][4];
+            for (int j = i + 1; j < 4; j++) q -= N[i][j] * x[j];
+            x[i] = q / N[i][i];
+        }
+        c[m] = (double)(x[0] * sc[0]);
+        for (int k = 0; k < 3; k++) c[4 + 3 * m + k] = (double)(x[1 + k] * sc[1 + k]);
+    }
+    double big = 1.0;
+    for (int i = 0; i < 32; i++) big = std::max(big, std::fabs(phi[i]));
+    for (int m = 0; m < 4; m++)
+        for (int a = 0; a < 8; a++) {
+            const double r = phi_compact<double>(c, m, a, shp[a * 3], shp[a * 3 + 1], shp[a * 3 + 2]);
+            if (!(std::fabs(r - phi[m * 8 + a]) <= 1e-11 * big)) return false;
+        }
+    return true;
+}
+
 // The element operators of one class (regular or PML) on the device: either
 // lossless classes (both the operator rows -- kOpW values -- and the
 // hourglass Phi rows -- 32 values -- collapse to at most n/16 distinct rows)
-// or, when they do not, one AoSoA array of kOpB values per element in blocks
-// of 64 elements: value k of element t at ((t / 64) * kOpB + k) * 64 + t % 64.
-template <typename R, typename E, typename OpRow, typename PhiRow>
-static int upload_operators(eqd_handle *h, E &X, int64_t n, bool no_classes, OpRow oprow, PhiRow phirow,
-                            int64_t *n_op_classes, int64_t *n_phi_classes)
+// or, when they do not, one AoSoA array of operator rows per element in
+// blocks of 64 elements: value k of element t at ((t / 64) * W + k) * 64 + t % 64.
+// When every element's Phi has the reference's form (fit_phi_compact), Phi is
+// carried in compact form: per-element rows of W = kOpBc values end with the
+// 16 compact values, and the class tables hold Phi rebuilt from them in the
+// kernels' precision and order (phi_compact), so both layouts compute the same
+// numbers.  Otherwise (or with EQD_FULL_PHI=1) rows of W = kOpB values carry
+// the given Phi.  shp and phi are the class's [24][n] and [32][n] arrays.
+template <typename R, typename E, typename OpRow>
+static int upload_operators(eqd_handle *h, E &X, int64_t n, bool no_classes, OpRow oprow,
+                            const std::vector<double> &shp, const std::vector<double> &phi, int64_t *n_op_classes,
+                            int64_t *n_phi_classes, int32_t *compact)
 {
+    std::vector<R> cmp;
+    int ok = 1;
+    const char *fe = std::getenv("EQD_FULL_PHI");
+    if (!(fe && fe[0] == '1') && n > 0) {
+        cmp.resize((size_t)n * 16);
+#pragma omp parallel for schedule(static) reduction(min : ok)
+        for (int64_t t = 0; t < n; t++) {
+            double sh[24], ph[32], c[16];
+            for (int i = 0; i < 24; i++) sh[i] = shp[i * n + t];
+            for (int i = 0; i < 32; i++) ph[i] = phi[i * n + t];
+            if (!fit_phi_compact(sh, ph, c)) ok = 0;
+            for (int i = 0; i < 16; i++) cmp[(size_t)t * 16 + i] = (R)c[i];
+        }
+    } else {
+        ok = 0;
+    }
+    *compact = ok;
+    auto phirow = [&](int64_t t, R *row) {
+        if (ok) {
+            for (int m = 0; m < 4; m++)
+                for (int a = 0; a < 8; a++)
+                    row[m * 8 + a] = phi_compact<R>(&cmp[(size_t)t * 16], m, a, (R)shp[(a * 3) * n + t],
+                                                    (R)shp[(a * 3 + 1) * n + t], (R)shp[(a * 3 + 2) * n + t]);
+        } else {
+            for (int i = 0; i < 32; i++) row[i] = (R)phi[i * n + t];
+        }
+    };
     std::vector<uint32_t> ocls, pcls;
     std::vector<R> otab, ptab;
     const int64_t maxc = no_classes ? 0 : std::max<int64_t>(1, n / 16);
-    const bool ok = classify_rows<R>(n, kOpW, oprow, maxc, ocls, otab) &&
-                    classify_rows<R>(n, 32, phirow, maxc, pcls, ptab);
+    const bool ok_cls = classify_rows<R>(n, kOpW, oprow, maxc, ocls, otab) &&
+                        classify_rows<R>(n, 32, phirow, maxc, pcls, ptab);
     int rc;
-    if (ok) {
+    X.opb_compact = 0;
+    if (ok_cls) {
         uint32_t *pc;
         R *pt;
         if ((rc = dupload(h, &pc, ocls)) || (rc = dupload(h, &pt, otab))) return rc;
@@ -235,19 +333,25 @@ static int upload_operators(eqd_handle *h, E &X, int64_t n, bool no_classes, OpR
         *n_phi_classes = X.n_phic;
         return 0;
     }
+    const int W = ok ? kOpBc : kOpB;
     const int64_t nb = (n + 63) / 64;
-    std::vector<R> b((size_t)(nb * 64 * kOpB), (R)0);
+    std::vector<R> b((size_t)(nb * 64 * W), (R)0);
 #pragma omp parallel for schedule(static)
     for (int64_t t = 0; t < n; t++) {
         R row[kOpB];
         oprow(t, row);
-        phirow(t, row + kOpW);
-        R *dst = b.data() + (t / 64) * kOpB * 64 + (t % 64);
-        for (int k = 0; k < kOpB; k++) dst[k * 64] = row[k];
+        if (ok) {
+            for (int i = 0; i < 16; i++) row[kOpW + i] = cmp[(size_t)t * 16 + i];
+        } else {
+            phirow(t, row + kOpW);
+        }
+        R *dst = b.data() + (t / 64) * W * 64 + (t % 64);
+        for (int k = 0; k < W; k++) dst[k * 64] = row[k];
     }
     R *pb;
     if ((rc = dupload(h, &pb, b))) return rc;
     X.opb = pb;
+    X.opb_compact = ok;
     X.optab = nullptr;
     X.phitab = nullptr;
     *n_op_classes = 0;
@@ -606,9 +710,7 @@ int eqd_commit(eqd_handle *h)
                     row[kOpVp] = (R)r_vp[t];
                     row[kOpRho] = (R)r_rho[t];
                     for (int i = 0; i < 6; i++) row[kOpSs + i] = (R)r_ss[i * nr + t];
-                }, [&](int64_t t, R *row) {
-                    for (int i = 0; i < 32; i++) row[i] = (R)r_phi[i * nr + t];
-                }, &h->op_classes_reg, &h->phi_classes_reg)))
+                }, r_shp, r_phi, &h->op_classes_reg, &h->phi_classes_reg, &h->phi_compact_reg)))
                 return r2;
             if (s.rdampm != 0.0 || s.C_elastic == 0) {
                 if ((r2 = up(&E.emass, r_em))) return r2;
@@ -700,9 +802,7 @@ int eqd_commit(eqd_handle *h)
                     row[kOpVp] = (R)p_vp[p];
                     row[kOpRho] = (R)p_rho[p];
                     for (int i = 0; i < 6; i++) row[kOpSs + i] = (R)p_ss[i * np + p];
-                }, [&](int64_t p, R *row) {
-                    for (int i = 0; i < 32; i++) row[i] = (R)p_phi[i * np + p];
-                }, &h->op_classes_pml, &h->phi_classes_pml)))
+                }, p_shp, p_phi, &h->op_classes_pml, &h->phi_classes_pml, &h->phi_compact_pml)))
                 return r2;
             if (s.rdampm != 0.0 || s.C_elastic == 0) {
                 if ((r2 = up(&P.emass, p_em))) return r2;
@@ -1265,6 +1365,8 @@ int eqd_layout_info(eqd_handle *h, const char *key, int64_t *value)
     else if (k == "phi_classes_regular") *value = h->phi_classes_reg;
     else if (k == "op_classes_pml") *value = h->op_classes_pml;
     else if (k == "phi_classes_pml") *value = h->phi_classes_pml;
+    else if (k == "phi_compact_regular") *value = h->phi_compact_reg;
+    else if (k == "phi_compact_pml") *value = h->phi_compact_pml;
     else return fail(h, EQD_EINVAL, "unknown layout key %s", key);
     return EQD_OK;
 }

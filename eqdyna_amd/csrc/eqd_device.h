@@ -42,15 +42,42 @@ struct NodeState {
 
 // layout of one operator row: dN_a/dx_k at kOpShp + a*3+k, det, lambda, mu,
 // Vp, rho (viscous hourglass only), SS (6); kOpB-wide per-element rows add
-// Phi_a of mode m at kOpW + m*8+a
+// Phi_a of mode m at kOpW + m*8+a, kOpBc-wide rows the compact hourglass
+// operators instead (phi_compact)
 constexpr int kOpShp = 0, kOpDet = 24, kOpLam = 25, kOpMu = 26, kOpVp = 27, kOpRho = 28, kOpSs = 29, kOpW = 35,
-              kOpB = 67;
+              kOpB = 67, kOpBc = kOpW + 16;
 
-// value k of element t's per-element operator row (AoSoA, blocks of 64 elements)
-template <typename R>
+// value k of element t's per-element operator row of W values (AoSoA, blocks
+// of 64 elements)
+template <typename R, int W = kOpB>
 __host__ __device__ __forceinline__ R opb_at(const R *opb, int64_t t, int k)
 {
-    return opb[((t >> 6) * kOpB + k) * 64 + (t & 63)];
+    return opb[((t >> 6) * W + k) * 64 + (t & 63)];
+}
+
+// Where an element kernel finds its operators: class tables in global memory
+// or per-element rows with Phi; class tables staged in LDS; per-element rows
+// with the compact hourglass operators.
+constexpr int kOpsRows = 0, kOpsLds = 1, kOpsCompact = 2;
+
+// Hourglass base vectors Gamma_m (ha of calcSSPhi4Hrgls,
+// src/assembleGlobalMass.f90:332-335): bit a set where Gamma_ma = -1.
+constexpr unsigned kGammaNeg[4] = {0x3Cu, 0x96u, 0xAAu, 0xA5u};
+
+// The reference's Phi_ma = (Gamma_ma - h_m . dN_a/dx) / s_m
+// (src/assembleGlobalMass.f90:351-372), with
+// h_m = sum_b Gamma_mb x_b and s_m the rms of the numerator over a, so four
+// numbers per mode carry its eight values: c[m] = 1/s_m and c[4 + 3m + k] =
+// h_mk / s_m.  Rebuilt here with no fused operations, in this order, on the
+// host and on the device alike.
+template <typename R>
+__host__ __device__ __forceinline__ R phi_compact(const R *c, int m, int a, R s0, R s1, R s2)
+{
+    const R r = ((kGammaNeg[m] >> a) & 1u) ? -c[m] : c[m];
+    R h = c[4 + 3 * m] * s0;
+    h = h + c[5 + 3 * m] * s1;
+    h = h + c[6 + 3 * m] * s2;
+    return r - h;
 }
 
 template <typename R>
@@ -67,13 +94,15 @@ struct RegElems {
     // Operators: with classes (optab set), elements whose operators are
     // bit-identical share one row of optab (kOpW values, at opc[t]) and of
     // phitab (32 values, at phic[t]); otherwise each element has its own
-    // kOpB-value row in opb (opb_at).
+    // row in opb (opb_at): kOpBc values with the compact hourglass operators
+    // when opb_compact is set, else kOpB values with Phi.
     const uint32_t *opc;
     const R *optab;
     const uint32_t *phic;
     const R *phitab;
     int32_t n_opc, n_phic;  // rows of optab and phitab
     const R *opb;
+    int32_t opb_compact;
 };
 
 template <typename R>
@@ -92,6 +121,7 @@ struct PmlElems {
     const R *phitab;
     int32_t n_opc, n_phic;
     const R *opb;
+    int32_t opb_compact;
 };
 
 struct PmlNodes {

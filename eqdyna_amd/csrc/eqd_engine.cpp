@@ -84,6 +84,14 @@ size_t class_lds(const X &E, size_t rsize)
     return b <= 16384 ? b : 0;
 }
 
+// where an element kind's kernel finds its operators (kOps*)
+template <typename X>
+int ops_of(const X &E, size_t lds)
+{
+    if (lds) return kOpsLds;
+    return E.opb && E.opb_compact ? kOpsCompact : kOpsRows;
+}
+
 // kernel variant of a scenario in a launch kind, and its dynamic LDS bytes
 template <typename R>
 int variant(const eqd_handle *h, int kind, size_t *lds)
@@ -94,11 +102,11 @@ int variant(const eqd_handle *h, int kind, size_t *lds)
     *lds = 0;
     if (kind == kKElem) {
         *lds = class_lds(elems<R>(h), sizeof(R));
-        return q | dp << 1 | hg << 2 | mass << 4 | (*lds ? 1 : 0) << 5;
+        return q | dp << 1 | hg << 2 | mass << 4 | ops_of(elems<R>(h), *lds) << 5;
     }
     if (kind == kKPml) {
         *lds = class_lds(pmls<R>(h), sizeof(R));
-        return hg | mass << 2 | (*lds ? 1 : 0) << 3;
+        return hg | mass << 2 | ops_of(pmls<R>(h), *lds) << 3;
     }
     if (kind == kKFault) return h->learn ? 1 : 0;
     return 0;
@@ -248,11 +256,11 @@ int enqueue_step(eqd_engine *e)
     timed_begin(e, kKElem, &ev, st);
     for (const Group &g : e->groups[kKElem])
         HIPCHK(e, launch_elem<R>(dM, g.g, g.sig & 1, (g.sig >> 1) & 1, (g.sig >> 2) & 3, (g.sig >> 4) & 1,
-                                 (g.sig >> 5) & 1, st));
+                                 (g.sig >> 5) & 3, st));
     timed_end(ev, st);
     timed_begin(e, kKPml, &ev, st);
     for (const Group &g : e->groups[kKPml])
-        HIPCHK(e, launch_pml<R>(dM, g.g, g.sig & 3, (g.sig >> 2) & 1, (g.sig >> 3) & 1, st));
+        HIPCHK(e, launch_pml<R>(dM, g.g, g.sig & 3, (g.sig >> 2) & 1, (g.sig >> 3) & 3, st));
     timed_end(ev, st);
     // fork: the fault pairs on the second stream, beside the node kernels
     // (EQD_SINGLE_STREAM=1 keeps them on the main stream, in step order)

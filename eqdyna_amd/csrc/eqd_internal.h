@@ -15,10 +15,10 @@
 
 namespace eqd {
 template <typename R>
-hipError_t launch_elem(const Member<R> *, const LaunchGrid &, int q, int dp, int hg, int mass, int lds_on,
+hipError_t launch_elem(const Member<R> *, const LaunchGrid &, int q, int dp, int hg, int mass, int ops,
                        hipStream_t);
 template <typename R>
-hipError_t launch_pml(const Member<R> *, const LaunchGrid &, int hg, int mass, int lds_on, hipStream_t);
+hipError_t launch_pml(const Member<R> *, const LaunchGrid &, int hg, int mass, int ops, hipStream_t);
 template <typename R>
 hipError_t launch_node3(const Member<R> *, const LaunchGrid &, hipStream_t);
 template <typename R>
@@ -111,6 +111,7 @@ struct eqd_handle {
     eqd::PmlElems<double> P64{};
     eqd::PmlElems<float> P32{};
     int64_t op_classes_reg = 0, phi_classes_reg = 0, op_classes_pml = 0, phi_classes_pml = 0;
+    int32_t phi_compact_reg = 0, phi_compact_pml = 0;  // hourglass operators in compact form
     eqd::PmlNodes PN{};
     eqd::FaultState F{};
     eqd::OffStations S{};

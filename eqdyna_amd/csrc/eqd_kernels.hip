@@ -66,13 +66,17 @@ __device__ __forceinline__ R sqr(R x) { return x * x; }
 #endif
 
 //
-// kLds: both class tables are staged in LDS (dynamic shared memory of
+// kOps: where the operators come from (kOpsRows, kOpsLds, kOpsCompact).  With
+// kOpsLds both class tables are staged in LDS (dynamic shared memory of
 // (n_opc * kOpW + n_phic * 32) values) at block start, so the operator reads
-// of the element loop are LDS reads instead of vector-memory loads.
-template <typename R, bool kQ, bool kDP, int kHG, bool kMass, bool kLds>
+// of the element loop are LDS reads; with kOpsCompact Phi is rebuilt per node
+// (phi_compact) from 16 values read once per element.
+template <typename R, bool kQ, bool kDP, int kHG, bool kMass, int kOps>
 __global__ void __launch_bounds__(256) elem_regular_kernel(const Member<R> *__restrict__ Ms,
                                                            const uint16_t *__restrict__ bmem, int remap)
 {
+    constexpr bool kLds = kOps == kOpsLds, kCmp = kOps == kOpsCompact;
+    constexpr int kW = kCmp ? kOpBc : kOpB;
     extern __shared__ __attribute__((aligned(16))) unsigned char eqd_smem[];
     typedef __attribute__((address_space(3))) R lds_R;
     const unsigned gb = launch_block(remap);
@@ -101,6 +105,7 @@ __global__ void __launch_bounds__(256) elem_regular_kernel(const Member<R> *__re
     constexpr bool kF32 = sizeof(R) != sizeof(double);
     auto SHP = [&](int i) -> R {
         if constexpr (kLds) return lrow[kOpShp + i];
+        else if constexpr (kCmp) return opb_at<R, kW>(E.opb, t, kOpShp + i);
         else return op ? op[kOpShp + i] : opb_at(E.opb, t, kOpShp + i);
     };
     auto PHI = [&](int m, int a) -> R {
@@ -110,8 +115,16 @@ __global__ void __launch_bounds__(256) elem_regular_kernel(const Member<R> *__re
     // scalar operator k of the element's row
     auto OPK = [&](int k) -> R {
         if constexpr (kLds) return lrow[k];
+        else if constexpr (kCmp) return opb_at<R, kW>(E.opb, t, k);
         else return op ? op[k] : opb_at(E.opb, t, k);
     };
+    // compact hourglass operators (kOpsCompact with the stiffness hourglass)
+    constexpr bool kCHg = kCmp && kHG == 1;
+    R cph[kCHg ? 16 : 1];
+    if constexpr (kCHg) {
+#pragma unroll
+        for (int i = 0; i < 16; i++) cph[i] = opb_at<R, kW>(E.opb, t, kOpW + i);
+    }
     const R rdk = (R)C.rdampk;
     // all eight node ids first (independent loads), then the nodes' v and u
     // one node ahead of the arithmetic, so a gather is always in flight
@@ -150,18 +163,26 @@ __global__ void __launch_bounds__(256) elem_regular_kernel(const Member<R> *__re
     // a node's operator values (dN/dx, and Phi of the four modes), loaded one
     // node ahead like its v and u
     constexpr int kNo = kHG == 1 ? 7 : 3;
+    auto set_phi = [&](int a, R o[kNo]) {
+        if constexpr (kCHg) {
+#pragma unroll
+            for (int m = 0; m < 4; m++) o[3 + m] = phi_compact<R>(cph, m, a, o[0], o[1], o[2]);
+        }
+    };
     auto load_op = [&](int a, R o[kNo]) {
 #pragma unroll
         for (int k = 0; k < 3; k++) o[k] = SHP(a * 3 + k);
-        if constexpr (kHG == 1) {
+        if constexpr (kHG == 1 && !kCmp) {
 #pragma unroll
             for (int m = 0; m < 4; m++) o[3 + m] = PHI(m, a);
         }
     };
-    // (from LDS the values are read where they are used)
+    // (from LDS the values are read where they are used; kept, only dN/dx
+    // when Phi is rebuilt from the compact operators)
     constexpr bool kKeep = !kLds && (EQD_KEEP_OPS == 2 || (EQD_KEEP_OPS == 1 && sizeof(R) == sizeof(float)));
+    constexpr int kNk = kCmp ? 3 : kNo;
     R opc_[kNo], opn_[kNo];
-    R keep[kKeep ? 8 : 1][kNo];
+    R keep[kKeep ? 8 : 1][kNk];
     if constexpr (!kLds) load_op(0, opc_);
 constexpr int kU1 = kKeep ? 8 : EQD_PREFETCH;
 #pragma unroll kU1
@@ -169,8 +190,9 @@ constexpr int kU1 = kKeep ? 8 : EQD_PREFETCH;
         const int b = a % P;
         if constexpr (kKeep) {
 #pragma unroll
-            for (int k = 0; k < kNo; k++) keep[a][k] = opc_[k];
+            for (int k = 0; k < kNk; k++) keep[a][k] = opc_[k];
         }
+        set_phi(a, opc_);
         R vl[3], dl[3];
 #pragma unroll
         for (int k = 0; k < 3; k++) {
@@ -337,10 +359,11 @@ constexpr int kU2 = kKeep ? 8 : EQD_ELEM_UNROLL;
             load_op(a, opc_);
         } else if constexpr (kKeep) {
 #pragma unroll
-            for (int k = 0; k < kNo; k++) opc_[k] = keep[a][k];
+            for (int k = 0; k < kNk; k++) opc_[k] = keep[a][k];
         } else {
             if (a < 7) load_op(a + 1, opn_);
         }
+        set_phi(a, opc_);
         R f[3] = {0, 0, 0};
         if (kMass) {
             const int nd = nds[a];
@@ -390,10 +413,12 @@ constexpr int kU2 = kKeep ? 8 : EQD_ELEM_UNROLL;
 // accumulates the velocity gradients and hourglass mode amplitudes; pass 2
 // forms each node's 12 (or collapsed 3) force slots, hourglass included, and
 // writes every slot once.
-template <typename R, int kHG, bool kMass, bool kLds>
+template <typename R, int kHG, bool kMass, int kOps>
 __global__ void __launch_bounds__(256) elem_pml_kernel(const Member<R> *__restrict__ Ms,
                                                        const uint16_t *__restrict__ bmem, int remap)
 {
+    constexpr bool kLds = kOps == kOpsLds, kCmp = kOps == kOpsCompact;
+    constexpr int kW = kCmp ? kOpBc : kOpB;
     extern __shared__ __attribute__((aligned(16))) unsigned char eqd_smem[];
     typedef __attribute__((address_space(3))) R lds_R;
     const unsigned gb = launch_block(remap);
@@ -422,6 +447,7 @@ __global__ void __launch_bounds__(256) elem_pml_kernel(const Member<R> *__restri
     constexpr bool kF64 = sizeof(R) == sizeof(double);
     auto SHP = [&](int i) -> R {
         if constexpr (kLds) return lrow[kOpShp + i];
+        else if constexpr (kCmp) return opb_at<R, kW>(E.opb, p, kOpShp + i);
         else return op ? op[kOpShp + i] : opb_at(E.opb, p, kOpShp + i);
     };
     auto PHI = [&](int m, int a) -> R {
@@ -430,8 +456,15 @@ __global__ void __launch_bounds__(256) elem_pml_kernel(const Member<R> *__restri
     };
     auto OPK = [&](int k) -> R {
         if constexpr (kLds) return lrow[k];
+        else if constexpr (kCmp) return opb_at<R, kW>(E.opb, p, k);
         else return op ? op[k] : opb_at(E.opb, p, k);
     };
+    constexpr bool kCHg = kCmp && kHG == 1;
+    R cph[kCHg ? 16 : 1];
+    if constexpr (kCHg) {
+#pragma unroll
+        for (int i = 0; i < 16; i++) cph[i] = opb_at<R, kW>(E.opb, p, kOpW + i);
+    }
     // node ids first, then each node's v and u one node ahead of the arithmetic
     int nds[8];
 #pragma unroll
@@ -462,24 +495,32 @@ __global__ void __launch_bounds__(256) elem_pml_kernel(const Member<R> *__restri
     R qv[3][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0, 0}};
     // a node's operator values, one node ahead when they come from memory
     constexpr int kNo = kHG == 1 ? 7 : 3;
+    auto set_phi = [&](int a, R o[kNo]) {
+        if constexpr (kCHg) {
+#pragma unroll
+            for (int m = 0; m < 4; m++) o[3 + m] = phi_compact<R>(cph, m, a, o[0], o[1], o[2]);
+        }
+    };
     auto load_op = [&](int a, R o[kNo]) {
 #pragma unroll
         for (int k = 0; k < 3; k++) o[k] = SHP(a * 3 + k);
-        if constexpr (kHG == 1) {
+        if constexpr (kHG == 1 && !kCmp) {
 #pragma unroll
             for (int m = 0; m < 4; m++) o[3 + m] = PHI(m, a);
         }
     };
     constexpr bool kKeep = !kLds && (EQD_KEEP_OPS_PML == 2 || (EQD_KEEP_OPS_PML == 1 && sizeof(R) == sizeof(float)));
     constexpr bool kStage = !kLds && !kKeep && EQD_STAGE_SHP_PML;
+    constexpr int kNk = kCmp ? 3 : kNo;
     lds_R *stage = (lds_R *)eqd_smem;
     const unsigned bd = blockDim.x, tx = threadIdx.x;
     // the force pass's operator values: dN/dx from the stage, Phi from memory
+    // (or rebuilt from the compact operators)
     auto load_op2 = [&](int a, R o[kNo]) {
         if constexpr (kStage) {
 #pragma unroll
             for (int k = 0; k < 3; k++) o[k] = stage[(a * 3 + k) * bd + tx];
-            if constexpr (kHG == 1) {
+            if constexpr (kHG == 1 && !kCmp) {
 #pragma unroll
                 for (int m = 0; m < 4; m++) o[3 + m] = PHI(m, a);
             }
@@ -488,7 +529,7 @@ __global__ void __launch_bounds__(256) elem_pml_kernel(const Member<R> *__restri
         }
     };
     R opc_[kNo], opn_[kNo];
-    R keep[kKeep ? 8 : 1][kNo];
+    R keep[kKeep ? 8 : 1][kNk];
     if constexpr (!kLds) load_op(0, opc_);
 constexpr int kU1 = kKeep ? 8 : EQD_PREFETCH;
 #pragma unroll kU1
@@ -499,8 +540,9 @@ constexpr int kU1 = kKeep ? 8 : EQD_PREFETCH;
         } else {
             if constexpr (kKeep) {
 #pragma unroll
-                for (int k = 0; k < kNo; k++) keep[a][k] = opc_[k];
+                for (int k = 0; k < kNk; k++) keep[a][k] = opc_[k];
             }
+            set_phi(a, opc_);
             if constexpr (kStage) {
 #pragma unroll
                 for (int k = 0; k < 3; k++) stage[(a * 3 + k) * bd + tx] = opc_[k];
@@ -628,10 +670,11 @@ constexpr int kU2 = kKeep ? 8 : EQD_ELEM_UNROLL;
             load_op(a, opc_);
         } else if constexpr (kKeep) {
 #pragma unroll
-            for (int k = 0; k < kNo; k++) opc_[k] = keep[a][k];
+            for (int k = 0; k < kNk; k++) opc_[k] = keep[a][k];
         } else {
             if (a < 7) load_op2(a + 1, opn_);
         }
+        set_phi(a, opc_);
         // hourglass force on this node, subtracted mode by mode
         R hg[3] = {0, 0, 0};
         if (kHG == 1) {
@@ -1464,19 +1507,22 @@ __global__ void __launch_bounds__(64) clock_kernel(const Member<R> *__restrict__
 static inline unsigned grid_for(int64_t n, int block) { return (unsigned)((n + block - 1) / block); }
 
 template <typename R>
-hipError_t launch_elem(const Member<R> *M, const LaunchGrid &g, int q, int dp, int hg, int mass, int lds_on,
+hipError_t launch_elem(const Member<R> *M, const LaunchGrid &g, int q, int dp, int hg, int mass, int ops,
                        hipStream_t s)
 {
     if (g.grid == 0) return hipSuccess;
     if (g.block < 64 || g.block > 256 || g.block % 64) return hipErrorInvalidValue;
+    if (ops < kOpsRows || ops > kOpsCompact) return hipErrorInvalidValue;
     // class tables in LDS (at most 16 KiB) or operators from memory
-    const size_t lds = lds_on ? g.lds : 0;
-#define EQD_LAUNCH(Q, DP, HG, MASS)                                                                        \
-    do {                                                                                                   \
-        if (lds_on)                                                                                        \
-            elem_regular_kernel<R, Q, DP, HG, MASS, true><<<g.grid, g.block, lds, s>>>(M, g.bmem, g.remap); \
-        else                                                                                               \
-            elem_regular_kernel<R, Q, DP, HG, MASS, false><<<g.grid, g.block, 0, s>>>(M, g.bmem, g.remap); \
+    const size_t lds = ops == kOpsLds ? g.lds : 0;
+#define EQD_LAUNCH(Q, DP, HG, MASS)                                                                               \
+    do {                                                                                                          \
+        if (ops == kOpsLds)                                                                                       \
+            elem_regular_kernel<R, Q, DP, HG, MASS, kOpsLds><<<g.grid, g.block, lds, s>>>(M, g.bmem, g.remap);    \
+        else if (ops == kOpsCompact)                                                                              \
+            elem_regular_kernel<R, Q, DP, HG, MASS, kOpsCompact><<<g.grid, g.block, 0, s>>>(M, g.bmem, g.remap);  \
+        else                                                                                                      \
+            elem_regular_kernel<R, Q, DP, HG, MASS, kOpsRows><<<g.grid, g.block, 0, s>>>(M, g.bmem, g.remap);     \
     } while (0)
 #define EQD_HG(Q, DP, MASS)                         \
     do {                                            \
@@ -1495,18 +1541,21 @@ hipError_t launch_elem(const Member<R> *M, const LaunchGrid &g, int q, int dp, i
 }
 
 template <typename R>
-hipError_t launch_pml(const Member<R> *M, const LaunchGrid &g, int hg, int mass, int lds_on, hipStream_t s)
+hipError_t launch_pml(const Member<R> *M, const LaunchGrid &g, int hg, int mass, int ops, hipStream_t s)
 {
     if (g.grid == 0) return hipSuccess;
     if (g.block < 64 || g.block > 256 || g.block % 64) return hipErrorInvalidValue;
+    if (ops < kOpsRows || ops > kOpsCompact) return hipErrorInvalidValue;
     // class tables, or each thread's staged dN/dx (24 values)
-    const size_t lds = lds_on ? g.lds : (EQD_STAGE_SHP_PML ? (size_t)g.block * 24 * sizeof(R) : 0);
-#define EQD_LAUNCH(HG, MASS)                                                                          \
-    do {                                                                                              \
-        if (lds_on)                                                                                   \
-            elem_pml_kernel<R, HG, MASS, true><<<g.grid, g.block, lds, s>>>(M, g.bmem, g.remap);     \
-        else                                                                                          \
-            elem_pml_kernel<R, HG, MASS, false><<<g.grid, g.block, lds, s>>>(M, g.bmem, g.remap);    \
+    const size_t lds = ops == kOpsLds ? g.lds : (EQD_STAGE_SHP_PML ? (size_t)g.block * 24 * sizeof(R) : 0);
+#define EQD_LAUNCH(HG, MASS)                                                                              \
+    do {                                                                                                  \
+        if (ops == kOpsLds)                                                                               \
+            elem_pml_kernel<R, HG, MASS, kOpsLds><<<g.grid, g.block, lds, s>>>(M, g.bmem, g.remap);       \
+        else if (ops == kOpsCompact)                                                                      \
+            elem_pml_kernel<R, HG, MASS, kOpsCompact><<<g.grid, g.block, lds, s>>>(M, g.bmem, g.remap);   \
+        else                                                                                              \
+            elem_pml_kernel<R, HG, MASS, kOpsRows><<<g.grid, g.block, lds, s>>>(M, g.bmem, g.remap);      \
     } while (0)
     if (mass) {
         if (hg == 1) EQD_LAUNCH(1, true);

@@ -141,7 +141,11 @@ int64_t eqd_device_bytes(eqd_handle *h);
  * "pml_elements", and the number of operator classes (elements whose stored
  * operators are bit-identical share one table row; 0 = per-element arrays):
  * "op_classes_regular", "phi_classes_regular", "op_classes_pml",
- * "phi_classes_pml".  Returns EQD_EINVAL for an unknown key. */
+ * "phi_classes_pml"; "phi_compact_regular", "phi_compact_pml": 1 when the
+ * hourglass Phi is carried as four numbers per mode (it has the form of
+ * src/assembleGlobalMass.f90:351-372 for every element of the class; then
+ * Phi agrees with the given one to 1e-11), 0 when the given Phi is kept.
+ * Returns EQD_EINVAL for an unknown key. */
 int eqd_layout_info(eqd_handle *h, const char *key, int64_t *value);
 
 const char *eqd_last_error(eqd_handle *h);

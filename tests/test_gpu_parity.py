@@ -140,3 +140,31 @@ def test_operator_classes_are_lossless(precision, monkeypatch):
     assert info["op_classes_regular"] > 0 and info2["op_classes_regular"] == 0
     for k in o1:
         assert np.array_equal(o1[k], o2[k]), k
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["box_layered", "box_q", "box_dip", "box_plastic"])
+def test_compact_phi_matches_given_phi(name, monkeypatch):
+    # Phi carried as four numbers per mode and rebuilt in the kernels, against
+    # the given Phi table (EQD_FULL_PHI=1), both as per-element rows: the two
+    # differ by the rounding of the rebuild only
+    print(f"case {name}", flush=True)
+    bundle, sc, _ = load_golden(name)
+    n = int(sc["nstep"])
+    monkeypatch.setenv("EQD_NO_OPERATOR_CLASSES", "1")
+    runs = {}
+    for full in ("0", "1"):
+        monkeypatch.setenv("EQD_FULL_PHI", full)
+        s = Solver(bundle, sc, "fp64")
+        info = s.layout_info()
+        assert info["phi_compact_regular"] == (full == "0")
+        if info["pml_elements"]:
+            assert info["phi_compact_pml"] == (full == "0")
+        s.run(n)
+        runs[full] = s.outputs()
+        s.close()
+    skip = undefined_slots(sc)
+    err = channel_errors(channels(runs["0"], skip), channels(runs["1"], skip))
+    worst = max(err.values())
+    print(f"[{name}] compact vs given Phi: worst {worst:.3e}", flush=True)
+    assert worst <= 1e-11, err
