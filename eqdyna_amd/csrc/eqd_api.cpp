@@ -719,7 +719,7 @@ int eqd_commit(eqd_handle *h)
             R *p;
             if ((r2 = dupload(h, &p, stc))) return r2;
             E.stress = p;
-            if ((r2 = dalloc(h, &p, 24 * nr))) return r2;
+            if ((r2 = dalloc(h, &p, (nr + 63) / 64 * 64 * kEfReg))) return r2;
             E.ef = p;
             E.porep = d_porep;
             E.pstrain = d_pstrain;
@@ -811,7 +811,7 @@ int eqd_commit(eqd_handle *h)
             R *q;
             if ((r2 = dupload(h, &q, stc))) return r2;
             P.stress = q;
-            if ((r2 = dalloc(h, &q, 96 * np))) return r2;
+            if ((r2 = dalloc(h, &q, (np + 63) / 64 * 64 * kEfPml))) return r2;
             P.ef = q;
             return 0;
         };

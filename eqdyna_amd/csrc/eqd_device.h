@@ -55,6 +55,16 @@ __host__ __device__ __forceinline__ R opb_at(const R *opb, int64_t t, int k)
     return opb[((t >> 6) * W + k) * 64 + (t & 63)];
 }
 
+// Slot s of element t's force rows, W slots per element: AoSoA in blocks of
+// 64 elements, so a wavefront's stores of one element row are one contiguous
+// region of W * 64 values.
+template <int W>
+__host__ __device__ __forceinline__ int64_t ef_at(int64_t t, int s)
+{
+    return ((t >> 6) * W + s) * 64 + (t & 63);
+}
+constexpr int kEfReg = 24, kEfPml = 96;
+
 // Where an element kernel finds its operators: class tables in global memory
 // or per-element rows with Phi; class tables staged in LDS; per-element rows
 // with the compact hourglass operators.
@@ -90,7 +100,7 @@ struct RegElems {
     double *pstrain;      // [n]      (plastic only)
     const uint8_t *qidx;  // [n]      (Q only) row of qtab
     const double *qtab;   // [16][6]  exp(-dt/tau), 1-exp(-dt/tau), 8*w_p, 8*w_s, c_p, c_s
-    R *ef;                // [24][n]  force on local node a, component c at (a*3+c)
+    R *ef;                // force on local node a, component c at slot a*3+c (ef_at<kEfReg>)
     // Operators: with classes (optab set), elements whose operators are
     // bit-identical share one row of optab (kOpW values, at opc[t]) and of
     // phitab (32 values, at phic[t]); otherwise each element has its own
@@ -113,7 +123,7 @@ struct PmlElems {
     const R *emass;
     const R *damp;         // [3][n] element-centre damping d_x, d_y, d_z (always per element)
     const uint8_t *mask12; // [n] bit a: local node a carries 12 components
-    R *ef;                 // [96][n] slot (a*12+k); 3-component nodes use k=0..2
+    R *ef;                 // slot a*12+k (ef_at<kEfPml>); 3-component nodes use k=0..2
     // operators as in RegElems
     const uint32_t *opc;
     const R *optab;

@@ -398,7 +398,7 @@ constexpr int kU2 = kKeep ? 8 : EQD_ELEM_UNROLL;
             }
         }
 #pragma unroll
-        for (int k = 0; k < 3; k++) E.ef[(a * 3 + k) * n + t] = f[k];
+        for (int k = 0; k < 3; k++) E.ef[ef_at<kEfReg>(t, a * 3 + k)] = f[k];
         if constexpr (!kLds && !kKeep) {
 #pragma unroll
             for (int k = 0; k < kNo; k++) opc_[k] = opn_[k];
@@ -724,14 +724,14 @@ constexpr int kU2 = kKeep ? 8 : EQD_ELEM_UNROLL;
         if (mask & (1u << a)) {
             // a PML node takes the 9 split components and, in 10-12, the rest
 #pragma unroll
-            for (int k = 0; k < 9; k++) E.ef[(a * 12 + k) * n + p] = f[k];
+            for (int k = 0; k < 9; k++) E.ef[ef_at<kEfPml>(p, a * 12 + k)] = f[k];
 #pragma unroll
-            for (int k = 0; k < 3; k++) E.ef[(a * 12 + 9 + k) * n + p] = f[9 + k] + hg[k];
+            for (int k = 0; k < 3; k++) E.ef[ef_at<kEfPml>(p, a * 12 + 9 + k)] = f[9 + k] + hg[k];
         } else {
             // a 3-component node takes the collapsed sums (src/assembleGlobalKU.f90:55-61)
-            E.ef[(a * 12 + 0) * n + p] = f[0] + f[1] + f[2] + f[9] + hg[0];
-            E.ef[(a * 12 + 1) * n + p] = f[3] + f[4] + f[5] + f[10] + hg[1];
-            E.ef[(a * 12 + 2) * n + p] = f[6] + f[7] + f[8] + f[11] + hg[2];
+            E.ef[ef_at<kEfPml>(p, a * 12 + 0)] = f[0] + f[1] + f[2] + f[9] + hg[0];
+            E.ef[ef_at<kEfPml>(p, a * 12 + 1)] = f[3] + f[4] + f[5] + f[10] + hg[1];
+            E.ef[ef_at<kEfPml>(p, a * 12 + 2)] = f[6] + f[7] + f[8] + f[11] + hg[2];
         }
         if constexpr (!kLds && !kKeep) {
 #pragma unroll
@@ -744,8 +744,7 @@ constexpr int kU2 = kKeep ? 8 : EQD_ELEM_UNROLL;
 // Gather of one 3-component node: sum of its elements' forces in ascending
 // element order (assembleGlobalKU's scatter-add, src/assembleGlobalKU.f90:28-63).
 template <typename R>
-__device__ __forceinline__ void gather3(const NodeState &N, int node, const R *ef, int64_t nreg,
-                                        const R *efp, int64_t npe, double F[3])
+__device__ __forceinline__ void gather3(const NodeState &N, int node, const R *ef, const R *efp, double F[3])
 {
     // entries in batches of eight: all entry loads, then all force loads,
     // then the sums in entry order (so at most two memory latencies per batch)
@@ -762,11 +761,10 @@ __device__ __forceinline__ void gather3(const NodeState &N, int node, const R *e
                 const int64_t idx = e[k] & kEntIdxMask;
                 const int a = (e[k] >> kEntLocalShift) & 7;
                 const bool pml = (e[k] & kEntPml) != 0;
-                const R *b = pml ? efp + (int64_t)(a * 12) * npe + idx : ef + (int64_t)(a * 3) * nreg + idx;
-                const int64_t st = pml ? npe : nreg;
+                const R *b = pml ? efp + ef_at<kEfPml>(idx, a * 12) : ef + ef_at<kEfReg>(idx, a * 3);
                 f[k][0] = b[0];
-                f[k][1] = b[st];
-                f[k][2] = b[2 * st];
+                f[k][1] = b[64];
+                f[k][2] = b[128];
             }
         }
 #pragma unroll
@@ -803,14 +801,13 @@ __global__ void __launch_bounds__(256) node3_kernel(const Member<R> *__restrict_
     if (i >= M.n_node3_free) return;
     const NodeState N = M.N;
     const R *ef = M.E.ef, *efp = M.P.ef;
-    const int64_t nreg = M.E.n, npe = M.P.n;
     const double dt = M.C.dt;
     const Clock ck = *M.clk;
     const int mode = ck.mode, next_step = ck.nt + 1;
     const int node = M.node3_free[i];
     const int64_t nn = N.nn;
     double F[3];
-    gather3<R>(N, node, ef, nreg, efp, npe, F);
+    gather3<R>(N, node, ef, efp, F);
     const double m = N.mass[node];
     if (mode == 1) {
 #pragma unroll
@@ -843,7 +840,6 @@ __global__ void __launch_bounds__(256) node_pml_kernel(const Member<R> *__restri
     const NodeState N = M.N;
     const PmlNodes P = M.PN;
     const R *efp = M.P.ef;
-    const int64_t npe = M.P.n;
     const double dt = M.C.dt;
     const Clock ck = *M.clk;
     const int mode = ck.mode, next_step = ck.nt + 1;
@@ -865,7 +861,7 @@ __global__ void __launch_bounds__(256) node_pml_kernel(const Member<R> *__restri
                 const int64_t idx = e[k] & kEntIdxMask;
                 const int a = (e[k] >> kEntLocalShift) & 7;
 #pragma unroll
-                for (int c = 0; c < 12; c++) f[k][c] = efp[(a * 12 + c) * npe + idx];
+                for (int c = 0; c < 12; c++) f[k][c] = efp[ef_at<kEfPml>(idx, a * 12 + c)];
             }
         }
 #pragma unroll
@@ -1041,7 +1037,6 @@ __global__ void __launch_bounds__(128) fault_kernel(const Member<R> *__restrict_
     const NodeState N = M.N;
     const StepConsts C = M.C;
     const R *ef = M.E.ef, *efp = M.P.ef;
-    const int64_t nreg = M.E.n, npe = M.P.n;
     const Clock ck = *M.clk;
     const int nt = ck.nt, mode = ck.mode;
     const double time = ck.time;
@@ -1062,8 +1057,8 @@ __global__ void __launch_bounds__(128) fault_kernel(const Member<R> *__restrict_
     auto UD = [&](int j) { return Fs.ud[j * nf + i]; };
     const double arn = Fs.arn[i];
     double Fsl[3], Fma[3];
-    gather3<R>(N, ns, ef, nreg, efp, npe, Fsl);
-    gather3<R>(N, nm, ef, nreg, efp, npe, Fma);
+    gather3<R>(N, ns, ef, efp, Fsl);
+    gather3<R>(N, nm, ef, efp, Fma);
     double nsd[3][2][3];
     {
     const double un[3] = {UN(0), UN(1), UN(2)}, us[3] = {US(0), US(1), US(2)}, ud[3] = {UD(0), UD(1), UD(2)};
