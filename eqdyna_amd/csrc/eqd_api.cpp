@@ -90,6 +90,19 @@ static int dupload(eqd_handle *h, T **p, const std::vector<T> &v)
     return 0;
 }
 
+// Component-major [w][n] values to the kernels' AoSoA blocks (blk_at<w>),
+// zero padded to whole blocks of 64 items.
+template <typename T, typename S>
+static std::vector<T> to_blocks(const std::vector<S> &cm, int64_t n, int w)
+{
+    std::vector<T> b((size_t)((n + 63) / 64 * 64 * w), T(0));
+    for (int64_t t = 0; t < n; t++)
+        for (int k = 0; k < w; k++) b[(size_t)(((t >> 6) * w + k) * 64 + (t & 63))] = (T)cm[(size_t)k * n + t];
+    return b;
+}
+static inline int64_t blocked_count(int64_t n, int w) { return (n + 63) / 64 * 64 * w; }
+static inline int64_t blocked_index(int64_t t, int k, int w) { return ((t >> 6) * w + k) * 64 + (t & 63); }
+
 // comdampv, src/comdampv.f90:1-120 (region 14 compares y with xmax0 as the
 // reference does).  Returns d_x, d_y, d_z.
 static void pml_node_damping(const eqd_scalars &s, double x2, double y, double z, double d[3])
@@ -681,7 +694,7 @@ int eqd_commit(eqd_handle *h)
         }
         int rc;
         int32_t *d_ien;
-        if ((rc = dupload(h, &d_ien, r_ien))) return rc;
+        if ((rc = dupload(h, &d_ien, to_blocks<int32_t>(r_ien, nr, 8)))) return rc;
         uint8_t *d_q = nullptr;
         double *d_qtab = nullptr, *d_porep, *d_pstrain;
         if ((rc = dupload(h, &d_q, r_q))) return rc;
@@ -690,10 +703,9 @@ int eqd_commit(eqd_handle *h)
         if ((rc = dupload(h, &d_pstrain, r_pstrain))) return rc;
         auto fill = [&](auto &E) -> int {
             using R = std::remove_pointer_t<decltype(E.ef)>;
-            auto up = [&](const R **dst, const std::vector<double> &src) -> int {
-                std::vector<R> c(src.begin(), src.end());
+            auto up = [&](const R **dst, const std::vector<double> &src, int w) -> int {
                 R *p;
-                int r2 = dupload(h, &p, c);
+                int r2 = dupload(h, &p, to_blocks<R>(src, nr, w));
                 *dst = p;
                 return r2;
             };
@@ -713,11 +725,10 @@ int eqd_commit(eqd_handle *h)
                 }, r_shp, r_phi, &h->op_classes_reg, &h->phi_classes_reg, &h->phi_compact_reg)))
                 return r2;
             if (s.rdampm != 0.0 || s.C_elastic == 0) {
-                if ((r2 = up(&E.emass, r_em))) return r2;
+                if ((r2 = up(&E.emass, r_em, 24))) return r2;
             }
-            std::vector<R> stc(r_st.begin(), r_st.end());
             R *p;
-            if ((r2 = dupload(h, &p, stc))) return r2;
+            if ((r2 = dupload(h, &p, to_blocks<R>(r_st, nr, nst)))) return r2;
             E.stress = p;
             if ((r2 = dalloc(h, &p, (nr + 63) / 64 * 64 * kEfReg))) return r2;
             E.ef = p;
@@ -772,14 +783,13 @@ int eqd_commit(eqd_handle *h)
         int rc;
         int32_t *pi;
         uint8_t *pm;
-        if ((rc = dupload(h, &pi, p_ien))) return rc;
+        if ((rc = dupload(h, &pi, to_blocks<int32_t>(p_ien, np, 8)))) return rc;
         if ((rc = dupload(h, &pm, p_mask))) return rc;
         auto fillp = [&](auto &P) -> int {
             using R = std::remove_pointer_t<decltype(P.ef)>;
-            auto up = [&](const R **dst, const std::vector<double> &src) -> int {
-                std::vector<R> c(src.begin(), src.end());
+            auto up = [&](const R **dst, const std::vector<double> &src, int w) -> int {
                 R *q;
-                int r2 = dupload(h, &q, c);
+                int r2 = dupload(h, &q, to_blocks<R>(src, np, w));
                 *dst = q;
                 return r2;
             };
@@ -793,7 +803,7 @@ int eqd_commit(eqd_handle *h)
             // the damping varies element by element across the layer: it stays a
             // per-element array, so the operator rows (as for regular elements)
             // collapse to a few classes
-            if ((r2 = up(&P.damp, p_damp))) return r2;
+            if ((r2 = up(&P.damp, p_damp, 3))) return r2;
             if ((r2 = upload_operators<R>(h, P, np, no_classes, [&](int64_t p, R *row) {
                     for (int i = 0; i < 24; i++) row[kOpShp + i] = (R)p_shp[i * np + p];
                     row[kOpDet] = (R)p_det[p];
@@ -805,11 +815,10 @@ int eqd_commit(eqd_handle *h)
                 }, p_shp, p_phi, &h->op_classes_pml, &h->phi_classes_pml, &h->phi_compact_pml)))
                 return r2;
             if (s.rdampm != 0.0 || s.C_elastic == 0) {
-                if ((r2 = up(&P.emass, p_em))) return r2;
+                if ((r2 = up(&P.emass, p_em, 24))) return r2;
             }
-            std::vector<R> stc(p_st.begin(), p_st.end());
             R *q;
-            if ((r2 = dupload(h, &q, stc))) return r2;
+            if ((r2 = dupload(h, &q, to_blocks<R>(p_st, np, 21)))) return r2;
             P.stress = q;
             if ((r2 = dalloc(h, &q, (np + 63) / 64 * 64 * kEfPml))) return r2;
             P.ef = q;
@@ -871,9 +880,9 @@ int eqd_commit(eqd_handle *h)
         PN.n = npn;
         int32_t *pn;
         if ((rc = dupload(h, &pn, h->pml_node_ids))) return rc; PN.node = pn;
-        if ((rc = dupload(h, &d, pdamp))) return rc; PN.damp = d;
-        if ((rc = dupload(h, &d, pv12))) return rc; PN.v12 = d;
-        if ((rc = dupload(h, &d, pacc))) return rc; PN.acc = d;
+        if ((rc = dupload(h, &d, to_blocks<double>(pdamp, npn, 3)))) return rc; PN.damp = d;
+        if ((rc = dupload(h, &d, to_blocks<double>(pv12, npn, 12)))) return rc; PN.v12 = d;
+        if ((rc = dupload(h, &d, to_blocks<double>(pacc, npn, 12)))) return rc; PN.acc = d;
     }
 
     // ---------------- fault
@@ -1049,7 +1058,7 @@ int eqd_get(eqd_handle *h, int32_t field, void *host, int64_t count)
         const bool isv = field == EQD_V1;
         std::vector<double> v, p;
         if ((rc = d2h(h, v, isv ? h->N.vel : h->N.acc, 3 * nn))) return rc;
-        if ((rc = d2h(h, p, isv ? h->PN.v12 : h->PN.acc, 12 * h->PN.n))) return rc;
+        if ((rc = d2h(h, p, isv ? h->PN.v12 : h->PN.acc, blocked_count(h->PN.n, 12)))) return rc;
         const auto &orig = h->fd[field];  // values for equations the device does not own (fixed PML nodes)
         for (int64_t i = 0; i < s.neq; i++) out[i] = orig.empty() ? 0.0 : orig[i];
         const int32_t *eqi = I32(h, EQD_EQNUMINDEXARR), *esl = I32(h, EQD_EQNUMSTARTINDEXLOC);
@@ -1058,7 +1067,7 @@ int eqd_get(eqd_handle *h, int32_t field, void *host, int64_t count)
         const int64_t np = h->PN.n;
         for (int64_t q = 0; q < np; q++) {
             const int32_t n = h->pml_node_ids[q];
-            for (int k = 0; k < 12; k++) out[eqi[esl[n] + k] - 1] = p[k * np + q];
+            for (int k = 0; k < 12; k++) out[eqi[esl[n] + k] - 1] = p[blocked_index(q, k, 12)];
         }
         return EQD_OK;
     }
@@ -1090,28 +1099,28 @@ int eqd_get(eqd_handle *h, int32_t field, void *host, int64_t count)
         std::vector<double> v;
         if (s.precision == EQD_PREC_FP32) {
             std::vector<float> f;
-            if ((rc = d2h(h, f, h->E32.stress, (int64_t)h->nreg_stress * nr))) return rc;
+            if ((rc = d2h(h, f, h->E32.stress, blocked_count(nr, h->nreg_stress)))) return rc;
             v.assign(f.begin(), f.end());
-        } else if ((rc = d2h(h, v, h->E64.stress, (int64_t)h->nreg_stress * nr))) {
+        } else if ((rc = d2h(h, v, h->E64.stress, blocked_count(nr, h->nreg_stress)))) {
             return rc;
         }
         for (int64_t t = 0; t < nr; t++) {
             const int64_t e = h->reg_ids[t];
             for (int i = 0; i < h->nreg_stress; i++) {
                 const int64_t idx = (int64_t)sci[e] + i;
-                if (idx < s.size_stress) out[idx] = v[i * nr + t];
+                if (idx < s.size_stress) out[idx] = v[blocked_index(t, i, h->nreg_stress)];
             }
         }
         if (s.precision == EQD_PREC_FP32) {
             std::vector<float> f;
-            if ((rc = d2h(h, f, h->P32.stress, 21 * np))) return rc;
+            if ((rc = d2h(h, f, h->P32.stress, blocked_count(np, 21)))) return rc;
             v.assign(f.begin(), f.end());
-        } else if ((rc = d2h(h, v, h->P64.stress, 21 * np))) {
+        } else if ((rc = d2h(h, v, h->P64.stress, blocked_count(np, 21)))) {
             return rc;
         }
         for (int64_t p = 0; p < np; p++) {
             const int64_t e = h->pml_ids[p];
-            for (int i = 0; i < 21; i++) out[(int64_t)sci[e] + i] = v[i * np + p];
+            for (int i = 0; i < 21; i++) out[(int64_t)sci[e] + i] = v[blocked_index(p, i, 21)];
         }
         return EQD_OK;
     }

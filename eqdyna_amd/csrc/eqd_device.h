@@ -55,11 +55,12 @@ __host__ __device__ __forceinline__ R opb_at(const R *opb, int64_t t, int k)
     return opb[((t >> 6) * W + k) * 64 + (t & 63)];
 }
 
-// Slot s of element t's force rows, W slots per element: AoSoA in blocks of
-// 64 elements, so a wavefront's stores of one element row are one contiguous
-// region of W * 64 values.
+// Value s of item t (element or PML node) of a per-item array of W values:
+// AoSoA in blocks of 64 items, so a wavefront's accesses of one item row are
+// one contiguous region of W * 64 values instead of W streams far apart.
+// Arrays in this layout hold (n + 63) / 64 * 64 * W values.
 template <int W>
-__host__ __device__ __forceinline__ int64_t ef_at(int64_t t, int s)
+__host__ __device__ __forceinline__ int64_t blk_at(int64_t t, int s)
 {
     return ((t >> 6) * W + s) * 64 + (t & 63);
 }
@@ -93,14 +94,14 @@ __host__ __device__ __forceinline__ R phi_compact(const R *c, int m, int a, R s0
 template <typename R>
 struct RegElems {
     int64_t n;
-    const int32_t *ien;   // [8][n]   0-based node ids
-    R *stress;            // [6 or 12][n]
-    const R *emass;       // [24][n]  (inertia/gravity term only)
+    const int32_t *ien;   // 8 per element (blk_at<8>), 0-based node ids
+    R *stress;            // 6, or 12 with Q, per element (blk_at)
+    const R *emass;       // 24 per element (blk_at<24>; inertia/gravity term only)
     const double *porep;  // [n]      (plastic only)
     double *pstrain;      // [n]      (plastic only)
     const uint8_t *qidx;  // [n]      (Q only) row of qtab
     const double *qtab;   // [16][6]  exp(-dt/tau), 1-exp(-dt/tau), 8*w_p, 8*w_s, c_p, c_s
-    R *ef;                // force on local node a, component c at slot a*3+c (ef_at<kEfReg>)
+    R *ef;                // force on local node a, component c at slot a*3+c (blk_at<kEfReg>)
     // Operators: with classes (optab set), elements whose operators are
     // bit-identical share one row of optab (kOpW values, at opc[t]) and of
     // phitab (32 values, at phic[t]); otherwise each element has its own
@@ -118,12 +119,12 @@ struct RegElems {
 template <typename R>
 struct PmlElems {
     int64_t n;
-    const int32_t *ien;    // [8][n]
-    R *stress;             // [21][n]: 15 split + 6 initial
-    const R *emass;
-    const R *damp;         // [3][n] element-centre damping d_x, d_y, d_z (always per element)
+    const int32_t *ien;    // 8 per element (blk_at<8>)
+    R *stress;             // 21 per element (blk_at<21>): 15 split + 6 initial
+    const R *emass;        // 24 per element (blk_at<24>)
+    const R *damp;         // 3 per element (blk_at<3>): element-centre damping d_x, d_y, d_z
     const uint8_t *mask12; // [n] bit a: local node a carries 12 components
-    R *ef;                 // slot a*12+k (ef_at<kEfPml>); 3-component nodes use k=0..2
+    R *ef;                 // slot a*12+k (blk_at<kEfPml>); 3-component nodes use k=0..2
     // operators as in RegElems
     const uint32_t *opc;
     const R *optab;
@@ -137,9 +138,9 @@ struct PmlElems {
 struct PmlNodes {
     int64_t n;
     const int32_t *node;  // [n]
-    const double *damp;   // [3][n] d_x, d_y, d_z at the node
-    double *v12;          // [12][n] split velocities (v1 of the 12 equations)
-    double *acc;          // [12][n] pending F/m
+    const double *damp;   // 3 per node (blk_at<3>): d_x, d_y, d_z at the node
+    double *v12;          // 12 per node (blk_at<12>): split velocities (v1 of the 12 equations)
+    double *acc;          // 12 per node (blk_at<12>): pending F/m
 };
 
 struct FaultState {

@@ -390,11 +390,13 @@ std::vector<Region> mutable_regions(const eqd_engine *e)
         add(h->N.disp, 3 * nn * sizeof(double));
         add(h->N.acc, 3 * nn * sizeof(double));
         add(h->N.nanflag, 2 * sizeof(int32_t));
-        add(e->fp32 ? (const void *)h->E32.stress : (const void *)h->E64.stress, (size_t)h->nreg_stress * h->n_reg * rs);
+        // element and PML-node arrays are in blocks of 64 items (blk_at)
+        auto blocked = [](int64_t n, int w) { return (size_t)((n + 63) / 64 * 64 * w); };
+        add(e->fp32 ? (const void *)h->E32.stress : (const void *)h->E64.stress, blocked(h->n_reg, h->nreg_stress) * rs);
         add(e->fp32 ? (const void *)h->E32.pstrain : (const void *)h->E64.pstrain, (size_t)h->n_reg * sizeof(double));
-        add(e->fp32 ? (const void *)h->P32.stress : (const void *)h->P64.stress, (size_t)21 * h->n_pml * rs);
-        add(h->PN.v12, 12 * (size_t)h->PN.n * sizeof(double));
-        add(h->PN.acc, 12 * (size_t)h->PN.n * sizeof(double));
+        add(e->fp32 ? (const void *)h->P32.stress : (const void *)h->P64.stress, blocked(h->n_pml, 21) * rs);
+        add(h->PN.v12, blocked(h->PN.n, 12) * sizeof(double));
+        add(h->PN.acc, blocked(h->PN.n, 12) * sizeof(double));
         if (h->F.nf > 0) {
             add(h->F.fric, 100 * (size_t)h->F.ld * sizeof(double));
             add(h->F.fnft, (size_t)h->F.nf * sizeof(double));

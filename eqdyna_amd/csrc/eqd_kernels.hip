@@ -77,6 +77,7 @@ __global__ void __launch_bounds__(256) elem_regular_kernel(const Member<R> *__re
 {
     constexpr bool kLds = kOps == kOpsLds, kCmp = kOps == kOpsCompact;
     constexpr int kW = kCmp ? kOpBc : kOpB;
+    constexpr int kNst = kQ ? 12 : 6;  // stress values per element (with the Q memory variables)
     extern __shared__ __attribute__((aligned(16))) unsigned char eqd_smem[];
     typedef __attribute__((address_space(3))) R lds_R;
     const unsigned gb = launch_block(remap);
@@ -130,7 +131,7 @@ __global__ void __launch_bounds__(256) elem_regular_kernel(const Member<R> *__re
     // one node ahead of the arithmetic, so a gather is always in flight
     int nds[8];
 #pragma unroll
-    for (int a = 0; a < 8; a++) nds[a] = E.ien[a * n + t];
+    for (int a = 0; a < 8; a++) nds[a] = E.ien[blk_at<8>(t, a)];
     // EQD_PREFETCH nodes in flight: slot a % EQD_PREFETCH holds node a's v and u
     constexpr int P = EQD_PREFETCH;
     double vq[P][3], uq[P][3];
@@ -264,7 +265,7 @@ constexpr int kU1 = kKeep ? 8 : EQD_PREFETCH;
 
     R stress[6], strdev[6];
 #pragma unroll
-    for (int i = 0; i < 6; i++) stress[i] = E.stress[i * n + t];
+    for (int i = 0; i < 6; i++) stress[i] = E.stress[blk_at<kNst>(t, i)];
     if (!kQ) {
 #pragma unroll
         for (int i = 0; i < 6; i++) {
@@ -279,14 +280,14 @@ constexpr int kU1 = kKeep ? 8 : EQD_PREFETCH;
         const R vols = strain[0] + strain[1] + strain[2];
         R anestr1[6], anestr[6];
 #pragma unroll
-        for (int i = 0; i < 6; i++) anestr1[i] = E.stress[(6 + i) * n + t];
+        for (int i = 0; i < 6; i++) anestr1[i] = E.stress[blk_at<kNst>(t, 6 + i)];
 #pragma unroll
         for (int i = 0; i < 3; i++)
             anestr[i] = e1 * anestr1[i] + e2 * ((R)2 * miuu * strain[i] * wks + (Mu * wkp - (R)2 * miuu * wks) * vols);
 #pragma unroll
         for (int i = 3; i < 6; i++) anestr[i] = e1 * anestr1[i] + e2 * (miuu * strain[i] * wks);
 #pragma unroll
-        for (int i = 0; i < 6; i++) E.stress[(6 + i) * n + t] = anestr[i];
+        for (int i = 0; i < 6; i++) E.stress[blk_at<kNst>(t, 6 + i)] = anestr[i];
 #pragma unroll
         for (int i = 0; i < 3; i++)
             stress[i] = (R)2 * miuu * strain[i] + (Mu - (R)2 * miuu) * vols - (R)0.5 * (anestr[i] + anestr1[i]);
@@ -326,7 +327,7 @@ constexpr int kU1 = kKeep ? 8 : EQD_PREFETCH;
         E.pstrain[t] = E.pstrain[t] + (double)pstrmag;
     }
 #pragma unroll
-    for (int i = 0; i < 6; i++) E.stress[i * n + t] = stress[i];
+    for (int i = 0; i < 6; i++) E.stress[blk_at<kNst>(t, i)] = stress[i];
 
     // f = -m*al (calcElemMass), + (-det*w) * B^T (sigma + rdampk*sigma_dot), - hourglass
     const R temp = -det * (R)C.w;
@@ -372,7 +373,7 @@ constexpr int kU2 = kKeep ? 8 : EQD_ELEM_UNROLL;
                 // absolute velocity here: the Rayleigh mass term is not translation invariant
                 R al = (R)(C.rdampm * N.vel[k * nn + nd]);
                 if (k == 2) al = al + (R)C.gterm;
-                f[k] = (R)0 - al * E.emass[(a * 3 + k) * n + t];
+                f[k] = (R)0 - al * E.emass[blk_at<24>(t, a * 3 + k)];
             }
         }
         const R n1 = opc_[0], n2 = opc_[1], n3 = opc_[2];
@@ -398,7 +399,7 @@ constexpr int kU2 = kKeep ? 8 : EQD_ELEM_UNROLL;
             }
         }
 #pragma unroll
-        for (int k = 0; k < 3; k++) E.ef[ef_at<kEfReg>(t, a * 3 + k)] = f[k];
+        for (int k = 0; k < 3; k++) E.ef[blk_at<kEfReg>(t, a * 3 + k)] = f[k];
         if constexpr (!kLds && !kKeep) {
 #pragma unroll
             for (int k = 0; k < kNo; k++) opc_[k] = opn_[k];
@@ -468,7 +469,7 @@ __global__ void __launch_bounds__(256) elem_pml_kernel(const Member<R> *__restri
     // node ids first, then each node's v and u one node ahead of the arithmetic
     int nds[8];
 #pragma unroll
-    for (int a = 0; a < 8; a++) nds[a] = E.ien[a * n + p];
+    for (int a = 0; a < 8; a++) nds[a] = E.ien[blk_at<8>(p, a)];
     constexpr int P = EQD_PREFETCH;
     double vq[P][3], uq[P][3];
 #pragma unroll
@@ -618,7 +619,7 @@ constexpr int kU1 = kKeep ? 8 : EQD_PREFETCH;
     for (int i = 3; i < 6; i++) srt[i] = miu * sr[i];
 
     const R dt = (R)C.dt;
-    const R d0 = E.damp[p], d1 = E.damp[n + p], d2 = E.damp[2 * n + p];
+    const R d0 = E.damp[blk_at<3>(p, 0)], d1 = E.damp[blk_at<3>(p, 1)], d2 = E.damp[blk_at<3>(p, 2)];
     const R l2m = lam + (R)2 * miu;
     const R coef[15] = {l2m, lam, lam, lam, l2m, lam, lam, lam, l2m, miu, miu, miu, miu, miu, miu};
     const R grad[15] = {Dx_vx, Dy_vy, Dz_vz, Dx_vx, Dy_vy, Dz_vz, Dx_vx, Dy_vy, Dz_vz,
@@ -628,11 +629,11 @@ constexpr int kU1 = kKeep ? 8 : EQD_PREFETCH;
 #pragma unroll
     for (int c = 0; c < 15; c++) {
         const R dd = dir[c] == 0 ? d0 : (dir[c] == 1 ? d1 : d2);
-        R sc = E.stress[c * n + p];
+        R sc = E.stress[blk_at<21>(p, c)];
         sc = coef[c] * grad[c] + ((R)1 / dt - dd / (R)2) * sc;
         sc = sc / ((R)1 / dt + dd / (R)2);
         sv[c] = sc;
-        E.stress[c * n + p] = sc;
+        E.stress[blk_at<21>(p, c)] = sc;
     }
     const R sxx = sv[0] + sv[1] + sv[2];
     const R syy = sv[3] + sv[4] + sv[5];
@@ -642,7 +643,7 @@ constexpr int kU1 = kKeep ? 8 : EQD_PREFETCH;
     const R syz = sv[13] + sv[14];
     R s0[6];
 #pragma unroll
-    for (int k = 0; k < 6; k++) s0[k] = E.stress[(15 + k) * n + p] + (R)C.rdampk * srt[k];
+    for (int k = 0; k < 6; k++) s0[k] = E.stress[blk_at<21>(p, 15 + k)] + (R)C.rdampk * srt[k];
     const R dw = det * (R)C.w;
     const unsigned mask = E.mask12[p];
     R g[4][3];
@@ -703,7 +704,7 @@ constexpr int kU2 = kKeep ? 8 : EQD_ELEM_UNROLL;
             for (int k = 0; k < 3; k++) {
                 R al = (R)(C.rdampm * N.vel[k * nn + nd]);
                 if (k == 2) al = al + (R)C.gterm;
-                f[9 + k] = (R)0 - al * E.emass[(a * 3 + k) * n + p];
+                f[9 + k] = (R)0 - al * E.emass[blk_at<24>(p, a * 3 + k)];
             }
         } else {
 #pragma unroll
@@ -724,14 +725,14 @@ constexpr int kU2 = kKeep ? 8 : EQD_ELEM_UNROLL;
         if (mask & (1u << a)) {
             // a PML node takes the 9 split components and, in 10-12, the rest
 #pragma unroll
-            for (int k = 0; k < 9; k++) E.ef[ef_at<kEfPml>(p, a * 12 + k)] = f[k];
+            for (int k = 0; k < 9; k++) E.ef[blk_at<kEfPml>(p, a * 12 + k)] = f[k];
 #pragma unroll
-            for (int k = 0; k < 3; k++) E.ef[ef_at<kEfPml>(p, a * 12 + 9 + k)] = f[9 + k] + hg[k];
+            for (int k = 0; k < 3; k++) E.ef[blk_at<kEfPml>(p, a * 12 + 9 + k)] = f[9 + k] + hg[k];
         } else {
             // a 3-component node takes the collapsed sums (src/assembleGlobalKU.f90:55-61)
-            E.ef[ef_at<kEfPml>(p, a * 12 + 0)] = f[0] + f[1] + f[2] + f[9] + hg[0];
-            E.ef[ef_at<kEfPml>(p, a * 12 + 1)] = f[3] + f[4] + f[5] + f[10] + hg[1];
-            E.ef[ef_at<kEfPml>(p, a * 12 + 2)] = f[6] + f[7] + f[8] + f[11] + hg[2];
+            E.ef[blk_at<kEfPml>(p, a * 12 + 0)] = f[0] + f[1] + f[2] + f[9] + hg[0];
+            E.ef[blk_at<kEfPml>(p, a * 12 + 1)] = f[3] + f[4] + f[5] + f[10] + hg[1];
+            E.ef[blk_at<kEfPml>(p, a * 12 + 2)] = f[6] + f[7] + f[8] + f[11] + hg[2];
         }
         if constexpr (!kLds && !kKeep) {
 #pragma unroll
@@ -761,7 +762,7 @@ __device__ __forceinline__ void gather3(const NodeState &N, int node, const R *e
                 const int64_t idx = e[k] & kEntIdxMask;
                 const int a = (e[k] >> kEntLocalShift) & 7;
                 const bool pml = (e[k] & kEntPml) != 0;
-                const R *b = pml ? efp + ef_at<kEfPml>(idx, a * 12) : ef + ef_at<kEfReg>(idx, a * 3);
+                const R *b = pml ? efp + blk_at<kEfPml>(idx, a * 12) : ef + blk_at<kEfReg>(idx, a * 3);
                 f[k][0] = b[0];
                 f[k][1] = b[64];
                 f[k][2] = b[128];
@@ -844,7 +845,7 @@ __global__ void __launch_bounds__(256) node_pml_kernel(const Member<R> *__restri
     const Clock ck = *M.clk;
     const int mode = ck.mode, next_step = ck.nt + 1;
     const int node = P.node[q];
-    const int64_t nn = N.nn, np = P.n;
+    const int64_t nn = N.nn;
     double F[12];
 #pragma unroll
     for (int k = 0; k < 12; k++) F[k] = 0.0;
@@ -861,7 +862,7 @@ __global__ void __launch_bounds__(256) node_pml_kernel(const Member<R> *__restri
                 const int64_t idx = e[k] & kEntIdxMask;
                 const int a = (e[k] >> kEntLocalShift) & 7;
 #pragma unroll
-                for (int c = 0; c < 12; c++) f[k][c] = efp[ef_at<kEfPml>(idx, a * 12 + c)];
+                for (int c = 0; c < 12; c++) f[k][c] = efp[blk_at<kEfPml>(idx, a * 12 + c)];
             }
         }
 #pragma unroll
@@ -875,20 +876,20 @@ __global__ void __launch_bounds__(256) node_pml_kernel(const Member<R> *__restri
     const double m = N.mass[node];
     if (mode == 1) {
 #pragma unroll
-        for (int k = 0; k < 12; k++) P.acc[k * np + q] = F[k] / m;
+        for (int k = 0; k < 12; k++) P.acc[blk_at<12>(q, k)] = F[k] / m;
         return;
     }
-    const double d[3] = {P.damp[q], P.damp[np + q], P.damp[2 * np + q]};
+    const double d[3] = {P.damp[blk_at<3>(q, 0)], P.damp[blk_at<3>(q, 1)], P.damp[blk_at<3>(q, 2)]};
     double v[12];
 #pragma unroll
     for (int k = 0; k < 9; k++) {
         const double dv = d[k % 3];
-        v[k] = (F[k] / m + P.v12[k * np + q] * (1.0 / dt - dv / 2.0)) / (1.0 / dt + dv / 2.0);
+        v[k] = (F[k] / m + P.v12[blk_at<12>(q, k)] * (1.0 / dt - dv / 2.0)) / (1.0 / dt + dv / 2.0);
     }
 #pragma unroll
-    for (int k = 9; k < 12; k++) v[k] = P.v12[k * np + q] + F[k] / m * dt;
+    for (int k = 9; k < 12; k++) v[k] = P.v12[blk_at<12>(q, k)] + F[k] / m * dt;
 #pragma unroll
-    for (int k = 0; k < 12; k++) P.v12[k * np + q] = v[k];
+    for (int k = 0; k < 12; k++) P.v12[blk_at<12>(q, k)] = v[k];
     double vel[3];
 #pragma unroll
     for (int k = 0; k < 3; k++) {
@@ -938,18 +939,18 @@ __global__ void __launch_bounds__(256) update_pml_from_acc_kernel(const Member<R
     const double dt = M.C.dt;
     const int next_step = M.clk->nt + 1;
     const int node = P.node[q];
-    const int64_t nn = N.nn, np = P.n;
-    const double d[3] = {P.damp[q], P.damp[np + q], P.damp[2 * np + q]};
+    const int64_t nn = N.nn;
+    const double d[3] = {P.damp[blk_at<3>(q, 0)], P.damp[blk_at<3>(q, 1)], P.damp[blk_at<3>(q, 2)]};
     double v[12];
 #pragma unroll
     for (int k = 0; k < 9; k++) {
         const double dv = d[k % 3];
-        v[k] = (P.acc[k * np + q] + P.v12[k * np + q] * (1.0 / dt - dv / 2.0)) / (1.0 / dt + dv / 2.0);
+        v[k] = (P.acc[blk_at<12>(q, k)] + P.v12[blk_at<12>(q, k)] * (1.0 / dt - dv / 2.0)) / (1.0 / dt + dv / 2.0);
     }
 #pragma unroll
-    for (int k = 9; k < 12; k++) v[k] = P.v12[k * np + q] + P.acc[k * np + q] * dt;
+    for (int k = 9; k < 12; k++) v[k] = P.v12[blk_at<12>(q, k)] + P.acc[blk_at<12>(q, k)] * dt;
 #pragma unroll
-    for (int k = 0; k < 12; k++) P.v12[k * np + q] = v[k];
+    for (int k = 0; k < 12; k++) P.v12[blk_at<12>(q, k)] = v[k];
     double vel[3];
 #pragma unroll
     for (int k = 0; k < 3; k++) {
