@@ -866,9 +866,9 @@ int eqd_commit(eqd_handle *h)
         int rc;
         double *d;
         h->N.nn = nn;
-        if ((rc = dupload(h, &d, vel))) return rc; h->N.vel = d;
-        if ((rc = dupload(h, &d, disp))) return rc; h->N.disp = d;
-        if ((rc = dupload(h, &d, acc))) return rc; h->N.acc = d;
+        if ((rc = dupload(h, &d, to_blocks<double>(vel, nn, 3)))) return rc; h->N.vel = d;
+        if ((rc = dupload(h, &d, to_blocks<double>(disp, nn, 3)))) return rc; h->N.disp = d;
+        if ((rc = dupload(h, &d, to_blocks<double>(acc, nn, 3)))) return rc; h->N.acc = d;
         if ((rc = dupload(h, &d, mass))) return rc; h->N.mass = d;
         int32_t *fl;
         if ((rc = dalloc(h, &fl, 2))) return rc;
@@ -1047,9 +1047,9 @@ int eqd_get(eqd_handle *h, int32_t field, void *host, int64_t count)
     case EQD_VELARR:
     case EQD_DISPARR: {
         std::vector<double> v;
-        if ((rc = d2h(h, v, field == EQD_VELARR ? h->N.vel : h->N.disp, 3 * nn))) return rc;
+        if ((rc = d2h(h, v, field == EQD_VELARR ? h->N.vel : h->N.disp, blocked_count(nn, 3)))) return rc;
         for (int64_t n = 0; n < nn; n++)
-            for (int k = 0; k < 3; k++) out[n * 3 + k] = v[k * nn + n];
+            for (int k = 0; k < 3; k++) out[n * 3 + k] = v[blocked_index(n, k, 3)];
         return EQD_OK;
     }
     case EQD_V1:
@@ -1057,13 +1057,13 @@ int eqd_get(eqd_handle *h, int32_t field, void *host, int64_t count)
         // v1 of 3-component nodes is velArr; of PML nodes the 12 split velocities
         const bool isv = field == EQD_V1;
         std::vector<double> v, p;
-        if ((rc = d2h(h, v, isv ? h->N.vel : h->N.acc, 3 * nn))) return rc;
+        if ((rc = d2h(h, v, isv ? h->N.vel : h->N.acc, blocked_count(nn, 3)))) return rc;
         if ((rc = d2h(h, p, isv ? h->PN.v12 : h->PN.acc, blocked_count(h->PN.n, 12)))) return rc;
         const auto &orig = h->fd[field];  // values for equations the device does not own (fixed PML nodes)
         for (int64_t i = 0; i < s.neq; i++) out[i] = orig.empty() ? 0.0 : orig[i];
         const int32_t *eqi = I32(h, EQD_EQNUMINDEXARR), *esl = I32(h, EQD_EQNUMSTARTINDEXLOC);
         for (int32_t n : h->node3_all)
-            for (int k = 0; k < 3; k++) out[eqi[esl[n] + k] - 1] = v[k * nn + n];
+            for (int k = 0; k < 3; k++) out[eqi[esl[n] + k] - 1] = v[blocked_index(n, k, 3)];
         const int64_t np = h->PN.n;
         for (int64_t q = 0; q < np; q++) {
             const int32_t n = h->pml_node_ids[q];

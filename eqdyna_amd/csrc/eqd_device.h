@@ -31,9 +31,9 @@ struct StepConsts {
 
 struct NodeState {
     int64_t nn;
-    double *vel;   // [3][nn]  velArr (== v1 for 3-dof nodes)
-    double *disp;  // [3][nn]  dispArr
-    double *acc;   // [3][nn]  pending F/m of 3-dof nodes
+    double *vel;   // 3 per node (blk_at<3>): velArr (== v1 for 3-dof nodes)
+    double *disp;  // 3 per node (blk_at<3>): dispArr
+    double *acc;   // 3 per node (blk_at<3>): pending F/m of 3-dof nodes
     const double *mass;    // [nn] nodal mass (nodalMassArr of the node's equations)
     const int32_t *rowptr; // [nn+1]
     const uint32_t *ent;   // CSR entries

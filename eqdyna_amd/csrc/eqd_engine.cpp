@@ -385,13 +385,13 @@ std::vector<Region> mutable_regions(const eqd_engine *e)
         if (p && b) r.push_back({const_cast<void *>(p), b, nullptr});
     };
     for (const eqd_handle *h : e->hs) {
-        const int64_t nn = h->N.nn;
-        add(h->N.vel, 3 * nn * sizeof(double));
-        add(h->N.disp, 3 * nn * sizeof(double));
-        add(h->N.acc, 3 * nn * sizeof(double));
-        add(h->N.nanflag, 2 * sizeof(int32_t));
-        // element and PML-node arrays are in blocks of 64 items (blk_at)
+        // node, element and PML-node arrays are in blocks of 64 items (blk_at)
         auto blocked = [](int64_t n, int w) { return (size_t)((n + 63) / 64 * 64 * w); };
+        const int64_t nn = h->N.nn;
+        add(h->N.vel, blocked(nn, 3) * sizeof(double));
+        add(h->N.disp, blocked(nn, 3) * sizeof(double));
+        add(h->N.acc, blocked(nn, 3) * sizeof(double));
+        add(h->N.nanflag, 2 * sizeof(int32_t));
         add(e->fp32 ? (const void *)h->E32.stress : (const void *)h->E64.stress, blocked(h->n_reg, h->nreg_stress) * rs);
         add(e->fp32 ? (const void *)h->E32.pstrain : (const void *)h->E64.pstrain, (size_t)h->n_reg * sizeof(double));
         add(e->fp32 ? (const void *)h->P32.stress : (const void *)h->P64.stress, blocked(h->n_pml, 21) * rs);

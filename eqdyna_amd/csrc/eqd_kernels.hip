@@ -139,8 +139,8 @@ __global__ void __launch_bounds__(256) elem_regular_kernel(const Member<R> *__re
     for (int b = 0; b < P; b++)
 #pragma unroll
         for (int k = 0; k < 3; k++) {
-            vq[b][k] = N.vel[k * nn + nds[b]];
-            uq[b][k] = N.disp[k * nn + nds[b]];
+            vq[b][k] = N.vel[blk_at<3>(nds[b], k)];
+            uq[b][k] = N.disp[blk_at<3>(nds[b], k)];
         }
     // FP32 bulk: velocities and displacements relative to the first node
     // (formed in FP64).  Strain, strain rate and the hourglass modes are
@@ -204,8 +204,8 @@ constexpr int kU1 = kKeep ? 8 : EQD_PREFETCH;
             const int nx = nds[a + P];
 #pragma unroll
             for (int k = 0; k < 3; k++) {
-                vq[b][k] = N.vel[k * nn + nx];
-                uq[b][k] = N.disp[k * nn + nx];
+                vq[b][k] = N.vel[blk_at<3>(nx, k)];
+                uq[b][k] = N.disp[blk_at<3>(nx, k)];
             }
         }
         if constexpr (kLds) {
@@ -371,7 +371,7 @@ constexpr int kU2 = kKeep ? 8 : EQD_ELEM_UNROLL;
 #pragma unroll
             for (int k = 0; k < 3; k++) {
                 // absolute velocity here: the Rayleigh mass term is not translation invariant
-                R al = (R)(C.rdampm * N.vel[k * nn + nd]);
+                R al = (R)(C.rdampm * N.vel[blk_at<3>(nd, k)]);
                 if (k == 2) al = al + (R)C.gterm;
                 f[k] = (R)0 - al * E.emass[blk_at<24>(t, a * 3 + k)];
             }
@@ -476,8 +476,8 @@ __global__ void __launch_bounds__(256) elem_pml_kernel(const Member<R> *__restri
     for (int b = 0; b < P; b++)
 #pragma unroll
         for (int k = 0; k < 3; k++) {
-            vq[b][k] = N.vel[k * nn + nds[b]];
-            uq[b][k] = N.disp[k * nn + nds[b]];
+            vq[b][k] = N.vel[blk_at<3>(nds[b], k)];
+            uq[b][k] = N.disp[blk_at<3>(nds[b], k)];
         }
     // velocities (relative to node 1 in the FP32 variant, see elem_regular_kernel)
     double v0[3] = {0.0, 0.0, 0.0}, u0[3] = {0.0, 0.0, 0.0};
@@ -567,8 +567,8 @@ constexpr int kU1 = kKeep ? 8 : EQD_PREFETCH;
             const int nx = nds[a + P];
 #pragma unroll
             for (int k = 0; k < 3; k++) {
-                vq[b][k] = N.vel[k * nn + nx];
-                if (kHG == 1) uq[b][k] = N.disp[k * nn + nx];
+                vq[b][k] = N.vel[blk_at<3>(nx, k)];
+                if (kHG == 1) uq[b][k] = N.disp[blk_at<3>(nx, k)];
             }
         }
         const R n1 = opc_[0], n2 = opc_[1], n3 = opc_[2];
@@ -702,7 +702,7 @@ constexpr int kU2 = kKeep ? 8 : EQD_ELEM_UNROLL;
             for (int k = 0; k < 9; k++) f[k] = 0;
 #pragma unroll
             for (int k = 0; k < 3; k++) {
-                R al = (R)(C.rdampm * N.vel[k * nn + nd]);
+                R al = (R)(C.rdampm * N.vel[blk_at<3>(nd, k)]);
                 if (k == 2) al = al + (R)C.gterm;
                 f[9 + k] = (R)0 - al * E.emass[blk_at<24>(p, a * 3 + k)];
             }
@@ -812,16 +812,16 @@ __global__ void __launch_bounds__(256) node3_kernel(const Member<R> *__restrict_
     const double m = N.mass[node];
     if (mode == 1) {
 #pragma unroll
-        for (int k = 0; k < 3; k++) N.acc[k * nn + node] = F[k] / m;
+        for (int k = 0; k < 3; k++) N.acc[blk_at<3>(node, k)] = F[k] / m;
         return;
     }
     double vn[3];
 #pragma unroll
     for (int k = 0; k < 3; k++) {
         const double a = F[k] / m;
-        const double v = N.vel[k * nn + node] + a * dt;
-        N.vel[k * nn + node] = v;
-        N.disp[k * nn + node] = N.disp[k * nn + node] + v * dt;
+        const double v = N.vel[blk_at<3>(node, k)] + a * dt;
+        N.vel[blk_at<3>(node, k)] = v;
+        N.disp[blk_at<3>(node, k)] = N.disp[blk_at<3>(node, k)] + v * dt;
         vn[k] = v;
     }
     flag_nan(N, node, next_step, vn[0], vn[1], vn[2]);
@@ -894,8 +894,8 @@ __global__ void __launch_bounds__(256) node_pml_kernel(const Member<R> *__restri
 #pragma unroll
     for (int k = 0; k < 3; k++) {
         vel[k] = v[3 * k] + v[3 * k + 1] + v[3 * k + 2] + v[9 + k];
-        N.vel[k * nn + node] = vel[k];
-        N.disp[k * nn + node] = N.disp[k * nn + node] + vel[k] * dt;
+        N.vel[blk_at<3>(node, k)] = vel[k];
+        N.disp[blk_at<3>(node, k)] = N.disp[blk_at<3>(node, k)] + vel[k] * dt;
     }
     flag_nan(N, node, next_step, vel[0], vel[1], vel[2]);
 }
@@ -918,9 +918,9 @@ __global__ void __launch_bounds__(256) update3_from_acc_kernel(const Member<R> *
     double vn[3];
 #pragma unroll
     for (int k = 0; k < 3; k++) {
-        const double v = N.vel[k * nn + node] + N.acc[k * nn + node] * dt;
-        N.vel[k * nn + node] = v;
-        N.disp[k * nn + node] = N.disp[k * nn + node] + v * dt;
+        const double v = N.vel[blk_at<3>(node, k)] + N.acc[blk_at<3>(node, k)] * dt;
+        N.vel[blk_at<3>(node, k)] = v;
+        N.disp[blk_at<3>(node, k)] = N.disp[blk_at<3>(node, k)] + v * dt;
         vn[k] = v;
     }
     flag_nan(N, node, next_step, vn[0], vn[1], vn[2]);
@@ -955,8 +955,8 @@ __global__ void __launch_bounds__(256) update_pml_from_acc_kernel(const Member<R
 #pragma unroll
     for (int k = 0; k < 3; k++) {
         vel[k] = v[3 * k] + v[3 * k + 1] + v[3 * k + 2] + v[9 + k];
-        N.vel[k * nn + node] = vel[k];
-        N.disp[k * nn + node] = N.disp[k * nn + node] + vel[k] * dt;
+        N.vel[blk_at<3>(node, k)] = vel[k];
+        N.disp[blk_at<3>(node, k)] = N.disp[blk_at<3>(node, k)] + vel[k] * dt;
     }
     flag_nan(N, node, next_step, vel[0], vel[1], vel[2]);
 }
@@ -1067,10 +1067,10 @@ __global__ void __launch_bounds__(128) fault_kernel(const Member<R> *__restrict_
     for (int k = 0; k < 3; k++) {
         q[k][0][0] = Fsl[k];
         q[k][1][0] = Fma[k];
-        q[k][0][1] = N.vel[k * nn + ns];
-        q[k][1][1] = N.vel[k * nn + nm];
-        q[k][0][2] = N.disp[k * nn + ns];
-        q[k][1][2] = N.disp[k * nn + nm];
+        q[k][0][1] = N.vel[blk_at<3>(ns, k)];
+        q[k][1][1] = N.vel[blk_at<3>(nm, k)];
+        q[k][0][2] = N.disp[blk_at<3>(ns, k)];
+        q[k][1][2] = N.disp[blk_at<3>(nm, k)];
     }
     for (int j = 0; j < 3; j++)
         for (int s = 0; s < 2; s++) {
@@ -1329,8 +1329,8 @@ __global__ void __launch_bounds__(128) fault_kernel(const Member<R> *__restrict_
         for (int j = 0; j < 3; j++) {
             Fsl[j] = (-xyzAcc[j] + xyzR[j] / mm) * mr;
             Fma[j] = (xyzAcc[j] + xyzR[j] / ms) * mr;
-            FR(31 + j) = N.vel[j * nn + nm] + (xyzAcc[j] + xyzR[j] / ms) * dt;
-            FR(34 + j) = N.vel[j * nn + ns] + (-xyzAcc[j] + xyzR[j] / mm) * dt;
+            FR(31 + j) = N.vel[blk_at<3>(nm, j)] + (xyzAcc[j] + xyzR[j] / ms) * dt;
+            FR(34 + j) = N.vel[blk_at<3>(ns, j)] + (-xyzAcc[j] + xyzR[j] / mm) * dt;
         }
     }
     if constexpr (kLearn) {
@@ -1368,21 +1368,21 @@ __global__ void __launch_bounds__(128) fault_kernel(const Member<R> *__restrict_
     const double msl = N.mass[ns], mma = N.mass[nm];
     if (mode == 1) {
         for (int k = 0; k < 3; k++) {
-            N.acc[k * nn + ns] = Fsl[k] / msl;
-            N.acc[k * nn + nm] = Fma[k] / mma;
+            N.acc[blk_at<3>(ns, k)] = Fsl[k] / msl;
+            N.acc[blk_at<3>(nm, k)] = Fma[k] / mma;
         }
         return;
     }
     double vs[3], vm[3];
     for (int k = 0; k < 3; k++) {
-        const double vs0 = N.vel[k * nn + ns], vm0 = N.vel[k * nn + nm];
-        const double us0 = N.disp[k * nn + ns], um0 = N.disp[k * nn + nm];
+        const double vs0 = N.vel[blk_at<3>(ns, k)], vm0 = N.vel[blk_at<3>(nm, k)];
+        const double us0 = N.disp[blk_at<3>(ns, k)], um0 = N.disp[blk_at<3>(nm, k)];
         vs[k] = vs0 + Fsl[k] / msl * dt;
         vm[k] = vm0 + Fma[k] / mma * dt;
-        N.vel[k * nn + ns] = vs[k];
-        N.vel[k * nn + nm] = vm[k];
-        N.disp[k * nn + ns] = us0 + vs[k] * dt;
-        N.disp[k * nn + nm] = um0 + vm[k] * dt;
+        N.vel[blk_at<3>(ns, k)] = vs[k];
+        N.vel[blk_at<3>(nm, k)] = vm[k];
+        N.disp[blk_at<3>(ns, k)] = us0 + vs[k] * dt;
+        N.disp[blk_at<3>(nm, k)] = um0 + vm[k] * dt;
     }
     flag_nan(N, ns, nt + 1, vs[0], vs[1], vs[2]);
     flag_nan(N, nm, nt + 1, vm[0], vm[1], vm[2]);
@@ -1409,9 +1409,9 @@ __global__ void __launch_bounds__(64) station_kernel(const Member<R> *__restrict
     const int i = r - 1;
     const int node = S.idh[i], k = S.idh[S.nrow + i], kind = S.idh[2 * S.nrow + i];
     if (kind == 1)
-        col[r] = N.disp[k * N.nn + node];
+        col[r] = N.disp[blk_at<3>(node, k)];
     else if (kind == 2)
-        col[r] = N.vel[k * N.nn + node];
+        col[r] = N.vel[blk_at<3>(node, k)];
 }
 
 // thermop, src/thermop.f90:1-40: pore pressure change fric(51) and
@@ -1481,7 +1481,7 @@ __global__ void __launch_bounds__(256) record_kernel(const Member<R> *__restrict
     const int node = rc.nodes[t];
     const int64_t nn = M.N.nn;
 #pragma unroll
-    for (int k = 0; k < 3; k++) rc.vel[(slot * rc.n_nodes + t) * 3 + k] = M.N.vel[k * nn + node];
+    for (int k = 0; k < 3; k++) rc.vel[(slot * rc.n_nodes + t) * 3 + k] = M.N.vel[blk_at<3>(node, k)];
 }
 
 // The step clock: nt += 1 and time += dt for every scenario of the launch,
