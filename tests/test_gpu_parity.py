@@ -168,3 +168,30 @@ def test_compact_phi_matches_given_phi(name, monkeypatch):
     worst = max(err.values())
     print(f"[{name}] compact vs given Phi: worst {worst:.3e}", flush=True)
     assert worst <= 1e-11, err
+
+
+@pytest.mark.gpu
+def test_phi_not_of_reference_form_keeps_its_table(monkeypatch):
+    # Phi moved off the reference's form (one value of every element): the
+    # library must keep the given Phi rows (no compact form) and compute
+    # exactly what the forced given-Phi layout computes
+    bundle, sc, _ = load_golden("box_layered")
+    b = dict(bundle)
+    phi = b["phi"].copy()
+    phi[:, 0, 5] += 1e-6
+    b["phi"] = phi
+    n = int(sc["nstep"])
+    monkeypatch.setenv("EQD_NO_OPERATOR_CLASSES", "1")
+    s = Solver(b, sc, "fp64")
+    info = s.layout_info()
+    s.run(n)
+    o = s.outputs()
+    s.close()
+    assert info["phi_compact_regular"] == 0 and info["phi_compact_pml"] == 0
+    monkeypatch.setenv("EQD_FULL_PHI", "1")
+    s = Solver(b, sc, "fp64")
+    s.run(n)
+    o2 = s.outputs()
+    s.close()
+    for k in o:
+        assert np.array_equal(o[k], o2[k], equal_nan=True), k
