@@ -820,8 +820,10 @@ int eqd_commit(eqd_handle *h)
             R *q;
             if ((r2 = dupload(h, &q, to_blocks<R>(p_st, np, 21)))) return r2;
             P.stress = q;
-            if ((r2 = dalloc(h, &q, (np + 63) / 64 * 64 * kEfPml))) return r2;
+            if ((r2 = dalloc(h, &q, blocked_count(np, kEfPml)))) return r2;
             P.ef = q;
+            if ((r2 = dalloc(h, &q, blocked_count(np, 6)))) return r2;
+            P.ssum = q;
             return 0;
         };
         if (fp32) {

@@ -64,7 +64,7 @@ __host__ __device__ __forceinline__ int64_t blk_at(int64_t t, int s)
 {
     return ((t >> 6) * W + s) * 64 + (t & 63);
 }
-constexpr int kEfReg = 24, kEfPml = 96;
+constexpr int kEfReg = 24, kEfPml = 48;
 
 // Where an element kernel finds its operators: class tables in global memory
 // or per-element rows with Phi; class tables staged in LDS; per-element rows
@@ -124,7 +124,11 @@ struct PmlElems {
     const R *emass;        // 24 per element (blk_at<24>)
     const R *damp;         // 3 per element (blk_at<3>): element-centre damping d_x, d_y, d_z
     const uint8_t *mask12; // [n] bit a: local node a carries 12 components
-    R *ef;                 // slot a*12+k (blk_at<kEfPml>); 3-component nodes use k=0..2
+    R *ef;                 // 6 slots per local node a (blk_at<kEfPml>, slot a*6+k): a 3-component
+                           // node's force at k = 0..2; a 12-component node's det*w*dN_a/dx at
+                           // k = 0..2 (its 9 split forces follow from ssum) and force
+                           // components 10-12 at k = 3..5
+    R *ssum;               // 6 per element (blk_at<6>): summed stresses xx, yy, zz, xy, xz, yz
     // operators as in RegElems
     const uint32_t *opc;
     const R *optab;

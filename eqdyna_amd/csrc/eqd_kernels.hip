@@ -641,6 +641,13 @@ constexpr int kU1 = kKeep ? 8 : EQD_PREFETCH;
     const R sxy = sv[9] + sv[10];
     const R sxz = sv[11] + sv[12];
     const R syz = sv[13] + sv[14];
+    // the summed stresses, for the PML nodes' split forces (node_pml_kernel)
+    E.ssum[blk_at<6>(p, 0)] = sxx;
+    E.ssum[blk_at<6>(p, 1)] = syy;
+    E.ssum[blk_at<6>(p, 2)] = szz;
+    E.ssum[blk_at<6>(p, 3)] = sxy;
+    E.ssum[blk_at<6>(p, 4)] = sxz;
+    E.ssum[blk_at<6>(p, 5)] = syz;
     R s0[6];
 #pragma unroll
     for (int k = 0; k < 6; k++) s0[k] = E.stress[blk_at<21>(p, 15 + k)] + (R)C.rdampk * srt[k];
@@ -723,16 +730,18 @@ constexpr int kU2 = kKeep ? 8 : EQD_ELEM_UNROLL;
         f[10] = f[10] - dw * (g2 * s0[1] + g3 * s0[3] + g1 * s0[5]);
         f[11] = f[11] - dw * (g3 * s0[2] + g2 * s0[3] + g1 * s0[4]);
         if (mask & (1u << a)) {
-            // a PML node takes the 9 split components and, in 10-12, the rest
+            // a PML node: its 9 split components are (0 - (dw g_i) s_j) with
+            // the summed stresses stored above, so the node kernel forms them
+            // from dw g (3 values) bit for bit; 10-12 are stored whole
 #pragma unroll
-            for (int k = 0; k < 9; k++) E.ef[blk_at<kEfPml>(p, a * 12 + k)] = f[k];
+            for (int k = 0; k < 3; k++) E.ef[blk_at<kEfPml>(p, a * 6 + k)] = dw * (k == 0 ? g1 : (k == 1 ? g2 : g3));
 #pragma unroll
-            for (int k = 0; k < 3; k++) E.ef[blk_at<kEfPml>(p, a * 12 + 9 + k)] = f[9 + k] + hg[k];
+            for (int k = 0; k < 3; k++) E.ef[blk_at<kEfPml>(p, a * 6 + 3 + k)] = f[9 + k] + hg[k];
         } else {
             // a 3-component node takes the collapsed sums (src/assembleGlobalKU.f90:55-61)
-            E.ef[blk_at<kEfPml>(p, a * 12 + 0)] = f[0] + f[1] + f[2] + f[9] + hg[0];
-            E.ef[blk_at<kEfPml>(p, a * 12 + 1)] = f[3] + f[4] + f[5] + f[10] + hg[1];
-            E.ef[blk_at<kEfPml>(p, a * 12 + 2)] = f[6] + f[7] + f[8] + f[11] + hg[2];
+            E.ef[blk_at<kEfPml>(p, a * 6 + 0)] = f[0] + f[1] + f[2] + f[9] + hg[0];
+            E.ef[blk_at<kEfPml>(p, a * 6 + 1)] = f[3] + f[4] + f[5] + f[10] + hg[1];
+            E.ef[blk_at<kEfPml>(p, a * 6 + 2)] = f[6] + f[7] + f[8] + f[11] + hg[2];
         }
         if constexpr (!kLds && !kKeep) {
 #pragma unroll
@@ -762,7 +771,7 @@ __device__ __forceinline__ void gather3(const NodeState &N, int node, const R *e
                 const int64_t idx = e[k] & kEntIdxMask;
                 const int a = (e[k] >> kEntLocalShift) & 7;
                 const bool pml = (e[k] & kEntPml) != 0;
-                const R *b = pml ? efp + blk_at<kEfPml>(idx, a * 12) : ef + blk_at<kEfReg>(idx, a * 3);
+                const R *b = pml ? efp + blk_at<kEfPml>(idx, a * 6) : ef + blk_at<kEfReg>(idx, a * 3);
                 f[k][0] = b[0];
                 f[k][1] = b[64];
                 f[k][2] = b[128];
@@ -840,7 +849,7 @@ __global__ void __launch_bounds__(256) node_pml_kernel(const Member<R> *__restri
     if (q >= M.PN.n) return;
     const NodeState N = M.N;
     const PmlNodes P = M.PN;
-    const R *efp = M.P.ef;
+    const R *efp = M.P.ef, *ssum = M.P.ssum;
     const double dt = M.C.dt;
     const Clock ck = *M.clk;
     const int mode = ck.mode, next_step = ck.nt + 1;
@@ -861,8 +870,23 @@ __global__ void __launch_bounds__(256) node_pml_kernel(const Member<R> *__restri
             if (j0 + k < end) {
                 const int64_t idx = e[k] & kEntIdxMask;
                 const int a = (e[k] >> kEntLocalShift) & 7;
+                // the 9 split components from dw g and the element's summed
+                // stresses, as elem_pml_kernel forms them; 10-12 as stored
+                const R q1 = efp[blk_at<kEfPml>(idx, a * 6 + 0)], q2 = efp[blk_at<kEfPml>(idx, a * 6 + 1)],
+                        q3 = efp[blk_at<kEfPml>(idx, a * 6 + 2)];
+                const R sxx = ssum[blk_at<6>(idx, 0)], syy = ssum[blk_at<6>(idx, 1)], szz = ssum[blk_at<6>(idx, 2)],
+                        sxy = ssum[blk_at<6>(idx, 3)], sxz = ssum[blk_at<6>(idx, 4)], syz = ssum[blk_at<6>(idx, 5)];
+                f[k][0] = (R)0 - q1 * sxx;
+                f[k][1] = (R)0 - q2 * sxy;
+                f[k][2] = (R)0 - q3 * sxz;
+                f[k][3] = (R)0 - q1 * sxy;
+                f[k][4] = (R)0 - q2 * syy;
+                f[k][5] = (R)0 - q3 * syz;
+                f[k][6] = (R)0 - q1 * sxz;
+                f[k][7] = (R)0 - q2 * syz;
+                f[k][8] = (R)0 - q3 * szz;
 #pragma unroll
-                for (int c = 0; c < 12; c++) f[k][c] = efp[blk_at<kEfPml>(idx, a * 12 + c)];
+                for (int c = 0; c < 3; c++) f[k][9 + c] = efp[blk_at<kEfPml>(idx, a * 6 + 3 + c)];
             }
         }
 #pragma unroll
