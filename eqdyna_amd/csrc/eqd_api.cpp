@@ -583,7 +583,7 @@ int eqd_commit(eqd_handle *h)
         }
     }
     const int64_t npn = (int64_t)h->pml_node_ids.size();
-    // transpose per-PML-node arrays to component-major
+    // per-PML-node arrays to component-major (then to blocks at upload)
     auto tr = [](const std::vector<double> &a, int64_t cnt, int w) {
         std::vector<double> b(a.size());
         for (int64_t q = 0; q < cnt; q++)

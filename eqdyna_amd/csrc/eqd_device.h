@@ -1,14 +1,17 @@
 // Device-side data layout of the eqdyna_amd time step.
 //
-// Everything the step touches stays resident in HBM in structure-of-arrays
-// form, indexed so that consecutive threads read consecutive addresses:
+// Everything the step touches stays resident in HBM in blocked
+// structure-of-arrays form (blk_at): value c of item t lives at
+// ((t / 64) * W + c) * 64 + t % 64, so consecutive threads read consecutive
+// addresses and one wavefront's W values of its 64 items form one contiguous
+// region:
 //  * elements are split by class (regular hex/wedge vs PML) and renumbered
-//    within the class in ascending reference order; component c of element t
-//    lives at base[c * n + t];
-//  * nodal arrays are component-major, base[k * nn + node];
+//    within the class in ascending reference order;
+//  * nodal arrays hold 3 values per node in the same blocks of 64 nodes;
 //  * the element-force buffers hold, per element and local node, the force the
-//    element contributes to that node; a node-to-element table (CSR, ascending
-//    element order) lets each node sum its contributions without atomics.
+//    element contributes to that node (for PML nodes in factored form); a
+//    node-to-element table (CSR, ascending element order) lets each node sum
+//    its contributions without atomics.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
