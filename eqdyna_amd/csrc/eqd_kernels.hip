@@ -414,8 +414,11 @@ constexpr int kU2 = kKeep ? 8 : EQD_ELEM_UNROLL;
 // accumulates the velocity gradients and hourglass mode amplitudes; pass 2
 // forms each node's 12 (or collapsed 3) force slots, hourglass included, and
 // writes every slot once.
+// At least 3 waves per SIMD in FP64 (4 in FP32): caps the FP64 per-element
+// variant at 168 VGPRs without spills (it would take 176, leaving 2 waves);
+// measured 0.324 -> 0.307 ms on the rough-fault box.
 template <typename R, int kHG, bool kMass, int kOps>
-__global__ void __launch_bounds__(256) elem_pml_kernel(const Member<R> *__restrict__ Ms,
+__global__ void __launch_bounds__(256, (sizeof(R) == 8 ? 3 : 4)) elem_pml_kernel(const Member<R> *__restrict__ Ms,
                                                        const uint16_t *__restrict__ bmem, int remap)
 {
     constexpr bool kLds = kOps == kOpsLds, kCmp = kOps == kOpsCompact;
