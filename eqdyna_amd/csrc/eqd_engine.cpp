@@ -202,10 +202,21 @@ int ensure_built(eqd_engine *e)
 // ---------------- per-kind timers (eager runs only)
 constexpr size_t kTimerPool = 256;
 
-void timed_begin(eqd_engine *e, int kind, hipEvent_t *stop, hipStream_t st)
+// The timer events of one launch of a kind in this step: the kind's first
+// launch records the start, its last the end, both from the dispatch itself
+// (no event packets of their own); null events when not timing.
+struct Stamp {
+    hipEvent_t a = nullptr, b = nullptr;
+    size_t n = 0;  // launches of the kind this step
+    hipEvent_t start(size_t i) const { return i == 0 ? a : nullptr; }
+    hipEvent_t stop(size_t i) const { return i + 1 == n ? b : nullptr; }
+};
+
+Stamp take_timer(eqd_engine *e, int kind, size_t launches)
 {
-    *stop = nullptr;
-    if (!e->timing) return;
+    Stamp p;
+    p.n = launches;
+    if (!e->timing || launches == 0) return p;
     KTimer &t = e->timers[kKindName[kind]];
     if (t.used == t.ev.size()) {
         hipEvent_t a, b;
@@ -213,14 +224,10 @@ void timed_begin(eqd_engine *e, int kind, hipEvent_t *stop, hipStream_t st)
         (void)hipEventCreate(&b);
         t.ev.push_back({a, b});
     }
-    (void)hipEventRecord(t.ev[t.used].first, st);
-    *stop = t.ev[t.used].second;
+    p.a = t.ev[t.used].first;
+    p.b = t.ev[t.used].second;
     t.used++;
-}
-
-void timed_end(hipEvent_t stop, hipStream_t st)
-{
-    if (stop) (void)hipEventRecord(stop, st);
+    return p;
 }
 
 // add the recorded launches' durations to the timers and recycle their events
@@ -247,21 +254,38 @@ int enqueue_step(eqd_engine *e)
 {
     const Member<R> *dM = (const Member<R> *)e->d_members;
     hipStream_t st = e->st;
-    hipEvent_t ev;
     HIPCHK(e, launch_clock<R>(dM, (int)e->hs.size(), st));
-    timed_begin(e, kKStation, &ev, st);
-    for (const Group &g : e->groups[kKStation]) HIPCHK(e, launch_station<R>(dM, g.g, st));
-    for (const Group &g : e->groups[kKRecord]) HIPCHK(e, launch_record<R>(dM, g.g, st));
-    timed_end(ev, st);
-    timed_begin(e, kKElem, &ev, st);
-    for (const Group &g : e->groups[kKElem])
-        HIPCHK(e, launch_elem<R>(dM, g.g, g.sig & 1, (g.sig >> 1) & 1, (g.sig >> 2) & 3, (g.sig >> 4) & 1,
-                                 (g.sig >> 5) & 3, st));
-    timed_end(ev, st);
-    timed_begin(e, kKPml, &ev, st);
-    for (const Group &g : e->groups[kKPml])
-        HIPCHK(e, launch_pml<R>(dM, g.g, g.sig & 3, (g.sig >> 2) & 1, (g.sig >> 3) & 3, st));
-    timed_end(ev, st);
+    {
+        const auto &gs = e->groups[kKStation], &gr = e->groups[kKRecord];
+        const Stamp p = take_timer(e, kKStation, gs.size() + gr.size());
+        size_t i = 0;
+        for (const Group &g : gs) {
+            HIPCHK(e, launch_station<R>(dM, g.g, st, p.start(i), p.stop(i)));
+            i++;
+        }
+        for (const Group &g : gr) {
+            HIPCHK(e, launch_record<R>(dM, g.g, st, p.start(i), p.stop(i)));
+            i++;
+        }
+    }
+    {
+        const auto &gs = e->groups[kKElem];
+        const Stamp p = take_timer(e, kKElem, gs.size());
+        for (size_t i = 0; i < gs.size(); i++) {
+            const Group &g = gs[i];
+            HIPCHK(e, launch_elem<R>(dM, g.g, g.sig & 1, (g.sig >> 1) & 1, (g.sig >> 2) & 3, (g.sig >> 4) & 1,
+                                     (g.sig >> 5) & 3, st, p.start(i), p.stop(i)));
+        }
+    }
+    {
+        const auto &gs = e->groups[kKPml];
+        const Stamp p = take_timer(e, kKPml, gs.size());
+        for (size_t i = 0; i < gs.size(); i++) {
+            const Group &g = gs[i];
+            HIPCHK(e, launch_pml<R>(dM, g.g, g.sig & 3, (g.sig >> 2) & 1, (g.sig >> 3) & 3, st, p.start(i),
+                                    p.stop(i)));
+        }
+    }
     // fork: the fault pairs on the second stream, beside the node kernels
     // (EQD_SINGLE_STREAM=1 keeps them on the main stream, in step order)
     hipStream_t fst = e->single_stream ? st : e->st2;
@@ -270,19 +294,29 @@ int enqueue_step(eqd_engine *e)
         HIPCHK(e, hipStreamWaitEvent(e->st2, e->ev_fork, 0));
     }
     // thermal pressurization (friclaw 5) before the fault pairs, src/driver.f90:27
-    timed_begin(e, kKTp, &ev, fst);
-    for (const Group &g : e->groups[kKTp]) HIPCHK(e, launch_thermop<R>(dM, g.g, fst));
-    timed_end(ev, fst);
-    timed_begin(e, kKFault, &ev, fst);
-    for (const Group &g : e->groups[kKFault]) HIPCHK(e, launch_fault<R>(dM, g.g, g.sig, fst));
-    timed_end(ev, fst);
+    {
+        const auto &gs = e->groups[kKTp];
+        const Stamp p = take_timer(e, kKTp, gs.size());
+        for (size_t i = 0; i < gs.size(); i++) HIPCHK(e, launch_thermop<R>(dM, gs[i].g, fst, p.start(i), p.stop(i)));
+    }
+    {
+        const auto &gs = e->groups[kKFault];
+        const Stamp p = take_timer(e, kKFault, gs.size());
+        for (size_t i = 0; i < gs.size(); i++)
+            HIPCHK(e, launch_fault<R>(dM, gs[i].g, gs[i].sig, fst, p.start(i), p.stop(i)));
+    }
     if (!e->single_stream) HIPCHK(e, hipEventRecord(e->ev_join, e->st2));
-    timed_begin(e, kKNode3, &ev, st);
-    for (const Group &g : e->groups[kKNode3]) HIPCHK(e, launch_node3<R>(dM, g.g, st));
-    timed_end(ev, st);
-    timed_begin(e, kKNodePml, &ev, st);
-    for (const Group &g : e->groups[kKNodePml]) HIPCHK(e, launch_node_pml<R>(dM, g.g, st));
-    timed_end(ev, st);
+    {
+        const auto &gs = e->groups[kKNode3];
+        const Stamp p = take_timer(e, kKNode3, gs.size());
+        for (size_t i = 0; i < gs.size(); i++) HIPCHK(e, launch_node3<R>(dM, gs[i].g, st, p.start(i), p.stop(i)));
+    }
+    {
+        const auto &gs = e->groups[kKNodePml];
+        const Stamp p = take_timer(e, kKNodePml, gs.size());
+        for (size_t i = 0; i < gs.size(); i++)
+            HIPCHK(e, launch_node_pml<R>(dM, gs[i].g, st, p.start(i), p.stop(i)));
+    }
     // join: the next step sees the fault pairs' update
     if (!e->single_stream) HIPCHK(e, hipStreamWaitEvent(st, e->ev_join, 0));
     return EQD_OK;

@@ -16,25 +16,32 @@
 namespace eqd {
 template <typename R>
 hipError_t launch_elem(const Member<R> *, const LaunchGrid &, int q, int dp, int hg, int mass, int ops,
-                       hipStream_t);
+                       hipStream_t, hipEvent_t t0 = nullptr, hipEvent_t t1 = nullptr);
 template <typename R>
-hipError_t launch_pml(const Member<R> *, const LaunchGrid &, int hg, int mass, int ops, hipStream_t);
+hipError_t launch_pml(const Member<R> *, const LaunchGrid &, int hg, int mass, int ops, hipStream_t,
+                      hipEvent_t t0 = nullptr, hipEvent_t t1 = nullptr);
 template <typename R>
-hipError_t launch_node3(const Member<R> *, const LaunchGrid &, hipStream_t);
+hipError_t launch_node3(const Member<R> *, const LaunchGrid &, hipStream_t, hipEvent_t t0 = nullptr,
+                        hipEvent_t t1 = nullptr);
 template <typename R>
-hipError_t launch_node_pml(const Member<R> *, const LaunchGrid &, hipStream_t);
+hipError_t launch_node_pml(const Member<R> *, const LaunchGrid &, hipStream_t, hipEvent_t t0 = nullptr,
+                           hipEvent_t t1 = nullptr);
 template <typename R>
 hipError_t launch_pending(const Member<R> *, const LaunchGrid &, const LaunchGrid &, hipStream_t);
 template <typename R>
-hipError_t launch_fault(const Member<R> *, const LaunchGrid &, int learn, hipStream_t);
+hipError_t launch_fault(const Member<R> *, const LaunchGrid &, int learn, hipStream_t, hipEvent_t t0 = nullptr,
+                        hipEvent_t t1 = nullptr);
 template <typename R>
-hipError_t launch_station(const Member<R> *, const LaunchGrid &, hipStream_t);
+hipError_t launch_station(const Member<R> *, const LaunchGrid &, hipStream_t, hipEvent_t t0 = nullptr,
+                          hipEvent_t t1 = nullptr);
 template <typename R>
 hipError_t launch_clock(const Member<R> *, int nmem, hipStream_t);
 template <typename R>
-hipError_t launch_thermop(const Member<R> *, const LaunchGrid &, hipStream_t);
+hipError_t launch_thermop(const Member<R> *, const LaunchGrid &, hipStream_t, hipEvent_t t0 = nullptr,
+                          hipEvent_t t1 = nullptr);
 template <typename R>
-hipError_t launch_record(const Member<R> *, const LaunchGrid &, hipStream_t);
+hipError_t launch_record(const Member<R> *, const LaunchGrid &, hipStream_t, hipEvent_t t0 = nullptr,
+                         hipEvent_t t1 = nullptr);
 hipError_t launch_mlp_prepare(const float *p, __bf16 *wb, float *w4r, hipStream_t s);
 
 struct KTimer {

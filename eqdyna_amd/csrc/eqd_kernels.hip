@@ -12,6 +12,8 @@
 //
 // Compiled with -ffp-contract=off: no FMA contraction, so each rounded
 // operation is the one the reference performs.
+#include <hip/hip_ext.h>
+
 #include "eqd_device.h"
 
 namespace eqd {
@@ -1529,9 +1531,20 @@ __global__ void __launch_bounds__(64) clock_kernel(const Member<R> *__restrict__
 // launchers (host side)
 static inline unsigned grid_for(int64_t n, int block) { return (unsigned)((n + block - 1) / block); }
 
+// One launch; with timer events t0 / t1 the dispatch itself records its start
+// and end (hipExtLaunchKernelGGL), so timing adds no packets to the stream.
+#define EQD_GO(KERNEL, G, B, L, S, ...)                                                           \
+    do {                                                                                          \
+        auto kfn_ = KERNEL;                                                                       \
+        if (t0 || t1)                                                                             \
+            hipExtLaunchKernelGGL(kfn_, dim3(G), dim3(B), (uint32_t)(L), S, t0, t1, 0, __VA_ARGS__); \
+        else                                                                                      \
+            kfn_<<<G, B, L, S>>>(__VA_ARGS__);                                                    \
+    } while (0)
+
 template <typename R>
 hipError_t launch_elem(const Member<R> *M, const LaunchGrid &g, int q, int dp, int hg, int mass, int ops,
-                       hipStream_t s)
+                       hipStream_t s, hipEvent_t t0, hipEvent_t t1)
 {
     if (g.grid == 0) return hipSuccess;
     if (g.block < 64 || g.block > 256 || g.block % 64) return hipErrorInvalidValue;
@@ -1541,11 +1554,11 @@ hipError_t launch_elem(const Member<R> *M, const LaunchGrid &g, int q, int dp, i
 #define EQD_LAUNCH(Q, DP, HG, MASS)                                                                               \
     do {                                                                                                          \
         if (ops == kOpsLds)                                                                                       \
-            elem_regular_kernel<R, Q, DP, HG, MASS, kOpsLds><<<g.grid, g.block, lds, s>>>(M, g.bmem, g.remap);    \
+            EQD_GO((elem_regular_kernel<R, Q, DP, HG, MASS, kOpsLds>), g.grid, g.block, lds, s, M, g.bmem, g.remap); \
         else if (ops == kOpsCompact)                                                                              \
-            elem_regular_kernel<R, Q, DP, HG, MASS, kOpsCompact><<<g.grid, g.block, 0, s>>>(M, g.bmem, g.remap);  \
+            EQD_GO((elem_regular_kernel<R, Q, DP, HG, MASS, kOpsCompact>), g.grid, g.block, 0, s, M, g.bmem, g.remap); \
         else                                                                                                      \
-            elem_regular_kernel<R, Q, DP, HG, MASS, kOpsRows><<<g.grid, g.block, 0, s>>>(M, g.bmem, g.remap);     \
+            EQD_GO((elem_regular_kernel<R, Q, DP, HG, MASS, kOpsRows>), g.grid, g.block, 0, s, M, g.bmem, g.remap); \
     } while (0)
 #define EQD_HG(Q, DP, MASS)                         \
     do {                                            \
@@ -1564,7 +1577,8 @@ hipError_t launch_elem(const Member<R> *M, const LaunchGrid &g, int q, int dp, i
 }
 
 template <typename R>
-hipError_t launch_pml(const Member<R> *M, const LaunchGrid &g, int hg, int mass, int ops, hipStream_t s)
+hipError_t launch_pml(const Member<R> *M, const LaunchGrid &g, int hg, int mass, int ops, hipStream_t s,
+                      hipEvent_t t0, hipEvent_t t1)
 {
     if (g.grid == 0) return hipSuccess;
     if (g.block < 64 || g.block > 256 || g.block % 64) return hipErrorInvalidValue;
@@ -1574,11 +1588,11 @@ hipError_t launch_pml(const Member<R> *M, const LaunchGrid &g, int hg, int mass,
 #define EQD_LAUNCH(HG, MASS)                                                                              \
     do {                                                                                                  \
         if (ops == kOpsLds)                                                                               \
-            elem_pml_kernel<R, HG, MASS, kOpsLds><<<g.grid, g.block, lds, s>>>(M, g.bmem, g.remap);       \
+            EQD_GO((elem_pml_kernel<R, HG, MASS, kOpsLds>), g.grid, g.block, lds, s, M, g.bmem, g.remap);     \
         else if (ops == kOpsCompact)                                                                      \
-            elem_pml_kernel<R, HG, MASS, kOpsCompact><<<g.grid, g.block, lds, s>>>(M, g.bmem, g.remap);   \
+            EQD_GO((elem_pml_kernel<R, HG, MASS, kOpsCompact>), g.grid, g.block, lds, s, M, g.bmem, g.remap); \
         else                                                                                              \
-            elem_pml_kernel<R, HG, MASS, kOpsRows><<<g.grid, g.block, lds, s>>>(M, g.bmem, g.remap);      \
+            EQD_GO((elem_pml_kernel<R, HG, MASS, kOpsRows>), g.grid, g.block, lds, s, M, g.bmem, g.remap);    \
     } while (0)
     if (mass) {
         if (hg == 1) EQD_LAUNCH(1, true);
@@ -1594,20 +1608,20 @@ hipError_t launch_pml(const Member<R> *M, const LaunchGrid &g, int hg, int mass,
 }
 
 template <typename R>
-hipError_t launch_node3(const Member<R> *M, const LaunchGrid &g, hipStream_t s)
+hipError_t launch_node3(const Member<R> *M, const LaunchGrid &g, hipStream_t s, hipEvent_t t0, hipEvent_t t1)
 {
     if (g.grid == 0) return hipSuccess;
     if (g.block < 64 || g.block > 256 || g.block % 64) return hipErrorInvalidValue;
-    node3_kernel<R><<<g.grid, g.block, 0, s>>>(M, g.bmem, g.remap);
+    EQD_GO(node3_kernel<R>, g.grid, g.block, 0, s, M, g.bmem, g.remap);
     return hipGetLastError();
 }
 
 template <typename R>
-hipError_t launch_node_pml(const Member<R> *M, const LaunchGrid &g, hipStream_t s)
+hipError_t launch_node_pml(const Member<R> *M, const LaunchGrid &g, hipStream_t s, hipEvent_t t0, hipEvent_t t1)
 {
     if (g.grid == 0) return hipSuccess;
     if (g.block < 64 || g.block > 256 || g.block % 64) return hipErrorInvalidValue;
-    node_pml_kernel<R><<<g.grid, g.block, 0, s>>>(M, g.bmem, g.remap);
+    EQD_GO(node_pml_kernel<R>, g.grid, g.block, 0, s, M, g.bmem, g.remap);
     return hipGetLastError();
 }
 
@@ -1620,42 +1634,43 @@ hipError_t launch_pending(const Member<R> *M, const LaunchGrid &g3, const Launch
 }
 
 template <typename R>
-hipError_t launch_fault(const Member<R> *M, const LaunchGrid &g, int learn, hipStream_t s)
+hipError_t launch_fault(const Member<R> *M, const LaunchGrid &g, int learn, hipStream_t s, hipEvent_t t0,
+                        hipEvent_t t1)
 {
     if (g.grid == 0) return hipSuccess;
     // the learned variant evaluates the network one wavefront at a time
     if (g.block != 64 && g.block != 128) return hipErrorInvalidValue;
     if (learn)
-        fault_kernel<R, true><<<g.grid, g.block, 0, s>>>(M, g.bmem);
+        EQD_GO((fault_kernel<R, true>), g.grid, g.block, 0, s, M, g.bmem);
     else
-        fault_kernel<R, false><<<g.grid, g.block, 0, s>>>(M, g.bmem);
+        EQD_GO((fault_kernel<R, false>), g.grid, g.block, 0, s, M, g.bmem);
     return hipGetLastError();
 }
 
 template <typename R>
-hipError_t launch_record(const Member<R> *M, const LaunchGrid &g, hipStream_t s)
+hipError_t launch_record(const Member<R> *M, const LaunchGrid &g, hipStream_t s, hipEvent_t t0, hipEvent_t t1)
 {
     if (g.grid == 0) return hipSuccess;
     if (g.block < 64 || g.block > 256 || g.block % 64) return hipErrorInvalidValue;
-    record_kernel<R><<<g.grid, g.block, 0, s>>>(M, g.bmem);
+    EQD_GO(record_kernel<R>, g.grid, g.block, 0, s, M, g.bmem);
     return hipGetLastError();
 }
 
 template <typename R>
-hipError_t launch_thermop(const Member<R> *M, const LaunchGrid &g, hipStream_t s)
+hipError_t launch_thermop(const Member<R> *M, const LaunchGrid &g, hipStream_t s, hipEvent_t t0, hipEvent_t t1)
 {
     if (g.grid == 0) return hipSuccess;
     if (g.block < 64 || g.block > 256 || g.block % 64) return hipErrorInvalidValue;
-    thermop_kernel<R><<<g.grid, g.block, 0, s>>>(M, g.bmem);
+    EQD_GO(thermop_kernel<R>, g.grid, g.block, 0, s, M, g.bmem);
     return hipGetLastError();
 }
 
 template <typename R>
-hipError_t launch_station(const Member<R> *M, const LaunchGrid &g, hipStream_t s)
+hipError_t launch_station(const Member<R> *M, const LaunchGrid &g, hipStream_t s, hipEvent_t t0, hipEvent_t t1)
 {
     if (g.grid == 0) return hipSuccess;
     if (g.block != 64) return hipErrorInvalidValue;
-    station_kernel<R><<<g.grid, 64, 0, s>>>(M, g.bmem);
+    EQD_GO(station_kernel<R>, g.grid, 64, 0, s, M, g.bmem);
     return hipGetLastError();
 }
 
@@ -1668,15 +1683,19 @@ hipError_t launch_clock(const Member<R> *M, int nmem, hipStream_t s)
 }
 
 #define EQD_INSTANTIATE(R)                                                                                     \
-    template hipError_t launch_elem<R>(const Member<R> *, const LaunchGrid &, int, int, int, int, int, hipStream_t); \
-    template hipError_t launch_pml<R>(const Member<R> *, const LaunchGrid &, int, int, int, hipStream_t);           \
-    template hipError_t launch_node3<R>(const Member<R> *, const LaunchGrid &, hipStream_t);                        \
-    template hipError_t launch_node_pml<R>(const Member<R> *, const LaunchGrid &, hipStream_t);                     \
-    template hipError_t launch_pending<R>(const Member<R> *, const LaunchGrid &, const LaunchGrid &, hipStream_t);  \
-    template hipError_t launch_fault<R>(const Member<R> *, const LaunchGrid &, int, hipStream_t);                   \
-    template hipError_t launch_station<R>(const Member<R> *, const LaunchGrid &, hipStream_t);                      \
-    template hipError_t launch_thermop<R>(const Member<R> *, const LaunchGrid &, hipStream_t);                      \
-    template hipError_t launch_record<R>(const Member<R> *, const LaunchGrid &, hipStream_t);                       \
+    template hipError_t launch_elem<R>(const Member<R> *, const LaunchGrid &, int, int, int, int, int, hipStream_t,    \
+                                       hipEvent_t, hipEvent_t);                                                       \
+    template hipError_t launch_pml<R>(const Member<R> *, const LaunchGrid &, int, int, int, hipStream_t, hipEvent_t,  \
+                                      hipEvent_t);                                                                    \
+    template hipError_t launch_node3<R>(const Member<R> *, const LaunchGrid &, hipStream_t, hipEvent_t, hipEvent_t);  \
+    template hipError_t launch_node_pml<R>(const Member<R> *, const LaunchGrid &, hipStream_t, hipEvent_t,            \
+                                           hipEvent_t);                                                               \
+    template hipError_t launch_pending<R>(const Member<R> *, const LaunchGrid &, const LaunchGrid &, hipStream_t);    \
+    template hipError_t launch_fault<R>(const Member<R> *, const LaunchGrid &, int, hipStream_t, hipEvent_t,           \
+                                        hipEvent_t);                                                                  \
+    template hipError_t launch_station<R>(const Member<R> *, const LaunchGrid &, hipStream_t, hipEvent_t, hipEvent_t);\
+    template hipError_t launch_thermop<R>(const Member<R> *, const LaunchGrid &, hipStream_t, hipEvent_t, hipEvent_t);\
+    template hipError_t launch_record<R>(const Member<R> *, const LaunchGrid &, hipStream_t, hipEvent_t, hipEvent_t); \
     template hipError_t launch_clock<R>(const Member<R> *, int, hipStream_t);
 EQD_INSTANTIATE(double)
 EQD_INSTANTIATE(float)
