@@ -408,7 +408,8 @@ def bench_c5(args) -> int:
     from eqdyna_amd.solver import Ensemble, Solver
     steps = args.steps if args.steps is not None else 200
     kt = 20
-    total = args.warmup + steps + kt
+    kl = 40  # an untimed launched (uncaptured) pass, for comparison with the replayed steps
+    total = args.warmup + steps + kl + kt
     cases = case.c5(n=args.scenarios, steps=total)
     members = []
     counts = {"element": 0, "element_pml": 0, "node": 0, "node_pml": 0}
@@ -443,6 +444,10 @@ def bench_c5(args) -> int:
     ens.run(steps)  # graph replay; returns after the stream has drained
     wall = time.perf_counter() - t0
     ens.set_graph(False)
+    t0 = time.perf_counter()
+    ens.run(kl)
+    wall_l = time.perf_counter() - t0
+    log(f"  launched without timers: {wall_l / kl * 1e3:.4f} ms per step (replayed {wall / steps * 1e3:.4f})")
     ens.set_timing(True)
     ens.run(kt)
     ktimes = {k: ens.kernel_time(k) for k in ("element", "element_pml", "fault", "node", "node_pml", "station")}
@@ -472,7 +477,10 @@ def bench_c5(args) -> int:
                    "scenarios": len(members), "elements": ne, "nodes": nn, "fault_pairs": nf,
                    "elements_min": min(sizes), "elements_max": max(sizes),
                    "slip_weakening": sum(1 for f in fric if f == 1), "rate_and_state": sum(1 for f in fric if f >= 3),
-                   "device_gb": round(dev_gb, 1), "tuner": tuned},
+                   "device_gb": round(dev_gb, 1), "tuner": tuned,
+                   "launched_untimed": {"steps": kl, "value": round(3.0 * nn * kl / wall_l / 1e6, 2),
+                                        "ms_per_step": round(wall_l / kl * 1e3, 4),
+                                        "note": "the same ensemble step launched, not replayed, no per-kernel events"}},
         "roofline": roof, "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

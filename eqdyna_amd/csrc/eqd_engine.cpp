@@ -339,11 +339,6 @@ int capture(eqd_engine *e)
     return EQD_OK;
 }
 
-// runs of at least this many steps replay a captured step unless the graph
-// mode is set explicitly (capturing costs about as much as a few dozen eager
-// steps save)
-constexpr int32_t kAutoGraphSteps = 64;
-
 // nsteps steps of every scenario; with mirrors the handles' host clocks
 // advance and NaN velocities are reported (the tuner's trial runs do neither)
 template <typename R>
@@ -365,7 +360,7 @@ int run(eqd_engine *e, int32_t nsteps, bool mirrors)
     const LaunchGrid none{};
     for (const Group &g : e->groups[kKPend3]) HIPCHK(e, launch_pending<R>(dM, g.g, none, st));
     for (const Group &g : e->groups[kKPendPml]) HIPCHK(e, launch_pending<R>(dM, none, g.g, st));
-    const bool graph = !e->timing && (e->graph_mode > 0 || (e->graph_mode < 0 && nsteps >= kAutoGraphSteps));
+    const bool graph = e->graph_on && !e->timing;
     if (graph && !e->exec) {
         int rc = capture<R>(e);
         if (rc) return rc;
@@ -695,7 +690,7 @@ int eqd_set_graph(eqd_handle *h, int32_t on)
 {
     if (!h) return EQD_EINVAL;
     if (!h->committed) return fail(h, EQD_ESTATE, "eqd_set_graph before eqd_commit");
-    h->eng->graph_mode = on < 0 ? -1 : (on ? 1 : 0);
+    h->eng->graph_on = on != 0;
     return EQD_OK;
 }
 
@@ -716,7 +711,7 @@ int eqd_ensemble_create(eqd_handle *const *members, int32_t n, eqd_ensemble **ou
         delete E;
         return rc;
     }
-    E->e.graph_mode = 1;
+    E->e.graph_on = true;
     for (eqd_handle *h : hs) {
         (void)hipStreamSynchronize(h->stream);
         h->ensemble = &E->e;
@@ -751,7 +746,7 @@ int eqd_ensemble_set_launch(eqd_ensemble *E, int32_t kind, int32_t block, int32_
 int eqd_ensemble_set_graph(eqd_ensemble *E, int32_t on)
 {
     if (!E) return EQD_EINVAL;
-    E->e.graph_mode = on < 0 ? -1 : (on ? 1 : 0);
+    E->e.graph_on = on != 0;
     return EQD_OK;
 }
 

@@ -86,9 +86,7 @@ struct eqd_engine {
     bool built = false;
     int64_t built_version = -1;  // sum of the scenarios' versions at the last build
     bool no_harvest = false;     // set while tuning
-    // 1: replay one captured step; 0: launch every step; -1: replay for runs of
-    // kAutoGraphSteps steps or more
-    int graph_mode = -1;
+    bool graph_on = false;  // replay one captured step instead of launching every step
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     bool timing = false;

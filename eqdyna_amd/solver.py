@@ -313,11 +313,10 @@ class Solver:
         self._check(self.L.eqd_tune(self.h, int(steps), buf, len(buf)))
         return json.loads(buf.value.decode())
 
-    def set_graph(self, on) -> None:
-        """True: capture one step as a HIP graph and replay it in run(); False:
-        launch every step; None (the default): replay for runs of 64 steps or
-        more."""
-        self._check(self.L.eqd_set_graph(self.h, -1 if on is None else (1 if on else 0)))
+    def set_graph(self, on: bool) -> None:
+        """True: capture one step as a HIP graph and replay it in run(); False
+        (the default): launch every step."""
+        self._check(self.L.eqd_set_graph(self.h, 1 if on else 0))
 
     def close(self) -> None:
         if getattr(self, "h", None):
@@ -422,8 +421,8 @@ class Ensemble:
     def set_launch(self, kind: str, block: int, remap: int = 0) -> None:
         self._check(self.L.eqd_ensemble_set_launch(self.e, KINDS[kind], int(block), int(remap)))
 
-    def set_graph(self, on) -> None:
-        self._check(self.L.eqd_ensemble_set_graph(self.e, -1 if on is None else (1 if on else 0)))
+    def set_graph(self, on: bool) -> None:
+        self._check(self.L.eqd_ensemble_set_graph(self.e, 1 if on else 0))
 
     def set_timing(self, on: bool) -> None:
         self._check(self.L.eqd_ensemble_set_timing(self.e, 1 if on else 0))

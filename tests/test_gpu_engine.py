@@ -169,22 +169,3 @@ def test_element_order_does_not_change_results(name, monkeypatch):
     monkeypatch.setenv("EQD_ELEMENT_ORDER", "1")
     got = eager_run(name, "fp64", [nstep])
     assert_same(got, ref, f"{name} brick order")
-
-
-@pytest.mark.gpu
-def test_default_graph_mode_matches_eager():
-    # by default runs of 64 steps or more replay a captured step and shorter
-    # runs launch every step; both give the bits of a run that launches every step
-    bundle, sc, _ = load_golden("box_rsf4")
-    nstep = int(sc["nstep"])
-    s = Solver(bundle, sc, "fp64", nstep=nstep + 70)
-    e = Solver(bundle, sc, "fp64", nstep=nstep + 70)
-    e.set_graph(False)
-    done = 0
-    for n in (5, nstep - 5, 70):
-        s.run(n)
-        e.run(n)
-        done += n
-        assert_same(outputs(s), outputs(e), f"default mode after {done} steps")
-    s.close()
-    e.close()
