@@ -22,6 +22,18 @@ template <typename R>
 __device__ __forceinline__ R sqr(R x) { return x * x; }
 
 
+// Element forces and stresses are stored non-temporal: a force is read once,
+// by the node kernels, a stress by its own element a step later, and
+// bypassing the caches keeps the node state (gathered by the next step's
+// element kernels) resident in the Infinity Cache instead.  Measured on the
+// rough-fault box (forces): FP32 step 0.718 -> 0.690 ms, FP64 3.7 % faster,
+// node kernel 0.159 -> 0.139 ms; the stresses add 0.2-0.6 %.
+template <typename R>
+__device__ __forceinline__ void store_nt(R *p, R v)
+{
+    __builtin_nontemporal_store(v, p);
+}
+
 // ------------------------------------------------------------------------
 // Regular (type 1, 11, 12, 13) elements: calcElemKU + calcElemMass + hrglss
 // for one element per thread.
@@ -289,7 +301,7 @@ constexpr int kU1 = kKeep ? 8 : EQD_PREFETCH;
 #pragma unroll
         for (int i = 3; i < 6; i++) anestr[i] = e1 * anestr1[i] + e2 * (miuu * strain[i] * wks);
 #pragma unroll
-        for (int i = 0; i < 6; i++) E.stress[blk_at<kNst>(t, 6 + i)] = anestr[i];
+        for (int i = 0; i < 6; i++) store_nt(E.stress + blk_at<kNst>(t, 6 + i), anestr[i]);
 #pragma unroll
         for (int i = 0; i < 3; i++)
             stress[i] = (R)2 * miuu * strain[i] + (Mu - (R)2 * miuu) * vols - (R)0.5 * (anestr[i] + anestr1[i]);
@@ -329,7 +341,7 @@ constexpr int kU1 = kKeep ? 8 : EQD_PREFETCH;
         E.pstrain[t] = E.pstrain[t] + (double)pstrmag;
     }
 #pragma unroll
-    for (int i = 0; i < 6; i++) E.stress[blk_at<kNst>(t, i)] = stress[i];
+    for (int i = 0; i < 6; i++) store_nt(E.stress + blk_at<kNst>(t, i), stress[i]);
 
     // f = -m*al (calcElemMass), + (-det*w) * B^T (sigma + rdampk*sigma_dot), - hourglass
     const R temp = -det * (R)C.w;
@@ -401,7 +413,7 @@ constexpr int kU2 = kKeep ? 8 : EQD_ELEM_UNROLL;
             }
         }
 #pragma unroll
-        for (int k = 0; k < 3; k++) E.ef[blk_at<kEfReg>(t, a * 3 + k)] = f[k];
+        for (int k = 0; k < 3; k++) store_nt(E.ef + blk_at<kEfReg>(t, a * 3 + k), f[k]);
         if constexpr (!kLds && !kKeep) {
 #pragma unroll
             for (int k = 0; k < kNo; k++) opc_[k] = opn_[k];
@@ -638,7 +650,7 @@ constexpr int kU1 = kKeep ? 8 : EQD_PREFETCH;
         sc = coef[c] * grad[c] + ((R)1 / dt - dd / (R)2) * sc;
         sc = sc / ((R)1 / dt + dd / (R)2);
         sv[c] = sc;
-        E.stress[blk_at<21>(p, c)] = sc;
+        store_nt(E.stress + blk_at<21>(p, c), sc);
     }
     const R sxx = sv[0] + sv[1] + sv[2];
     const R syy = sv[3] + sv[4] + sv[5];
@@ -739,14 +751,14 @@ constexpr int kU2 = kKeep ? 8 : EQD_ELEM_UNROLL;
             // the summed stresses stored above, so the node kernel forms them
             // from dw g (3 values) bit for bit; 10-12 are stored whole
 #pragma unroll
-            for (int k = 0; k < 3; k++) E.ef[blk_at<kEfPml>(p, a * 6 + k)] = dw * (k == 0 ? g1 : (k == 1 ? g2 : g3));
+            for (int k = 0; k < 3; k++) store_nt(E.ef + blk_at<kEfPml>(p, a * 6 + k), dw * (k == 0 ? g1 : (k == 1 ? g2 : g3)));
 #pragma unroll
-            for (int k = 0; k < 3; k++) E.ef[blk_at<kEfPml>(p, a * 6 + 3 + k)] = f[9 + k] + hg[k];
+            for (int k = 0; k < 3; k++) store_nt(E.ef + blk_at<kEfPml>(p, a * 6 + 3 + k), f[9 + k] + hg[k]);
         } else {
             // a 3-component node takes the collapsed sums (src/assembleGlobalKU.f90:55-61)
-            E.ef[blk_at<kEfPml>(p, a * 6 + 0)] = f[0] + f[1] + f[2] + f[9] + hg[0];
-            E.ef[blk_at<kEfPml>(p, a * 6 + 1)] = f[3] + f[4] + f[5] + f[10] + hg[1];
-            E.ef[blk_at<kEfPml>(p, a * 6 + 2)] = f[6] + f[7] + f[8] + f[11] + hg[2];
+            store_nt(E.ef + blk_at<kEfPml>(p, a * 6 + 0), f[0] + f[1] + f[2] + f[9] + hg[0]);
+            store_nt(E.ef + blk_at<kEfPml>(p, a * 6 + 1), f[3] + f[4] + f[5] + f[10] + hg[1]);
+            store_nt(E.ef + blk_at<kEfPml>(p, a * 6 + 2), f[6] + f[7] + f[8] + f[11] + hg[2]);
         }
         if constexpr (!kLds && !kKeep) {
 #pragma unroll
