@@ -51,11 +51,13 @@ constexpr int kOpShp = 0, kOpDet = 24, kOpLam = 25, kOpMu = 26, kOpVp = 27, kOpR
               kOpB = 67, kOpBc = kOpW + 16;
 
 // value k of element t's per-element operator row of W values (AoSoA, blocks
-// of 64 elements)
+// of 64 elements), loaded non-temporal: each value is read once per step, and
+// keeping it out of the caches leaves them to the node state the element
+// kernels gather
 template <typename R, int W = kOpB>
-__host__ __device__ __forceinline__ R opb_at(const R *opb, int64_t t, int k)
+__device__ __forceinline__ R opb_at(const R *opb, int64_t t, int k)
 {
-    return opb[((t >> 6) * W + k) * 64 + (t & 63)];
+    return __builtin_nontemporal_load(&opb[((t >> 6) * W + k) * 64 + (t & 63)]);
 }
 
 // Value s of item t (element or PML node) of a per-item array of W values:

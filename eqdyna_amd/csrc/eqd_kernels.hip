@@ -34,6 +34,15 @@ __device__ __forceinline__ void store_nt(R *p, R v)
     __builtin_nontemporal_store(v, p);
 }
 
+// The element kernels' other once-per-step streams (IEN, stresses, operator
+// rows) are loaded non-temporal for the same reason: c4 FP32 step 0.688 ->
+// 0.664 ms, FP64 3.2 % faster, c2 FP32 1.1 %.
+template <typename T>
+__device__ __forceinline__ T load_nt(const T *p)
+{
+    return __builtin_nontemporal_load(p);
+}
+
 // ------------------------------------------------------------------------
 // Regular (type 1, 11, 12, 13) elements: calcElemKU + calcElemMass + hrglss
 // for one element per thread.
@@ -145,7 +154,7 @@ __global__ void __launch_bounds__(256) elem_regular_kernel(const Member<R> *__re
     // one node ahead of the arithmetic, so a gather is always in flight
     int nds[8];
 #pragma unroll
-    for (int a = 0; a < 8; a++) nds[a] = E.ien[blk_at<8>(t, a)];
+    for (int a = 0; a < 8; a++) nds[a] = load_nt(E.ien + blk_at<8>(t, a));
     // EQD_PREFETCH nodes in flight: slot a % EQD_PREFETCH holds node a's v and u
     constexpr int P = EQD_PREFETCH;
     double vq[P][3], uq[P][3];
@@ -279,7 +288,7 @@ constexpr int kU1 = kKeep ? 8 : EQD_PREFETCH;
 
     R stress[6], strdev[6];
 #pragma unroll
-    for (int i = 0; i < 6; i++) stress[i] = E.stress[blk_at<kNst>(t, i)];
+    for (int i = 0; i < 6; i++) stress[i] = load_nt(E.stress + blk_at<kNst>(t, i));
     if (!kQ) {
 #pragma unroll
         for (int i = 0; i < 6; i++) {
@@ -294,7 +303,7 @@ constexpr int kU1 = kKeep ? 8 : EQD_PREFETCH;
         const R vols = strain[0] + strain[1] + strain[2];
         R anestr1[6], anestr[6];
 #pragma unroll
-        for (int i = 0; i < 6; i++) anestr1[i] = E.stress[blk_at<kNst>(t, 6 + i)];
+        for (int i = 0; i < 6; i++) anestr1[i] = load_nt(E.stress + blk_at<kNst>(t, 6 + i));
 #pragma unroll
         for (int i = 0; i < 3; i++)
             anestr[i] = e1 * anestr1[i] + e2 * ((R)2 * miuu * strain[i] * wks + (Mu * wkp - (R)2 * miuu * wks) * vols);
@@ -486,7 +495,7 @@ __global__ void __launch_bounds__(256, (sizeof(R) == 8 ? 3 : 4)) elem_pml_kernel
     // node ids first, then each node's v and u one node ahead of the arithmetic
     int nds[8];
 #pragma unroll
-    for (int a = 0; a < 8; a++) nds[a] = E.ien[blk_at<8>(p, a)];
+    for (int a = 0; a < 8; a++) nds[a] = load_nt(E.ien + blk_at<8>(p, a));
     constexpr int P = EQD_PREFETCH;
     double vq[P][3], uq[P][3];
 #pragma unroll
@@ -646,7 +655,7 @@ constexpr int kU1 = kKeep ? 8 : EQD_PREFETCH;
 #pragma unroll
     for (int c = 0; c < 15; c++) {
         const R dd = dir[c] == 0 ? d0 : (dir[c] == 1 ? d1 : d2);
-        R sc = E.stress[blk_at<21>(p, c)];
+        R sc = load_nt(E.stress + blk_at<21>(p, c));
         sc = coef[c] * grad[c] + ((R)1 / dt - dd / (R)2) * sc;
         sc = sc / ((R)1 / dt + dd / (R)2);
         sv[c] = sc;
@@ -667,7 +676,7 @@ constexpr int kU1 = kKeep ? 8 : EQD_PREFETCH;
     E.ssum[blk_at<6>(p, 5)] = syz;
     R s0[6];
 #pragma unroll
-    for (int k = 0; k < 6; k++) s0[k] = E.stress[blk_at<21>(p, 15 + k)] + (R)C.rdampk * srt[k];
+    for (int k = 0; k < 6; k++) s0[k] = load_nt(E.stress + blk_at<21>(p, 15 + k)) + (R)C.rdampk * srt[k];
     const R dw = det * (R)C.w;
     const unsigned mask = E.mask12[p];
     R g[4][3];
