@@ -891,6 +891,81 @@ def c5_scenario(seed: int, steps: int = 200, index: int = 0) -> Case:
     return c
 
 
+def box_c5(seed: int, rsf: bool, steps: int = 60, name: str = "box_c5") -> Case:
+    """A c5 scenario drawn at fixture size: the same kinds of draw as
+    c5_scenario (box dimensions, a seeded five-layer medium, tau_0 carrying a
+    von Karman field of 10 % standard deviation, a hypocentre drawn on the
+    fault, slip weakening or slip-law rate-and-state friction with a seeded a
+    field and a velocity-strengthening rim) on a box of 16-24 x 12-16 x 9-12
+    hexahedra at 100 m with a 3-element PML, correlation length 300 m, and the
+    stress levels of box_sw / box_rsf so that the rupture starts within the
+    fixture's steps."""
+    rng = np.random.default_rng(seed)
+    d, npml = 100.0, 3
+    nxe = 2 * int(rng.integers(8, 13))    # 16..24 elements along strike
+    nye = 2 * int(rng.integers(6, 9))     # 12..16 across
+    nze = int(rng.integers(9, 13))        # 9..12 in depth
+    c = Case(name=name)
+    c.dx = c.dy = c.dz = d
+    c.nPML, c.R = npml, 0.01
+    c.enlarging_ratio = 1.0
+    c.xmax = (nxe // 2 - npml) * d
+    c.xmin = -c.xmax
+    half_y = nye // 2 - npml - 1
+    c.nuni_y_minus = c.nuni_y_plus = half_y
+    c.ymax = (half_y + 1) * d
+    c.ymin = -c.ymax
+    c.zmin, c.zmax = -(nze - npml) * d, 0.0
+    c.fxmax = c.xmax - d
+    c.fxmin = -c.fxmax
+    c.fymin = c.fymax = 0.0
+    c.fzmin, c.fzmax = c.zmin + d, 0.0
+    c.nmat, c.n2mat = 5, 4
+    c.mat = layered_medium(seed + 7, max_depth=-c.zmin)
+    c.vp, c.vs, c.rou = c.mat[-1, 1], c.mat[-1, 2], c.mat[-1, 3]
+    c.roumax, c.vmaxPML = float(c.mat[:, 3].max()), float(c.mat[:, 1].max())
+    c.dt = 0.25 * d / c.vmaxPML
+    c.term = steps * c.dt
+    c.xsource = float(np.round(rng.uniform(c.fxmin + 2 * d, c.fxmax - 2 * d), -2))
+    c.zsource = float(np.round(rng.uniform(c.fzmin + 2 * d, -2 * d), -2))
+    c.ysource = 0.0
+    field = von_karman_field(c.nfx, c.nfz, d, 300.0, 0.75, seed + 13)
+    o = c.new_ofv()
+    if rsf:
+        c.friclaw, c.tpv, c.C_nuclea = 4, 104, 1
+        c.nucR, c.nucT, c.nucdtau0 = 3.0 * d, 10.0 * c.dt, 40.0e6
+        b, L, f0, v0, fw, vw, vini, sn, tau_mean = 0.012, 0.02, 0.6, 1.0e-6, 0.1, 0.1, 1.0e-12, -60.0e6, 24.0e6
+        a_field = 0.008 + 0.008 * rng.random((c.nfz, c.nfx))
+        for ix, x in enumerate(c.fx):
+            for iz, z in enumerate(c.fz):
+                a = a_field[iz, ix]
+                if abs(x) > c.fxmax - 0.01 or z < c.fzmin + 0.01:
+                    a = 0.02
+                tau = tau_mean * (1.0 + 0.1 * field[iz, ix])
+                o[1, iz, ix], o[2, iz, ix], o[3, iz, ix] = 0.6, 0.45, 0.3
+                o[7, iz, ix], o[8, iz, ix] = sn, tau
+                o[9, iz, ix], o[10, iz, ix], o[11, iz, ix] = a, b, L
+                o[12, iz, ix], o[13, iz, ix], o[14, iz, ix], o[15, iz, ix] = v0, f0, fw, vw
+                o[46, iz, ix] = vini
+                o[20, iz, ix] = state_steady_state(a, b, L, v0, f0, tau, sn, vini, 4)
+    else:
+        c.friclaw, c.tpv, c.C_nuclea = 1, 8, 0
+        mus, mud, sn, tau_mean = 0.60, 0.45, -100.0e6, 55.0e6
+        for ix, x in enumerate(c.fx):
+            for iz, z in enumerate(c.fz):
+                o[1, iz, ix] = mus
+                if abs(abs(x) - c.fxmax) < 0.01 or abs(z - c.fzmin) < 0.01:
+                    o[1, iz, ix] = 1.0e4
+                o[2, iz, ix], o[3, iz, ix], o[7, iz, ix] = mud, 0.02, sn
+                tau = min(tau_mean * (1.0 + 0.1 * field[iz, ix]), 0.98 * mus * -sn)
+                if abs(x - c.xsource) <= 1.5 * d and abs(z - c.zsource) <= 1.5 * d:
+                    tau = 62.0e6
+                o[8, iz, ix] = tau
+    c.st_on_fault = [[c.xsource / 1e3, c.zsource / 1e3], [0.0, 0.0]]
+    c.st_off_fault = [[0.0, 2 * d / 1e3, 0.0], [c.xsource / 1e3, -2 * d / 1e3, 0.0]]
+    return c
+
+
 def c5(seed: int = 5, n: int = 64, steps: int = 200) -> List[Case]:
     """c5, the batched ensemble: n scenarios of uneven size and physics
     (c5_scenario) drawn from one seed."""

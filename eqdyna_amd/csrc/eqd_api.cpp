@@ -872,9 +872,9 @@ int eqd_commit(eqd_handle *h)
         if ((rc = dupload(h, &d, to_blocks<double>(disp, nn, 3)))) return rc; h->N.disp = d;
         if ((rc = dupload(h, &d, to_blocks<double>(acc, nn, 3)))) return rc; h->N.acc = d;
         if ((rc = dupload(h, &d, mass))) return rc; h->N.mass = d;
-        int32_t *fl;
-        if ((rc = dalloc(h, &fl, 2))) return rc;
-        HIPCHK(h, hipMemset(fl, 0, 2 * sizeof(int32_t)));
+        unsigned long long *fl;
+        if ((rc = dalloc(h, &fl, 1))) return rc;
+        HIPCHK(h, hipMemset(fl, 0xFF, sizeof(unsigned long long)));
         h->N.nanflag = fl;
         if ((rc = dupload(h, &h->d_node3_free, h->node3_free))) return rc;
         if ((rc = dupload(h, &h->d_node3_all, h->node3_all))) return rc;
@@ -1151,10 +1151,11 @@ int eqd_clock(eqd_handle *h, int32_t *nt, double *time_elapsed)
 int eqd_nan_info(eqd_handle *h, int32_t *node, int32_t *nt)
 {
     if (!h || !h->committed) return EQD_EINVAL;
-    int32_t flag[2];
-    HIPCHK(h, hipMemcpy(flag, h->N.nanflag, sizeof(flag), hipMemcpyDeviceToHost));
-    if (node) *node = flag[0];
-    if (nt) *nt = flag[1];
+    unsigned long long key;
+    HIPCHK(h, hipMemcpy(&key, h->N.nanflag, sizeof(key), hipMemcpyDeviceToHost));
+    const bool none = key == ~0ull;
+    if (node) *node = none ? 0 : (int32_t)(uint32_t)(key & 0xFFFFFFFFu);
+    if (nt) *nt = none ? 0 : (int32_t)(uint32_t)(key >> 32);
     return EQD_OK;
 }
 

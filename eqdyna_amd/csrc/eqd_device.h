@@ -40,7 +40,7 @@ struct NodeState {
     const double *mass;    // [nn] nodal mass (nodalMassArr of the node's equations)
     const int32_t *rowptr; // [nn+1]
     const uint32_t *ent;   // CSR entries
-    int32_t *nanflag;      // [2] node+1, step
+    unsigned long long *nanflag; // first NaN velocity: step << 32 | node + 1 (all ones: none)
 };
 
 // layout of one operator row: dN_a/dx_k at kOpShp + a*3+k, det, lambda, mu,

@@ -385,21 +385,33 @@ int run(eqd_engine *e, int32_t nsteps, bool mirrors)
         if (rc) return rc;
     }
     if (!mirrors) return EQD_OK;
+    // The NaN flag is read once per run, so a run goes on to its end past a
+    // NaN; the handle's clock then stops at the step of the first NaN (the
+    // reference stops there, src/driver.f90:147-152) and the device state is
+    // that of the run's end, which the caller should not continue from.
+    int bad = -1;
+    int32_t bad_node = 0, bad_step = 0;
     for (size_t m = 0; m < e->hs.size(); m++) {
         eqd_handle *h = e->hs[m];
-        h->nt += nsteps;
-        for (int32_t k = 0; k < nsteps; k++) h->time = h->time + h->s.dt;
-    }
-    for (size_t m = 0; m < e->hs.size(); m++) {
-        eqd_handle *h = e->hs[m];
-        int32_t flag[2];
-        HIPCHK(e, hipMemcpy(flag, h->N.nanflag, sizeof(flag), hipMemcpyDeviceToHost));
-        if (flag[0] != 0) {
-            if (e->hs.size() > 1)
-                return fail(e, EQD_ENAN, "scenario %d: Velocity NaN at node %d at time step %d", (int)m, flag[0],
-                            flag[1]);
-            return fail(e, EQD_ENAN, "Velocity NaN at node %d at time step %d", flag[0], flag[1]);
+        unsigned long long key;
+        HIPCHK(e, hipMemcpy(&key, h->N.nanflag, sizeof(key), hipMemcpyDeviceToHost));
+        int32_t steps = nsteps;
+        if (key != ~0ull) {
+            const int32_t node = (int32_t)(uint32_t)(key & 0xFFFFFFFFu), step = (int32_t)(uint32_t)(key >> 32);
+            steps = std::max(0, std::min(nsteps, step - h->nt));
+            if (bad < 0) {
+                bad = (int)m;
+                bad_node = node;
+                bad_step = step;
+            }
         }
+        h->nt += steps;
+        for (int32_t k = 0; k < steps; k++) h->time = h->time + h->s.dt;
+    }
+    if (bad >= 0) {
+        if (e->hs.size() > 1)
+            return fail(e, EQD_ENAN, "scenario %d: Velocity NaN at node %d at time step %d", bad, bad_node, bad_step);
+        return fail(e, EQD_ENAN, "Velocity NaN at node %d at time step %d", bad_node, bad_step);
     }
     return EQD_OK;
 }
@@ -425,7 +437,7 @@ std::vector<Region> mutable_regions(const eqd_engine *e)
         add(h->N.vel, blocked(nn, 3) * sizeof(double));
         add(h->N.disp, blocked(nn, 3) * sizeof(double));
         add(h->N.acc, blocked(nn, 3) * sizeof(double));
-        add(h->N.nanflag, 2 * sizeof(int32_t));
+        add(h->N.nanflag, sizeof(unsigned long long));
         add(e->fp32 ? (const void *)h->E32.stress : (const void *)h->E64.stress, blocked(h->n_reg, h->nreg_stress) * rs);
         add(e->fp32 ? (const void *)h->E32.pstrain : (const void *)h->E64.pstrain, (size_t)h->n_reg * sizeof(double));
         add(e->fp32 ? (const void *)h->P32.stress : (const void *)h->P64.stress, blocked(h->n_pml, 21) * rs);

@@ -814,13 +814,15 @@ __device__ __forceinline__ void gather3(const NodeState &N, int node, const R *e
     }
 }
 
+// First NaN velocity: the reference stops at the first node, in node order, of
+// the first step whose update gives a NaN (src/driver.f90:147-152).  The key
+// (step << 32 | node + 1) is lowered atomically, so the earliest step and then
+// the lowest node wins whatever thread or stream finds it first.
 __device__ __forceinline__ void flag_nan(const NodeState &N, int node, int step, double v0, double v1, double v2)
 {
     if (v0 != v0 || v1 != v1 || v2 != v2) {
-        if (N.nanflag[0] == 0) {
-            N.nanflag[0] = node + 1;
-            N.nanflag[1] = step;
-        }
+        const unsigned long long key = ((unsigned long long)(uint32_t)step << 32) | (uint32_t)(node + 1);
+        atomicMin(N.nanflag, key);
     }
 }
 

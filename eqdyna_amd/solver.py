@@ -230,6 +230,13 @@ class Solver:
         self._check(self.L.eqd_clock(self.h, C.byref(nt), C.byref(t)))
         return nt.value, t.value
 
+    def nan_info(self):
+        """(node, step) of the first NaN velocity (1-based node id; (0, 0) if none)."""
+        node = C.c_int32()
+        nt = C.c_int32()
+        self._check(self.L.eqd_nan_info(self.h, C.byref(node), C.byref(nt)))
+        return node.value, nt.value
+
     def kernel_time(self, name: str):
         ms = C.c_double()
         n = C.c_int64()

@@ -385,10 +385,44 @@ static void calc_pml_elem_ku(const eqdo_scalars *s, const eqdo_state *st, int e,
     }
 }
 
+/* Force grouping.  0 (default): the reference's, every element's stiffness
+ * force added to the nodes first (assembleGlobalKU), then every element's
+ * hourglass force mode by mode (hrglss).  1: the grouping of the project's GPU
+ * kernels, kept here only to measure what that regrouping alone changes: an
+ * element's stiffness and hourglass (C_hg = 1) forces on a node are combined
+ * before the node sums its elements -- regular elements subtract mode m's
+ * force from their stiffness force mode by mode, PML elements sum the four
+ * modes from zero and add them to components 10-12 (or to the collapsed
+ * three-component sum, itself formed before the node adds it). */
+static int g_grouping = 0;
+
+void eqdo_set_grouping(int32_t g) { g_grouping = g; }
+
+/* hourglass force of mode m (1-based) of element e on its eight nodes,
+ * fhr[a][k], as src/hrglss.f90:20-44 forms it */
+static void hourglass_mode(const eqdo_scalars *s, const eqdo_state *st, int e, int m, double fhr[8][3])
+{
+    double dl[3][8];
+    for (int i = 1; i <= 8; i++)
+        for (int j = 1; j <= 3; j++) dl[j - 1][i - 1] = DISP(j, IEN(i, e)) + s->rdampk * VEL(j, IEN(i, e));
+    double phid[3];
+    for (int i = 0; i < 3; i++) {
+        phid[i] = 0.0;
+        for (int j = 1; j <= 8; j++) phid[i] = phid[i] + PHI(j, m, e) * dl[i][j - 1];
+    }
+    for (int i = 1; i <= 8; i++) {
+        double p = PHI(i, m, e);
+        fhr[i - 1][0] = p * (SS(1, e) * phid[0] + SS(2, e) * phid[1] + SS(3, e) * phid[2]);
+        fhr[i - 1][1] = p * (SS(2, e) * phid[0] + SS(4, e) * phid[1] + SS(5, e) * phid[2]);
+        fhr[i - 1][2] = p * (SS(3, e) * phid[0] + SS(5, e) * phid[1] + SS(6, e) * phid[2]);
+    }
+}
+
 /* ---------------------------------------------------------- assembleGlobalKU
  * src/assembleGlobalKU.f90:3-67, with calcElemMass (src/calcElemMass.f90). */
 static void assemble_global_ku(const eqdo_scalars *s, eqdo_state *st)
 {
+    const int regroup = g_grouping == 1 && s->C_hg == 1;
     const double gterm = (1.0 - (double)s->C_elastic) * s->grav * (s->roumax - (s->gamar + 1.0) * s->rhow) / s->roumax;
     for (int e = 1; e <= s->ne; e++) {
         double al[24], elresf[24], vl[24], dl[24], ex[24];
@@ -410,6 +444,13 @@ static void assemble_global_ku(const eqdo_scalars *s, eqdo_state *st)
             double *stress = st->stressArr + st->stressCompIndexArr[e - 1];
             calc_elem_ku(s, st, e, vl, dl, stress, elresf, -st->eledet[e - 1], st->eleporep[e - 1], &pstrinc, ex);
             st->pstrain[e - 1] = st->pstrain[e - 1] + pstrinc;
+            if (regroup) {
+                for (int m = 1; m <= 4; m++) {
+                    double fhr[8][3];
+                    hourglass_mode(s, st, e, m, fhr);
+                    for (int i = 0; i < 24; i++) elresf[i] = elresf[i] - fhr[i / 3][i % 3];
+                }
+            }
             for (int i = 1; i <= 8; i++)
                 for (int j = 1; j <= 3; j++) {
                     int eq = EQI(ESL(IEN(i, e)) + j);
@@ -422,6 +463,31 @@ static void assemble_global_ku(const eqdo_scalars *s, eqdo_state *st)
                 for (int j = 1; j <= 3; j++) efPML[(i - 1) * 12 + 9 + j - 1] = elresf[(i - 1) * 3 + j - 1];
             double *sv = st->stressArr + st->stressCompIndexArr[e - 1];
             calc_pml_elem_ku(s, st, e, vl, efPML, sv, ex);
+            if (regroup) {
+                double hg[8][3] = {{0}}, fhr[8][3];
+                for (int m = 1; m <= 4; m++) {
+                    hourglass_mode(s, st, e, m, fhr);
+                    for (int a = 0; a < 8; a++)
+                        for (int k = 0; k < 3; k++) hg[a][k] = hg[a][k] - fhr[a][k];
+                }
+                for (int i = 1; i <= 8; i++) {
+                    int n = IEN(i, e);
+                    const double *fi = efPML + (i - 1) * 12;
+                    if (NDOF(n) == 12) {
+                        for (int j = 1; j <= 12; j++) {
+                            int eq = EQI(ESL(n) + j);
+                            double f = j <= 9 ? fi[j - 1] : fi[j - 1] + hg[i - 1][j - 10];
+                            if (eq > 0) FORCE(eq) = FORCE(eq) + f;
+                        }
+                    } else if (NDOF(n) == 3) {
+                        for (int k = 0; k < 3; k++) {
+                            int eq = EQI(ESL(n) + k + 1);
+                            FORCE(eq) = FORCE(eq) + (fi[3 * k] + fi[3 * k + 1] + fi[3 * k + 2] + fi[9 + k] + hg[i - 1][k]);
+                        }
+                    }
+                }
+                continue;
+            }
             for (int i = 1; i <= 8; i++) {
                 int n = IEN(i, e);
                 const double *fi = efPML + (i - 1) * 12;
@@ -447,6 +513,7 @@ static void assemble_global_ku(const eqdo_scalars *s, eqdo_state *st)
  * src/hrglss.f90:3-100 */
 static void hourglass(const eqdo_scalars *s, eqdo_state *st)
 {
+    if (g_grouping == 1 && s->C_hg == 1) return; /* already added per element */
     static const int fi[4][8] = {{1, 1, -1, -1, -1, -1, 1, 1}, {1, -1, -1, 1, -1, 1, 1, -1},
                                  {1, -1, 1, -1, 1, -1, 1, -1}, {1, -1, 1, -1, -1, 1, -1, 1}};
     for (int e = 1; e <= s->ne; e++) {

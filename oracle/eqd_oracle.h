@@ -65,6 +65,8 @@ typedef struct {
 /* Advance nsteps time steps.  Returns 0, or 1 when a NaN velocity appears
  * (the reference stops at src/driver.f90:147-152; err_node/err_nt say where). */
 int eqdo_run(const eqdo_scalars *s, eqdo_state *st, int32_t nsteps);
+/* 0: the reference's force grouping (default); 1: the GPU kernels' (see eqd_oracle.c) */
+void eqdo_set_grouping(int32_t g);
 
 /* Individual pieces, exposed so tests can pin them in isolation. */
 void eqdo_comdampv(const eqdo_scalars *s, double x, double y, double z, double dv[9]);

@@ -27,6 +27,7 @@ def lib():
         _lib = C.CDLL(LIB_PATH)
         _lib.eqdo_run.restype = C.c_int
         _lib.eqdo_run.argtypes = [C.c_void_p, C.c_void_p, C.c_int32]
+        _lib.eqdo_set_grouping.argtypes = [C.c_int32]
         _lib.eqdo_comdampv.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double, C.POINTER(C.c_double)]
         _lib.eqdo_qconstant.argtypes = [C.c_double, C.c_int32, C.c_double, C.POINTER(C.c_double),
                                         C.POINTER(C.c_double), C.POINTER(C.c_double)]
@@ -85,10 +86,14 @@ def make_scalars(sc: Dict[str, float], nsteps_total: int | None = None) -> Scala
     return s
 
 
-def run_oracle(bundle: Dict[str, np.ndarray], sc: Dict[str, float], nsteps: int | None = None) -> Dict[str, np.ndarray]:
+def run_oracle(bundle: Dict[str, np.ndarray], sc: Dict[str, float], nsteps: int | None = None,
+               grouping: int = 0) -> Dict[str, np.ndarray]:
     """Run ``nsteps`` steps (default: the case's nstep) from the bundle's
-    initial state; returns copies of the evolved state and station series."""
+    initial state; returns copies of the evolved state and station series.
+    ``grouping`` 1 combines each element's stiffness and hourglass forces
+    before the node sums, as the GPU kernels do (0: the reference's order)."""
     L = lib()
+    L.eqdo_set_grouping(int(grouping))
     nstep = int(sc["nstep"]) if nsteps is None else int(nsteps)
     s = make_scalars(sc, nstep)
     keep = {}

@@ -43,6 +43,8 @@ def cases():
         case.box_dip(name="box_dip"),
         case.box_rough(name="box_rough"),
         case.box_tp(name="box_tp"),
+        case.box_c5(11, rsf=False, name="box_c5sw"),
+        case.box_c5(12, rsf=True, name="box_c5rsf"),
     ]
 
 

@@ -9,7 +9,7 @@ import pytest
 from eqdyna_amd.solver import Solver
 from fixtures import channel_errors, channels, load_golden, undefined_slots
 
-ELASTIC = ["box_sw", "box_tw", "box_layered", "box_rsf3", "box_rsf4", "box_q", "box_hg2", "box_dip", "box_rough",
+ELASTIC = ["box_sw", "box_tw", "box_layered", "box_rsf3", "box_rsf4", "box_q", "box_hg2", "box_dip", "box_rough", "box_c5sw", "box_c5rsf",
            "box_tp"]
 
 
@@ -38,5 +38,33 @@ def test_fp32_error_vs_fp64_reference(name):
     ch = channel_errors(channels(out, skip), channels(ref, skip))
     worst = max(ch, key=ch.get)
     print(f"[{name}] fp32 vs fp64 reference: {e}; worst channel {worst} {ch[worst]:.2e}", flush=True)
+    assert e["rupture_time_mean_dt"] <= 0.5
+    assert e["peak_slip_rate_rel"] <= 0.01
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["box_rough", "box_rsf4"])
+def test_fp32_learned_mode_vs_fp64_reference(name):
+    # the headline fast mode exactly as the benchmark runs it: FP32 bulk plus
+    # the shipped network (eqdyna_amd/rsf_guess_c3.npz) with the faster
+    # iteration policy, k = 6 iterations before the plain-guess fallback,
+    # gate 0.5; judged against the reference's FP64 run of the same inputs
+    from eqdyna_amd import learned
+    params, _ = learned.load()
+    print(f"case {name}", flush=True)
+    bundle, sc, ref = load_golden(name)
+    s = Solver(bundle, sc, "fp32")
+    s.set_guess(params, k_fallback=6, gate=0.5)
+    s.run(int(sc["nstep"]))
+    out = s.outputs()
+    st = s.newton_stats()
+    s.close()
+    e = fp32_errors(out, ref, sc["dt"])
+    ch = channel_errors(channels(out), channels(ref))
+    worst = max(ch, key=ch.get)
+    print(f"[{name}] fp32+learned vs fp64 reference: {e}; worst channel {worst} {ch[worst]:.2e}; "
+          f"newton {st}", flush=True)
+    assert st["solves"] == int(sc["nftnd"]) * int(sc["nstep"])
+    assert e["rupture_flips"] == 0
     assert e["rupture_time_mean_dt"] <= 0.5
     assert e["peak_slip_rate_rel"] <= 0.01

@@ -36,3 +36,11 @@ def test_builder_bitwise_vs_reference_meshgen(name):
         got = np.asarray(b[k])
         assert got.shape == ref.shape, f"{name}: {k} shape {got.shape} != {ref.shape}"
         assert got.dtype == ref.dtype and np.array_equal(got, ref), f"{name}: {k} differs from the reference meshgen"
+
+
+def test_builder_bitwise_vs_reference_c1_full_size():
+    # c1 at its full size (432,000 hexahedra): every input array the
+    # reference's one-rank run handed to its time loop, by SHA-256
+    import test_gpu_size as T
+    ref, sha = T.load_size("c1")
+    T.build_inputs("c1", sha)
