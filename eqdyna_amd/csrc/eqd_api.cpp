@@ -490,6 +490,29 @@ int eqd_commit(eqd_handle *h)
     if (s.noff > 0 && !h->src.count(EQD_IDHIST)) return fail(h, EQD_EINVAL, "idhist was not set");
     if (s.C_Q == 1 && s.C_elastic == 0) return fail(h, EQD_EINVAL, "Q model can only work with elastic code");
     if (s.friclaw < 1 || s.friclaw > 5) return fail(h, EQD_EINVAL, "friclaw %d is not supported (1-5)", s.friclaw);
+    // Split-node pairs, checked before anything touches the device.  Every
+    // pair updates its two nodes in one thread, all pairs at once (the
+    // reference goes through them one after another), so a node may belong to
+    // one pair only and a pair needs two distinct nodes.
+    {
+        const int32_t *nsmp = I32(h, EQD_NSMP), *ndof = I32(h, EQD_NUMOFDOFPERNODEARR);
+        std::vector<uint8_t> seen(s.nf > 0 ? s.nn : 0, 0);
+        for (int64_t i = 0; i < s.nf; i++) {
+            if (nsmp[i * 2] == nsmp[i * 2 + 1])
+                return fail(h, EQD_EINVAL, "pair %lld: slave and master are the same node %d", (long long)i + 1,
+                            nsmp[i * 2]);
+            for (int j = 0; j < 2; j++) {
+                const int32_t nd = nsmp[i * 2 + j];
+                if (nd < 1 || nd > s.nn)
+                    return fail(h, EQD_EINVAL, "nsmp(%d,%lld) = %d out of range", j + 1, (long long)i + 1, nd);
+                if (ndof[nd - 1] != 3) return fail(h, EQD_EINVAL, "split node %d is not a 3-component node", nd);
+                if (seen[nd - 1])
+                    return fail(h, EQD_EINVAL, "node %d belongs to more than one split-node pair (pair %lld)", nd,
+                                (long long)i + 1);
+                seen[nd - 1] = 1;
+            }
+        }
+    }
     HIPCHK(h, hipSetDevice(s.device));
     // the handle's own engine (streams, launch tables); built at the first run
     if (!h->eng) {
@@ -523,14 +546,8 @@ int eqd_commit(eqd_handle *h)
     // ---------------- nodes
     std::vector<double> mass(nn, 0.0);
     std::vector<uint8_t> is_split(nn, 0);
-    for (int64_t i = 0; i < s.nf; i++) {
-        for (int j = 0; j < 2; j++) {
-            const int32_t nd = nsmp[i * 2 + j];
-            if (nd < 1 || nd > nn) return fail(h, EQD_EINVAL, "nsmp(%d,%lld) = %d out of range", j + 1, (long long)i + 1, nd);
-            if (ndof[nd - 1] != 3) return fail(h, EQD_EINVAL, "split node %d is not a 3-component node", nd);
-            is_split[nd - 1] = 1;
-        }
-    }
+    for (int64_t i = 0; i < s.nf; i++)
+        for (int j = 0; j < 2; j++) is_split[nsmp[i * 2 + j] - 1] = 1;
     h->pml_node_index.assign(nn, -1);
     std::vector<double> vel(3 * nn, 0.0), disp(3 * nn, 0.0), acc(3 * nn, 0.0);
     std::vector<double> pdamp, pv12, pacc;

@@ -34,3 +34,23 @@ def test_create_rejects_bad_scalars():
     s = solver.Scalars()
     h = ctypes.c_void_p()
     assert lib.eqd_create(ctypes.byref(s), ctypes.byref(h)) != 0  # nn = ne = 0
+
+
+@pytest.mark.parametrize("kind", ["shared", "same"])
+def test_commit_rejects_bad_split_node_pairs(kind):
+    # all pairs update their two nodes at once, so a node in two pairs, or a
+    # pair of one node, is rejected before anything touches the device
+    from fixtures import load_golden
+    bundle, sc, _ = load_golden("box_sw")
+    b = dict(bundle)
+    nsmp = b["nsmp"].copy()  # (nf, 2): slave, master
+    if kind == "shared":
+        nsmp[3, 0] = nsmp[2, 0]
+        msg = "more than one split-node pair"
+    else:
+        nsmp[3, 1] = nsmp[3, 0]
+        msg = "same node"
+    b["nsmp"] = nsmp
+    with pytest.raises(solver.EqdError) as e:
+        solver.Solver(b, sc)
+    assert e.value.code == -1 and msg in str(e.value), str(e.value)
