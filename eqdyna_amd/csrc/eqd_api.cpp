@@ -885,8 +885,27 @@ int eqd_commit(eqd_handle *h)
         int rc;
         double *d;
         h->N.nn = nn;
-        if ((rc = dupload(h, &d, to_blocks<double>(vel, nn, 3)))) return rc; h->N.vel = d;
-        if ((rc = dupload(h, &d, to_blocks<double>(disp, nn, 3)))) return rc; h->N.disp = d;
+        if (s.precision == EQD_PREC_FP32) {
+            // FP32 velocity, displacement as the pair hi + lo (eqd_device.h,
+            // NodeState); split nodes keep FP64 copies in the fault's arrays
+            std::vector<float> hi(3 * nn), lo(3 * nn);
+            for (int64_t j = 0; j < 3 * nn; j++) {
+                hi[j] = (float)disp[j];
+                lo[j] = (float)(disp[j] - (double)hi[j]);
+            }
+            float *f;
+            if ((rc = dupload(h, &f, to_blocks<float>(vel, nn, 3)))) return rc; h->N.vel32 = f;
+            if ((rc = dupload(h, &f, to_blocks<float>(hi, nn, 3)))) return rc; h->N.dhi = f;
+            if ((rc = dupload(h, &f, to_blocks<float>(lo, nn, 3)))) return rc; h->N.dlo = f;
+            std::vector<int32_t> slot(nn, -1);
+            for (int64_t i = 0; i < s.nf; i++)
+                for (int j = 0; j < 2; j++) slot[nsmp[i * 2 + j] - 1] = (int32_t)(2 * i + j);
+            int32_t *ps;
+            if ((rc = dupload(h, &ps, slot))) return rc; h->N.pair_slot = ps;
+        } else {
+            if ((rc = dupload(h, &d, to_blocks<double>(vel, nn, 3)))) return rc; h->N.vel = d;
+            if ((rc = dupload(h, &d, to_blocks<double>(disp, nn, 3)))) return rc; h->N.disp = d;
+        }
         if ((rc = dupload(h, &d, to_blocks<double>(acc, nn, 3)))) return rc; h->N.acc = d;
         if ((rc = dupload(h, &d, mass))) return rc; h->N.mass = d;
         unsigned long long *fl;
@@ -900,7 +919,12 @@ int eqd_commit(eqd_handle *h)
         int32_t *pn;
         if ((rc = dupload(h, &pn, h->pml_node_ids))) return rc; PN.node = pn;
         if ((rc = dupload(h, &d, to_blocks<double>(pdamp, npn, 3)))) return rc; PN.damp = d;
-        if ((rc = dupload(h, &d, to_blocks<double>(pv12, npn, 12)))) return rc; PN.v12 = d;
+        if (s.precision == EQD_PREC_FP32) {
+            float *f;
+            if ((rc = dupload(h, &f, to_blocks<float>(pv12, npn, 12)))) return rc; PN.v12f = f;
+        } else {
+            if ((rc = dupload(h, &d, to_blocks<double>(pv12, npn, 12)))) return rc; PN.v12 = d;
+        }
         if ((rc = dupload(h, &d, to_blocks<double>(pacc, npn, 12)))) return rc; PN.acc = d;
     }
 
@@ -942,6 +966,19 @@ int eqd_commit(eqd_handle *h)
         int rc;
         int32_t *pi;
         double *d;
+        if (s.precision == EQD_PREC_FP32 && nf > 0) {
+            // FP64 velocity and displacement of the split nodes
+            std::vector<double> sv(6 * ld, 0.0), su(6 * ld, 0.0);
+            for (int64_t i = 0; i < nf; i++)
+                for (int side = 0; side < 2; side++)
+                    for (int k = 0; k < 3; k++) {
+                        const int64_t n = fn[side * nf + i];
+                        sv[(side * 3 + k) * ld + i] = vel[k * nn + n];
+                        su[(side * 3 + k) * ld + i] = disp[k * nn + n];
+                    }
+            if ((rc = dupload(h, &d, sv))) return rc; F.sv = d;
+            if ((rc = dupload(h, &d, su))) return rc; F.su = d;
+        }
         if ((rc = dupload(h, &pi, fn))) return rc; F.nsmp = pi;
         if ((rc = dupload(h, &pi, sta))) return rc; F.station = pi;
         if ((rc = dupload(h, &d, un))) return rc; F.un = d;
@@ -1018,7 +1055,7 @@ int eqd_commit(eqd_handle *h)
     // keep what eqd_get needs to rebuild reference-layout arrays; the caller's
     // buffers are not referenced after this point
     for (int f : {EQD_STRESSARR, EQD_PSTRAIN, EQD_V1, EQD_NODALFORCEARR}) keep_copy(h, f, h->fd);
-    for (int f : {EQD_EQNUMINDEXARR, EQD_EQNUMSTARTINDEXLOC, EQD_STRESSCOMPINDEXARR}) keep_copy(h, f, h->fi);
+    for (int f : {EQD_EQNUMINDEXARR, EQD_EQNUMSTARTINDEXLOC, EQD_STRESSCOMPINDEXARR, EQD_NSMP}) keep_copy(h, f, h->fi);
     h->src.clear();
     h->committed = true;
     h->version++;
@@ -1047,6 +1084,35 @@ static int d2h(eqd_handle *h, std::vector<T> &v, const T *d, int64_t count)
     return 0;
 }
 
+// Velocities (vel) or displacements of every node in blocked layout
+// (blk_at<3>) at the precision the device keeps: the FP64 arrays, or in the
+// FP32 bulk mode the FP32 values (displacement hi + lo) with the split nodes'
+// FP64 state from the fault's arrays.
+static int node_state(eqd_handle *h, bool vel, std::vector<double> &out)
+{
+    const int64_t nn = h->s.nn;
+    int rc;
+    if (!h->N.vel32) return d2h(h, out, vel ? h->N.vel : h->N.disp, blocked_count(nn, 3));
+    std::vector<float> a, b;
+    if ((rc = d2h(h, a, vel ? h->N.vel32 : h->N.dhi, blocked_count(nn, 3)))) return rc;
+    out.assign(a.begin(), a.end());
+    if (!vel) {
+        if ((rc = d2h(h, b, h->N.dlo, blocked_count(nn, 3)))) return rc;
+        for (size_t j = 0; j < out.size(); j++) out[j] = (double)a[j] + (double)b[j];
+    }
+    const int64_t nf = h->F.nf, ld = h->F.ld;
+    if (nf > 0) {
+        std::vector<double> st;
+        if ((rc = d2h(h, st, vel ? h->F.sv : h->F.su, 6 * ld))) return rc;
+        const int32_t *nsmp = I32(h, EQD_NSMP);
+        for (int64_t i = 0; i < nf; i++)
+            for (int side = 0; side < 2; side++)
+                for (int k = 0; k < 3; k++)
+                    out[blocked_index(nsmp[i * 2 + side] - 1, k, 3)] = st[(side * 3 + k) * ld + i];
+    }
+    return EQD_OK;
+}
+
 extern "C" {
 
 int eqd_get(eqd_handle *h, int32_t field, void *host, int64_t count)
@@ -1066,7 +1132,7 @@ int eqd_get(eqd_handle *h, int32_t field, void *host, int64_t count)
     case EQD_VELARR:
     case EQD_DISPARR: {
         std::vector<double> v;
-        if ((rc = d2h(h, v, field == EQD_VELARR ? h->N.vel : h->N.disp, blocked_count(nn, 3)))) return rc;
+        if ((rc = node_state(h, field == EQD_VELARR, v))) return rc;
         for (int64_t n = 0; n < nn; n++)
             for (int k = 0; k < 3; k++) out[n * 3 + k] = v[blocked_index(n, k, 3)];
         return EQD_OK;
@@ -1076,8 +1142,18 @@ int eqd_get(eqd_handle *h, int32_t field, void *host, int64_t count)
         // v1 of 3-component nodes is velArr; of PML nodes the 12 split velocities
         const bool isv = field == EQD_V1;
         std::vector<double> v, p;
-        if ((rc = d2h(h, v, isv ? h->N.vel : h->N.acc, blocked_count(nn, 3)))) return rc;
-        if ((rc = d2h(h, p, isv ? h->PN.v12 : h->PN.acc, blocked_count(h->PN.n, 12)))) return rc;
+        if (isv) {
+            if ((rc = node_state(h, true, v))) return rc;
+        } else if ((rc = d2h(h, v, h->N.acc, blocked_count(nn, 3)))) {
+            return rc;
+        }
+        if (isv && h->PN.v12f) {
+            std::vector<float> f;
+            if ((rc = d2h(h, f, h->PN.v12f, blocked_count(h->PN.n, 12)))) return rc;
+            p.assign(f.begin(), f.end());
+        } else if ((rc = d2h(h, p, isv ? h->PN.v12 : h->PN.acc, blocked_count(h->PN.n, 12)))) {
+            return rc;
+        }
         const auto &orig = h->fd[field];  // values for equations the device does not own (fixed PML nodes)
         for (int64_t i = 0; i < s.neq; i++) out[i] = orig.empty() ? 0.0 : orig[i];
         const int32_t *eqi = I32(h, EQD_EQNUMINDEXARR), *esl = I32(h, EQD_EQNUMSTARTINDEXLOC);

@@ -32,10 +32,20 @@ struct StepConsts {
     int32_t C_elastic, C_Q, C_hg, C_nuclea, friclaw, TPV, insertFaultType, nucfault;
 };
 
+// Node state.  FP64 mode: vel and disp in FP64.  FP32 bulk mode: vel32 in
+// FP32 and the displacement as an unevaluated FP32 pair dhi + dlo (dhi the
+// displacement rounded to FP32, dlo what that rounding dropped), about 48
+// significant bits, so the accumulation u += v dt keeps FP64-like accuracy
+// while the element kernels gather 4-byte v and dhi; the split nodes of the
+// fault keep their own FP64 state (FaultState::sv, su), of which vel32, dhi
+// and dlo hold the rounded copy.
 struct NodeState {
     int64_t nn;
-    double *vel;   // 3 per node (blk_at<3>): velArr (== v1 for 3-dof nodes)
-    double *disp;  // 3 per node (blk_at<3>): dispArr
+    double *vel;   // 3 per node (blk_at<3>): velArr (== v1 for 3-dof nodes); FP64 mode
+    double *disp;  // 3 per node (blk_at<3>): dispArr; FP64 mode
+    float *vel32;  // 3 per node (blk_at<3>); FP32 bulk mode
+    float *dhi, *dlo;  // 3 per node (blk_at<3>); FP32 bulk mode
+    const int32_t *pair_slot;  // [nn] FP32 bulk mode: 2 * pair + (0 slave, 1 master) of a split node, else -1
     double *acc;   // 3 per node (blk_at<3>): pending F/m of 3-dof nodes
     const double *mass;    // [nn] nodal mass (nodalMassArr of the node's equations)
     const int32_t *rowptr; // [nn+1]
@@ -148,7 +158,8 @@ struct PmlNodes {
     int64_t n;
     const int32_t *node;  // [n]
     const double *damp;   // 3 per node (blk_at<3>): d_x, d_y, d_z at the node
-    double *v12;          // 12 per node (blk_at<12>): split velocities (v1 of the 12 equations)
+    double *v12;          // 12 per node (blk_at<12>): split velocities (v1 of the 12 equations); FP64 mode
+    float *v12f;          // the same in FP32; FP32 bulk mode
     double *acc;          // 12 per node (blk_at<12>): pending F/m
 };
 
@@ -172,6 +183,8 @@ struct FaultState {
     const double *arn;      // [nf]
     const double *fms, *fmm; // [nf] fnms of slave and master
     const double *xyz;      // [3][nf] slave coordinates
+    double *sv, *su;        // FP32 bulk mode: FP64 velocity and displacement of the pair's nodes,
+                            // [(side * 3 + k) * ld + pair], side 0 slave, 1 master (else null)
     double *fric;           // [100][ld]
     double *fnft;           // [nf]
     const int32_t *station; // [nf] on-fault station (0-based) or -1

@@ -436,15 +436,21 @@ std::vector<Region> mutable_regions(const eqd_engine *e)
         const int64_t nn = h->N.nn;
         add(h->N.vel, blocked(nn, 3) * sizeof(double));
         add(h->N.disp, blocked(nn, 3) * sizeof(double));
+        add(h->N.vel32, blocked(nn, 3) * sizeof(float));
+        add(h->N.dhi, blocked(nn, 3) * sizeof(float));
+        add(h->N.dlo, blocked(nn, 3) * sizeof(float));
         add(h->N.acc, blocked(nn, 3) * sizeof(double));
         add(h->N.nanflag, sizeof(unsigned long long));
         add(e->fp32 ? (const void *)h->E32.stress : (const void *)h->E64.stress, blocked(h->n_reg, h->nreg_stress) * rs);
         add(e->fp32 ? (const void *)h->E32.pstrain : (const void *)h->E64.pstrain, (size_t)h->n_reg * sizeof(double));
         add(e->fp32 ? (const void *)h->P32.stress : (const void *)h->P64.stress, blocked(h->n_pml, 21) * rs);
         add(h->PN.v12, blocked(h->PN.n, 12) * sizeof(double));
+        add(h->PN.v12f, blocked(h->PN.n, 12) * sizeof(float));
         add(h->PN.acc, blocked(h->PN.n, 12) * sizeof(double));
         if (h->F.nf > 0) {
             add(h->F.fric, 100 * (size_t)h->F.ld * sizeof(double));
+            add(h->F.sv, 6 * (size_t)h->F.ld * sizeof(double));
+            add(h->F.su, 6 * (size_t)h->F.ld * sizeof(double));
             add(h->F.fnft, (size_t)h->F.nf * sizeof(double));
             add(h->F.nstat, 5 * (size_t)h->F.ld * sizeof(int32_t));
             add(h->F.hist, (size_t)(h->s.non > 0 ? h->s.non : 1) * h->s.nstep * 12 * sizeof(double));

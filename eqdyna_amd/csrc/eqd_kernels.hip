@@ -43,6 +43,108 @@ __device__ __forceinline__ T load_nt(const T *p)
     return __builtin_nontemporal_load(p);
 }
 
+// Node state in the precision of the mode (R): FP64 arrays, or FP32
+// velocity and the FP32 displacement pair (eqd_device.h, NodeState).
+template <typename R>
+__device__ __forceinline__ double node_v(const NodeState &N, int node, int k)
+{
+    if constexpr (sizeof(R) == sizeof(float)) return (double)N.vel32[blk_at<3>(node, k)];
+    else return N.vel[blk_at<3>(node, k)];
+}
+
+template <typename R>
+__device__ __forceinline__ double node_u(const NodeState &N, int node, int k)
+{
+    if constexpr (sizeof(R) == sizeof(float)) return (double)N.dhi[blk_at<3>(node, k)] + (double)N.dlo[blk_at<3>(node, k)];
+    else return N.disp[blk_at<3>(node, k)];
+}
+
+// velocity and displacement of node component k as the element kernels
+// gather them: the stored values in the mode's precision (FP32 bulk: the
+// rounded displacement dhi)
+template <typename R>
+__device__ __forceinline__ R gather_v(const NodeState &N, int node, int k)
+{
+    if constexpr (sizeof(R) == sizeof(float)) return N.vel32[blk_at<3>(node, k)];
+    else return N.vel[blk_at<3>(node, k)];
+}
+
+template <typename R>
+__device__ __forceinline__ R gather_u(const NodeState &N, int node, int k)
+{
+    if constexpr (sizeof(R) == sizeof(float)) return N.dhi[blk_at<3>(node, k)];
+    else return N.disp[blk_at<3>(node, k)];
+}
+
+// store a node component's new velocity and displacement (computed in FP64)
+template <typename R>
+__device__ __forceinline__ void node_store(const NodeState &N, int node, int k, double v, double u)
+{
+    const int64_t j = blk_at<3>(node, k);
+    if constexpr (sizeof(R) == sizeof(float)) {
+        const float hi = (float)u;
+        N.vel32[j] = (float)v;
+        N.dhi[j] = hi;
+        N.dlo[j] = (float)(u - (double)hi);
+    } else {
+        N.vel[j] = v;
+        N.disp[j] = u;
+    }
+}
+
+// split velocity k of PML node q
+template <typename R>
+__device__ __forceinline__ double pml_v(const PmlNodes &P, int64_t q, int k)
+{
+    if constexpr (sizeof(R) == sizeof(float)) return (double)P.v12f[blk_at<12>(q, k)];
+    else return P.v12[blk_at<12>(q, k)];
+}
+
+template <typename R>
+__device__ __forceinline__ void pml_store(const PmlNodes &P, int64_t q, int k, double v)
+{
+    if constexpr (sizeof(R) == sizeof(float)) P.v12f[blk_at<12>(q, k)] = (float)v;
+    else P.v12[blk_at<12>(q, k)] = v;
+}
+
+// A node's state at full precision: the FP32 bulk mode keeps split nodes in
+// the fault's FP64 arrays (pair_slot says where); every other node is read
+// as node_v / node_u read it.
+template <typename R>
+__device__ __forceinline__ double exact_v(const Member<R> &M, int node, int k)
+{
+    if constexpr (sizeof(R) == sizeof(float)) {
+        const int ps = M.N.pair_slot[node];
+        if (ps >= 0) return M.F.sv[((ps & 1) * 3 + k) * M.F.ld + (ps >> 1)];
+    }
+    return node_v<R>(M.N, node, k);
+}
+
+template <typename R>
+__device__ __forceinline__ double exact_u(const Member<R> &M, int node, int k)
+{
+    if constexpr (sizeof(R) == sizeof(float)) {
+        const int ps = M.N.pair_slot[node];
+        if (ps >= 0) return M.F.su[((ps & 1) * 3 + k) * M.F.ld + (ps >> 1)];
+    }
+    return node_u<R>(M.N, node, k);
+}
+
+// store a node's new state at full precision (and its rounded copy)
+template <typename R>
+__device__ __forceinline__ void exact_store(const Member<R> &M, int node, int k, double v, double u)
+{
+    if constexpr (sizeof(R) == sizeof(float)) {
+        const int ps = M.N.pair_slot[node];
+        if (ps >= 0) {
+            const int64_t j = ((ps & 1) * 3 + k) * M.F.ld + (ps >> 1);
+            M.F.sv[j] = v;
+            M.F.su[j] = u;
+        }
+    }
+    node_store<R>(M.N, node, k, v, u);
+}
+
 // ------------------------------------------------------------------------
 // Regular (type 1, 11, 12, 13) elements: calcElemKU + calcElemMass + hrglss
 // for one element per thread.
@@ -157,20 +259,20 @@ __global__ void __launch_bounds__(256) elem_regular_kernel(const Member<R> *__re
     for (int a = 0; a < 8; a++) nds[a] = load_nt(E.ien + blk_at<8>(t, a));
     // EQD_PREFETCH nodes in flight: slot a % EQD_PREFETCH holds node a's v and u
     constexpr int P = EQD_PREFETCH;
-    double vq[P][3], uq[P][3];
+    R vq[P][3], uq[P][3];
 #pragma unroll
     for (int b = 0; b < P; b++)
 #pragma unroll
         for (int k = 0; k < 3; k++) {
-            vq[b][k] = N.vel[blk_at<3>(nds[b], k)];
-            uq[b][k] = N.disp[blk_at<3>(nds[b], k)];
+            vq[b][k] = gather_v<R>(N, nds[b], k);
+            uq[b][k] = gather_u<R>(N, nds[b], k);
         }
-    // FP32 bulk: velocities and displacements relative to the first node
-    // (formed in FP64).  Strain, strain rate and the hourglass modes are
-    // blind to a common translation (sum_a dN_a/dx = 0, Phi orthogonal to
-    // constants), so only the rounding changes, and it is now relative to the
+    // FP32 bulk: velocities and displacements relative to the first node.
+    // Strain, strain rate and the hourglass modes are blind to a common
+    // translation (sum_a dN_a/dx = 0, Phi orthogonal to constants), so only
+    // the rounding changes: the arithmetic then rounds relative to the
     // deformation rather than to the absolute motion.
-    double v0[3] = {0, 0, 0}, u0[3] = {0, 0, 0};
+    R v0[3] = {0, 0, 0}, u0[3] = {0, 0, 0};
     if constexpr (kF32) {
 #pragma unroll
         for (int k = 0; k < 3; k++) {
@@ -220,15 +322,15 @@ constexpr int kU1 = kKeep ? 8 : EQD_PREFETCH;
         R vl[3], dl[3];
 #pragma unroll
         for (int k = 0; k < 3; k++) {
-            vl[k] = (R)(vq[b][k] - v0[k]);
-            dl[k] = (R)(uq[b][k] - u0[k]);
+            vl[k] = vq[b][k] - v0[k];
+            dl[k] = uq[b][k] - u0[k];
         }
         if (a + P < 8) {
             const int nx = nds[a + P];
 #pragma unroll
             for (int k = 0; k < 3; k++) {
-                vq[b][k] = N.vel[blk_at<3>(nx, k)];
-                uq[b][k] = N.disp[blk_at<3>(nx, k)];
+                vq[b][k] = gather_v<R>(N, nx, k);
+                uq[b][k] = gather_u<R>(N, nx, k);
             }
         }
         if constexpr (kLds) {
@@ -394,7 +496,7 @@ constexpr int kU2 = kKeep ? 8 : EQD_ELEM_UNROLL;
 #pragma unroll
             for (int k = 0; k < 3; k++) {
                 // absolute velocity here: the Rayleigh mass term is not translation invariant
-                R al = (R)(C.rdampm * N.vel[blk_at<3>(nd, k)]);
+                R al = (R)(C.rdampm * node_v<R>(N, nd, k));
                 if (k == 2) al = al + (R)C.gterm;
                 f[k] = (R)0 - al * E.emass[blk_at<24>(t, a * 3 + k)];
             }
@@ -497,16 +599,16 @@ __global__ void __launch_bounds__(256, (sizeof(R) == 8 ? 3 : 4)) elem_pml_kernel
 #pragma unroll
     for (int a = 0; a < 8; a++) nds[a] = load_nt(E.ien + blk_at<8>(p, a));
     constexpr int P = EQD_PREFETCH;
-    double vq[P][3], uq[P][3];
+    R vq[P][3], uq[P][3];
 #pragma unroll
     for (int b = 0; b < P; b++)
 #pragma unroll
         for (int k = 0; k < 3; k++) {
-            vq[b][k] = N.vel[blk_at<3>(nds[b], k)];
-            uq[b][k] = N.disp[blk_at<3>(nds[b], k)];
+            vq[b][k] = gather_v<R>(N, nds[b], k);
+            uq[b][k] = gather_u<R>(N, nds[b], k);
         }
     // velocities (relative to node 1 in the FP32 variant, see elem_regular_kernel)
-    double v0[3] = {0.0, 0.0, 0.0}, u0[3] = {0.0, 0.0, 0.0};
+    R v0[3] = {0, 0, 0}, u0[3] = {0, 0, 0};
     if (!kF64) {
 #pragma unroll
         for (int k = 0; k < 3; k++) {
@@ -579,22 +681,22 @@ constexpr int kU1 = kKeep ? 8 : EQD_PREFETCH;
         R va[3];
         R dlh[3];
 #pragma unroll
-        for (int k = 0; k < 3; k++) va[k] = (R)(vq[b][k] - v0[k]);
+        for (int k = 0; k < 3; k++) va[k] = vq[b][k] - v0[k];
         if (kHG == 1) {
 #pragma unroll
             for (int k = 0; k < 3; k++) {
                 if (kF64)
                     dlh[k] = (R)(uq[b][k] + C.rdampk * (double)va[k]);
                 else
-                    dlh[k] = (R)((uq[b][k] - u0[k]) + C.rdampk * (vq[b][k] - v0[k]));
+                    dlh[k] = (R)((double)(uq[b][k] - u0[k]) + C.rdampk * (double)va[k]);
             }
         }
         if (a + P < 8) {
             const int nx = nds[a + P];
 #pragma unroll
             for (int k = 0; k < 3; k++) {
-                vq[b][k] = N.vel[blk_at<3>(nx, k)];
-                if (kHG == 1) uq[b][k] = N.disp[blk_at<3>(nx, k)];
+                vq[b][k] = gather_v<R>(N, nx, k);
+                if (kHG == 1) uq[b][k] = gather_u<R>(N, nx, k);
             }
         }
         const R n1 = opc_[0], n2 = opc_[1], n3 = opc_[2];
@@ -735,7 +837,7 @@ constexpr int kU2 = kKeep ? 8 : EQD_ELEM_UNROLL;
             for (int k = 0; k < 9; k++) f[k] = 0;
 #pragma unroll
             for (int k = 0; k < 3; k++) {
-                R al = (R)(C.rdampm * N.vel[blk_at<3>(nd, k)]);
+                R al = (R)(C.rdampm * node_v<R>(N, nd, k));
                 if (k == 2) al = al + (R)C.gterm;
                 f[9 + k] = (R)0 - al * E.emass[blk_at<24>(p, a * 3 + k)];
             }
@@ -856,9 +958,8 @@ __global__ void __launch_bounds__(256) node3_kernel(const Member<R> *__restrict_
 #pragma unroll
     for (int k = 0; k < 3; k++) {
         const double a = F[k] / m;
-        const double v = N.vel[blk_at<3>(node, k)] + a * dt;
-        N.vel[blk_at<3>(node, k)] = v;
-        N.disp[blk_at<3>(node, k)] = N.disp[blk_at<3>(node, k)] + v * dt;
+        const double v = node_v<R>(N, node, k) + a * dt;
+        node_store<R>(N, node, k, v, node_u<R>(N, node, k) + v * dt);
         vn[k] = v;
     }
     flag_nan(N, node, next_step, vn[0], vn[1], vn[2]);
@@ -936,18 +1037,17 @@ __global__ void __launch_bounds__(256) node_pml_kernel(const Member<R> *__restri
 #pragma unroll
     for (int k = 0; k < 9; k++) {
         const double dv = d[k % 3];
-        v[k] = (F[k] / m + P.v12[blk_at<12>(q, k)] * (1.0 / dt - dv / 2.0)) / (1.0 / dt + dv / 2.0);
+        v[k] = (F[k] / m + pml_v<R>(P, q, k) * (1.0 / dt - dv / 2.0)) / (1.0 / dt + dv / 2.0);
     }
 #pragma unroll
-    for (int k = 9; k < 12; k++) v[k] = P.v12[blk_at<12>(q, k)] + F[k] / m * dt;
+    for (int k = 9; k < 12; k++) v[k] = pml_v<R>(P, q, k) + F[k] / m * dt;
 #pragma unroll
-    for (int k = 0; k < 12; k++) P.v12[blk_at<12>(q, k)] = v[k];
+    for (int k = 0; k < 12; k++) pml_store<R>(P, q, k, v[k]);
     double vel[3];
 #pragma unroll
     for (int k = 0; k < 3; k++) {
         vel[k] = v[3 * k] + v[3 * k + 1] + v[3 * k + 2] + v[9 + k];
-        N.vel[blk_at<3>(node, k)] = vel[k];
-        N.disp[blk_at<3>(node, k)] = N.disp[blk_at<3>(node, k)] + vel[k] * dt;
+        node_store<R>(N, node, k, vel[k], node_u<R>(N, node, k) + vel[k] * dt);
     }
     flag_nan(N, node, next_step, vel[0], vel[1], vel[2]);
 }
@@ -970,9 +1070,8 @@ __global__ void __launch_bounds__(256) update3_from_acc_kernel(const Member<R> *
     double vn[3];
 #pragma unroll
     for (int k = 0; k < 3; k++) {
-        const double v = N.vel[blk_at<3>(node, k)] + N.acc[blk_at<3>(node, k)] * dt;
-        N.vel[blk_at<3>(node, k)] = v;
-        N.disp[blk_at<3>(node, k)] = N.disp[blk_at<3>(node, k)] + v * dt;
+        const double v = exact_v(M, node, k) + N.acc[blk_at<3>(node, k)] * dt;
+        exact_store(M, node, k, v, exact_u(M, node, k) + v * dt);
         vn[k] = v;
     }
     flag_nan(N, node, next_step, vn[0], vn[1], vn[2]);
@@ -997,18 +1096,17 @@ __global__ void __launch_bounds__(256) update_pml_from_acc_kernel(const Member<R
 #pragma unroll
     for (int k = 0; k < 9; k++) {
         const double dv = d[k % 3];
-        v[k] = (P.acc[blk_at<12>(q, k)] + P.v12[blk_at<12>(q, k)] * (1.0 / dt - dv / 2.0)) / (1.0 / dt + dv / 2.0);
+        v[k] = (P.acc[blk_at<12>(q, k)] + pml_v<R>(P, q, k) * (1.0 / dt - dv / 2.0)) / (1.0 / dt + dv / 2.0);
     }
 #pragma unroll
-    for (int k = 9; k < 12; k++) v[k] = P.v12[blk_at<12>(q, k)] + P.acc[blk_at<12>(q, k)] * dt;
+    for (int k = 9; k < 12; k++) v[k] = pml_v<R>(P, q, k) + P.acc[blk_at<12>(q, k)] * dt;
 #pragma unroll
-    for (int k = 0; k < 12; k++) P.v12[blk_at<12>(q, k)] = v[k];
+    for (int k = 0; k < 12; k++) pml_store<R>(P, q, k, v[k]);
     double vel[3];
 #pragma unroll
     for (int k = 0; k < 3; k++) {
         vel[k] = v[3 * k] + v[3 * k + 1] + v[3 * k + 2] + v[9 + k];
-        N.vel[blk_at<3>(node, k)] = vel[k];
-        N.disp[blk_at<3>(node, k)] = N.disp[blk_at<3>(node, k)] + vel[k] * dt;
+        node_store<R>(N, node, k, vel[k], node_u<R>(N, node, k) + vel[k] * dt);
     }
     flag_nan(N, node, next_step, vel[0], vel[1], vel[2]);
 }
@@ -1109,6 +1207,17 @@ __global__ void __launch_bounds__(128) fault_kernel(const Member<R> *__restrict_
     auto US = [&](int j) { return Fs.us[j * nf + i]; };
     auto UD = [&](int j) { return Fs.ud[j * nf + i]; };
     const double arn = Fs.arn[i];
+    // velocity and displacement of the pair's nodes (side 0 slave, 1 master)
+    // at full precision: the FP32 bulk mode keeps them in the fault's FP64
+    // arrays, the FP64 mode in the node arrays
+    auto PV = [&](int side, int node, int k) -> double {
+        if constexpr (sizeof(R) == sizeof(float)) return Fs.sv[(side * 3 + k) * ld + i];
+        else return N.vel[blk_at<3>(node, k)];
+    };
+    auto PU = [&](int side, int node, int k) -> double {
+        if constexpr (sizeof(R) == sizeof(float)) return Fs.su[(side * 3 + k) * ld + i];
+        else return N.disp[blk_at<3>(node, k)];
+    };
     double Fsl[3], Fma[3];
     gather3<R>(N, ns, ef, efp, Fsl);
     gather3<R>(N, nm, ef, efp, Fma);
@@ -1119,10 +1228,10 @@ __global__ void __launch_bounds__(128) fault_kernel(const Member<R> *__restrict_
     for (int k = 0; k < 3; k++) {
         q[k][0][0] = Fsl[k];
         q[k][1][0] = Fma[k];
-        q[k][0][1] = N.vel[blk_at<3>(ns, k)];
-        q[k][1][1] = N.vel[blk_at<3>(nm, k)];
-        q[k][0][2] = N.disp[blk_at<3>(ns, k)];
-        q[k][1][2] = N.disp[blk_at<3>(nm, k)];
+        q[k][0][1] = PV(0, ns, k);
+        q[k][1][1] = PV(1, nm, k);
+        q[k][0][2] = PU(0, ns, k);
+        q[k][1][2] = PU(1, nm, k);
     }
     for (int j = 0; j < 3; j++)
         for (int s = 0; s < 2; s++) {
@@ -1381,8 +1490,8 @@ __global__ void __launch_bounds__(128) fault_kernel(const Member<R> *__restrict_
         for (int j = 0; j < 3; j++) {
             Fsl[j] = (-xyzAcc[j] + xyzR[j] / mm) * mr;
             Fma[j] = (xyzAcc[j] + xyzR[j] / ms) * mr;
-            FR(31 + j) = N.vel[blk_at<3>(nm, j)] + (xyzAcc[j] + xyzR[j] / ms) * dt;
-            FR(34 + j) = N.vel[blk_at<3>(ns, j)] + (-xyzAcc[j] + xyzR[j] / mm) * dt;
+            FR(31 + j) = PV(1, nm, j) + (xyzAcc[j] + xyzR[j] / ms) * dt;
+            FR(34 + j) = PV(0, ns, j) + (-xyzAcc[j] + xyzR[j] / mm) * dt;
         }
     }
     if constexpr (kLearn) {
@@ -1427,14 +1536,19 @@ __global__ void __launch_bounds__(128) fault_kernel(const Member<R> *__restrict_
     }
     double vs[3], vm[3];
     for (int k = 0; k < 3; k++) {
-        const double vs0 = N.vel[blk_at<3>(ns, k)], vm0 = N.vel[blk_at<3>(nm, k)];
-        const double us0 = N.disp[blk_at<3>(ns, k)], um0 = N.disp[blk_at<3>(nm, k)];
+        const double vs0 = PV(0, ns, k), vm0 = PV(1, nm, k);
+        const double us0 = PU(0, ns, k), um0 = PU(1, nm, k);
         vs[k] = vs0 + Fsl[k] / msl * dt;
         vm[k] = vm0 + Fma[k] / mma * dt;
-        N.vel[blk_at<3>(ns, k)] = vs[k];
-        N.vel[blk_at<3>(nm, k)] = vm[k];
-        N.disp[blk_at<3>(ns, k)] = us0 + vs[k] * dt;
-        N.disp[blk_at<3>(nm, k)] = um0 + vm[k] * dt;
+        const double us1 = us0 + vs[k] * dt, um1 = um0 + vm[k] * dt;
+        if constexpr (sizeof(R) == sizeof(float)) {
+            Fs.sv[k * ld + i] = vs[k];
+            Fs.sv[(3 + k) * ld + i] = vm[k];
+            Fs.su[k * ld + i] = us1;
+            Fs.su[(3 + k) * ld + i] = um1;
+        }
+        node_store<R>(N, ns, k, vs[k], us1);
+        node_store<R>(N, nm, k, vm[k], um1);
     }
     flag_nan(N, ns, nt + 1, vs[0], vs[1], vs[2]);
     flag_nan(N, nm, nt + 1, vm[0], vm[1], vm[2]);
@@ -1461,9 +1575,9 @@ __global__ void __launch_bounds__(64) station_kernel(const Member<R> *__restrict
     const int i = r - 1;
     const int node = S.idh[i], k = S.idh[S.nrow + i], kind = S.idh[2 * S.nrow + i];
     if (kind == 1)
-        col[r] = N.disp[blk_at<3>(node, k)];
+        col[r] = exact_u(M, node, k);
     else if (kind == 2)
-        col[r] = N.vel[blk_at<3>(node, k)];
+        col[r] = exact_v(M, node, k);
 }
 
 // thermop, src/thermop.f90:1-40: pore pressure change fric(51) and
@@ -1533,7 +1647,7 @@ __global__ void __launch_bounds__(256) record_kernel(const Member<R> *__restrict
     const int node = rc.nodes[t];
     const int64_t nn = M.N.nn;
 #pragma unroll
-    for (int k = 0; k < 3; k++) rc.vel[(slot * rc.n_nodes + t) * 3 + k] = M.N.vel[blk_at<3>(node, k)];
+    for (int k = 0; k < 3; k++) rc.vel[(slot * rc.n_nodes + t) * 3 + k] = exact_v(M, node, k);
 }
 
 // The step clock: nt += 1 and time += dt for every scenario of the launch,
