@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "eqd_internal.h"
+#include "eqd_precompute.h"
 
 using namespace eqd;
 
@@ -292,10 +293,45 @@ static bool fit_phi_compact(const double shp[24], const double phi[32], double c
 // kernels' precision and order (phi_compact), so both layouts compute the same
 // numbers.  Otherwise (or with EQD_FULL_PHI=1) rows of W = kOpB values carry
 // the given Phi.  shp and phi are the class's [24][n] and [32][n] arrays.
-template <typename R, typename E, typename OpRow>
+// The geometry row of one hexahedron (kOpG values: the coordinates of local
+// nodes 1..7 relative to node 0, lambda, mu, Vp, rho), if its given
+// operators are those of its coordinates: the one-time precompute
+// (eqd_precompute.h) repeated here from xl must give dN/dx, det, SS and Phi
+// within 1e-12 of each quantity's size.
+template <typename R>
+static bool geometry_row(const double xl[3][8], double lam, double mu, double vp, double rho, double w,
+                         const double *shp, double det, const double *ss, const double *phi, R row[kOpG])
+{
+    double sh[24], dt, em[24], s6[6], ph[32];
+    if (!eqd::elem_precompute(xl, false, rho, lam, mu, w, sh, &dt, em, s6, ph)) return false;
+    auto close = [](const double *a, const double *b, int k) {
+        double m = 0, d = 0;
+        for (int i = 0; i < k; i++) {
+            m = std::max(m, std::fabs(b[i]));
+            d = std::max(d, std::fabs(a[i] - b[i]));
+        }
+        return d <= 1e-12 * m;
+    };
+    if (!close(sh, shp, 24) || !close(&dt, &det, 1) || !close(s6, ss, 6) || !close(ph, phi, 32)) return false;
+    for (int a = 1; a < 8; a++)
+        for (int j = 0; j < 3; j++) row[(a - 1) * 3 + j] = (R)(xl[j][a] - xl[j][0]);
+    row[kGeoMat] = (R)lam;
+    row[kGeoMat + 1] = (R)mu;
+    row[kGeoMat + 2] = (R)vp;
+    row[kGeoMat + 3] = (R)rho;
+    return true;
+}
+
+//
+// Geometry rows (FP32 bulk mode, allow_geom): where neither class tables nor
+// anything shorter applies, and geomrow(t, row) confirms that the element's
+// given operators are those of its corner coordinates and fills its row
+// (kOpG values), the kernels rebuild the operators from the coordinates
+// (kOpsGeom): 25 values per element instead of 51.
+template <typename R, typename E, typename OpRow, typename GeomRow>
 static int upload_operators(eqd_handle *h, E &X, int64_t n, bool no_classes, OpRow oprow,
                             const std::vector<double> &shp, const std::vector<double> &phi, int64_t *n_op_classes,
-                            int64_t *n_phi_classes, int32_t *compact)
+                            int64_t *n_phi_classes, int32_t *compact, bool allow_geom, GeomRow geomrow)
 {
     std::vector<R> cmp;
     int ok = 1;
@@ -346,8 +382,35 @@ static int upload_operators(eqd_handle *h, E &X, int64_t n, bool no_classes, OpR
         *n_phi_classes = X.n_phic;
         return 0;
     }
-    const int W = ok ? kOpBc : kOpB;
     const int64_t nb = (n + 63) / 64;
+    X.opb_geom = 0;
+    if (allow_geom && ok) {
+        std::vector<R> g((size_t)(nb * 64 * kOpG), (R)0);
+        int gok = 1;
+#pragma omp parallel for schedule(static) reduction(min : gok)
+        for (int64_t t = 0; t < n; t++) {
+            R row[kOpG];
+            if (!geomrow(t, row)) {
+                gok = 0;
+                continue;
+            }
+            R *dst = g.data() + (t / 64) * kOpG * 64 + (t % 64);
+            for (int k = 0; k < kOpG; k++) dst[k * 64] = row[k];
+        }
+        if (gok) {
+            R *pb;
+            if ((rc = dupload(h, &pb, g))) return rc;
+            X.opb = pb;
+            X.opb_compact = 1;
+            X.opb_geom = 1;
+            X.optab = nullptr;
+            X.phitab = nullptr;
+            *n_op_classes = 0;
+            *n_phi_classes = 0;
+            return 0;
+        }
+    }
+    const int W = ok ? kOpBc : kOpB;
     std::vector<R> b((size_t)(nb * 64 * W), (R)0);
 #pragma omp parallel for schedule(static)
     for (int64_t t = 0; t < n; t++) {
@@ -643,6 +706,14 @@ int eqd_commit(eqd_handle *h)
     const bool fp32 = s.precision == EQD_PREC_FP32;
     const char *nc_env = std::getenv("EQD_NO_OPERATOR_CLASSES");
     const bool no_classes = nc_env && nc_env[0] == '1';
+    // geometry rows: FP32 bulk mode, hexahedra only (EQD_GEOM_OPS=0 keeps the
+    // stored operators, 2 rebuilds the regular elements' only)
+    const char *go_env = std::getenv("EQD_GEOM_OPS");
+    const bool geom_ok = fp32 && !(go_env && go_env[0] == '0');
+    const bool geom_pml = geom_ok && !(go_env && go_env[0] == '2');
+    bool reg_wedges = false;
+    for (int32_t e : h->reg_ids)
+        if (etype[e] == 11 || etype[e] == 12) reg_wedges = true;
 
     // regular class SoA
     {
@@ -739,7 +810,17 @@ int eqd_commit(eqd_handle *h)
                     row[kOpVp] = (R)r_vp[t];
                     row[kOpRho] = (R)r_rho[t];
                     for (int i = 0; i < 6; i++) row[kOpSs + i] = (R)r_ss[i * nr + t];
-                }, r_shp, r_phi, &h->op_classes_reg, &h->phi_classes_reg, &h->phi_compact_reg)))
+                }, r_shp, r_phi, &h->op_classes_reg, &h->phi_classes_reg, &h->phi_compact_reg,
+                geom_ok && !reg_wedges, [&](int64_t t, R *row) {
+                    const int64_t e = h->reg_ids[t];
+                    double xl[3][8], sh[24], s6[6], ph[32];
+                    for (int a = 0; a < 8; a++)
+                        for (int j = 0; j < 3; j++) xl[j][a] = coor[(int64_t)(ien[e * 8 + a] - 1) * 3 + j];
+                    for (int i = 0; i < 24; i++) sh[i] = r_shp[i * nr + t];
+                    for (int i = 0; i < 6; i++) s6[i] = r_ss[i * nr + t];
+                    for (int i = 0; i < 32; i++) ph[i] = r_phi[i * nr + t];
+                    return geometry_row<R>(xl, r_lam[t], r_mu[t], r_vp[t], r_rho[t], s.w, sh, r_det[t], s6, ph, row);
+                })))
                 return r2;
             if (s.rdampm != 0.0 || s.C_elastic == 0) {
                 if ((r2 = up(&E.emass, r_em, 24))) return r2;
@@ -829,7 +910,17 @@ int eqd_commit(eqd_handle *h)
                     row[kOpVp] = (R)p_vp[p];
                     row[kOpRho] = (R)p_rho[p];
                     for (int i = 0; i < 6; i++) row[kOpSs + i] = (R)p_ss[i * np + p];
-                }, p_shp, p_phi, &h->op_classes_pml, &h->phi_classes_pml, &h->phi_compact_pml)))
+                }, p_shp, p_phi, &h->op_classes_pml, &h->phi_classes_pml, &h->phi_compact_pml,
+                geom_pml, [&](int64_t p, R *row) {
+                    const int64_t e = h->pml_ids[p];
+                    double xl[3][8], sh[24], s6[6], ph[32];
+                    for (int a = 0; a < 8; a++)
+                        for (int j = 0; j < 3; j++) xl[j][a] = coor[(int64_t)(ien[e * 8 + a] - 1) * 3 + j];
+                    for (int i = 0; i < 24; i++) sh[i] = p_shp[i * np + p];
+                    for (int i = 0; i < 6; i++) s6[i] = p_ss[i * np + p];
+                    for (int i = 0; i < 32; i++) ph[i] = p_phi[i * np + p];
+                    return geometry_row<R>(xl, p_lam[p], p_mu[p], p_vp[p], p_rho[p], s.w, sh, p_det[p], s6, ph, row);
+                })))
                 return r2;
             if (s.rdampm != 0.0 || s.C_elastic == 0) {
                 if ((r2 = up(&P.emass, p_em, 24))) return r2;
@@ -1472,6 +1563,8 @@ int eqd_layout_info(eqd_handle *h, const char *key, int64_t *value)
     else if (k == "phi_classes_pml") *value = h->phi_classes_pml;
     else if (k == "phi_compact_regular") *value = h->phi_compact_reg;
     else if (k == "phi_compact_pml") *value = h->phi_compact_pml;
+    else if (k == "geom_ops_regular") *value = h->s.precision == EQD_PREC_FP32 ? h->E32.opb_geom : 0;
+    else if (k == "geom_ops_pml") *value = h->s.precision == EQD_PREC_FP32 ? h->P32.opb_geom : 0;
     else return fail(h, EQD_EINVAL, "unknown layout key %s", key);
     return EQD_OK;
 }

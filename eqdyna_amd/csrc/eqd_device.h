@@ -59,6 +59,11 @@ struct NodeState {
 // operators instead (phi_compact)
 constexpr int kOpShp = 0, kOpDet = 24, kOpLam = 25, kOpMu = 26, kOpVp = 27, kOpRho = 28, kOpSs = 29, kOpW = 35,
               kOpB = 67, kOpBc = kOpW + 16;
+// Geometry rows (kOpsGeom, FP32 bulk mode on meshes whose elements all
+// differ): the coordinates of local nodes 1..7 relative to node 0 at
+// (a - 1) * 3 + j, then lambda, mu, Vp, rho at kOpG - 4 ..; the kernels
+// rebuild dN/dx, det, SS and the compact hourglass operators from them.
+constexpr int kOpG = 25, kGeoMat = 21;
 
 // value k of element t's per-element operator row of W values (AoSoA, blocks
 // of 64 elements), loaded non-temporal: each value is read once per step, and
@@ -83,8 +88,8 @@ constexpr int kEfReg = 24, kEfPml = 48;
 
 // Where an element kernel finds its operators: class tables in global memory
 // or per-element rows with Phi; class tables staged in LDS; per-element rows
-// with the compact hourglass operators.
-constexpr int kOpsRows = 0, kOpsLds = 1, kOpsCompact = 2;
+// with the compact hourglass operators; per-element geometry rows.
+constexpr int kOpsRows = 0, kOpsLds = 1, kOpsCompact = 2, kOpsGeom = 3;
 
 // Hourglass base vectors Gamma_m (ha of calcSSPhi4Hrgls,
 // src/assembleGlobalMass.f90:332-335): bit a set where Gamma_ma = -1.
@@ -129,6 +134,7 @@ struct RegElems {
     int32_t n_opc, n_phic;  // rows of optab and phitab
     const R *opb;
     int32_t opb_compact;
+    int32_t opb_geom;  // opb holds geometry rows (kOpG values)
 };
 
 template <typename R>
@@ -152,6 +158,7 @@ struct PmlElems {
     int32_t n_opc, n_phic;
     const R *opb;
     int32_t opb_compact;
+    int32_t opb_geom;
 };
 
 struct PmlNodes {

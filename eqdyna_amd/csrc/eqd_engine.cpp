@@ -89,6 +89,7 @@ template <typename X>
 int ops_of(const X &E, size_t lds)
 {
     if (lds) return kOpsLds;
+    if (E.opb && E.opb_geom) return kOpsGeom;
     return E.opb && E.opb_compact ? kOpsCompact : kOpsRows;
 }
 

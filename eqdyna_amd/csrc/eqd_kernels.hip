@@ -92,6 +92,121 @@ __device__ __forceinline__ void node_store(const NodeState &N, int node, int k, 
     }
 }
 
+// ------------------------------------------------------------------------
+// Operators of one hexahedron rebuilt from its corner coordinates (kOpsGeom):
+// the quantities of the one-time precompute (calcGlobalShapeFunc, vlm and
+// calcSSPhi4Hrgls; src/calcGlobalShapeFunc.f90:1-75, src/library.f90:60-93,
+// src/assembleGlobalMass.f90:328-374) in the kernel's precision, from the
+// coordinates of local nodes 1..7 relative to node 0, so the values are
+// rounded relative to the element's size.  dN/dx at the centre, the Jacobian
+// determinant, the hourglass stiffness SS and the compact hourglass
+// operators (1/s_m, h_m/s_m; phi_compact).
+template <typename R>
+struct GeoOps {
+    R shp[24];
+    R det;
+    R ss[6];
+    R cph[16];
+};
+
+template <typename R>
+__device__ __forceinline__ void geo_ops(const R *opb, int64_t t, R lam, R mu, GeoOps<R> &G)
+{
+    // sign of the trilinear shape function of node a in direction i at the centre
+    constexpr int ac[8][3] = {{-1, -1, -1}, {1, -1, -1}, {1, 1, -1}, {-1, 1, -1},
+                              {-1, -1, 1},  {1, -1, 1},  {1, 1, 1},  {-1, 1, 1}};
+    R x[3][8];
+#pragma unroll
+    for (int j = 0; j < 3; j++) x[j][0] = 0;
+#pragma unroll
+    for (int a = 1; a < 8; a++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) x[j][a] = opb_at<R, kOpG>(opb, t, (a - 1) * 3 + j);
+    // xs[j][i] = d x_j / d xi_i = sum_a ac[a][i] x_j(a) / 8
+    R xs[3][3];
+#pragma unroll
+    for (int j = 0; j < 3; j++)
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            R s = 0;
+#pragma unroll
+            for (int a = 1; a < 8; a++) s = ac[a][i] > 0 ? s + x[j][a] : s - x[j][a];
+            xs[j][i] = s * (R)0.125;
+        }
+    const R c11 = xs[1][1] * xs[2][2] - xs[1][2] * xs[2][1];
+    const R c12 = xs[1][2] * xs[2][0] - xs[1][0] * xs[2][2];
+    const R c13 = xs[1][0] * xs[2][1] - xs[1][1] * xs[2][0];
+    const R c21 = xs[2][1] * xs[0][2] - xs[2][2] * xs[0][1];
+    const R c22 = xs[2][2] * xs[0][0] - xs[2][0] * xs[0][2];
+    const R c23 = xs[2][0] * xs[0][1] - xs[2][1] * xs[0][0];
+    const R c31 = xs[0][1] * xs[1][2] - xs[0][2] * xs[1][1];
+    const R c32 = xs[0][2] * xs[1][0] - xs[0][0] * xs[1][2];
+    const R c33 = xs[0][0] * xs[1][1] - xs[0][1] * xs[1][0];
+    const R det = xs[0][0] * c11 + xs[0][1] * c12 + xs[0][2] * c13;
+    const R rdet = (R)1 / det;
+    const R q = (R)0.125 * rdet;
+#pragma unroll
+    for (int a = 0; a < 8; a++) {
+        const R s1 = (R)ac[a][0], s2 = (R)ac[a][1], s3 = (R)ac[a][2];
+        G.shp[a * 3 + 0] = (s1 * c11 + s2 * c12 + s3 * c13) * q;
+        G.shp[a * 3 + 1] = (s1 * c21 + s2 * c22 + s3 * c23) * q;
+        G.shp[a * 3 + 2] = (s1 * c31 + s2 * c32 + s3 * c33) * q;
+    }
+    G.det = det;
+    // element volume (vlm, src/library.f90:60-93)
+    constexpr int it[8][8] = {{1, 2, 3, 4, 5, 6, 7, 8}, {2, 3, 4, 1, 6, 7, 8, 5}, {3, 4, 1, 2, 7, 8, 5, 6},
+                              {4, 1, 2, 3, 8, 5, 6, 7}, {5, 8, 7, 6, 1, 4, 3, 2}, {6, 5, 8, 7, 2, 1, 4, 3},
+                              {7, 6, 5, 8, 3, 2, 1, 4}, {8, 7, 6, 5, 4, 3, 2, 1}};
+    R vol = 0;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+#define GY(r) x[1][it[i][(r) - 1] - 1]
+#define GZ(r) x[2][it[i][(r) - 1] - 1]
+        const R bb = GY(2) * (GZ(6) - GZ(3) + GZ(5) - GZ(4)) + GY(3) * (GZ(2) - GZ(4)) +
+                     GY(4) * (GZ(3) - GZ(8) + GZ(2) - GZ(5)) + GY(5) * (GZ(8) - GZ(6) + GZ(4) - GZ(2)) +
+                     GY(6) * (GZ(5) - GZ(2)) + GY(8) * (GZ(4) - GZ(5));
+#undef GY
+#undef GZ
+        vol = vol + x[0][i] * bb;
+    }
+    vol = vol / (R)12;
+    // SS = co * X^T X with X the inverse Jacobian (calcSSPhi4Hrgls)
+    const R ce = (R)16 * (mu * ((R)3 * lam + (R)2 * mu) / (lam + mu)) / (R)15;
+    const R co = ce * vol / (R)48;
+    const R X[3][3] = {{c11 * rdet, c21 * rdet, c31 * rdet}, {c12 * rdet, c22 * rdet, c32 * rdet},
+                       {c13 * rdet, c23 * rdet, c33 * rdet}};
+    G.ss[0] = co * (X[0][0] * X[0][0] + X[1][0] * X[1][0] + X[2][0] * X[2][0]);
+    G.ss[1] = co * (X[0][0] * X[0][1] + X[1][0] * X[1][1] + X[2][0] * X[2][1]);
+    G.ss[2] = co * (X[0][0] * X[0][2] + X[1][0] * X[1][2] + X[2][0] * X[2][2]);
+    G.ss[3] = co * (X[0][1] * X[0][1] + X[1][1] * X[1][1] + X[2][1] * X[2][1]);
+    G.ss[4] = co * (X[0][1] * X[0][2] + X[1][1] * X[1][2] + X[2][1] * X[2][2]);
+    G.ss[5] = co * (X[0][2] * X[0][2] + X[1][2] * X[1][2] + X[2][2] * X[2][2]);
+    // compact hourglass operators: h_m = sum_b Gamma_mb x_b, s_m the rms over
+    // a of Gamma_ma - h_m . dN_a/dx
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+        R h[3];
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            R s = 0;
+#pragma unroll
+            for (int b = 1; b < 8; b++) s = ((kGammaNeg[m] >> b) & 1u) ? s - x[j][b] : s + x[j][b];
+            h[j] = s;
+        }
+        R sq = 0;
+#pragma unroll
+        for (int a = 0; a < 8; a++) {
+            const R g = ((kGammaNeg[m] >> a) & 1u) ? (R)-1 : (R)1;
+            const R ph = g - (h[0] * G.shp[a * 3] + h[1] * G.shp[a * 3 + 1] + h[2] * G.shp[a * 3 + 2]);
+            sq = sq + ph * ph;
+        }
+        const R rs = (R)1 / sqrt(sq / (R)8);
+        G.cph[m] = rs;
+#pragma unroll
+        for (int j = 0; j < 3; j++) G.cph[4 + 3 * m + j] = h[j] * rs;
+    }
+}
+
 // split velocity k of PML node q
 template <typename R>
 __device__ __forceinline__ double pml_v(const PmlNodes &P, int64_t q, int k)
@@ -189,19 +304,33 @@ __device__ __forceinline__ void exact_store(const Member<R> &M, int node, int k,
 #ifndef EQD_STAGE_SHP_PML
 #define EQD_STAGE_SHP_PML 1
 #endif
+// Waves per SIMD the geometry-row variants (kOpsGeom) are bounded to.
+// Measured on the rough-fault box (FP32, 600 steps): regular elements
+// 0.150 ms unbounded (130 VGPRs, 3 waves) against 0.138 ms at 4 waves; PML
+// elements 0.154 ms at 4 waves (spilling) against 0.115 ms at 3 waves.  The
+// regular variants with Q, plasticity or an inertia term would spill at 4
+// waves and stay unbounded.
+#ifndef EQD_REG_GEO_WAVES
+#define EQD_REG_GEO_WAVES 4
+#endif
+#ifndef EQD_PML_GEO_WAVES
+#define EQD_PML_GEO_WAVES 3
+#endif
 
 //
 // kOps: where the operators come from (kOpsRows, kOpsLds, kOpsCompact).  With
 // kOpsLds both class tables are staged in LDS (dynamic shared memory of
 // (n_opc * kOpW + n_phic * 32) values) at block start, so the operator reads
 // of the element loop are LDS reads; with kOpsCompact Phi is rebuilt per node
-// (phi_compact) from 16 values read once per element.
+// (phi_compact) from 16 values read once per element; with kOpsGeom all
+// operators are rebuilt from the element's corner coordinates (geo_ops).
 template <typename R, bool kQ, bool kDP, int kHG, bool kMass, int kOps>
-__global__ void __launch_bounds__(256) elem_regular_kernel(const Member<R> *__restrict__ Ms,
-                                                           const uint16_t *__restrict__ bmem, int remap)
+__global__ void __launch_bounds__(256, (kOps == kOpsGeom && !kQ && !kDP && !kMass ? EQD_REG_GEO_WAVES : 1))
+    elem_regular_kernel(const Member<R> *__restrict__ Ms, const uint16_t *__restrict__ bmem, int remap)
 {
-    constexpr bool kLds = kOps == kOpsLds, kCmp = kOps == kOpsCompact;
-    constexpr int kW = kCmp ? kOpBc : kOpB;
+    constexpr bool kGeo = kOps == kOpsGeom;
+    constexpr bool kLds = kOps == kOpsLds, kCmp = kOps == kOpsCompact || kGeo;
+    constexpr int kW = kGeo ? kOpG : (kCmp ? kOpBc : kOpB);
     constexpr int kNst = kQ ? 12 : 6;  // stress values per element (with the Q memory variables)
     extern __shared__ __attribute__((aligned(16))) unsigned char eqd_smem[];
     typedef __attribute__((address_space(3))) R lds_R;
@@ -229,8 +358,12 @@ __global__ void __launch_bounds__(256) elem_regular_kernel(const Member<R> *__re
     }
     const int64_t n = E.n, nn = N.nn;
     constexpr bool kF32 = sizeof(R) != sizeof(double);
+    GeoOps<R> geo;
+    if constexpr (kGeo)
+        geo_ops<R>(E.opb, t, opb_at<R, kOpG>(E.opb, t, kGeoMat), opb_at<R, kOpG>(E.opb, t, kGeoMat + 1), geo);
     auto SHP = [&](int i) -> R {
-        if constexpr (kLds) return lrow[kOpShp + i];
+        if constexpr (kGeo) return geo.shp[i];
+        else if constexpr (kLds) return lrow[kOpShp + i];
         else if constexpr (kCmp) return opb_at<R, kW>(E.opb, t, kOpShp + i);
         else return op ? op[kOpShp + i] : opb_at(E.opb, t, kOpShp + i);
     };
@@ -240,16 +373,27 @@ __global__ void __launch_bounds__(256) elem_regular_kernel(const Member<R> *__re
     };
     // scalar operator k of the element's row
     auto OPK = [&](int k) -> R {
-        if constexpr (kLds) return lrow[k];
-        else if constexpr (kCmp) return opb_at<R, kW>(E.opb, t, k);
-        else return op ? op[k] : opb_at(E.opb, t, k);
+        if constexpr (kGeo) {
+            if (k == kOpDet) return geo.det;
+            if (k >= kOpSs) return geo.ss[k - kOpSs];
+            return opb_at<R, kOpG>(E.opb, t, kGeoMat + (k - kOpLam));
+        } else if constexpr (kLds) {
+            return lrow[k];
+        } else if constexpr (kCmp) {
+            return opb_at<R, kW>(E.opb, t, k);
+        } else {
+            return op ? op[k] : opb_at(E.opb, t, k);
+        }
     };
-    // compact hourglass operators (kOpsCompact with the stiffness hourglass)
+    // compact hourglass operators (kOpsCompact or kOpsGeom with the stiffness hourglass)
     constexpr bool kCHg = kCmp && kHG == 1;
     R cph[kCHg ? 16 : 1];
     if constexpr (kCHg) {
 #pragma unroll
-        for (int i = 0; i < 16; i++) cph[i] = opb_at<R, kW>(E.opb, t, kOpW + i);
+        for (int i = 0; i < 16; i++) {
+            if constexpr (kGeo) cph[i] = geo.cph[i];
+            else cph[i] = opb_at<R, kW>(E.opb, t, kOpW + i);
+        }
     }
     const R rdk = (R)C.rdampk;
     // all eight node ids first (independent loads), then the nodes' v and u
@@ -305,12 +449,12 @@ __global__ void __launch_bounds__(256) elem_regular_kernel(const Member<R> *__re
     };
     // (from LDS the values are read where they are used; kept, only dN/dx
     // when Phi is rebuilt from the compact operators)
-    constexpr bool kKeep = !kLds && (EQD_KEEP_OPS == 2 || (EQD_KEEP_OPS == 1 && sizeof(R) == sizeof(float)));
+    constexpr bool kKeep = !kLds && !kGeo && (EQD_KEEP_OPS == 2 || (EQD_KEEP_OPS == 1 && sizeof(R) == sizeof(float)));
     constexpr int kNk = kCmp ? 3 : kNo;
     R opc_[kNo], opn_[kNo];
     R keep[kKeep ? 8 : 1][kNk];
     if constexpr (!kLds) load_op(0, opc_);
-constexpr int kU1 = kKeep ? 8 : EQD_PREFETCH;
+constexpr int kU1 = (kKeep || kGeo) ? 8 : EQD_PREFETCH;
 #pragma unroll kU1
     for (int a = 0; a < 8; a++) {
         const int b = a % P;
@@ -478,7 +622,7 @@ constexpr int kU1 = kKeep ? 8 : EQD_PREFETCH;
     // pass 2: one node's force at a time (operators again one node ahead, or
     // kept from pass 1)
     if constexpr (!kLds && !kKeep) load_op(0, opc_);
-constexpr int kU2 = kKeep ? 8 : EQD_ELEM_UNROLL;
+constexpr int kU2 = (kKeep || kGeo) ? 8 : EQD_ELEM_UNROLL;
 #pragma unroll kU2
     for (int a = 0; a < 8; a++) {
         if constexpr (kLds) {
@@ -543,11 +687,12 @@ constexpr int kU2 = kKeep ? 8 : EQD_ELEM_UNROLL;
 // variant at 168 VGPRs without spills (it would take 176, leaving 2 waves);
 // measured 0.324 -> 0.307 ms on the rough-fault box.
 template <typename R, int kHG, bool kMass, int kOps>
-__global__ void __launch_bounds__(256, (sizeof(R) == 8 ? 3 : 4)) elem_pml_kernel(const Member<R> *__restrict__ Ms,
-                                                       const uint16_t *__restrict__ bmem, int remap)
+__global__ void __launch_bounds__(256, (sizeof(R) == 8 ? 3 : (kOps == kOpsGeom ? EQD_PML_GEO_WAVES : 4)))
+    elem_pml_kernel(const Member<R> *__restrict__ Ms, const uint16_t *__restrict__ bmem, int remap)
 {
-    constexpr bool kLds = kOps == kOpsLds, kCmp = kOps == kOpsCompact;
-    constexpr int kW = kCmp ? kOpBc : kOpB;
+    constexpr bool kGeo = kOps == kOpsGeom;
+    constexpr bool kLds = kOps == kOpsLds, kCmp = kOps == kOpsCompact || kGeo;
+    constexpr int kW = kGeo ? kOpG : (kCmp ? kOpBc : kOpB);
     extern __shared__ __attribute__((aligned(16))) unsigned char eqd_smem[];
     typedef __attribute__((address_space(3))) R lds_R;
     const unsigned gb = launch_block(remap);
@@ -574,8 +719,12 @@ __global__ void __launch_bounds__(256, (sizeof(R) == 8 ? 3 : 4)) elem_pml_kernel
     }
     const int64_t n = E.n, nn = N.nn;
     constexpr bool kF64 = sizeof(R) == sizeof(double);
+    GeoOps<R> geo;
+    if constexpr (kGeo)
+        geo_ops<R>(E.opb, p, opb_at<R, kOpG>(E.opb, p, kGeoMat), opb_at<R, kOpG>(E.opb, p, kGeoMat + 1), geo);
     auto SHP = [&](int i) -> R {
-        if constexpr (kLds) return lrow[kOpShp + i];
+        if constexpr (kGeo) return geo.shp[i];
+        else if constexpr (kLds) return lrow[kOpShp + i];
         else if constexpr (kCmp) return opb_at<R, kW>(E.opb, p, kOpShp + i);
         else return op ? op[kOpShp + i] : opb_at(E.opb, p, kOpShp + i);
     };
@@ -584,15 +733,26 @@ __global__ void __launch_bounds__(256, (sizeof(R) == 8 ? 3 : 4)) elem_pml_kernel
         else return pht ? pht[m * 8 + a] : opb_at(E.opb, p, kOpW + m * 8 + a);
     };
     auto OPK = [&](int k) -> R {
-        if constexpr (kLds) return lrow[k];
-        else if constexpr (kCmp) return opb_at<R, kW>(E.opb, p, k);
-        else return op ? op[k] : opb_at(E.opb, p, k);
+        if constexpr (kGeo) {
+            if (k == kOpDet) return geo.det;
+            if (k >= kOpSs) return geo.ss[k - kOpSs];
+            return opb_at<R, kOpG>(E.opb, p, kGeoMat + (k - kOpLam));
+        } else if constexpr (kLds) {
+            return lrow[k];
+        } else if constexpr (kCmp) {
+            return opb_at<R, kW>(E.opb, p, k);
+        } else {
+            return op ? op[k] : opb_at(E.opb, p, k);
+        }
     };
     constexpr bool kCHg = kCmp && kHG == 1;
     R cph[kCHg ? 16 : 1];
     if constexpr (kCHg) {
 #pragma unroll
-        for (int i = 0; i < 16; i++) cph[i] = opb_at<R, kW>(E.opb, p, kOpW + i);
+        for (int i = 0; i < 16; i++) {
+            if constexpr (kGeo) cph[i] = geo.cph[i];
+            else cph[i] = opb_at<R, kW>(E.opb, p, kOpW + i);
+        }
     }
     // node ids first, then each node's v and u one node ahead of the arithmetic
     int nds[8];
@@ -638,8 +798,8 @@ __global__ void __launch_bounds__(256, (sizeof(R) == 8 ? 3 : 4)) elem_pml_kernel
             for (int m = 0; m < 4; m++) o[3 + m] = PHI(m, a);
         }
     };
-    constexpr bool kKeep = !kLds && (EQD_KEEP_OPS_PML == 2 || (EQD_KEEP_OPS_PML == 1 && sizeof(R) == sizeof(float)));
-    constexpr bool kStage = !kLds && !kKeep && EQD_STAGE_SHP_PML;
+    constexpr bool kKeep = !kLds && !kGeo && (EQD_KEEP_OPS_PML == 2 || (EQD_KEEP_OPS_PML == 1 && sizeof(R) == sizeof(float)));
+    constexpr bool kStage = !kLds && !kKeep && !kGeo && EQD_STAGE_SHP_PML;
     constexpr int kNk = kCmp ? 3 : kNo;
     lds_R *stage = (lds_R *)eqd_smem;
     const unsigned bd = blockDim.x, tx = threadIdx.x;
@@ -660,7 +820,7 @@ __global__ void __launch_bounds__(256, (sizeof(R) == 8 ? 3 : 4)) elem_pml_kernel
     R opc_[kNo], opn_[kNo];
     R keep[kKeep ? 8 : 1][kNk];
     if constexpr (!kLds) load_op(0, opc_);
-constexpr int kU1 = kKeep ? 8 : EQD_PREFETCH;
+constexpr int kU1 = (kKeep || kGeo) ? 8 : EQD_PREFETCH;
 #pragma unroll kU1
     for (int a = 0; a < 8; a++) {
         const int b = a % P;
@@ -799,7 +959,7 @@ constexpr int kU1 = kKeep ? 8 : EQD_PREFETCH;
     }
 
     if constexpr (!kLds && !kKeep) load_op2(0, opc_);
-constexpr int kU2 = kKeep ? 8 : EQD_ELEM_UNROLL;
+constexpr int kU2 = (kKeep || kGeo) ? 8 : EQD_ELEM_UNROLL;
 #pragma unroll kU2
     for (int a = 0; a < 8; a++) {
         if constexpr (kLds) {
@@ -1685,13 +1845,18 @@ hipError_t launch_elem(const Member<R> *M, const LaunchGrid &g, int q, int dp, i
 {
     if (g.grid == 0) return hipSuccess;
     if (g.block < 64 || g.block > 256 || g.block % 64) return hipErrorInvalidValue;
-    if (ops < kOpsRows || ops > kOpsCompact) return hipErrorInvalidValue;
+    if (ops < kOpsRows || ops > kOpsGeom) return hipErrorInvalidValue;
+    // geometry rows exist in the FP32 bulk mode only
+    if (ops == kOpsGeom && sizeof(R) != sizeof(float)) return hipErrorInvalidValue;
+    constexpr int kGeoOr = sizeof(R) == sizeof(float) ? kOpsGeom : kOpsCompact;
     // class tables in LDS (at most 16 KiB) or operators from memory
     const size_t lds = ops == kOpsLds ? g.lds : 0;
 #define EQD_LAUNCH(Q, DP, HG, MASS)                                                                               \
     do {                                                                                                          \
         if (ops == kOpsLds)                                                                                       \
             EQD_GO((elem_regular_kernel<R, Q, DP, HG, MASS, kOpsLds>), g.grid, g.block, lds, s, M, g.bmem, g.remap); \
+        else if (ops == kOpsGeom)                                                                                 \
+            EQD_GO((elem_regular_kernel<R, Q, DP, HG, MASS, kGeoOr>), g.grid, g.block, 0, s, M, g.bmem, g.remap); \
         else if (ops == kOpsCompact)                                                                              \
             EQD_GO((elem_regular_kernel<R, Q, DP, HG, MASS, kOpsCompact>), g.grid, g.block, 0, s, M, g.bmem, g.remap); \
         else                                                                                                      \
@@ -1719,13 +1884,19 @@ hipError_t launch_pml(const Member<R> *M, const LaunchGrid &g, int hg, int mass,
 {
     if (g.grid == 0) return hipSuccess;
     if (g.block < 64 || g.block > 256 || g.block % 64) return hipErrorInvalidValue;
-    if (ops < kOpsRows || ops > kOpsCompact) return hipErrorInvalidValue;
-    // class tables, or each thread's staged dN/dx (24 values)
-    const size_t lds = ops == kOpsLds ? g.lds : (EQD_STAGE_SHP_PML ? (size_t)g.block * 24 * sizeof(R) : 0);
+    if (ops < kOpsRows || ops > kOpsGeom) return hipErrorInvalidValue;
+    if (ops == kOpsGeom && sizeof(R) != sizeof(float)) return hipErrorInvalidValue;
+    constexpr int kGeoOr = sizeof(R) == sizeof(float) ? kOpsGeom : kOpsCompact;
+    // class tables, or each thread's staged dN/dx (24 values); geometry rows
+    // keep dN/dx in registers
+    const size_t lds = ops == kOpsLds ? g.lds
+                                      : (ops == kOpsGeom ? 0 : (EQD_STAGE_SHP_PML ? (size_t)g.block * 24 * sizeof(R) : 0));
 #define EQD_LAUNCH(HG, MASS)                                                                              \
     do {                                                                                                  \
         if (ops == kOpsLds)                                                                               \
             EQD_GO((elem_pml_kernel<R, HG, MASS, kOpsLds>), g.grid, g.block, lds, s, M, g.bmem, g.remap);     \
+        else if (ops == kOpsGeom)                                                                         \
+            EQD_GO((elem_pml_kernel<R, HG, MASS, kGeoOr>), g.grid, g.block, lds, s, M, g.bmem, g.remap);  \
         else if (ops == kOpsCompact)                                                                      \
             EQD_GO((elem_pml_kernel<R, HG, MASS, kOpsCompact>), g.grid, g.block, lds, s, M, g.bmem, g.remap); \
         else                                                                                              \

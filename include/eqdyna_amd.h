@@ -144,7 +144,11 @@ int64_t eqd_device_bytes(eqd_handle *h);
  * "phi_classes_pml"; "phi_compact_regular", "phi_compact_pml": 1 when the
  * hourglass Phi is carried as four numbers per mode (it has the form of
  * src/assembleGlobalMass.f90:351-372 for every element of the class; then
- * Phi agrees with the given one to 1e-11), 0 when the given Phi is kept.
+ * Phi agrees with the given one to 1e-11), 0 when the given Phi is kept;
+ * "geom_ops_regular", "geom_ops_pml": 1 when the FP32 bulk mode rebuilds the
+ * class's operators from the elements' corner coordinates in the kernels
+ * (meshes whose elements all differ; the given operators must be those of
+ * the coordinates, EQD_GEOM_OPS=0 in the environment keeps them stored).
  * Returns EQD_EINVAL for an unknown key. */
 int eqd_layout_info(eqd_handle *h, const char *key, int64_t *value);
 

@@ -68,3 +68,34 @@ def test_fp32_learned_mode_vs_fp64_reference(name):
     assert e["rupture_flips"] == 0
     assert e["rupture_time_mean_dt"] <= 0.5
     assert e["peak_slip_rate_rel"] <= 0.01
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["box_rough", "box_rsf4"])
+def test_geometry_operators_vs_stored_operators(name, monkeypatch):
+    # FP32 bulk mode on a mesh whose elements all differ rebuilds each
+    # element's operators from its corner coordinates in the kernels; with
+    # EQD_GEOM_OPS=0 it reads the stored (FP64-computed, FP32-rounded) rows.
+    # Both are judged against the reference's FP64 run with the same bars.
+    print(f"case {name}", flush=True)
+    bundle, sc, ref = load_golden(name)
+    if name == "box_rsf4":
+        monkeypatch.setenv("EQD_NO_OPERATOR_CLASSES", "1")  # a planar mesh: force per-element operators
+    res = {}
+    for geo in ("1", "0"):
+        monkeypatch.setenv("EQD_GEOM_OPS", geo)
+        s = Solver(bundle, sc, "fp32")
+        info = s.layout_info()
+        s.run(int(sc["nstep"]))
+        out = s.outputs()
+        s.close()
+        assert info["geom_ops_regular"] == int(geo) and info["geom_ops_pml"] == int(geo), info
+        e = fp32_errors(out, ref, sc["dt"])
+        ch = channel_errors(channels(out), channels(ref))
+        worst = max(ch, key=ch.get)
+        print(f"[{name}] geometry rows {geo}: {e}; worst channel {worst} {ch[worst]:.2e}", flush=True)
+        assert e["rupture_flips"] == 0 and e["rupture_time_mean_dt"] <= 0.5 and e["peak_slip_rate_rel"] <= 0.01
+        res[geo] = out
+    d = channel_errors(channels(res["1"]), channels(res["0"]))
+    w = max(d, key=d.get)
+    print(f"[{name}] geometry rows against stored rows: worst channel {w} {d[w]:.2e}", flush=True)

@@ -176,6 +176,7 @@ def test_operator_classes_are_lossless(precision, monkeypatch):
     o1 = s1.outputs()
     s1.close()
     monkeypatch.setenv("EQD_NO_OPERATOR_CLASSES", "1")
+    monkeypatch.setenv("EQD_GEOM_OPS", "0")  # stored per-element rows, not rebuilt from coordinates
     s2 = Solver(bundle, sc, precision)
     info2 = s2.layout_info()
     s2.run(n)
