@@ -13,9 +13,11 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <mutex>
 #include <vector>
 
 #include "../../include/eqdyna_amd.h"
+
 #include "eqd_mlp.h"
 
 namespace eqd {
@@ -283,6 +285,35 @@ bool alloc_net(DevMem &D, MlpBuffers &B)
     return B.p && B.wb && B.w4r;
 }
 
+// ---------------- scratch of the allocation-free entry point
+// The network's prepared operands (bf16 weights w1, w2, w3 and their
+// transposes, and w4 as float) of eqd_mlp_forward_noalloc live in this
+// file-scope device array: kNoAllocScratch bytes per device, held until the
+// process ends.  Host code never uses its name as an address: the device
+// address comes from hipGetSymbolAddress, looked up once per device.
+constexpr size_t kWbCount = 64 * 16 + 4 * 4096;                       // bf16 values
+constexpr size_t kW4rOffset = (kWbCount * sizeof(__bf16) + 255) / 256 * 256;  // bytes
+constexpr size_t kNoAllocScratch = kW4rOffset + 64 * sizeof(float);
+static_assert(kNoAllocScratch == EQD_MLP_NOALLOC_SCRATCH_BYTES, "scratch size and header disagree");
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratch];
+
+constexpr int kMaxDevices = 64;
+void *g_scratch_ptr[kMaxDevices] = {};
+std::mutex g_scratch_mu;
+
+// device address of this device's scratch array (device current), or null
+void *scratch_on(int device)
+{
+    if (device < 0 || device >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void *ptr = nullptr;
+        if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess) return nullptr;
+        g_scratch_ptr[device] = ptr;
+    }
+    return g_scratch_ptr[device];
+}
+
 int check_params(const float *params)
 {
     const float *nrm = params + kOffNorm;
@@ -315,6 +346,25 @@ int eqd_mlp_forward(const float *params, const float *X, int64_t n, float *r_out
     if (hipGetLastError() != hipSuccess) return EQD_EHIP;
     if (hipMemcpy(r_out, dy, (size_t)n * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess) return EQD_EHIP;
     return EQD_OK;
+}
+
+int eqd_mlp_forward_noalloc(const float *params, const float *X, int64_t n, float *r_out, int32_t device,
+                            void *stream)
+{
+    if (!params || (n > 0 && (!X || !r_out)) || n < 0) return EQD_EINVAL;
+    if (n > (int64_t)0xFFFFFFFF * 64) return EQD_EINVAL;  // one launch covers at most 2^32 - 1 waves
+    if (n == 0) return EQD_OK;
+    if (hipSetDevice(device) != hipSuccess) return EQD_EHIP;
+    unsigned char *sc = (unsigned char *)scratch_on(device);
+    if (!sc) return EQD_EHIP;
+    MlpBuffers B;
+    B.p = const_cast<float *>(params);
+    B.wb = (__bf16 *)sc;
+    B.w4r = (float *)(sc + kW4rOffset);
+    const hipStream_t st = (hipStream_t)stream;
+    mlp_prepare_kernel<<<16, 256, 0, st>>>(B.p, B.wb, B.w4r);
+    mlp_forward_kernel<<<(unsigned)((n + 63) / 64), 64, 0, st>>>(B.net(), X, n, r_out);
+    return hipGetLastError() == hipSuccess ? EQD_OK : EQD_EHIP;
 }
 
 int eqd_mlp_grad(const float *params, const float *X, const float *logv, int64_t n, float *grad_out,

@@ -23,7 +23,8 @@ EXPORTS = ["eqd_create", "eqd_set", "eqd_commit", "eqd_run", "eqd_get", "eqd_clo
            "eqd_mlp_grad", "eqd_mlp_train", "eqd_set_launch", "eqd_get_launch", "eqd_tune", "eqd_set_graph",
            "eqd_ensemble_create", "eqd_ensemble_run", "eqd_ensemble_tune", "eqd_ensemble_set_launch",
            "eqd_ensemble_set_graph", "eqd_ensemble_set_timing", "eqd_ensemble_kernel_time",
-           "eqd_ensemble_last_error", "eqd_ensemble_destroy", "eqd_precompute", "eqd_record", "eqd_record_get"]
+           "eqd_ensemble_last_error", "eqd_ensemble_destroy", "eqd_precompute", "eqd_record", "eqd_record_get",
+           "eqd_mlp_forward_noalloc"]
 
 # launch kinds (EQD_KIND_*)
 KINDS = {"element": 0, "element_pml": 1, "node": 2, "node_pml": 3, "fault": 4}
@@ -114,6 +115,7 @@ def lib():
         L.eqd_harvest_get.argtypes = [C.c_void_p, fp, C.c_int64, C.POINTER(C.c_int64)]
         L.eqd_newton_stats.argtypes = [C.c_void_p, C.POINTER(C.c_int64)]
         L.eqd_mlp_forward.argtypes = [fp, fp, C.c_int64, fp, C.c_int32]
+        L.eqd_mlp_forward_noalloc.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_int32, C.c_void_p]
         L.eqd_mlp_grad.argtypes = [fp, fp, fp, C.c_int64, fp, fp, C.c_int32]
         L.eqd_mlp_train.argtypes = [fp, fp, C.c_int64, C.POINTER(TrainOpts), fp, fp]
         i32p = C.POINTER(C.c_int32)
@@ -357,6 +359,16 @@ def mlp_forward(params, X, device: int = 0) -> np.ndarray:
     if rc != 0:
         raise EqdError(rc, "eqd_mlp_forward failed")
     return out
+
+
+def mlp_forward_noalloc(params_ptr: int, X_ptr: int, n: int, out_ptr: int, device: int = 0, stream: int = 0) -> None:
+    """eqd_mlp_forward_noalloc: device arrays given by address (params
+    float32[EQD_MLP_NPARAMS], X float32[n][6], out float32[n]), enqueued on the
+    HIP stream `stream` (0: the null stream); returns without waiting."""
+    rc = lib().eqd_mlp_forward_noalloc(C.c_void_p(params_ptr), C.c_void_p(X_ptr), int(n), C.c_void_p(out_ptr),
+                                       int(device), C.c_void_p(stream))
+    if rc != 0:
+        raise EqdError(rc, "eqd_mlp_forward_noalloc failed")
 
 
 def mlp_grad(params, X, r, device: int = 0):

@@ -202,6 +202,20 @@ int eqd_newton_stats(eqd_handle *h, int64_t stats[5]);
 /* Standalone use of the network on the GPU (host arrays, device `device`):
  * X[n][6] raw inputs, r[n] outputs. */
 int eqd_mlp_forward(const float *params, const float *X, int64_t n, float *r_out, int32_t device);
+/* eqd_mlp_forward for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference.  params[EQD_MLP_NPARAMS],
+ * X[n][6] and r_out[n] are DEVICE arrays of the caller; the work is enqueued
+ * on `stream` (a hipStream_t; null: the null stream) and the call returns
+ * without waiting.  It allocates, frees and copies nothing: the network's
+ * prepared operands go to a library-owned scratch array of
+ * EQD_MLP_NOALLOC_SCRATCH_BYTES on each device it runs on, held until the
+ * process ends and shared by every call on that device, so calls that
+ * overlap on one device race (serialize them, e.g. on one stream).  n may be
+ * any size up to 64 * (2^32 - 1) rows (EQD_EINVAL beyond). */
+#define EQD_MLP_NOALLOC_SCRATCH_BYTES 35072
+int eqd_mlp_forward_noalloc(const float *params, const float *X, int64_t n, float *r_out, int32_t device,
+                            void *stream);
 /* Mean squared error of the normalised output against the normalised
  * targets r[n], and its gradient with respect to the first EQD_MLP_NTRAIN
  * parameters, over all n samples. */
