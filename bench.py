@@ -20,12 +20,12 @@ A "step" is one explicit time step (src/driver.f90:9-34) over the whole mesh,
 with the mesh resident in HBM.  Prints one JSON line:
   value       Mdof-steps/s = 3 * N_nodes * steps / t / 1e6 (N_nodes = grid nodes +
               split master nodes; PML split components are not counted)
-  roofline    the run's dominant kernel (largest total time): algorithmic bytes
-              per launch (stored-operator model, per unit: regular hexahedron
-              648 B FP64 / 340 B FP32 (+96/48 B Q memory variables), PML
-              hexahedron 912 / 470 B, 3-component node 200 / 152 B, PML node
-              530 / 300 B) divided by its average launch time, measured with HIP
-              events on the library's stream over the timed steps; peak = 8 TB/s.
+  roofline    the run's dominant kernel (largest total time): its algorithmic
+              bytes per launch (kernel_bytes: every array it needs, in the layout
+              the library chose for the mesh, moved once) divided by its average
+              launch time, measured with HIP events its own dispatches record
+              over the timed steps; peak = 8 TB/s, beside a measured device copy;
+              and the whole step's algorithmic bytes over the time per step.
   cpu_baseline the reference Fortran (oracle/_ref/eqd_ref) on MPI ranks on the
               host's cores, on a bounded sample of the same workload.
 """
@@ -43,14 +43,6 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Mdof-steps/s: plain FP64 vs FP32+learned RSF guess at equal rupture-time error"
 HBM_PEAK_GBS = 8000.0
-# algorithmic bytes per unit of each kernel (stored-operator model)
-UNIT_BYTES = {
-    "element": {"fp64": 648, "fp32": 340},
-    "element_pml": {"fp64": 912, "fp32": 470},
-    "node": {"fp64": 200, "fp32": 152},
-    "node_pml": {"fp64": 530, "fp32": 300},
-}
-Q_BYTES = {"fp64": 96, "fp32": 48}
 KERNEL_NAMES = {"element": "elem_regular_kernel", "element_pml": "elem_pml_kernel", "node": "node3_kernel",
                 "node_pml": "node_pml_kernel"}
 GUESS = {"k_fallback": 6, "gate": 0.5}
@@ -86,14 +78,20 @@ def build_workload(name: str, steps_total: int):
 
 
 def load_traffic(workload: str, precision: str, kernel: str):
+    """Counted bytes per launch of a kernel from profiles/kernel_traffic.json
+    (rocprofv3 FETCH_SIZE and WRITE_SIZE passes of the same workload) and
+    where they come from, or (None, None)."""
     p = os.path.join(ROOT, "profiles", "kernel_traffic.json")
     if not os.path.exists(p):
-        return None
+        return None, None
     try:
-        e = json.load(open(p)).get(f"{workload}_{precision}_{kernel}")
-        return float(e["bytes_per_launch"]) if e else None
+        d = json.load(open(p))
+        e = d.get(f"{workload}_{precision}_{kernel}")
+        if not e:
+            return None, None
+        return float(e["bytes_per_launch"]), f"profiles/kernel_traffic.json ({d.get('_collected', 'undated')})"
     except Exception:  # noqa: BLE001
-        return None
+        return None, None
 
 
 def run_one(bundle, sc, precision: str, warmup: int, steps: int, guess=None, tune: int = 0):
@@ -144,24 +142,107 @@ def run_graph(bundle, sc, precision: str, warmup: int, steps: int, guess=None, l
     return wall
 
 
-def roofline(run, counts, precision: str, q: bool, workload: str):
-    """Roofline of the run's dominant kernel (largest total time)."""
-    tot = {k: ms * n for k, (ms, n) in run["ktimes"].items() if k in UNIT_BYTES}
+def kernel_bytes(bundle, sc, layout: dict, precision: str) -> dict:
+    """Algorithmic bytes per launch of each bulk kernel: every array the
+    kernel needs, in the layout the library chose for this mesh
+    (layout_info), read or written once -- each element's operators (class
+    indices when elements share class tables held in LDS; stored rows with
+    the full or the compact hourglass operators; or the geometry rows the
+    FP32 mode rebuilds them from), connectivity, stresses and forces once per
+    element, each node's state once per kernel that touches it, each
+    element-to-node force once.  Counts come from the mesh itself."""
+    import numpy as np
+    f32 = precision == "fp32"
+    r = 4 if f32 else 8               # element operator / stress / force values
+    nv = 4 if f32 else 8              # node velocity values
+    nu_r, nu_w = (4, 8) if f32 else (8, 8)  # displacement: gathered (FP32: the rounded part), updated (hi + lo)
+    ien = np.asarray(bundle["nodeElemIdRelation"]).reshape(-1, 8) - 1
+    et = np.asarray(bundle["elemTypeArr"])
+    ndof = np.asarray(bundle["numOfDofPerNodeArr"])
+    pml = et == 2
+    nr, npl = int((~pml).sum()), int(pml.sum())
+    split = np.zeros(len(ndof), dtype=bool)
+    split[np.asarray(bundle["nsmp"]).ravel() - 1] = True
+    q, dp, hg = int(sc["C_Q"]) == 1, int(sc["C_elastic"]) == 0, int(sc["C_hg"])
+    mass_term = float(sc.get("rdampm", 0.0)) != 0.0 or dp
+
+    def ops(kind: str) -> int:
+        if layout.get(f"op_classes_{kind}", 0) > 0:
+            return 8  # two 32-bit class indices; the tables sit in LDS
+        if layout.get(f"geom_ops_{kind}", 0):
+            return 23 * r  # 21 relative coordinates, lambda, mu
+        hg_vals = (16 if layout.get(f"phi_compact_{kind}", 0) else 32) if hg == 1 else 2
+        return (24 + 3 + (6 if hg == 1 else 0) + hg_vals) * r
+
+    occ = np.bincount(ien.ravel(), minlength=len(ndof))        # element-node entries per node
+    occ_reg = np.bincount(ien[~pml].ravel(), minlength=len(ndof))
+    occ_pml = np.bincount(ien[pml].ravel(), minlength=len(ndof))
+    node3 = (ndof == 3) & ~split
+    nodepml = (ndof == 12) & (occ_pml > 0)
+    out = {}
+    ns = 12 if q else 6
+    out["element"] = (nr * (32 + ops("regular") + 2 * ns * r + 24 * r + (1 if q else 0) + (24 if dp else 0)
+                            + (24 * r if mass_term else 0))
+                      + int((occ_reg > 0).sum()) * 3 * (nv + nu_r))
+    # PML element: a PML node takes 6 values of its slot, a 3-component node 3
+    pml_slots = int(np.where(ndof[ien[pml]] == 12, 6, 3).sum()) if npl else 0
+    out["element_pml"] = (npl * (32 + ops("pml") + 2 * 21 * r + 3 * r + 1 + 6 * r + (24 * r if mass_term else 0))
+                          + pml_slots * r + int((occ_pml > 0).sum()) * 3 * (nv + nu_r))
+    n3, e3 = int(node3.sum()), int(occ[node3].sum())
+    out["node"] = n3 * (4 + 4 + 8 + 3 * 2 * nv + 3 * 2 * nu_w) + e3 * (4 + 3 * r)
+    npn, ep = int(nodepml.sum()), int(occ[nodepml].sum())
+    out["node_pml"] = (npn * (4 + 4 + 8 + 24 + 12 * 2 * nv + 3 * 2 * nv + 3 * 2 * nu_w)
+                       + ep * (4 + 6 * r) + npl * 6 * r)
+    return {k: int(v) for k, v in out.items()}
+
+
+def measure_copy_gbs(nbytes: int = 1 << 32, reps: int = 5) -> float:
+    """Device-to-device copy bandwidth of this GPU, (read + write) bytes over
+    the best of `reps` timed copy kernels of `nbytes` (torch's copy kernel,
+    timed with HIP events), in a separate process so that torch's HIP runtime
+    never shares a process with libeqd.so's."""
+    import subprocess
+    code = ("import torch,json;n=%d//4;a=torch.ones(n,device='cuda');b=torch.empty_like(a);b.copy_(a);"
+            "e0=torch.cuda.Event(enable_timing=True);e1=torch.cuda.Event(enable_timing=True);best=0.0\n"
+            "for _ in range(%d):\n e0.record();b.copy_(a);e1.record();e1.synchronize();"
+            "best=max(best,2*n*4/(e0.elapsed_time(e1)*1e-3)/1e9)\nprint(json.dumps(best))" % (nbytes, reps))
+    try:
+        p = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=300)
+        return float(p.stdout.strip().splitlines()[-1]) if p.returncode == 0 else 0.0
+    except Exception:  # noqa: BLE001
+        return 0.0
+
+
+def roofline(run, kbytes: dict, precision: str, workload: str, ms_per_step: float, copy_gbs: float):
+    """Roofline of the run's dominant kernel (largest total time): its
+    algorithmic bytes per launch (kernel_bytes) over its average launch time
+    from the HIP events its own dispatches record; beside it the whole step's
+    algorithmic bytes over the measured time per step, and the measured copy
+    bandwidth."""
+    tot = {k: ms * n for k, (ms, n) in run["ktimes"].items() if k in kbytes}
     k = max(tot, key=tot.get)
     ms = run["ktimes"][k][0]
-    per = UNIT_BYTES[k][precision] + (Q_BYTES[precision] if (q and k == "element") else 0)
-    alg = per * counts[k]
+    alg = kbytes[k]
     achieved = alg / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
-    traffic = load_traffic(workload, precision, k)
+    traffic, source = load_traffic(workload, precision, k)
+    step_bytes = sum(kbytes.values())
     out = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-           "kernel": KERNEL_NAMES[k], "algorithmic_bytes_per_unit": per, "units_per_launch": int(counts[k]),
-           "algorithmic_bytes_per_launch": int(alg), "avg_launch_ms": round(ms, 5)}
+           "kernel": KERNEL_NAMES[k], "algorithmic_bytes_per_launch": int(alg), "avg_launch_ms": round(ms, 5),
+           "copy_gbs_measured": round(copy_gbs, 1),
+           "frac_of_copy": round(achieved / copy_gbs, 4) if copy_gbs > 0 else None,
+           "step": {"algorithmic_bytes": int(step_bytes), "ms": round(ms_per_step, 5),
+                    "gbs": round(step_bytes / (ms_per_step * 1e-3) / 1e9, 1) if ms_per_step > 0 else None,
+                    "frac": round(step_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if ms_per_step > 0 else None,
+                    "per_kernel_bytes": kbytes}}
     if traffic and ms > 0:
-        # the counted HBM bytes of the same kernel over its measured launch time
-        # (the model above leaves out the element's force stores and node gathers)
+        # bytes the L2 fetched from and wrote to memory for the same kernel
+        # (rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled by a device copy), over
+        # the same launch time; an upper bound on DRAM bytes (the Infinity
+        # Cache serves some of the fetches)
+        out["traffic_source"] = source
         out["traffic_gbs"] = round(traffic / (ms * 1e-3) / 1e9, 1)
-        out["traffic_frac"] = round(out["traffic_gbs"] / HBM_PEAK_GBS, 4)
+        out["traffic_over_algorithmic"] = round(traffic / alg, 3)
     return out
 
 
@@ -181,12 +262,14 @@ def cpu_baseline_reference(workload: str, ranks: int, steps: int):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import refrun
     from eqdyna_amd import case, mesh
-    if not refrun.available() or not os.path.exists(refrun.MPIEXEC):
+    binary = refrun.REF_BIN_O3 if os.path.exists(refrun.REF_BIN_O3) else refrun.REF_BIN
+    build = "amdflang -O3" if binary == refrun.REF_BIN_O3 else "amdflang -O2 (parity build; the -O3 build is absent)"
+    if not os.path.exists(binary) or not os.path.exists(refrun.MPIEXEC):
         return None
     if workload in ("c3", "c4"):
         c = case.c3(steps=steps)
         sample = (f"c3 at full size (3,456,000 hexahedra, rough fault, slip-law RSF), first {steps} of 3000 "
-                  f"steps, plain Newton guess")
+                  f"steps (through the onset of rupture), plain Newton guess")
     elif workload == "c1":
         c = case.c1(steps=steps)
         sample = f"c1 at full size (432,000 hexahedra at 200 m, SW fault), first {steps} steps"
@@ -204,11 +287,11 @@ def cpu_baseline_reference(workload: str, ranks: int, steps: int):
     c.nx, c.ny, c.nz = fac[ranks]
     with tempfile.TemporaryDirectory() as d:
         t = time.perf_counter()
-        _, _, _, t_loop = refrun.run_reference(c, d, ranks, dump=False, timeout=900)
+        _, _, _, t_loop = refrun.run_reference(c, d, ranks, dump=False, timeout=900, binary=binary)
         log(f"reference on {ranks} ranks: t_loop {t_loop:.2f} s ({time.perf_counter() - t:.1f} s with meshing); "
             f"start-up phases {refrun.LAST_PHASES}")
     return {"value": 3.0 * nn * steps / t_loop / 1e6, "unit": "Mdof-steps/s", "cores": ranks, "kind": "reference",
-            "sample": sample + f", reference Fortran (amdflang -O2) on {ranks} MPI ranks, time loop only",
+            "sample": sample + f", reference Fortran ({build}) on {ranks} MPI ranks, time loop only",
             "startup_s": dict(refrun.LAST_PHASES)}
 
 
@@ -227,8 +310,11 @@ def cpu_baseline_port(steps: int):
 
 
 def calibrate_copy(nbytes: int) -> int:
-    """A plain copy of known size: its FETCH_SIZE / WRITE_SIZE counters give the
-    scale factor between counter readings and bytes moved."""
+    """Kernels of known traffic for scaling the FETCH_SIZE / WRITE_SIZE
+    counters to bytes: a plain copy of `nbytes` (read nbytes, write nbytes),
+    then a gather of the same array's 128-byte rows in a random order
+    (index_select: read nbytes in whole lines plus 8 bytes of index per row,
+    write nbytes), the access pattern of the node and element gathers."""
     import torch
     n = nbytes // 4
     a = torch.empty(n, dtype=torch.float32, device="cuda")
@@ -236,8 +322,13 @@ def calibrate_copy(nbytes: int) -> int:
     b = torch.empty_like(a)
     for _ in range(3):
         b.copy_(a)
+    rows = a.view(-1, 32)
+    perm = torch.randperm(rows.shape[0], device="cuda")
+    for _ in range(3):
+        torch.index_select(rows, 0, perm, out=b.view(-1, 32))
     torch.cuda.synchronize()
-    print(json.dumps({"copy_bytes": n * 4, "launches": 3}), flush=True)
+    print(json.dumps({"copy_bytes": n * 4, "gather_read_bytes": n * 4 + 8 * rows.shape[0],
+                      "gather_write_bytes": n * 4, "launches": 3}), flush=True)
     return 0
 
 
@@ -259,7 +350,9 @@ def main() -> int:
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"], help="c1-c3 only")
     ap.add_argument("--cpu-baseline", default="auto", choices=["auto", "reference", "port", "none"])
     ap.add_argument("--cpu-ranks", type=int, default=16)
-    ap.add_argument("--cpu-steps", type=int, default=60)
+    ap.add_argument("--cpu-steps", type=int, default=60, help="reference steps timed (c3/c4: at least this, through "
+                    "the onset of rupture unless --cpu-steps-auto 0)")
+    ap.add_argument("--cpu-steps-auto", type=int, default=1)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run a device-to-device copy of BYTES (for scaling HBM counters) and exit")
     args = ap.parse_args()
@@ -277,12 +370,9 @@ def main() -> int:
     total = args.warmup + steps
     c, bundle, sc, desc = build_workload(args.workload, total)
     et = bundle["elemTypeArr"]
-    ndof = bundle["numOfDofPerNodeArr"]
-    counts = {"element": int((et != 2).sum()), "element_pml": int((et == 2).sum()),
-              "node_pml": int((ndof == 12).sum()),
-              # split pairs are updated by the fault kernel, not the node kernel
-              "node": int((ndof == 3).sum()) - 2 * int(sc["nftnd"])}
+    n_regular = int((et != 2).sum())
     nn = int(sc["totalNumOfNodes"])
+    sc_dt = float(sc["dt"])
     extra = {}
     if args.precompute:
         extra["device_precompute"] = bench_precompute(bundle)
@@ -323,13 +413,22 @@ def main() -> int:
         log(f"graph replay: {precision} {g:.1f} Mdof-steps/s")
         extra["graph_replay"] = {"value": round(g, 2), "ms_per_step": round(wg / steps * 1e3, 4),
                                  "note": "same steps replayed from one captured step; no per-kernel events"}
-    roof = roofline(run, counts, precision, c.C_Q == 1, args.workload)
+    kbytes = kernel_bytes(bundle, sc, run["layout"], precision)
+    copy_gbs = measure_copy_gbs()
+    log(f"device copy: {copy_gbs:.1f} GB/s (read + write)")
+    roof = roofline(run, kbytes, precision, args.workload, run["wall"] / steps * 1e3, copy_gbs)
     del bundle
 
     cpu = None
+    cpu_steps = args.cpu_steps
+    if args.workload in ("c3", "c4") and args.cpu_steps_auto:
+        # through the onset of rupture: 40 steps past the first ruptured node of this run
+        ft = run["fnft"][run["fnft"] < 5000]
+        if ft.size:
+            cpu_steps = int(min(400, max(args.cpu_steps, round(float(ft.min()) / sc_dt) + 40)))
     if args.cpu_baseline in ("auto", "reference"):
         try:
-            cpu = cpu_baseline_reference(args.workload, args.cpu_ranks, args.cpu_steps)
+            cpu = cpu_baseline_reference(args.workload, args.cpu_ranks, cpu_steps)
         except Exception as e:  # noqa: BLE001
             log(f"reference CPU baseline failed: {e}")
             cpu = None
@@ -346,9 +445,9 @@ def main() -> int:
         "scaling": "weak", "vs_baseline": None, "dtype": dtype,
         "data": "synthetic (seeded case generator; no external data)",
         "config": {"workload": desc, "elements": int(sc["totalNumOfElements"]), "nodes": nn,
-                   "regular_elements": counts["element"], "fault_pairs": int(sc["nftnd"]), "precision": precision,
+                   "regular_elements": n_regular, "fault_pairs": int(sc["nftnd"]), "precision": precision,
                    "steps_run": run["nt"],
-                   "operator_classes": {k: v for k, v in run["layout"].items() if "classes" in k},
+                   "layout": run["layout"],
                    "launch_shapes": {k: {"block": b, "remap": r} for k, (b, r) in run["launch"].items()}, **extra},
         "roofline": roof, "cpu_baseline": cpu,
     }
@@ -412,22 +511,19 @@ def bench_c5(args) -> int:
     total = args.warmup + steps + kl + kt
     cases = case.c5(n=args.scenarios, steps=total)
     members = []
-    counts = {"element": 0, "element_pml": 0, "node": 0, "node_pml": 0}
+    kbytes = {"element": 0, "element_pml": 0, "node": 0, "node_pml": 0}
     nn = ne = nf = 0
     sizes = []
     t = time.perf_counter()
     for c in cases:
         bundle, sc = mesh.build_case(c, nthreads=min(16, os.cpu_count() or 1))
-        et, ndof = bundle["elemTypeArr"], bundle["numOfDofPerNodeArr"]
-        counts["element"] += int((et != 2).sum())
-        counts["element_pml"] += int((et == 2).sum())
-        counts["node_pml"] += int((ndof == 12).sum())
-        counts["node"] += int((ndof == 3).sum()) - 2 * int(sc["nftnd"])
         nn += int(sc["totalNumOfNodes"])
         ne += int(sc["totalNumOfElements"])
         nf += int(sc["nftnd"])
         sizes.append(int(sc["totalNumOfElements"]))
         members.append(Solver(bundle, sc, "fp32", device=0, nstep=total))
+        for k, v in kernel_bytes(bundle, sc, members[-1].layout_info(), "fp32").items():
+            kbytes[k] += v
         del bundle
     dev_gb = sum(m.device_bytes() for m in members) / 1e9
     log(f"c5: {len(members)} scenarios, {ne} elements, {nn} nodes, {nf} fault pairs, {dev_gb:.1f} GB on the device, "
@@ -458,7 +554,9 @@ def bench_c5(args) -> int:
     for m in members:
         m.close()
     value = 3.0 * nn * steps / wall / 1e6
-    roof = roofline({"ktimes": ktimes}, counts, "fp32", False, "c5")
+    copy_gbs = measure_copy_gbs()
+    log(f"device copy: {copy_gbs:.1f} GB/s (read + write)")
+    roof = roofline({"ktimes": ktimes}, kbytes, "fp32", "c5", wall / steps * 1e3, copy_gbs)
     roof["kernel_times_from"] = f"uncaptured pass of {kt} steps after the timed steps"
     cpu = None
     if args.cpu_baseline in ("auto", "reference"):

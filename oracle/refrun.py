@@ -12,6 +12,8 @@ from eqdyna_amd.bundle import load_arrays, load_scalars
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 REF_BIN = os.path.join(HERE, "_ref", "eqd_ref")
+# timing-only build (-O3), the CPU baseline; never used for parity
+REF_BIN_O3 = os.path.join(HERE, "_ref", "eqd_ref_o3")
 MPIEXEC = "/opt/conda/bin/mpiexec"
 
 
@@ -22,12 +24,13 @@ def available() -> bool:
 LAST_PHASES = {}
 
 
-def run_reference(case, workdir: str, nranks: int = 1, dump: bool = True, timeout: int = 3600):
-    """Write the case inputs into ``workdir``, run the reference on ``nranks``
-    MPI ranks (ranks = case.nx*case.ny*case.nz must hold), and return
-    (bundle_in, scalars, ref_out, t_loop_seconds).  Dumps need one rank."""
-    if not available():
-        raise FileNotFoundError(f"{REF_BIN} is not built (make -f oracle/ref.mk)")
+def run_reference(case, workdir: str, nranks: int = 1, dump: bool = True, timeout: int = 3600, binary: str = REF_BIN):
+    """Write the case inputs into ``workdir``, run the reference (``binary``:
+    the parity build, or REF_BIN_O3 for timing) on ``nranks`` MPI ranks
+    (ranks = case.nx*case.ny*case.nz must hold), and return (bundle_in,
+    scalars, ref_out, t_loop_seconds).  Dumps need one rank."""
+    if not os.path.exists(binary):
+        raise FileNotFoundError(f"{binary} is not built (make -f oracle/ref.mk)")
     if nranks != case.nx * case.ny * case.nz:
         raise ValueError("nranks must equal case.nx*case.ny*case.nz")
     if dump and nranks != 1:
@@ -35,9 +38,9 @@ def run_reference(case, workdir: str, nranks: int = 1, dump: bool = True, timeou
     case.write_inputs(workdir, dump=dump)
     env = dict(os.environ, OMP_NUM_THREADS="1")
     if nranks == 1:
-        cmd = [REF_BIN]
+        cmd = [binary]
     else:
-        cmd = [MPIEXEC, "-n", str(nranks), REF_BIN]
+        cmd = [MPIEXEC, "-n", str(nranks), binary]
     p = subprocess.run(cmd, cwd=workdir, env=env, capture_output=True, text=True, timeout=timeout)
     if p.returncode != 0 or "EQD_REF" not in p.stdout:
         raise RuntimeError(f"reference run failed ({p.returncode}):\n{p.stdout[-3000:]}\n{p.stderr[-3000:]}")
