@@ -332,6 +332,199 @@ def tpv8() -> Case:
     return c
 
 
+TPV1053D_ON = [[0.0, -3.0], [0.0, -7.5], [0.0, -12.0], [9.0, -7.5], [12.0, -3.0], [12.0, -12.0], [15.0, -7.5],
+               [18.0, -7.5], [-9.0, -7.5], [-12.0, -3.0], [-12.0, -12.0], [-15.0, -7.5], [-18.0, -7.5]]
+TPV1053D_OFF = [[0, 9, 0], [0, -9, 0], [12, 6, 0], [12, -6, 0], [-12, 6, 0], [-12, -6, 0]]
+
+
+def tpv104() -> Case:
+    """SCEC TPV104 as configured by case_input/test.tpv104/user_defined_params.py
+    (the case behind test.reference.results/test.tpv104): slip law with strong
+    velocity weakening (friclaw 4), smooth-boxcar fields of a and Vw, TPV104
+    nucleation, the class defaults of scripts/defaultParameters.py otherwise
+    (stations included), 2 x 2 x 1 ranks."""
+    c = Case(name="tpv104")
+    c.xmin, c.xmax = -38.0e3, 38.0e3
+    c.ymin, c.ymax = -20.0e3, 22.0e3
+    c.zmin, c.zmax = -38.0e3, 0.0e3
+    c.fxmin, c.fxmax = -18.0e3, 18.0e3
+    c.fymin, c.fymax = 0.0, 0.0
+    c.fzmin, c.fzmax = -18.0e3, 0.0e3
+    c.xsource, c.ysource, c.zsource = 0.0, 0.0, -7.5e3
+    c.term = 5.0
+    c.dx = c.dy = c.dz = 500.0
+    c.dt = 0.5 * c.dx / c.vp
+    c.friclaw = 4
+    c.tpv = 104
+    c.nucR = 3.0e3
+    c.nucdtau0 = 45.0e6
+    c.nx, c.ny, c.nz = 2, 2, 1
+    o = c.new_ofv()
+    for ix, x in enumerate(c.fx):
+        for iz, z in enumerate(c.fz):
+            o[1, iz, ix], o[2, iz, ix], o[3, iz, ix] = 0.18, 0.12, 0.3
+            o[7, iz, ix], o[8, iz, ix] = -120.0e6, 40.0e6
+            t1 = boxcar_b1(x, 15.0e3, 3.0e3)
+            t2 = boxcar_b1(-z - 7.5e3, 15.0e3 / 2.0, 3.0e3)
+            o[9, iz, ix] = 0.01 + (1.0 - t1 * t2) * 0.01
+            o[10, iz, ix], o[11, iz, ix], o[12, iz, ix], o[13, iz, ix] = 0.014, 0.4, 1e-6, 0.6
+            o[14, iz, ix] = 0.2
+            o[15, iz, ix] = 0.1 + 0.9 * (1.0 - t1 * t2)
+            o[16, iz, ix] = 4.0e-4 + 1.0 * (1.0 - t1 * t2)
+            o[17, iz, ix], o[18, iz, ix], o[19, iz, ix] = 1.0e-6, 2.7e6, 0.1e6
+            o[40, iz, ix], o[41, iz, ix], o[42, iz, ix] = 0.02, 483.15, 80.0e6
+            o[46, iz, ix] = 1e-16
+            o[20, iz, ix] = state_steady_state(o[9, iz, ix], o[10, iz, ix], o[11, iz, ix], o[12, iz, ix],
+                                               o[13, iz, ix], o[8, iz, ix], o[7, iz, ix], o[46, iz, ix], 4)
+    c.st_on_fault = [list(s) for s in TPV1053D_ON]
+    c.st_off_fault = [list(s) for s in TPV1053D_OFF]
+    return c
+
+
+def _meng2023(name: str) -> Case:
+    """Shared set-up of the meng2023 cases (case_input/test.meng2023a,
+    test.meng2023cb): a five-layer medium, time weakening (friclaw 2) with the
+    TPV201-style forced nucleation, 400 m cells, 2 x 2 x 1 ranks.  roumax and
+    vmaxPML keep the class defaults because scripts/defaultParameters.py
+    evaluates them before the case sets its layers."""
+    c = Case(name=name)
+    c.xmin, c.xmax = -30.0e3, 30.0e3
+    c.ymin, c.ymax = -28.0e3, 32.0e3
+    c.zmin, c.zmax = -34.0e3, 0.0e3
+    c.fxmin, c.fxmax = -6.0e3, 6.0e3
+    c.fymin, c.fymax = 0.0, 0.0
+    c.fzmin, c.fzmax = -8.0e3, 0.0e3
+    c.xsource, c.ysource, c.zsource = 0.0, 0.0, -3.4e3
+    c.dx = c.dy = c.dz = 400.0
+    c.nuni_y_plus = c.nuni_y_minus = 10
+    c.enlarging_ratio = 1.025
+    c.nmat, c.n2mat = 5, 4
+    c.mat = np.array([[1.19e3, 2.74e3, 1.45e3, 2.1e3], [2.01e3, 5.75e3, 3.06e3, 2.4e3],
+                      [4.94e3, 5.72e3, 3.4e3, 2.6e3], [10.94e3, 6.18e3, 3.62e3, 2.8e3], [1.e6, 6.32e3, 3.67e3, 2.8e3]])
+    c.term = 5.0
+    c.dt = 0.5 * c.dx / 6.32e3
+    c.friclaw = 2
+    c.tpv = 201
+    c.nucR = 25.e3
+    c.nucRuptVel = 1.5e3
+    c.nucdtau0 = -9999.
+    c.nx, c.ny, c.nz = 2, 2, 1
+    c.st_on_fault = [list(s) for s in TPV1053D_ON]
+    c.st_off_fault = [list(s) for s in TPV1053D_OFF]
+    c.new_ofv()
+    return c
+
+
+def meng2023a() -> Case:
+    """case_input/test.meng2023a/user_defined_params.py: a rupturable patch
+    of 4.5 x 4.5 km with a tapered dynamic friction, locked elsewhere."""
+    c = _meng2023("meng2023a")
+    o = c.ofv
+    fs, d0, t0 = 0.4, 0.3, 0.2
+    for ix, x in enumerate(c.fx):
+        for iz, z in enumerate(c.fz):
+            o[1, iz, ix] = o[2, iz, ix] = 1000.
+            if abs(x) < 2.25e3 and z < -3.4e3 + 2.25e3 and z > -3.4e3 - 2.25e3:
+                t1 = t2 = 1.
+                if abs(x) >= 1.25e3:
+                    t1 = (2.25e3 - abs(x)) / 1.0e3
+                if abs(z - -3.4e3) >= 1.25e3:
+                    t2 = (2.25e3 - abs(z - -3.4e3)) / 1.0e3
+                o[1, iz, ix] = fs
+                o[2, iz, ix] = fs - 0.1 * t1 * t2
+            o[3, iz, ix], o[5, iz, ix] = d0, t0
+            o[7, iz, ix], o[8, iz, ix] = -100.0e6, 35.0e6
+    return c
+
+
+def meng2023cb() -> Case:
+    """case_input/test.meng2023cb/user_defined_params.py: a 6 x 6 km patch
+    with four strengthening barriers."""
+    c = _meng2023("meng2023cb")
+    o = c.ofv
+    fs, d0, t0 = 0.4, 0.3, 0.2
+    for ix, x in enumerate(c.fx):
+        for iz, z in enumerate(c.fz):
+            o[1, iz, ix], o[2, iz, ix] = 0.4, 0.8
+            if abs(x) < 3.e3 and z > -6.4e3 and z < -0.4e3:
+                t1 = t2 = 1.
+                for cx, cz in ((2.0e3, -3.4e3), (-2.0e3, -3.4e3), (0.0, -1.4e3), (0.0, -5.4e3)):
+                    if abs(x - cx) < 1.0e3 and abs(z - cz) < 1.0e3:
+                        t1, t2 = 1., -4.
+                o[1, iz, ix] = fs
+                o[2, iz, ix] = fs - 0.1 * t1 * t2
+            o[3, iz, ix], o[5, iz, ix] = d0, t0
+            o[7, iz, ix], o[8, iz, ix] = -100.0e6, 35.0e6
+    return c
+
+
+def tpv10() -> Case:
+    """SCEC TPV10 as configured by case_input/test.tpv10/user_defined_params.py:
+    a 60-degree dipping planar fault inserted as a shifted-node interface
+    (insertFaultType 1) with the grid and time step of scripts/defaultParameters.py
+    mod4dip, slip weakening with cohesion, depth-dependent normal and dip-direction
+    shear stress, 2 x 2 x 1 ranks.  bFault_Rough_Geometry.txt follows the
+    construction of scripts/generateFaultInterface for a planar fault: y offset
+    (nz - 1 - j) dx cos(dip) at fault row j, the slopes by numpy.gradient, every
+    value as printed with '%f'."""
+    c = Case(name="tpv10")
+    dip = 60.0
+    c.dip = dip
+    c.xmin, c.xmax = -30.0e3, 30.0e3
+    c.ymin, c.ymax = -20.0e3, 40.0e3
+    c.zmin, c.zmax = -35.0e3, 0.0e3
+    c.fxmin, c.fxmax = -15.0e3, 15.0e3
+    c.fzmin, c.fzmax = -15.0e3, 0.0e3
+    c.xsource = 0.0
+    c.dx = c.dy = 500.
+    c.nmat = 1
+    c.vp, c.vs, c.rou = 5716, 3300, 2700
+    c.term = 5.
+    c.C_elastic = 1
+    c.insertFaultType = 1
+    c.friclaw = 1
+    c.tpv = 10
+    c.C_nuclea = 0
+    c.nucR = 1.5e3
+    s, co = math.sin(dip / 180. * math.pi), math.cos(dip / 180. * math.pi)
+    c.fzmin = -15.0e3 * s
+    c.ysource, c.zsource = 12.0e3 * co, -12.0e3 * s
+    c.dz = c.dx * s
+    c.dt = 0.5 * c.dz / c.vp
+    c.nx, c.ny, c.nz = 2, 2, 1
+    o = c.new_ofv()
+    for ix, x in enumerate(c.fx):
+        for iz, z in enumerate(c.fz):
+            dd = abs(z) / math.sin(abs(dip) / 180. * math.pi)
+            o[1, iz, ix] = 0.76
+            if abs(abs(x) - c.fxmax) < 0.01 or abs(z - c.fzmin) < 0.01:
+                o[1, iz, ix] = 1000.
+            o[2, iz, ix], o[3, iz, ix], o[4, iz, ix] = 0.448, 0.5, 0.2e6
+            o[7, iz, ix] = -7378. * dd
+            o[8, iz, ix] = 0.
+            o[49, iz, ix] = -abs(0.55 * o[7, iz, ix])
+            if abs(x - c.xsource) <= 1.5e3 and abs(z - c.zsource) <= 1.5e3 * math.sin(abs(dip) / 180. * math.pi):
+                o[49, iz, ix] = -0.2e6 - abs((0.76 + 0.0057) * o[7, iz, ix])
+    C = math.sin(dip / 180. * math.pi)
+    c.st_on_fault = [[0.0, -1.5 * C], [0.0, -3.0 * C], [0.0, -4.5 * C], [0.0, -7.5 * C], [0.0, 0.0], [4.5, 0.0],
+                     [12.0, 0.0], [0.0, -12.0 * C], [4.5, -7.5 * C], [12.0, -7.5 * C]]
+    t = math.tan(abs(dip) / 180. * math.pi)
+    c.st_off_fault = [[0, 1, 0], [0, -1, 0], [0, 2, 0], [0, -2, 0], [0, 3, 0], [0, -3, 0], [12, 3, 0], [12, -3, 0],
+                      [0, 0.5 + 0.3 / t, -0.3], [0, -0.5 + 0.3 / t, -0.3]]
+    nx, nz = c.nfx, c.nfz
+    surf = np.zeros((nz, nx))
+    for i in range(nx):
+        for j in range(nz):
+            surf[j, i] = surf[j, i] + (nz - 1 - j) * c.dx * co
+    ddx = np.gradient(surf / c.dx, axis=1)
+    ddz = np.gradient(surf / c.dz, axis=0)
+    pr = lambda v: float("%f" % v)  # noqa: E731  (the generator writes '%f')
+    c.rough_nx, c.rough_nz = nx, nz
+    c.rough_origin = (pr(c.fxmin), pr(c.fzmin), pr(c.dx))
+    c.rough = np.array([[pr(surf[j, i]), pr(ddx[j, i]), pr(ddz[j, i])] for i in range(nx) for j in range(nz)])
+    return c
+
+
 def box_sw(nx_half: int = 6, nz: int = 6, ny_half: int = 3, d: float = 200.0,
            npml: int = 3, steps: int = 40, friclaw: int = 1, layered: bool = False,
            name: str = "box_sw") -> Case:

@@ -6,7 +6,7 @@ import numpy as np
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 CASES = ["box_sw", "box_tw", "box_layered", "box_rsf3", "box_rsf4", "box_q", "box_plastic", "box_hg2",
-         "box_dip", "box_rough", "box_tp", "box_c5sw", "box_c5rsf"]
+         "box_dip", "box_rough", "box_tp", "box_c5sw", "box_c5rsf", "box_tpv2802", "box_tpv202"]
 
 # fric slots the time step writes (1-based, src/faulting.f90)
 FRIC_SLOTS = [20, 23, 24, 31, 32, 33, 34, 35, 36, 47, 48, 71, 72, 73, 74, 75, 76, 77, 78, 79, 80]

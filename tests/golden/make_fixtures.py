@@ -31,6 +31,15 @@ def cases():
     q.C_Q = 1
     hg2 = case.box_sw(name="box_hg2")
     hg2.C_hg = 2
+    # TPV2802 nucleation (src/faulting.f90:388-402): the state variable and
+    # fric(23) reset from the trial traction at the first step, then the
+    # smooth stress perturbation
+    n2802 = case.box_rsf(friclaw=4, name="box_tpv2802")
+    n2802.tpv = 2802
+    n2802.ofv[20] += 0.02  # off steady state: only the step-1 reset brings it back
+    # TPV202 forced rupture front of slip weakening (src/faulting.f90:426-439)
+    n202 = case.box_sw(name="box_tpv202")
+    n202.C_nuclea, n202.tpv = 1, 202
     return [
         case.box_sw(name="box_sw"),
         case.box_sw(friclaw=2, name="box_tw"),
@@ -45,6 +54,8 @@ def cases():
         case.box_tp(name="box_tp"),
         case.box_c5(11, rsf=False, name="box_c5sw"),
         case.box_c5(12, rsf=True, name="box_c5rsf"),
+        n2802,
+        n202,
     ]
 
 

@@ -1,7 +1,8 @@
 """The reference build (oracle/_ref, compiled from the reference's own
-sources) against the results the reference publishes for two of its cases
-(test.reference.results/test.tpv8, test.tpv1053d: frt.txt of the ranks that
-hold fault nodes, E18.7 text, src/library_output.f90:160-205).  This pins the
+sources) against the results the reference publishes for its cases
+(test.reference.results/test.tpv8, tpv1053d, tpv104, tpv10, meng2023a,
+meng2023cb: frt.txt of the ranks that hold fault nodes, E18.7 text,
+src/library_output.f90:160-205).  This pins the
 build that writes the golden fixtures.  Runs only where the reference tree
 and the build are present (the CPU container), on 4 MPI ranks as published.
 
@@ -23,7 +24,8 @@ from eqdyna_amd import case
 PUBLISHED = "/root/reference/test.reference.results"
 
 
-@pytest.mark.parametrize("name,skip_cols", [("tpv8", [7]), ("tpv1053d", [7, 22])])
+@pytest.mark.parametrize("name,skip_cols", [("tpv8", [7]), ("tpv1053d", [7, 22]), ("tpv104", [7]), ("tpv10", [7]),
+                                            ("meng2023a", [7]), ("meng2023cb", [7])])
 def test_reference_build_reproduces_published_results(name, skip_cols):
     if not refrun.available() or not os.path.isdir(os.path.join(PUBLISHED, "test." + name)):
         pytest.skip("needs the reference tree and its build (CPU container only)")
