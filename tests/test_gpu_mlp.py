@@ -143,3 +143,11 @@ def test_learned_guess_vs_reference(name):
         assert st["iterations"] < plain["iterations"], arm
     gain = 1.0 - res["policy_learned"][0]["iterations"] / res["policy_plain"][0]["iterations"]
     print(f"[{name}] network gain in iterations over the plain-guess policy: {100 * gain:.1f} %", flush=True)
+    # What the network gains over the same policy without it, trained on this
+    # case's own 5,460 harvested solves: on the ageing-law case it saves
+    # iterations (measured 9.7 %); on the two slip-law cases, whose state
+    # jumps at the front, it does not (measured -4.4 % on box_rsf4 and
+    # -31.5 % on box_rough): the gate lets through proposals that start Newton
+    # further from the root than the previous step's rate.
+    if name == "box_rsf3":
+        assert gain > 0.05, gain
