@@ -504,6 +504,20 @@ static void keep_copy(eqd_handle *h, int f, std::map<int, std::vector<T>> &dst)
     dst[f].assign(p, p + sp.count);
 }
 
+namespace eqd {
+
+int settle(eqd_handle *h)
+{
+    if (!h || h->inflight == 0) return EQD_OK;
+    const int32_t n = h->inflight;
+    h->inflight = 0;
+    const int rc = engine_finish(h->eng, n);
+    if (rc) h->err = h->eng->err;
+    return rc;
+}
+
+}  // namespace eqd
+
 extern "C" {
 
 int eqd_create(const eqd_scalars *s, eqd_handle **out)
@@ -522,6 +536,7 @@ int eqd_create(const eqd_scalars *s, eqd_handle **out)
 int eqd_set(eqd_handle *h, int32_t field, const void *host, int64_t count)
 {
     if (!h) return EQD_EINVAL;
+    SETTLE(h);
     FieldSpec sp;
     if (!field_spec(h->s, field, sp)) return fail(h, EQD_EINVAL, "unknown field %d", field);
     if (count != sp.count)
@@ -1161,10 +1176,37 @@ extern "C" int eqd_run(eqd_handle *h, int32_t nsteps)
     if (!h->committed) return fail(h, EQD_ESTATE, "eqd_run before eqd_commit");
     if (h->ensemble) return fail(h, EQD_ESTATE, "the handle belongs to an ensemble: run the ensemble");
     if (nsteps < 0) return fail(h, EQD_EINVAL, "nsteps < 0");
+    SETTLE(h);
     if (nsteps == 0) return EQD_OK;
     const int rc = engine_run(h->eng, nsteps);
     if (rc) h->err = h->eng->err;
     return rc;
+}
+
+extern "C" int eqd_run_async(eqd_handle *h, int32_t nsteps)
+{
+    if (!h) return EQD_EINVAL;
+    if (!h->committed) return fail(h, EQD_ESTATE, "eqd_run_async before eqd_commit");
+    if (h->ensemble) return fail(h, EQD_ESTATE, "the handle belongs to an ensemble: run the ensemble");
+    if (nsteps < 0) return fail(h, EQD_EINVAL, "nsteps < 0");
+    SETTLE(h);
+    if (nsteps == 0) return EQD_OK;
+    const int rc = engine_launch(h->eng, nsteps);
+    if (rc) {
+        h->err = h->eng->err;
+        // whatever was enqueued finishes before the error is returned
+        (void)hipStreamSynchronize(h->eng->st);
+        (void)hipStreamSynchronize(h->eng->st2);
+        return rc;
+    }
+    h->inflight = nsteps;
+    return EQD_OK;
+}
+
+extern "C" int eqd_wait(eqd_handle *h)
+{
+    if (!h) return EQD_EINVAL;
+    return settle(h);
 }
 
 template <typename T>
@@ -1209,6 +1251,7 @@ extern "C" {
 int eqd_get(eqd_handle *h, int32_t field, void *host, int64_t count)
 {
     if (!h) return EQD_EINVAL;
+    SETTLE(h);
     if (!h->committed) return fail(h, EQD_ESTATE, "eqd_get before eqd_commit");
     FieldSpec sp;
     if (!field_spec(h->s, field, sp)) return fail(h, EQD_EINVAL, "unknown field %d", field);
@@ -1327,6 +1370,7 @@ int eqd_get(eqd_handle *h, int32_t field, void *host, int64_t count)
 int eqd_clock(eqd_handle *h, int32_t *nt, double *time_elapsed)
 {
     if (!h) return EQD_EINVAL;
+    SETTLE(h);
     if (nt) *nt = h->nt;
     if (time_elapsed) *time_elapsed = h->time;
     return EQD_OK;
@@ -1335,6 +1379,7 @@ int eqd_clock(eqd_handle *h, int32_t *nt, double *time_elapsed)
 int eqd_nan_info(eqd_handle *h, int32_t *node, int32_t *nt)
 {
     if (!h || !h->committed) return EQD_EINVAL;
+    (void)settle(h);  // the NaN a run in flight found is reported below
     unsigned long long key;
     HIPCHK(h, hipMemcpy(&key, h->N.nanflag, sizeof(key), hipMemcpyDeviceToHost));
     const bool none = key == ~0ull;
@@ -1346,6 +1391,7 @@ int eqd_nan_info(eqd_handle *h, int32_t *node, int32_t *nt)
 int eqd_kernel_time(eqd_handle *h, const char *name, double *avg_ms, int64_t *launches)
 {
     if (!h || !name) return EQD_EINVAL;
+    SETTLE(h);
     if (!h->eng) return fail(h, EQD_ESTATE, "eqd_kernel_time before eqd_commit");
     auto it = h->eng->timers.find(name);
     if (it == h->eng->timers.end() || it->second.launches == 0) {
@@ -1361,6 +1407,7 @@ int eqd_kernel_time(eqd_handle *h, const char *name, double *avg_ms, int64_t *la
 int eqd_set_timing(eqd_handle *h, int32_t on)
 {
     if (!h) return EQD_EINVAL;
+    SETTLE(h);
     h->timing = on != 0;
     if (h->eng) h->eng->timing = h->timing;
     return EQD_OK;
@@ -1371,6 +1418,7 @@ int64_t eqd_device_bytes(eqd_handle *h) { return h ? h->bytes : 0; }
 int eqd_set_guess(eqd_handle *h, const float *params, int64_t nparams, int32_t k_fallback, float gate)
 {
     if (!h) return EQD_EINVAL;
+    SETTLE(h);
     if (!h->committed) return fail(h, EQD_ESTATE, "eqd_set_guess before eqd_commit");
     if (k_fallback < 0 || k_fallback > 20) return fail(h, EQD_EINVAL, "k_fallback %d outside 0..20", k_fallback);
     if (!params || nparams == 0) {
@@ -1421,6 +1469,7 @@ int eqd_set_guess(eqd_handle *h, const float *params, int64_t nparams, int32_t k
 int eqd_harvest(eqd_handle *h, int32_t every, int64_t capacity)
 {
     if (!h) return EQD_EINVAL;
+    SETTLE(h);
     if (!h->committed) return fail(h, EQD_ESTATE, "eqd_harvest before eqd_commit");
     if (capacity < 0 || (capacity > 0 && every < 1)) return fail(h, EQD_EINVAL, "bad harvest every/capacity");
     HIPCHK(h, hipSetDevice(h->s.device));
@@ -1448,6 +1497,7 @@ int eqd_harvest(eqd_handle *h, int32_t every, int64_t capacity)
 int eqd_harvest_get(eqd_handle *h, float *out, int64_t max_rows, int64_t *rows)
 {
     if (!h || !rows || max_rows < 0 || (max_rows > 0 && !out)) return EQD_EINVAL;
+    SETTLE(h);
     if (!h->committed) return fail(h, EQD_ESTATE, "eqd_harvest_get before eqd_commit");
     HIPCHK(h, hipSetDevice(h->s.device));
     HIPCHK(h, hipStreamSynchronize(h->stream));
@@ -1462,42 +1512,61 @@ int eqd_harvest_get(eqd_handle *h, float *out, int64_t max_rows, int64_t *rows)
     return EQD_OK;
 }
 
-// record index of the next record step after step nt
-static int32_t next_record(int32_t nt, int32_t every) { return nt <= 0 ? 0 : (nt - 1) / every + 1; }
-
-int eqd_record(eqd_handle *h, const int32_t *node_ids, int32_t n_nodes, int32_t every, int32_t capacity)
+static void free_records(eqd_handle *h)
 {
-    if (!h) return EQD_EINVAL;
-    if (!h->committed) return fail(h, EQD_ESTATE, "eqd_record before eqd_commit");
-    if (n_nodes < 0 || (n_nodes > 0 && !node_ids) || capacity < 0 || (capacity > 0 && every < 1))
-        return fail(h, EQD_EINVAL, "bad record nodes/every/capacity");
-    for (int32_t i = 0; i < n_nodes; i++)
-        if (node_ids[i] < 1 || node_ids[i] > h->s.nn) return fail(h, EQD_EINVAL, "record node %d out of range", i + 1);
-    HIPCHK(h, hipSetDevice(h->s.device));
-    HIPCHK(h, hipStreamSynchronize(h->stream));
+    RecStream &q = h->rq;
+    if (q.cst) (void)hipStreamSynchronize(q.cst);
+    for (const RecSeg &g : q.segs)
+        if (g.ev) (void)hipEventDestroy(g.ev);
+    for (hipEvent_t ev : q.spare) (void)hipEventDestroy(ev);
+    if (q.cst) (void)hipStreamDestroy(q.cst);
+    if (q.hvel) (void)hipHostFree(q.hvel);
+    if (q.hslip) (void)hipHostFree(q.hslip);
+    q = RecStream{};
     for (void *&p : h->rec_bufs) {
         if (p) (void)hipFree(p);
         p = nullptr;
     }
     h->d_rec = nullptr;
     h->rec = Recorder{};
+}
+
+int eqd_record(eqd_handle *h, const int32_t *node_ids, int32_t n_nodes, int32_t every, int32_t capacity)
+{
+    if (!h) return EQD_EINVAL;
+    if (!h->committed) return fail(h, EQD_ESTATE, "eqd_record before eqd_commit");
+    SETTLE(h);
+    if (n_nodes < 0 || (n_nodes > 0 && !node_ids) || capacity < 0 || (capacity > 0 && every < 1))
+        return fail(h, EQD_EINVAL, "bad record nodes/every/capacity");
+    for (int32_t i = 0; i < n_nodes; i++)
+        if (node_ids[i] < 1 || node_ids[i] > h->s.nn) return fail(h, EQD_EINVAL, "record node %d out of range", i + 1);
+    HIPCHK(h, hipSetDevice(h->s.device));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    free_records(h);
     h->version++;
     if (capacity == 0) return EQD_OK;
     std::vector<int32_t> ids(node_ids, node_ids + n_nodes);
     for (int32_t &v : ids) v -= 1;
     const size_t nv = sizeof(double) * 3 * (size_t)capacity * (size_t)std::max(n_nodes, 1);
     const size_t ns = sizeof(double) * (size_t)capacity * (size_t)std::max<int64_t>(h->F.nf, 1);
+    RecStream &q = h->rq;
     if (hipMalloc(&h->rec_bufs[0], sizeof(Recorder) + sizeof(int32_t) * std::max(n_nodes, 1)) != hipSuccess ||
-        hipMalloc(&h->rec_bufs[1], nv) != hipSuccess || hipMalloc(&h->rec_bufs[2], ns) != hipSuccess)
+        hipMalloc(&h->rec_bufs[1], nv) != hipSuccess || hipMalloc(&h->rec_bufs[2], ns) != hipSuccess ||
+        hipHostMalloc((void **)&q.hvel, nv, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void **)&q.hslip, ns, hipHostMallocDefault) != hipSuccess) {
+        free_records(h);
         return fail(h, EQD_EOOM, "record buffers of %d records", capacity);
+    }
+    // the copies run on a stream of their own (default priority), beside the steps
+    HIPCHK(h, hipStreamCreateWithFlags(&q.cst, hipStreamNonBlocking));
     Recorder &R = h->rec;
     R.every = every;
     R.capacity = capacity;
     R.n_nodes = n_nodes;
-    R.base = next_record(h->nt, every);  // the first record is the next record step
     R.nodes = (const int32_t *)((char *)h->rec_bufs[0] + sizeof(Recorder));
     R.vel = (double *)h->rec_bufs[1];
     R.slip_rate = h->F.nf > 0 ? (double *)h->rec_bufs[2] : nullptr;
+    q.drained = q.next = next_record(h->nt, every);  // the first record is the next record step
     if (n_nodes > 0)
         HIPCHK(h, hipMemcpy((void *)R.nodes, ids.data(), sizeof(int32_t) * n_nodes, hipMemcpyHostToDevice));
     h->d_rec = (Recorder *)h->rec_bufs[0];
@@ -1512,31 +1581,44 @@ int eqd_record_get(eqd_handle *h, double *vel, double *slip_rate, int32_t max_re
     *count = 0;
     if (!h->d_rec) return EQD_OK;
     HIPCHK(h, hipSetDevice(h->s.device));
-    HIPCHK(h, hipStreamSynchronize(h->stream));
-    Recorder &R = h->rec;
-    const int32_t have = next_record(h->nt, R.every) - R.base;  // record steps since the last drain
-    if (std::min(have, R.capacity) > max_records)
-        return fail(h, EQD_EINVAL, "%d records waiting, room for %d", std::min(have, R.capacity), max_records);
-    const int32_t n = std::min(have, R.capacity);
-    if (n > 0) {
-        if (vel && R.n_nodes > 0)
-            HIPCHK(h, hipMemcpy(vel, R.vel, sizeof(double) * 3 * R.n_nodes * (size_t)n, hipMemcpyDeviceToHost));
-        if (slip_rate && R.slip_rate)
-            HIPCHK(h, hipMemcpy(slip_rate, R.slip_rate, sizeof(double) * h->F.nf * (size_t)n, hipMemcpyDeviceToHost));
+    RecStream &q = h->rq;
+    const Recorder &R = h->rec;
+    // every finished run's records; a run still in flight keeps its own
+    const size_t nseg = q.segs.size() - (h->inflight > 0 && !q.segs.empty() ? 1 : 0);
+    int32_t n = 0, dropped = 0;
+    for (size_t i = 0; i < nseg; i++) {
+        n += q.segs[i].end - q.segs[i].begin;
+        dropped += q.segs[i].next - q.segs[i].end;
+    }
+    if (n > max_records) return fail(h, EQD_EINVAL, "%d records waiting, room for %d", n, max_records);
+    const int64_t nv = 3 * (int64_t)R.n_nodes, nf = h->F.nf;
+    int32_t k = 0;
+    for (size_t i = 0; i < nseg; i++) {
+        RecSeg g = q.segs.front();
+        q.segs.pop_front();
+        if (g.ev) {
+            const hipError_t ec = hipEventSynchronize(g.ev);
+            q.spare.push_back(g.ev);
+            if (ec != hipSuccess) return fail(h, EQD_EHIP, "record copy: %s", hipGetErrorString(ec));
+        }
+        for (int32_t r = g.begin; r < g.end; r++, k++) {
+            const int64_t a = r % R.capacity;
+            if (vel && nv > 0) memcpy(vel + k * nv, q.hvel + a * nv, sizeof(double) * nv);
+            if (slip_rate && nf > 0) memcpy(slip_rate + k * nf, q.hslip + a * nf, sizeof(double) * nf);
+        }
+        q.drained = g.next;
     }
     *count = n;
-    // the buffer starts over at the next record step
-    R.base = next_record(h->nt, R.every);
-    HIPCHK(h, hipMemcpy(h->d_rec, &R, sizeof(Recorder), hipMemcpyHostToDevice));
-    if (have > n)
+    if (dropped > 0)
         return fail(h, EQD_EINVAL, "%d record steps since the last eqd_record_get, capacity %d: the last %d dropped",
-                    have, R.capacity, have - n);
+                    n + dropped, R.capacity, dropped);
     return EQD_OK;
 }
 
 int eqd_newton_stats(eqd_handle *h, int64_t stats[5])
 {
     if (!h || !stats) return EQD_EINVAL;
+    SETTLE(h);
     if (!h->committed) return fail(h, EQD_ESTATE, "eqd_newton_stats before eqd_commit");
     HIPCHK(h, hipSetDevice(h->s.device));
     HIPCHK(h, hipStreamSynchronize(h->stream));
@@ -1575,6 +1657,7 @@ int eqd_destroy(eqd_handle *h)
 {
     if (!h) return EQD_OK;
     if (h->ensemble) return fail(h, EQD_ESTATE, "the handle belongs to an ensemble: destroy the ensemble first");
+    (void)settle(h);
     (void)hipSetDevice(h->s.device);
     if (h->eng) {
         engine_free(h->eng);
@@ -1582,8 +1665,7 @@ int eqd_destroy(eqd_handle *h)
     }
     for (void *p : h->allocs) (void)hipFree(p);
     if (h->harvest) (void)hipFree(h->harvest);
-    for (void *p : h->rec_bufs)
-        if (p) (void)hipFree(p);
+    free_records(h);
     delete h;
     return EQD_OK;
 }

@@ -217,7 +217,7 @@ struct Clock {
     int32_t nt;       // current step
     int32_t mode;     // 1 on the last step of a run: store F/m instead of the fused update
     int32_t run_end;  // the step at which the current run ends
-    int32_t pad;
+    int32_t rec_lo;   // first record index the host can take in this run (Recorder)
 };
 
 // launch kinds of one step
@@ -227,10 +227,13 @@ enum LaunchKind : int {
 
 // Ground-motion and source-evolution records (the data of output_gm and
 // output_src_evol, src/library_output.f90:267-312): at every step nt with
-// (nt - 1) % every == 0 the velocities of the listed nodes and fric(47) of
-// every pair go to record slot (nt - 1) / every - base (dropped past capacity).
+// (nt - 1) % every == 0, record r = (nt - 1) / every holds the velocities of
+// the listed nodes and fric(47) of every pair.  The device buffers are a ring
+// of `capacity` records (slot r % capacity); a run writes only the records in
+// [clk->rec_lo, clk->rec_lo + capacity), the ones whose slots the host has
+// already copied out, and drops the rest.
 struct Recorder {
-    int32_t every, capacity, n_nodes, base;
+    int32_t every, capacity, n_nodes, pad;
     const int32_t *nodes;  // [n_nodes] 0-based node ids
     double *vel;           // [capacity][n_nodes][3]
     double *slip_rate;     // [capacity][nf]

@@ -340,16 +340,61 @@ int capture(eqd_engine *e)
     return EQD_OK;
 }
 
-// nsteps steps of every scenario; with mirrors the handles' host clocks
-// advance and NaN velocities are reported (the tuner's trial runs do neither)
+// ---------------- records (eqd_record): the window a run may write, and the
+// copies of its records into the pinned host ring behind its last step
+
+bool recording(const eqd_engine *e, const eqd_handle *h) { return h->d_rec && !e->no_harvest; }
+
+// Enqueue (on the handle's copy stream, behind everything st holds so far)
+// the copy of the run's records r0 <= r < r1 that fit the window starting at
+// lo, and queue the run's segment.
+int rec_copy(eqd_engine *e, eqd_handle *h, int32_t r0, int32_t r1, int32_t lo)
+{
+    RecStream &q = h->rq;
+    const Recorder &R = h->rec;
+    RecSeg g{r0, std::max(r0, std::min(r1, lo + R.capacity)), r1, nullptr};
+    if (g.end > g.begin) {
+        if (q.spare.empty()) {
+            hipEvent_t ev;
+            HIPCHK(e, hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+            q.spare.push_back(ev);
+        }
+        g.ev = q.spare.back();
+        q.spare.pop_back();
+        HIPCHK(e, hipEventRecord(e->ev_rec, e->st));
+        HIPCHK(e, hipStreamWaitEvent(q.cst, e->ev_rec, 0));
+        const int64_t nv = 3 * (int64_t)R.n_nodes, nf = h->F.nf;
+        for (int32_t r = g.begin; r < g.end;) {
+            const int32_t a = r % R.capacity, n = std::min(g.end - r, R.capacity - a);  // up to the ring's end
+            if (nv > 0)
+                HIPCHK(e, hipMemcpyAsync(q.hvel + a * nv, R.vel + a * nv, sizeof(double) * nv * n,
+                                         hipMemcpyDeviceToHost, q.cst));
+            if (nf > 0 && R.slip_rate)
+                HIPCHK(e, hipMemcpyAsync(q.hslip + a * nf, R.slip_rate + a * nf, sizeof(double) * nf * n,
+                                         hipMemcpyDeviceToHost, q.cst));
+            r += n;
+        }
+        HIPCHK(e, hipEventRecord(g.ev, q.cst));
+    }
+    q.segs.push_back(g);
+    q.next = r1;
+    return EQD_OK;
+}
+
+// nsteps steps of every scenario, enqueued (finish waits for them); with
+// mirrors the records are copied out (the tuner's trial runs do neither)
 template <typename R>
-int run(eqd_engine *e, int32_t nsteps, bool mirrors)
+int launch(eqd_engine *e, int32_t nsteps, bool mirrors)
 {
     const Member<R> *dM = (const Member<R> *)e->d_members;
     hipStream_t st = e->st;
-    // clocks from the host mirrors; the run ends at nt + nsteps
-    for (eqd_handle *h : e->hs) {
-        const Clock c = {h->time, h->nt, 0, h->nt + nsteps, 0};
+    // clocks from the host mirrors; the run ends at nt + nsteps, and writes
+    // the records from the first one not yet handed over
+    std::vector<int32_t> lo(e->hs.size(), 0);
+    for (size_t m = 0; m < e->hs.size(); m++) {
+        eqd_handle *h = e->hs[m];
+        if (mirrors && recording(e, h)) lo[m] = h->rq.drained;
+        const Clock c = {h->time, h->nt, 0, h->nt + nsteps, lo[m]};
         HIPCHK(e, hipMemcpy(h->d_clock, &c, sizeof(Clock), hipMemcpyHostToDevice));
     }
     for (auto &kv : e->timers) {
@@ -379,7 +424,22 @@ int run(eqd_engine *e, int32_t nsteps, bool mirrors)
             }
         }
     }
-    HIPCHK(e, hipStreamSynchronize(st));
+    if (mirrors)
+        for (size_t m = 0; m < e->hs.size(); m++) {
+            eqd_handle *h = e->hs[m];
+            if (!recording(e, h)) continue;
+            const int32_t every = h->rec.every;
+            int rc = rec_copy(e, h, next_record(h->nt, every), next_record(h->nt + nsteps, every), lo[m]);
+            if (rc) return rc;
+        }
+    return EQD_OK;
+}
+
+// Wait for the run that launch enqueued; with mirrors the handles' host
+// clocks advance and NaN velocities are reported.
+int finish(eqd_engine *e, int32_t nsteps, bool mirrors)
+{
+    HIPCHK(e, hipStreamSynchronize(e->st));
     HIPCHK(e, hipStreamSynchronize(e->st2));
     if (e->timing) {
         int rc = drain_timers(e);
@@ -408,6 +468,14 @@ int run(eqd_engine *e, int32_t nsteps, bool mirrors)
         }
         h->nt += steps;
         for (int32_t k = 0; k < steps; k++) h->time = h->time + h->s.dt;
+        if (steps < nsteps && recording(e, h) && !h->rq.segs.empty()) {
+            // the records after the NaN step are not the reference's: the
+            // run's segment ends there and the next run writes from there on
+            RecSeg &g = h->rq.segs.back();
+            g.next = std::max(g.begin, next_record(h->nt, h->rec.every));
+            g.end = std::min(g.end, g.next);
+            h->rq.next = g.next;
+        }
     }
     if (bad >= 0) {
         if (e->hs.size() > 1)
@@ -415,6 +483,13 @@ int run(eqd_engine *e, int32_t nsteps, bool mirrors)
         return fail(e, EQD_ENAN, "Velocity NaN at node %d at time step %d", bad_node, bad_step);
     }
     return EQD_OK;
+}
+
+template <typename R>
+int run(eqd_engine *e, int32_t nsteps, bool mirrors)
+{
+    const int rc = launch<R>(e, nsteps, mirrors);
+    return rc ? rc : finish(e, nsteps, mirrors);
 }
 
 // device state a step changes, for the tuner's save and restore
@@ -556,6 +631,8 @@ int tune_impl(eqd_engine *e, int32_t steps, std::string &report)
 
 namespace eqd {
 
+int32_t next_record(int32_t nt, int32_t every) { return nt <= 0 ? 0 : (nt - 1) / every + 1; }
+
 int engine_init(eqd_engine *e, const std::vector<eqd_handle *> &hs)
 {
     if (hs.empty()) return fail(e, EQD_EINVAL, "no scenarios");
@@ -585,12 +662,13 @@ int engine_init(eqd_engine *e, const std::vector<eqd_handle *> &hs)
     HIPCHK(e, hipStreamCreateWithFlags(&e->st2, hipStreamNonBlocking));
     HIPCHK(e, hipEventCreateWithFlags(&e->ev_fork, hipEventDisableTiming));
     HIPCHK(e, hipEventCreateWithFlags(&e->ev_join, hipEventDisableTiming));
+    HIPCHK(e, hipEventCreateWithFlags(&e->ev_rec, hipEventDisableTiming));
     const size_t msize = e->fp32 ? sizeof(Member<float>) : sizeof(Member<double>);
     HIPCHK(e, hipMalloc(&e->d_members, msize * hs.size()));
     return EQD_OK;
 }
 
-int engine_run(eqd_engine *e, int32_t nsteps)
+int engine_launch(eqd_engine *e, int32_t nsteps)
 {
     if (nsteps < 0) return fail(e, EQD_EINVAL, "nsteps < 0");
     if (nsteps == 0) return EQD_OK;
@@ -603,7 +681,20 @@ int engine_run(eqd_engine *e, int32_t nsteps)
     }
     int rc = ensure_built(e);
     if (rc) return rc;
-    return e->fp32 ? run<float>(e, nsteps, true) : run<double>(e, nsteps, true);
+    return e->fp32 ? launch<float>(e, nsteps, true) : launch<double>(e, nsteps, true);
+}
+
+int engine_finish(eqd_engine *e, int32_t nsteps)
+{
+    if (nsteps <= 0) return EQD_OK;
+    HIPCHK(e, hipSetDevice(e->device));
+    return finish(e, nsteps, true);
+}
+
+int engine_run(eqd_engine *e, int32_t nsteps)
+{
+    const int rc = engine_launch(e, nsteps);
+    return rc ? rc : engine_finish(e, nsteps);
 }
 
 int engine_tune(eqd_engine *e, int32_t steps, std::string &report)
@@ -645,6 +736,8 @@ void engine_free(eqd_engine *e)
     e->timers.clear();
     if (e->ev_fork) (void)hipEventDestroy(e->ev_fork);
     if (e->ev_join) (void)hipEventDestroy(e->ev_join);
+    if (e->ev_rec) (void)hipEventDestroy(e->ev_rec);
+    e->ev_rec = nullptr;
     if (e->st2) (void)hipStreamDestroy(e->st2);
     if (e->st) (void)hipStreamDestroy(e->st);
     e->ev_fork = e->ev_join = nullptr;
@@ -675,6 +768,7 @@ int eqd_tune(eqd_handle *h, int32_t steps, char *report, int64_t report_cap)
 {
     if (!h) return EQD_EINVAL;
     if (!h->committed) return fail(h, EQD_ESTATE, "eqd_tune before eqd_commit");
+    SETTLE(h);
     if (h->ensemble) return fail(h, EQD_ESTATE, "the handle belongs to an ensemble: tune the ensemble");
     std::string r;
     const int rc = engine_tune(h->eng, steps, r);
@@ -690,6 +784,7 @@ int eqd_set_launch(eqd_handle *h, int32_t kind, int32_t block, int32_t remap)
 {
     if (!h) return EQD_EINVAL;
     if (!h->committed) return fail(h, EQD_ESTATE, "eqd_set_launch before eqd_commit");
+    SETTLE(h);
     const int rc = engine_set_launch(h->eng, kind, block, remap);
     if (rc) h->err = h->eng->err;
     return rc;
@@ -709,6 +804,7 @@ int eqd_set_graph(eqd_handle *h, int32_t on)
 {
     if (!h) return EQD_EINVAL;
     if (!h->committed) return fail(h, EQD_ESTATE, "eqd_set_graph before eqd_commit");
+    SETTLE(h);
     h->eng->graph_on = on != 0;
     return EQD_OK;
 }
@@ -723,6 +819,7 @@ int eqd_ensemble_create(eqd_handle *const *members, int32_t n, eqd_ensemble **ou
         for (int j = 0; j < i; j++)
             if (hs[j] == hs[i]) return EQD_EINVAL;
     }
+    for (eqd_handle *h : hs) SETTLE(h);
     eqd_ensemble *E = new eqd_ensemble();
     const int rc = engine_init(&E->e, hs);
     if (rc) {

@@ -6,6 +6,7 @@
 
 #include <cstdarg>
 #include <cstdio>
+#include <deque>
 #include <map>
 #include <string>
 #include <vector>
@@ -79,6 +80,7 @@ struct eqd_engine {
     int device = 0;
     hipStream_t st = nullptr, st2 = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    hipEvent_t ev_rec = nullptr;  // a run's end, for the record copies
     bool single_stream = false;
     void *d_members = nullptr;  // Member<R>[hs.size()]
     std::vector<eqd::Group> groups[eqd::kKinds];
@@ -93,6 +95,32 @@ struct eqd_engine {
     std::map<std::string, eqd::KTimer> timers;
     std::string err;
 };
+
+namespace eqd {
+
+// The records of one run: [begin, end) were copied (or are being copied) into
+// the pinned host ring on the handle's copy stream, `ev` marks the copy's end,
+// and [end, next) were dropped for want of room (next is the following run's
+// first record).
+struct RecSeg {
+    int32_t begin, end, next;
+    hipEvent_t ev;
+};
+
+// Host side of the record ring: the device writes record r into slot
+// r % capacity of its ring, each run's records are copied to the same slot of
+// the pinned host ring on `cst` behind the run's last step, and eqd_record_get
+// hands them over from there.  drained: records before it were handed over
+// (or dropped); the window a run may write is [drained, drained + capacity).
+struct RecStream {
+    hipStream_t cst = nullptr;
+    double *hvel = nullptr, *hslip = nullptr;  // pinned [capacity][n_nodes][3], [capacity][nf]
+    int32_t drained = 0, next = 0;
+    std::deque<RecSeg> segs;
+    std::vector<hipEvent_t> spare;  // finished copy events for reuse
+};
+
+}  // namespace eqd
 
 struct eqd_handle {
     eqd_scalars s{};
@@ -131,10 +159,13 @@ struct eqd_handle {
     __bf16 *mlp_wb = nullptr;
     // harvest buffer of training rows (its schedule is in F)
     float *harvest = nullptr;
-    // ground-motion / source-evolution records: host copy and device objects
+    // ground-motion / source-evolution records: host copy and device objects,
+    // and the copies of finished records into pinned host memory
     eqd::Recorder rec{};
     eqd::Recorder *d_rec = nullptr;
     void *rec_bufs[3] = {nullptr, nullptr, nullptr};
+    eqd::RecStream rq{};
+    int32_t inflight = 0;  // steps of a run launched by eqd_run_async and not yet waited for
     bool timing = false;  // per-kernel event timing of the handle's own runs
     // bumped by every change the engine's launches depend on
     int64_t version = 0;
@@ -174,6 +205,12 @@ inline int fail(eqd_engine *e, int code, const char *fmt, ...)
 
 }  // namespace eqd
 
+#define SETTLE(h)                                                                                      \
+    do {                                                                                               \
+        const int s_ = eqd::settle(h);                                                                 \
+        if (s_) return s_;                                                                             \
+    } while (0)
+
 #define HIPCHK(h, x)                                                                                   \
     do {                                                                                               \
         hipError_t e_ = (x);                                                                           \
@@ -185,6 +222,12 @@ namespace eqd {
 // engine entry points (eqd_engine.cpp)
 int engine_init(eqd_engine *e, const std::vector<eqd_handle *> &hs);
 int engine_run(eqd_engine *e, int32_t nsteps);
+int engine_launch(eqd_engine *e, int32_t nsteps);  // enqueue a run (engine_finish waits for it)
+int engine_finish(eqd_engine *e, int32_t nsteps);
+int32_t next_record(int32_t nt, int32_t every);      // first record index at a step after nt
+// Wait for the handle's run in flight (eqd_run_async), if any: its clock
+// advances and its error (a NaN) is returned here.
+int settle(eqd_handle *h);
 int engine_tune(eqd_engine *e, int32_t steps, std::string &report);
 int engine_set_launch(eqd_engine *e, int kind, int block, int remap);
 void engine_free(eqd_engine *e);

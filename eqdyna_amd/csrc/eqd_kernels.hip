@@ -1679,10 +1679,9 @@ __global__ void __launch_bounds__(128) fault_kernel(const Member<R> *__restrict_
     // source-evolution record (output_src_evol, src/library_output.f90:297-312)
     if (M.rec) {
         const Recorder rc = *M.rec;
-        if ((nt - 1) % rc.every == 0) {
-            const int64_t slot = (nt - 1) / rc.every - rc.base;
-            if (slot >= 0 && slot < rc.capacity) rc.slip_rate[slot * nf + i] = FR(47);
-        }
+        const int32_t r = (nt - 1) / rc.every, lo = M.clk->rec_lo;
+        if ((nt - 1) % rc.every == 0 && r >= lo && r - lo < rc.capacity)
+            rc.slip_rate[(int64_t)(r % rc.capacity) * nf + i] = FR(47);
     }
 #undef FR
     // F/m (src/driver.f90:29) and the pair's update for the next step
@@ -1802,8 +1801,9 @@ __global__ void __launch_bounds__(256) record_kernel(const Member<R> *__restrict
     if (t >= rc.n_nodes) return;
     const int nt = M.clk->nt;
     if ((nt - 1) % rc.every != 0) return;
-    const int64_t slot = (nt - 1) / rc.every - rc.base;
-    if (slot < 0 || slot >= rc.capacity) return;
+    const int32_t r = (nt - 1) / rc.every, lo = M.clk->rec_lo;
+    if (r < lo || r - lo >= rc.capacity) return;
+    const int64_t slot = r % rc.capacity;
     const int node = rc.nodes[t];
     const int64_t nn = M.N.nn;
 #pragma unroll

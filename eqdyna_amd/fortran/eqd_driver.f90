@@ -61,6 +61,15 @@ module eqd_abi
             type(c_ptr), value :: h
             integer(c_int32_t), value :: nsteps
         end function eqd_run
+        integer(c_int) function eqd_run_async(h, nsteps) bind(C, name='eqd_run_async')
+            import :: c_ptr, c_int, c_int32_t
+            type(c_ptr), value :: h
+            integer(c_int32_t), value :: nsteps
+        end function eqd_run_async
+        integer(c_int) function eqd_wait(h) bind(C, name='eqd_wait')
+            import :: c_ptr, c_int
+            type(c_ptr), value :: h
+        end function eqd_wait
         integer(c_int) function eqd_get(h, field, host, count) bind(C, name='eqd_get')
             import :: c_ptr, c_int, c_int32_t, c_int64_t
             type(c_ptr), value :: h
@@ -133,11 +142,11 @@ subroutine driver
     type(c_ptr) :: h
     integer(c_int) :: rc
     integer(c_int32_t) :: ntc, badnode, badnt, chunk, done, nrec, irec, inode
-    integer(c_int32_t), parameter :: maxrec = 16
+    integer(c_int32_t), parameter :: maxrec = 32  ! the records of two chunks
     real(c_double) :: tc
     real(c_double), allocatable :: recv(:, :, :), recs(:, :)
     integer(c_int64_t) :: nf, non, noff
-    integer :: k
+    character(len=8) :: envval
 
     nf = nftnd(1)
     non = min(numOfOnFaultStCount, size(anonfs, 2))
@@ -208,19 +217,15 @@ subroutine driver
                        'record')
     endif
     ! Advance in chunks that end where the reference prints progress
-    ! (src/driver.f90:13-17), at most 100 steps (10 records).
+    ! (src/driver.f90:13-17), at most 100 steps (10 records).  Each chunk goes
+    ! to the device before the records of the one before it are written, so
+    ! the files are written while the device steps (eqd_run_async).
     done = 0
-    do while (done < nstep)
-        chunk = min(nstep - done, 100_c_int32_t)
-        if (me == masterProcsId) then
-            k = done + 1
-            if (mod(k, 100) == 1) then
-                write(*,*) '=     Current time in dynamic rupture (GPU)                         ='
-                write(*,'(X,A,40X,f7.3,4X,A)') '=', timeElapsed + dt, 's'
-            endif
-            chunk = min(chunk, int(100 - mod(done, 100), c_int32_t))
-        endif
-        rc = eqd_run(h, chunk)
+    chunk = min(nstep, 100_c_int32_t)
+    call progress(done)
+    call eqd_check(h, eqd_run_async(h, chunk), 'run')
+    do while (chunk > 0)
+        rc = eqd_wait(h)
         if (rc == -2) then
             rc = eqd_nan_info(h, badnode, badnt)
             write(*,*) 'Velocity NaN at point ', meshCoor(1,badnode), meshCoor(2,badnode), meshCoor(3,badnode), &
@@ -232,7 +237,13 @@ subroutine driver
         rc = eqd_clock(h, ntc, tc)
         nt = ntc
         timeElapsed = tc
+        chunk = min(nstep - done, int(100 - mod(done, 100), c_int32_t))
+        if (chunk > 0) then
+            call progress(done)
+            call eqd_check(h, eqd_run_async(h, chunk), 'run')
+        endif
         if (outputGroundMotion == 1) then
+            ! the records of the chunk that just ended (not of the one in flight)
             call eqd_check(h, eqd_record_get(h, recv, recs, maxrec, nrec), 'record_get')
             if (nrec > 0 .and. surface_nnode > 0) then
                 open(unit=10009+me, file='gm'//mm, status='unknown', position='append', access='stream')
@@ -268,4 +279,84 @@ subroutine driver
     if (noff > 0) call eqd_check(h, eqd_get(h, EQD_OFFFAULTHIST, OffFaultStGramSCEC, &
                                              int(size(OffFaultStGramSCEC), c_int64_t)), 'get off-fault stations')
     rc = eqd_destroy(h)
+    call get_environment_variable('EQD_FULL_PRECISION_OUTPUT', envval)
+    if (trim(envval) == '1') call output_full_precision
+
+contains
+
+    ! the progress lines of src/driver.f90:13-17, before the step after `done`
+    subroutine progress(done)
+        integer(c_int32_t), intent(in) :: done
+        if (me == masterProcsId .and. mod(done + 1, 100) == 1) then
+            write(*,*) '=     Current time in dynamic rupture (GPU)                         ='
+            write(*,'(X,A,40X,f7.3,4X,A)') '=', timeElapsed + dt, 's'
+        endif
+    end subroutine progress
 end subroutine driver
+
+! Full-precision companions of the reference's text outputs, written when
+! EQD_FULL_PRECISION_OUTPUT=1: the same rows and columns as output_frt
+! (src/library_output.f90:160-205), output_onfault_st (:16-96) and
+! output_offfault_st (:98-158) in ES25.16E3 (17 significant digits, which
+! read back to the same doubles) instead of E18.7 / E16.7, in files named
+! like the reference's with '.full' before the extension.  The reference's
+! own files are written as before.
+subroutine output_full_precision
+    use globalvar
+    implicit none
+    integer (kind = 4) :: i, j, k
+    character(len=8) :: cst, cdp, cbd
+    character(len=*), parameter :: fmt8 = '(8es25.16e3)', fmt11 = '(11es25.16e3)', fmt7 = '(7es25.16e3)'
+
+    if (nftnd(1) > 0) then
+        open(unit=40004+me, file='frt.full.txt'//mm, status='unknown')
+        do i = 1, nftnd(1)
+            write(40004+me, '(22es25.16e3)') (meshCoor(j, nsmp(1,i,1)), j = 1, 3), fnft(i,1), &
+                (fric(j,i,1), j = 71, 76), fric(47,i,1), fric(78,i,1), fric(79,i,1), fric(80,i,1), &
+                (fric(j,i,1), j = 31, 36), fric(20,i,1), fric(23,i,1)
+        enddo
+        close(40004+me)
+    endif
+    do i = 1, numOfOnFaultStCount
+        k = anonfs(3,i)
+        if (k /= 1) cycle  ! the reference names main-fault stations only
+        write(cst,'(i3.3)') nint(xonfs(1,anonfs(2,i),k)/100.d0)
+        write(cdp,'(i3.3)') nint(abs(xonfs(2,anonfs(2,i),k))/dsin(fltxyz(2,4,1))/100.d0)
+        open(51, file='faultst'//trim(adjustl(cst))//'dp'//trim(adjustl(cdp))//'.full.txt', status='unknown')
+        if (friclaw >= 3) then
+            write(51,'(a)') '# t h-slip h-slip-rate h-shear-stress v-slip v-slip-rate v-shear-stress n-stress psi' // &
+                            ' temperature pressure'
+            do j = 1, nstep
+                write(51, fmt11) onFaultQuantHistSCECForm(1,j,i), onFaultQuantHistSCECForm(5,j,i), &
+                    onFaultQuantHistSCECForm(2,j,i), onFaultQuantHistSCECForm(8,j,i)/1.0d6, &
+                    -onFaultQuantHistSCECForm(6,j,i), -onFaultQuantHistSCECForm(3,j,i), &
+                    -onFaultQuantHistSCECForm(9,j,i)/1.0d6, -onFaultQuantHistSCECForm(10,j,i)/1.0d6, &
+                    onFaultQuantHistSCECForm(4,j,i), onFaultQuantHistSCECForm(12,j,i), &
+                    onFaultQuantHistSCECForm(11,j,i)/1.0d6
+            enddo
+        else
+            write(51,'(a)') '# t h-slip h-slip-rate h-shear-stress v-slip v-slip-rate v-shear-stress n-stress'
+            do j = 1, nstep
+                write(51, fmt8) onFaultQuantHistSCECForm(1,j,i), onFaultQuantHistSCECForm(5,j,i), &
+                    onFaultQuantHistSCECForm(2,j,i), onFaultQuantHistSCECForm(8,j,i)/1.0d6, &
+                    -onFaultQuantHistSCECForm(6,j,i), -onFaultQuantHistSCECForm(3,j,i), &
+                    -onFaultQuantHistSCECForm(9,j,i)/1.0d6, -onFaultQuantHistSCECForm(10,j,i)/1.0d6
+            enddo
+        endif
+        close(51)
+    enddo
+    do i = 1, numOfOffFaultStCount
+        write(cbd,'(i4.3)') int(x4nds(2,OffFaultStNodeIdIndex(1,i))/100.d0)
+        write(cst,'(i4.3)') int(x4nds(1,OffFaultStNodeIdIndex(1,i))/100.d0)
+        write(cdp,'(i4.3)') int(abs(x4nds(3,OffFaultStNodeIdIndex(1,i)))/100.d0)
+        open(51, file='body'//trim(adjustl(cbd))//'st'//trim(adjustl(cst))//'dp'//trim(adjustl(cdp))//'.full.txt', &
+             status='unknown')
+        write(51,'(a)') '# t h-disp h-vel v-disp v-vel n-disp n-vel'
+        do j = 1, nstep
+            write(51, fmt7) OffFaultStGramSCEC(1,j), OffFaultStGramSCEC((i-1)*6+2,j), &
+                OffFaultStGramSCEC((i-1)*6+3,j), -OffFaultStGramSCEC((i-1)*6+6,j), &
+                -OffFaultStGramSCEC((i-1)*6+7,j), OffFaultStGramSCEC((i-1)*6+4,j), OffFaultStGramSCEC((i-1)*6+5,j)
+        enddo
+        close(51)
+    enddo
+end subroutine output_full_precision

@@ -17,7 +17,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("EQD_LIB", os.path.join(HERE, "libeqd.so"))
 
-EXPORTS = ["eqd_create", "eqd_set", "eqd_commit", "eqd_run", "eqd_get", "eqd_clock", "eqd_nan_info",
+EXPORTS = ["eqd_create", "eqd_set", "eqd_commit", "eqd_run", "eqd_run_async", "eqd_wait", "eqd_get", "eqd_clock", "eqd_nan_info",
            "eqd_kernel_time", "eqd_set_timing", "eqd_device_bytes", "eqd_layout_info", "eqd_last_error",
            "eqd_destroy", "eqd_set_guess", "eqd_harvest", "eqd_harvest_get", "eqd_newton_stats", "eqd_mlp_forward",
            "eqd_mlp_grad", "eqd_mlp_train", "eqd_set_launch", "eqd_get_launch", "eqd_tune", "eqd_set_graph",
@@ -98,6 +98,8 @@ def lib():
         L.eqd_set.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_int64]
         L.eqd_commit.argtypes = [C.c_void_p]
         L.eqd_run.argtypes = [C.c_void_p, C.c_int32]
+        L.eqd_run_async.argtypes = [C.c_void_p, C.c_int32]
+        L.eqd_wait.argtypes = [C.c_void_p]
         L.eqd_get.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_int64]
         L.eqd_clock.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_double)]
         L.eqd_nan_info.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
@@ -212,6 +214,13 @@ class Solver:
 
     def run(self, nsteps: int) -> None:
         self._check(self.L.eqd_run(self.h, int(nsteps)))
+
+    def run_async(self, nsteps: int) -> None:
+        """Enqueue nsteps steps and return at once (wait() finishes the run)."""
+        self._check(self.L.eqd_run_async(self.h, int(nsteps)))
+
+    def wait(self) -> None:
+        self._check(self.L.eqd_wait(self.h))
 
     def set_timing(self, on: bool) -> None:
         self._check(self.L.eqd_set_timing(self.h, 1 if on else 0))

@@ -116,6 +116,17 @@ int eqd_commit(eqd_handle *h);
  * becomes NaN; eqd_nan_info then names the node and step. */
 int eqd_run(eqd_handle *h, int32_t nsteps);
 
+/* eqd_run in two halves, so the host can work while the device steps (the
+ * reference's loop is synchronous, src/driver.f90:9-34; these replace the
+ * `call driver` of src/eqdyna3d.f90:72 in a host that overlaps its output
+ * writing with the next chunk of steps).  eqd_run_async enqueues nsteps steps
+ * and returns; eqd_wait waits for them and returns what eqd_run would have
+ * (EQD_ENAN ...).  While a run is in flight eqd_record_get hands over the
+ * records of the runs before it without waiting for it; every other call on
+ * the handle first waits for the run (and returns its error, if any). */
+int eqd_run_async(eqd_handle *h, int32_t nsteps);
+int eqd_wait(eqd_handle *h);
+
 /* Copy one array back into reference layout. */
 int eqd_get(eqd_handle *h, int32_t field, void *host, int64_t count);
 
@@ -242,11 +253,18 @@ int eqd_mlp_train(const float *X, const float *r, int64_t n, const eqd_mlp_train
  * longer copies all of velArr and fric back to write a few nodes.  At every
  * step nt with (nt - 1) % every == 0 the velocities of node_ids (1-based, e.g.
  * surfaceNodeIdArr) and fric(47) of every pair go to the next of `capacity`
- * records; capacity 0 stops recording.  eqd_record_get copies the records
- * since the last call (vel [count][n_nodes][3], slip_rate [count][nf], either
- * may be NULL) and empties the buffer.  EQD_EINVAL if more records wait than
- * max_records (nothing is copied), or if more record steps passed than the
- * capacity holds (the first `capacity` are copied, the rest were dropped). */
+ * records; capacity 0 stops recording.  The records live in a device ring of
+ * `capacity` records; at the end of every run they are copied, on a stream of
+ * their own and without stopping the steps, into a pinned host ring of the
+ * same size, from which eqd_record_get hands them over: a run writes device
+ * slots whose records are already on the host while the host drains the
+ * records of the runs before it.  eqd_record_get copies the records since the
+ * last call (vel [count][n_nodes][3], slip_rate [count][nf], either may be
+ * NULL), in step order, and empties the buffer; with a run in flight
+ * (eqd_run_async) the records of that run stay for the next call.  EQD_EINVAL
+ * if more records wait than max_records (nothing is copied), or if more
+ * record steps passed than the capacity holds (the first `capacity` are
+ * copied, the rest were dropped). */
 int eqd_record(eqd_handle *h, const int32_t *node_ids, int32_t n_nodes, int32_t every, int32_t capacity);
 int eqd_record_get(eqd_handle *h, double *vel, double *slip_rate, int32_t max_records, int32_t *count);
 
