@@ -38,9 +38,11 @@ def test_builder_bitwise_vs_reference_meshgen(name):
         assert got.dtype == ref.dtype and np.array_equal(got, ref), f"{name}: {k} differs from the reference meshgen"
 
 
-def test_builder_bitwise_vs_reference_c1_full_size():
-    # c1 at its full size (432,000 hexahedra): every input array the
-    # reference's one-rank run handed to its time loop, by SHA-256
+@pytest.mark.parametrize("name", ["c1", "c3"])
+def test_builder_bitwise_vs_reference_full_size(name):
+    # c1 (432,000 hexahedra) and c3 (3,456,000, rough fault) at full size:
+    # every input array the reference's one-rank run handed to its time
+    # loop, by SHA-256
     import test_gpu_size as T
-    ref, sha = T.load_size("c1")
-    T.build_inputs("c1", sha)
+    ref, sha = T.load_size(name)
+    T.build_inputs(name, sha)
