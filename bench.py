@@ -340,6 +340,8 @@ def main() -> int:
     ap.add_argument("--workload", default="c4", choices=["c4", "c3", "c2", "c1", "c5"])
     ap.add_argument("--tune", type=int, default=4, help="steps per candidate of the launch-shape tuner (0: off)")
     ap.add_argument("--scenarios", type=int, default=64, help="c5 only: ensemble size")
+    ap.add_argument("--c5-guess", default="off", choices=["on", "off"],
+                    help="c5 only: the rate-and-state scenarios start Newton from the learned guess")
     ap.add_argument("--host-startup", type=int, default=0, metavar="RANKS",
                     help="only time the reference host's start-up (meshgen, assembleGlobalMass) on the workload's "
                          "full mesh on RANKS MPI ranks, and the device precompute of the same mesh, then exit")
@@ -510,6 +512,10 @@ def bench_c5(args) -> int:
     kl = 40  # an untimed launched (uncaptured) pass, for comparison with the replayed steps
     total = args.warmup + steps + kl + kt
     cases = case.c5(n=args.scenarios, steps=total)
+    guess = None
+    if args.c5_guess == "on":
+        from eqdyna_amd import learned
+        guess, _ = learned.load()
     members = []
     kbytes = {"element": 0, "element_pml": 0, "node": 0, "node_pml": 0}
     nn = ne = nf = 0
@@ -522,6 +528,8 @@ def bench_c5(args) -> int:
         nf += int(sc["nftnd"])
         sizes.append(int(sc["totalNumOfElements"]))
         members.append(Solver(bundle, sc, "fp32", device=0, nstep=total))
+        if guess is not None and c.friclaw >= 3:
+            members[-1].set_guess(guess, GUESS["k_fallback"], GUESS["gate"])
         for k, v in kernel_bytes(bundle, sc, members[-1].layout_info(), "fp32").items():
             kbytes[k] += v
         del bundle
@@ -575,6 +583,8 @@ def bench_c5(args) -> int:
                    "scenarios": len(members), "elements": ne, "nodes": nn, "fault_pairs": nf,
                    "elements_min": min(sizes), "elements_max": max(sizes),
                    "slip_weakening": sum(1 for f in fric if f == 1), "rate_and_state": sum(1 for f in fric if f >= 3),
+                   "rate_and_state_guess": ({**GUESS, "weights": "eqdyna_amd/rsf_guess_c3.npz"} if guess is not None
+                                            else "plain (the reference's Newton start)"),
                    "device_gb": round(dev_gb, 1), "tuner": tuned,
                    "launched_untimed": {"steps": kl, "value": round(3.0 * nn * kl / wall_l / 1e6, 2),
                                         "ms_per_step": round(wall_l / kl * 1e3, 4),

@@ -316,6 +316,15 @@ __device__ __forceinline__ void exact_store(const Member<R> &M, int node, int k,
 #ifndef EQD_PML_GEO_WAVES
 #define EQD_PML_GEO_WAVES 3
 #endif
+// Waves per SIMD the fault kernel is bounded to (1: unbounded, 256 VGPRs).
+// Bounded to 2 (28-68 bytes per lane spilled) it measured the same on the
+// 64-scenario ensemble (12.90 against 12.88 ms per step, ~9,650 fault waves)
+// and on the rough-fault box (0.552 against 0.550 ms): the pairs' kernel runs
+// beside the node kernels and fills the slots they leave, so its occupancy is
+// not what bounds the step.
+#ifndef EQD_FAULT_WAVES
+#define EQD_FAULT_WAVES 1
+#endif
 
 //
 // kOps: where the operators come from (kOpsRows, kOpsLds, kOpsCompact).  With
@@ -1338,7 +1347,8 @@ __device__ __forceinline__ void slip_law(double V2, double &psi, const RsfParams
 // then: lanes past the last pair read pair nf-1 and write only their own
 // padding rows of fric, then leave).
 template <typename R, bool kLearn>
-__global__ void __launch_bounds__(128) fault_kernel(const Member<R> *__restrict__ Ms, const uint16_t *__restrict__ bmem)
+__global__ void __launch_bounds__(128, EQD_FAULT_WAVES) fault_kernel(const Member<R> *__restrict__ Ms,
+                                                                      const uint16_t *__restrict__ bmem)
 {
     __shared__ __attribute__((aligned(16))) __bf16 mlp_lds0[kLearn ? 2 * 64 * kLd0 : 8];
     __shared__ __attribute__((aligned(16))) __bf16 mlp_lds[kLearn ? 2 * 64 * kLd : 8];
