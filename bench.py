@@ -24,7 +24,8 @@ with the mesh resident in HBM.  Prints one JSON line:
               bytes per launch (kernel_bytes: every array it needs, in the layout
               the library chose for the mesh, moved once) divided by its average
               launch time, measured with HIP events its own dispatches record
-              over the timed steps; peak = 8 TB/s, beside a measured device copy;
+              on every --timing-every-th timed step; peak = 8 TB/s, beside a
+              measured device copy;
               and the whole step's algorithmic bytes over the time per step.
   cpu_baseline the reference Fortran (oracle/_ref/eqd_ref) on MPI ranks on the
               host's cores, on a bounded sample of the same workload.
@@ -45,6 +46,7 @@ METRIC = "Mdof-steps/s: plain FP64 vs FP32+learned RSF guess at equal rupture-ti
 HBM_PEAK_GBS = 8000.0
 KERNEL_NAMES = {"element": "elem_regular_kernel", "element_pml": "elem_pml_kernel", "node": "node3_kernel",
                 "node_pml": "node_pml_kernel"}
+KERNEL_KEY = {v: k for k, v in KERNEL_NAMES.items()}
 GUESS = {"k_fallback": 6, "gate": 0.5}
 
 
@@ -94,9 +96,9 @@ def load_traffic(workload: str, precision: str, kernel: str):
         return None, None
 
 
-def run_one(bundle, sc, precision: str, warmup: int, steps: int, guess=None, tune: int = 0):
+def run_one(bundle, sc, precision: str, warmup: int, steps: int, guess=None, tune: int = 0, timing_every: int = 10):
     """One timed run: warm-up steps, the tuner (state restored), then `steps`
-    timed with per-kernel events."""
+    timed, with per-kernel events on every `timing_every`-th of them."""
     from eqdyna_amd.solver import Solver
     t = time.perf_counter()
     s = Solver(bundle, sc, precision, device=0, nstep=warmup + steps)
@@ -111,7 +113,7 @@ def run_one(bundle, sc, precision: str, warmup: int, steps: int, guess=None, tun
         tuned = s.tune(tune)
         log(f"  tuner: " + ", ".join(f"{k} block {v['block']} remap {v['remap']} ({v['ms']:.4f} ms)"
                                      for k, v in tuned.items()))
-    s.set_timing(True)
+    s.set_timing(timing_every)
     s.run(min(warmup, 1))  # the launch tables for the chosen shapes are built here, not in the timed steps
     t0 = time.perf_counter()
     s.run(steps)  # returns after the stream has drained
@@ -338,6 +340,9 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None, help="timed steps (c4: 2990, others: 200)")
     ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--timing-every", type=int, default=10,
+                    help="per-kernel HIP events on every k-th timed step (1: every step; the events cost device "
+                         "time at the kernel boundaries they mark)")
     ap.add_argument("--workload", default="c4", choices=["c4", "c3", "c2", "c1", "c5"])
     ap.add_argument("--tune", type=int, default=4, help="steps per candidate of the launch-shape tuner (0: off)")
     ap.add_argument("--scenarios", type=int, default=64, help="c5 only: ensemble size")
@@ -382,8 +387,9 @@ def main() -> int:
     if args.workload == "c4":
         from eqdyna_amd import learned
         params, meta = learned.load()
-        ref = run_one(bundle, sc, "fp64", args.warmup, steps, tune=args.tune)
-        run = run_one(bundle, sc, "fp32", args.warmup, steps, guess=params, tune=args.tune)
+        ref = run_one(bundle, sc, "fp64", args.warmup, steps, tune=args.tune, timing_every=args.timing_every)
+        run = run_one(bundle, sc, "fp32", args.warmup, steps, guess=params, tune=args.tune,
+                      timing_every=args.timing_every)
         precision = "fp32"
         v64 = 3.0 * nn * steps / ref["wall"] / 1e6
         n64, n32 = ref["newton"], run["newton"]
@@ -398,7 +404,7 @@ def main() -> int:
         })
     else:
         precision = args.precision
-        run = run_one(bundle, sc, precision, args.warmup, steps, tune=args.tune)
+        run = run_one(bundle, sc, precision, args.warmup, steps, tune=args.tune, timing_every=args.timing_every)
     value = 3.0 * nn * steps / run["wall"] / 1e6
     if "fp64_value" in extra:
         extra["ratio_fp32_learned_vs_fp64"] = round(value / extra["fp64_value"], 4)
@@ -420,6 +426,8 @@ def main() -> int:
     copy_gbs = measure_copy_gbs()
     log(f"device copy: {copy_gbs:.1f} GB/s (read + write)")
     roof = roofline(run, kbytes, precision, args.workload, run["wall"] / steps * 1e3, copy_gbs)
+    roof["kernel_times_from"] = (f"HIP events of the kernels' own dispatches on every {args.timing_every}-th "
+                                 f"timed step ({run['ktimes'][KERNEL_KEY[roof['kernel']]][1]} launches)")
     del bundle
 
     cpu = None

@@ -1408,7 +1408,7 @@ int eqd_set_timing(eqd_handle *h, int32_t on)
 {
     if (!h) return EQD_EINVAL;
     SETTLE(h);
-    h->timing = on != 0;
+    h->timing = on < 0 ? 0 : on;
     if (h->eng) h->eng->timing = h->timing;
     return EQD_OK;
 }

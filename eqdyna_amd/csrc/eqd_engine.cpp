@@ -217,7 +217,7 @@ Stamp take_timer(eqd_engine *e, int kind, size_t launches)
 {
     Stamp p;
     p.n = launches;
-    if (!e->timing || launches == 0) return p;
+    if (!e->sample || launches == 0) return p;
     KTimer &t = e->timers[kKindName[kind]];
     if (t.used == t.ev.size()) {
         hipEvent_t a, b;
@@ -415,7 +415,9 @@ int launch(eqd_engine *e, int32_t nsteps, bool mirrors)
         if (graph) {
             HIPCHK(e, hipGraphLaunch(e->exec, st));
         } else {
+            e->sample = e->timing > 0 && k % e->timing == 0;
             int rc = enqueue_step<R>(e);
+            e->sample = false;
             if (rc) return rc;
             if (e->timing) {
                 size_t most = 0;
@@ -565,8 +567,8 @@ int tune_impl(eqd_engine *e, int32_t steps, std::string &report)
             return fail(e, EQD_EOOM, "tuner backup of %zu bytes", x.bytes);
         }
     int rc = copy_regions(e, regs, true);
-    const bool timing = e->timing;
-    e->timing = true;
+    const int32_t timing = e->timing;
+    e->timing = 1;
     e->no_harvest = true;
     struct Cand {
         int block, remap;
@@ -869,7 +871,7 @@ int eqd_ensemble_set_graph(eqd_ensemble *E, int32_t on)
 int eqd_ensemble_set_timing(eqd_ensemble *E, int32_t on)
 {
     if (!E) return EQD_EINVAL;
-    E->e.timing = on != 0;
+    E->e.timing = on < 0 ? 0 : on;
     return EQD_OK;
 }
 

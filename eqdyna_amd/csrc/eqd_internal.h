@@ -91,7 +91,8 @@ struct eqd_engine {
     bool graph_on = false;  // replay one captured step instead of launching every step
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    bool timing = false;
+    int32_t timing = 0;   // per-kernel event timing: 0 off, k: every k-th step of a run
+    bool sample = false;  // the step being enqueued carries timer events
     std::map<std::string, eqd::KTimer> timers;
     std::string err;
 };
@@ -166,7 +167,7 @@ struct eqd_handle {
     void *rec_bufs[3] = {nullptr, nullptr, nullptr};
     eqd::RecStream rq{};
     int32_t inflight = 0;  // steps of a run launched by eqd_run_async and not yet waited for
-    bool timing = false;  // per-kernel event timing of the handle's own runs
+    int32_t timing = 0;  // per-kernel event timing of the handle's own runs (every k-th step)
     // bumped by every change the engine's launches depend on
     int64_t version = 0;
     eqd_engine *eng = nullptr;       // the handle's own engine (one scenario)

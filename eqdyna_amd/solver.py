@@ -222,8 +222,9 @@ class Solver:
     def wait(self) -> None:
         self._check(self.L.eqd_wait(self.h))
 
-    def set_timing(self, on: bool) -> None:
-        self._check(self.L.eqd_set_timing(self.h, 1 if on else 0))
+    def set_timing(self, on) -> None:
+        """Per-kernel event timing: False/0 off, True/1 every step, k every k-th step."""
+        self._check(self.L.eqd_set_timing(self.h, int(on)))
 
     def get(self, name: str) -> np.ndarray:
         fid, dt = FIELDS[name]
@@ -453,8 +454,8 @@ class Ensemble:
     def set_graph(self, on: bool) -> None:
         self._check(self.L.eqd_ensemble_set_graph(self.e, 1 if on else 0))
 
-    def set_timing(self, on: bool) -> None:
-        self._check(self.L.eqd_ensemble_set_timing(self.e, 1 if on else 0))
+    def set_timing(self, on) -> None:
+        self._check(self.L.eqd_ensemble_set_timing(self.e, int(on)))
 
     def kernel_time(self, name: str):
         ms = C.c_double()

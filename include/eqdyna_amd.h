@@ -142,7 +142,10 @@ int eqd_nan_info(eqd_handle *h, int32_t *node, int32_t *nt);
  * "node_pml", "station". */
 int eqd_kernel_time(eqd_handle *h, const char *name, double *avg_ms, int64_t *launches);
 
-/* Enable (1) or disable (0) the per-launch event timing used by eqd_kernel_time. */
+/* Per-launch event timing used by eqd_kernel_time: 0 off, k >= 1 on every
+ * k-th step of each run (1: every step).  The events come from the
+ * dispatches themselves and cost device time at every kernel boundary they
+ * mark, so sampling keeps a timed run close to an untimed one. */
 int eqd_set_timing(eqd_handle *h, int32_t on);
 
 /* Device bytes held by the handle. */

@@ -64,6 +64,26 @@ def test_graph_replay_matches_eager(name, precision):
 
 
 @pytest.mark.gpu
+def test_sampled_timing_counts_and_keeps_results():
+    # per-kernel events on every 4th step of a run: the timers count exactly
+    # those launches, and timing never changes what the steps compute
+    bundle, sc, _ = load_golden("box_rsf4")
+    n = int(sc["nstep"])
+    s = Solver(bundle, sc)
+    s.set_timing(4)
+    s.run(n)
+    for k in ("element", "element_pml", "node", "node_pml", "fault"):
+        ms, launches = s.kernel_time(k)
+        assert launches == (n + 3) // 4 and ms > 0, (k, ms, launches)
+    timed = outputs(s)
+    s.close()
+    s = Solver(bundle, sc)
+    s.run(n)
+    assert_same(outputs(s), timed, "sampled timing")
+    s.close()
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("name", ["box_rsf4", "box_layered"])
 def test_launch_shapes_do_not_change_results(name):
     print(f"case {name}", flush=True)
