@@ -66,7 +66,7 @@ int64_t count_of(const eqd_handle *h, int kind)
     case kKNode3: return (int64_t)h->node3_free.size();
     case kKNodePml: return h->PN.n;
     case kKFault: return h->F.nf;
-    case kKStation: return h->S.nrow > 0 ? (int64_t)h->S.nrow + 1 : 0;
+    case kKStation: return 0;  // launched with the clock, one block per scenario (launch_step_start)
     case kKPend3: return (int64_t)h->node3_all.size();
     case kKPendPml: return h->PN.n;
     case kKTp: return h->F.tph ? h->F.nf * 64 : 0;  // one wavefront per pair
@@ -255,19 +255,15 @@ int enqueue_step(eqd_engine *e)
 {
     const Member<R> *dM = (const Member<R> *)e->d_members;
     hipStream_t st = e->st;
-    HIPCHK(e, launch_clock<R>(dM, (int)e->hs.size(), st));
     {
-        const auto &gs = e->groups[kKStation], &gr = e->groups[kKRecord];
-        const Stamp p = take_timer(e, kKStation, gs.size() + gr.size());
-        size_t i = 0;
-        for (const Group &g : gs) {
-            HIPCHK(e, launch_station<R>(dM, g.g, st, p.start(i), p.stop(i)));
-            i++;
-        }
-        for (const Group &g : gr) {
-            HIPCHK(e, launch_record<R>(dM, g.g, st, p.start(i), p.stop(i)));
-            i++;
-        }
+        // clock advance and off-fault stations, one block per scenario
+        const Stamp p = take_timer(e, kKStation, 1);
+        HIPCHK(e, launch_step_start<R>(dM, (int)e->hs.size(), st, p.start(0), p.stop(0)));
+    }
+    {
+        const auto &gr = e->groups[kKRecord];
+        const Stamp p = take_timer(e, kKRecord, gr.size());
+        for (size_t i = 0; i < gr.size(); i++) HIPCHK(e, launch_record<R>(dM, gr[i].g, st, p.start(i), p.stop(i)));
     }
     {
         const auto &gs = e->groups[kKElem];

@@ -33,10 +33,8 @@ template <typename R>
 hipError_t launch_fault(const Member<R> *, const LaunchGrid &, int learn, hipStream_t, hipEvent_t t0 = nullptr,
                         hipEvent_t t1 = nullptr);
 template <typename R>
-hipError_t launch_station(const Member<R> *, const LaunchGrid &, hipStream_t, hipEvent_t t0 = nullptr,
-                          hipEvent_t t1 = nullptr);
-template <typename R>
-hipError_t launch_clock(const Member<R> *, int nmem, hipStream_t);
+hipError_t launch_step_start(const Member<R> *, int nmem, hipStream_t, hipEvent_t t0 = nullptr,
+                             hipEvent_t t1 = nullptr);
 template <typename R>
 hipError_t launch_thermop(const Member<R> *, const LaunchGrid &, hipStream_t, hipEvent_t t0 = nullptr,
                           hipEvent_t t1 = nullptr);

@@ -1723,30 +1723,45 @@ __global__ void __launch_bounds__(128, EQD_FAULT_WAVES) fault_kernel(const Membe
     flag_nan(N, nm, nt + 1, vm[0], vm[1], vm[2]);
 }
 
-// storeOffFaultStData, src/driver.f90:157-180
+// The start of a step, one block per scenario: the clock advance of
+// src/driver.f90:9-11 (nt += 1, timeElapsed += dt with the reference's FP64
+// addition, and the run's last-step flag), then storeOffFaultStData
+// (src/driver.f90:157-180) at the new clock: time and the recorded
+// displacement or velocity of every off-fault station row.  One kernel
+// instead of two saves a kernel boundary per step; every later kernel of the
+// step reads the advanced clock from device memory.
 template <typename R>
-__global__ void __launch_bounds__(64) station_kernel(const Member<R> *__restrict__ Ms, const uint16_t *__restrict__ bmem)
+__global__ void __launch_bounds__(64) step_start_kernel(const Member<R> *__restrict__ Ms)
 {
-    const unsigned gb = blockIdx.x;
-    const Member<R> &M = Ms[member_of(bmem, gb)];
-    const OffStations S = M.S;
-    const NodeState N = M.N;
-    const int r = (int)(gb - (unsigned)M.blk0[kKStation]) * blockDim.x + threadIdx.x;
-    if (r > S.nrow) return;
-    const Clock ck = *M.clk;
-    const int nt = ck.nt;
-    const double time = ck.time;
-    double *col = S.hist + (int64_t)(nt - 1) * (S.nrow + 1);
-    if (r == 0) {
-        col[0] = time;
-        return;
+    const Member<R> &M = Ms[blockIdx.x];
+    __shared__ double s_time;
+    __shared__ int s_nt;
+    if (threadIdx.x == 0) {
+        Clock *c = M.clk;
+        const int nt = c->nt + 1;
+        const double time = c->time + M.C.dt;
+        c->nt = nt;
+        c->time = time;
+        c->mode = nt == c->run_end ? 1 : 0;
+        s_nt = nt;
+        s_time = time;
     }
-    const int i = r - 1;
-    const int node = S.idh[i], k = S.idh[S.nrow + i], kind = S.idh[2 * S.nrow + i];
-    if (kind == 1)
-        col[r] = exact_u(M, node, k);
-    else if (kind == 2)
-        col[r] = exact_v(M, node, k);
+    __syncthreads();
+    const OffStations S = M.S;
+    if (S.nrow <= 0) return;
+    double *col = S.hist + (int64_t)(s_nt - 1) * (S.nrow + 1);
+    for (int r = threadIdx.x; r <= S.nrow; r += blockDim.x) {
+        if (r == 0) {
+            col[0] = s_time;
+            continue;
+        }
+        const int i = r - 1;
+        const int node = S.idh[i], k = S.idh[S.nrow + i], kind = S.idh[2 * S.nrow + i];
+        if (kind == 1)
+            col[r] = exact_u(M, node, k);
+        else if (kind == 2)
+            col[r] = exact_v(M, node, k);
+    }
 }
 
 // thermop, src/thermop.f90:1-40: pore pressure change fric(51) and
@@ -1818,20 +1833,6 @@ __global__ void __launch_bounds__(256) record_kernel(const Member<R> *__restrict
     const int64_t nn = M.N.nn;
 #pragma unroll
     for (int k = 0; k < 3; k++) rc.vel[(slot * rc.n_nodes + t) * 3 + k] = exact_v(M, node, k);
-}
-
-// The step clock: nt += 1 and time += dt for every scenario of the launch,
-// and the last-step flag of the run (src/driver.f90:9-11).
-template <typename R>
-__global__ void __launch_bounds__(64) clock_kernel(const Member<R> *__restrict__ Ms, int nmem)
-{
-    const int m = blockIdx.x * blockDim.x + threadIdx.x;
-    if (m >= nmem) return;
-    Clock *c = Ms[m].clk;
-    const int nt = c->nt + 1;
-    c->nt = nt;
-    c->time = c->time + Ms[m].C.dt;
-    c->mode = nt == c->run_end ? 1 : 0;
 }
 
 // ------------------------------------------------------------------------
@@ -1984,19 +1985,10 @@ hipError_t launch_thermop(const Member<R> *M, const LaunchGrid &g, hipStream_t s
 }
 
 template <typename R>
-hipError_t launch_station(const Member<R> *M, const LaunchGrid &g, hipStream_t s, hipEvent_t t0, hipEvent_t t1)
-{
-    if (g.grid == 0) return hipSuccess;
-    if (g.block != 64) return hipErrorInvalidValue;
-    EQD_GO(station_kernel<R>, g.grid, 64, 0, s, M, g.bmem);
-    return hipGetLastError();
-}
-
-template <typename R>
-hipError_t launch_clock(const Member<R> *M, int nmem, hipStream_t s)
+hipError_t launch_step_start(const Member<R> *M, int nmem, hipStream_t s, hipEvent_t t0, hipEvent_t t1)
 {
     if (nmem <= 0) return hipSuccess;
-    clock_kernel<R><<<grid_for(nmem, 64), 64, 0, s>>>(M, nmem);
+    EQD_GO(step_start_kernel<R>, nmem, 64, 0, s, M);
     return hipGetLastError();
 }
 
@@ -2011,10 +2003,9 @@ hipError_t launch_clock(const Member<R> *M, int nmem, hipStream_t s)
     template hipError_t launch_pending<R>(const Member<R> *, const LaunchGrid &, const LaunchGrid &, hipStream_t);    \
     template hipError_t launch_fault<R>(const Member<R> *, const LaunchGrid &, int, hipStream_t, hipEvent_t,           \
                                         hipEvent_t);                                                                  \
-    template hipError_t launch_station<R>(const Member<R> *, const LaunchGrid &, hipStream_t, hipEvent_t, hipEvent_t);\
+    template hipError_t launch_step_start<R>(const Member<R> *, int, hipStream_t, hipEvent_t, hipEvent_t);           \
     template hipError_t launch_thermop<R>(const Member<R> *, const LaunchGrid &, hipStream_t, hipEvent_t, hipEvent_t);\
-    template hipError_t launch_record<R>(const Member<R> *, const LaunchGrid &, hipStream_t, hipEvent_t, hipEvent_t); \
-    template hipError_t launch_clock<R>(const Member<R> *, int, hipStream_t);
+    template hipError_t launch_record<R>(const Member<R> *, const LaunchGrid &, hipStream_t, hipEvent_t, hipEvent_t);
 EQD_INSTANTIATE(double)
 EQD_INSTANTIATE(float)
 #undef EQD_INSTANTIATE
