@@ -158,7 +158,8 @@ def kernel_bytes(bundle, sc, layout: dict, precision: str) -> dict:
     f32 = precision == "fp32"
     r = 4 if f32 else 8               # element operator / stress / force values
     nv = 4 if f32 else 8              # node velocity values
-    nu_r, nu_w = (4, 8) if f32 else (8, 8)  # displacement: gathered (FP32: the rounded part), updated (hi + lo)
+    nu_r, nu_w = (4, 6) if f32 else (8, 8)  # displacement: gathered (FP32: hi), read and written (hi + bf16 lo)
+    mb = 4 if f32 else 8              # nodal mass as the node kernels read it
     ien = np.asarray(bundle["nodeElemIdRelation"]).reshape(-1, 8) - 1
     et = np.asarray(bundle["elemTypeArr"])
     ndof = np.asarray(bundle["numOfDofPerNodeArr"])
@@ -192,9 +193,9 @@ def kernel_bytes(bundle, sc, layout: dict, precision: str) -> dict:
     out["element_pml"] = (npl * (32 + ops("pml") + 2 * 21 * r + 3 * r + 1 + 6 * r + (24 * r if mass_term else 0))
                           + pml_slots * r + int((occ_pml > 0).sum()) * 3 * (nv + nu_r))
     n3, e3 = int(node3.sum()), int(occ[node3].sum())
-    out["node"] = n3 * (4 + 4 + 8 + 3 * 2 * nv + 3 * 2 * nu_w) + e3 * (4 + 3 * r)
+    out["node"] = n3 * (4 + 4 + mb + 3 * 2 * nv + 3 * 2 * nu_w) + e3 * (4 + 3 * r)
     npn, ep = int(nodepml.sum()), int(occ[nodepml].sum())
-    out["node_pml"] = (npn * (4 + 4 + 8 + 24 + 12 * 2 * nv + 3 * 2 * nv + 3 * 2 * nu_w)
+    out["node_pml"] = (npn * (4 + 4 + mb + 3 * mb + 12 * 2 * nv + 3 * 2 * nv + 3 * 2 * nu_w)
                        + ep * (4 + 6 * r) + npl * 6 * r)
     return {k: int(v) for k, v in out.items()}
 

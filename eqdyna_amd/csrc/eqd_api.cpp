@@ -104,6 +104,22 @@ static std::vector<T> to_blocks(const std::vector<S> &cm, int64_t n, int w)
 static inline int64_t blocked_count(int64_t n, int w) { return (n + 63) / 64 * 64 * w; }
 static inline int64_t blocked_index(int64_t t, int k, int w) { return ((t >> 6) * w + k) * 64 + (t & 63); }
 
+// bfloat16 bits of a finite float (round to nearest even) and back, as the
+// kernels store the FP32 bulk mode's displacement residual
+static inline uint16_t bf16_bits(float x)
+{
+    uint32_t u;
+    memcpy(&u, &x, 4);
+    return (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+}
+static inline float bf16_value(uint16_t h)
+{
+    const uint32_t u = (uint32_t)h << 16;
+    float x;
+    memcpy(&x, &u, 4);
+    return x;
+}
+
 // comdampv, src/comdampv.f90:1-120 (region 14 compares y with xmax0 as the
 // reference does).  Returns d_x, d_y, d_z.
 static void pml_node_damping(const eqd_scalars &s, double x2, double y, double z, double d[3])
@@ -994,15 +1010,19 @@ int eqd_commit(eqd_handle *h)
         if (s.precision == EQD_PREC_FP32) {
             // FP32 velocity, displacement as the pair hi + lo (eqd_device.h,
             // NodeState); split nodes keep FP64 copies in the fault's arrays
-            std::vector<float> hi(3 * nn), lo(3 * nn);
+            std::vector<float> hi(3 * nn), m32(nn);
+            std::vector<uint16_t> lo(3 * nn);
             for (int64_t j = 0; j < 3 * nn; j++) {
                 hi[j] = (float)disp[j];
-                lo[j] = (float)(disp[j] - (double)hi[j]);
+                lo[j] = bf16_bits((float)(disp[j] - (double)hi[j]));
             }
+            for (int64_t n = 0; n < nn; n++) m32[n] = (float)mass[n];
             float *f;
+            uint16_t *l;
             if ((rc = dupload(h, &f, to_blocks<float>(vel, nn, 3)))) return rc; h->N.vel32 = f;
             if ((rc = dupload(h, &f, to_blocks<float>(hi, nn, 3)))) return rc; h->N.dhi = f;
-            if ((rc = dupload(h, &f, to_blocks<float>(lo, nn, 3)))) return rc; h->N.dlo = f;
+            if ((rc = dupload(h, &l, to_blocks<uint16_t>(lo, nn, 3)))) return rc; h->N.dlo = l;
+            if ((rc = dupload(h, &f, m32))) return rc; h->N.mass32 = f;
             std::vector<int32_t> slot(nn, -1);
             for (int64_t i = 0; i < s.nf; i++)
                 for (int j = 0; j < 2; j++) slot[nsmp[i * 2 + j] - 1] = (int32_t)(2 * i + j);
@@ -1025,6 +1045,11 @@ int eqd_commit(eqd_handle *h)
         int32_t *pn;
         if ((rc = dupload(h, &pn, h->pml_node_ids))) return rc; PN.node = pn;
         if ((rc = dupload(h, &d, to_blocks<double>(pdamp, npn, 3)))) return rc; PN.damp = d;
+        if (s.precision == EQD_PREC_FP32) {
+            std::vector<float> d32(pdamp.begin(), pdamp.end());
+            float *f;
+            if ((rc = dupload(h, &f, to_blocks<float>(d32, npn, 3)))) return rc; PN.damp32 = f;
+        }
         if (s.precision == EQD_PREC_FP32) {
             float *f;
             if ((rc = dupload(h, &f, to_blocks<float>(pv12, npn, 12)))) return rc; PN.v12f = f;
@@ -1226,12 +1251,13 @@ static int node_state(eqd_handle *h, bool vel, std::vector<double> &out)
     const int64_t nn = h->s.nn;
     int rc;
     if (!h->N.vel32) return d2h(h, out, vel ? h->N.vel : h->N.disp, blocked_count(nn, 3));
-    std::vector<float> a, b;
+    std::vector<float> a;
+    std::vector<uint16_t> b;
     if ((rc = d2h(h, a, vel ? h->N.vel32 : h->N.dhi, blocked_count(nn, 3)))) return rc;
     out.assign(a.begin(), a.end());
     if (!vel) {
         if ((rc = d2h(h, b, h->N.dlo, blocked_count(nn, 3)))) return rc;
-        for (size_t j = 0; j < out.size(); j++) out[j] = (double)a[j] + (double)b[j];
+        for (size_t j = 0; j < out.size(); j++) out[j] = (double)a[j] + (double)bf16_value(b[j]);
     }
     const int64_t nf = h->F.nf, ld = h->F.ld;
     if (nf > 0) {

@@ -44,10 +44,15 @@ struct NodeState {
     double *vel;   // 3 per node (blk_at<3>): velArr (== v1 for 3-dof nodes); FP64 mode
     double *disp;  // 3 per node (blk_at<3>): dispArr; FP64 mode
     float *vel32;  // 3 per node (blk_at<3>); FP32 bulk mode
-    float *dhi, *dlo;  // 3 per node (blk_at<3>); FP32 bulk mode
+    // displacement in the FP32 bulk mode, 3 per node (blk_at<3>): dhi the FP32
+    // value the element kernels gather, dlo the residual u - dhi as bfloat16
+    // bits, so the node kernels carry u to about 32 significant bits
+    float *dhi;
+    uint16_t *dlo;
     const int32_t *pair_slot;  // [nn] FP32 bulk mode: 2 * pair + (0 slave, 1 master) of a split node, else -1
     double *acc;   // 3 per node (blk_at<3>): pending F/m of 3-dof nodes
     const double *mass;    // [nn] nodal mass (nodalMassArr of the node's equations)
+    const float *mass32;   // [nn] the same rounded to FP32, read by the FP32 bulk mode's node kernels
     const int32_t *rowptr; // [nn+1]
     const uint32_t *ent;   // CSR entries
     unsigned long long *nanflag; // first NaN velocity: step << 32 | node + 1 (all ones: none)
@@ -165,6 +170,7 @@ struct PmlNodes {
     int64_t n;
     const int32_t *node;  // [n]
     const double *damp;   // 3 per node (blk_at<3>): d_x, d_y, d_z at the node
+    const float *damp32;  // the same rounded to FP32, read by the FP32 bulk mode
     double *v12;          // 12 per node (blk_at<12>): split velocities (v1 of the 12 equations); FP64 mode
     float *v12f;          // the same in FP32; FP32 bulk mode
     double *acc;          // 12 per node (blk_at<12>): pending F/m

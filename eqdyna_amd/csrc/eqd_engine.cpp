@@ -512,7 +512,7 @@ std::vector<Region> mutable_regions(const eqd_engine *e)
         add(h->N.disp, blocked(nn, 3) * sizeof(double));
         add(h->N.vel32, blocked(nn, 3) * sizeof(float));
         add(h->N.dhi, blocked(nn, 3) * sizeof(float));
-        add(h->N.dlo, blocked(nn, 3) * sizeof(float));
+        add(h->N.dlo, blocked(nn, 3) * sizeof(uint16_t));
         add(h->N.acc, blocked(nn, 3) * sizeof(double));
         add(h->N.nanflag, sizeof(unsigned long long));
         add(e->fp32 ? (const void *)h->E32.stress : (const void *)h->E64.stress, blocked(h->n_reg, h->nreg_stress) * rs);

@@ -52,11 +52,40 @@ __device__ __forceinline__ double node_v(const NodeState &N, int node, int k)
     else return N.vel[blk_at<3>(node, k)];
 }
 
+// bfloat16 bits <-> float (round to nearest even; the residuals converted
+// here are finite)
+__device__ __forceinline__ float bf16_float(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+__device__ __forceinline__ uint16_t float_bf16(float x)
+{
+    const uint32_t u = __float_as_uint(x);
+    return (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+}
+
 template <typename R>
 __device__ __forceinline__ double node_u(const NodeState &N, int node, int k)
 {
-    if constexpr (sizeof(R) == sizeof(float)) return (double)N.dhi[blk_at<3>(node, k)] + (double)N.dlo[blk_at<3>(node, k)];
+    if constexpr (sizeof(R) == sizeof(float))
+        return (double)N.dhi[blk_at<3>(node, k)] + (double)bf16_float(N.dlo[blk_at<3>(node, k)]);
     else return N.disp[blk_at<3>(node, k)];
+}
+
+// a PML node's damping coefficient: FP64, or in the FP32 bulk mode its FP32
+// rounding (it scales the FP32 split velocities)
+template <typename R>
+__device__ __forceinline__ double pml_damp(const PmlNodes &P, int64_t q, int k)
+{
+    if constexpr (sizeof(R) == sizeof(float)) return (double)P.damp32[blk_at<3>(q, k)];
+    else return P.damp[blk_at<3>(q, k)];
+}
+
+// the nodal mass as the node kernels divide by it: FP64, or in the FP32 bulk
+// mode its FP32 rounding (a relative change of at most 2^-24, below the
+// rounding of the FP32 velocity it feeds)
+template <typename R>
+__device__ __forceinline__ double node_mass(const NodeState &N, int node)
+{
+    if constexpr (sizeof(R) == sizeof(float)) return (double)N.mass32[node];
+    else return N.mass[node];
 }
 
 // velocity and displacement of node component k as the element kernels
@@ -85,7 +114,7 @@ __device__ __forceinline__ void node_store(const NodeState &N, int node, int k, 
         const float hi = (float)u;
         N.vel32[j] = (float)v;
         N.dhi[j] = hi;
-        N.dlo[j] = (float)(u - (double)hi);
+        N.dlo[j] = float_bf16((float)(u - (double)hi));
     } else {
         N.vel[j] = v;
         N.disp[j] = u;
@@ -1117,7 +1146,7 @@ __global__ void __launch_bounds__(256) node3_kernel(const Member<R> *__restrict_
     const int64_t nn = N.nn;
     double F[3];
     gather3<R>(N, node, ef, efp, F);
-    const double m = N.mass[node];
+    const double m = node_mass<R>(N, node);
     if (mode == 1) {
 #pragma unroll
         for (int k = 0; k < 3; k++) N.acc[blk_at<3>(node, k)] = F[k] / m;
@@ -1195,13 +1224,13 @@ __global__ void __launch_bounds__(256) node_pml_kernel(const Member<R> *__restri
             }
         }
     }
-    const double m = N.mass[node];
+    const double m = node_mass<R>(N, node);
     if (mode == 1) {
 #pragma unroll
         for (int k = 0; k < 12; k++) P.acc[blk_at<12>(q, k)] = F[k] / m;
         return;
     }
-    const double d[3] = {P.damp[blk_at<3>(q, 0)], P.damp[blk_at<3>(q, 1)], P.damp[blk_at<3>(q, 2)]};
+    const double d[3] = {pml_damp<R>(P, q, 0), pml_damp<R>(P, q, 1), pml_damp<R>(P, q, 2)};
     double v[12];
 #pragma unroll
     for (int k = 0; k < 9; k++) {
@@ -1260,7 +1289,7 @@ __global__ void __launch_bounds__(256) update_pml_from_acc_kernel(const Member<R
     const int next_step = M.clk->nt + 1;
     const int node = P.node[q];
     const int64_t nn = N.nn;
-    const double d[3] = {P.damp[blk_at<3>(q, 0)], P.damp[blk_at<3>(q, 1)], P.damp[blk_at<3>(q, 2)]};
+    const double d[3] = {pml_damp<R>(P, q, 0), pml_damp<R>(P, q, 1), pml_damp<R>(P, q, 2)};
     double v[12];
 #pragma unroll
     for (int k = 0; k < 9; k++) {
