@@ -1002,6 +1002,25 @@ int eqd_commit(eqd_handle *h)
         if ((rc = dupload(h, &d_ent, ent))) return rc;
         h->N.rowptr = d_rp;
         h->N.ent = d_ent;
+        // the free 3-component nodes' entries again, slot-major by position,
+        // so that a wavefront's load of slot k reads consecutive words
+        // (EQD_NODE_ELL=0 keeps the CSR gather)
+        const char *ell_env = std::getenv("EQD_NODE_ELL");
+        bool ell_ok = !(ell_env && ell_env[0] == '0');
+        const int64_t n3 = (int64_t)h->node3_free.size();
+        for (int64_t i = 0; i < n3 && ell_ok; i++) {
+            const int32_t n = h->node3_free[i];
+            if (rowptr[n + 1] - rowptr[n] > kEll3) ell_ok = false;
+        }
+        h->d_ell3 = nullptr;
+        if (ell_ok && n3 > 0) {
+            std::vector<uint32_t> ell((size_t)blocked_count(n3, kEll3), kEntNone);
+            for (int64_t i = 0; i < n3; i++) {
+                const int32_t n = h->node3_free[i];
+                for (int j = rowptr[n]; j < rowptr[n + 1]; j++) ell[blocked_index(i, j - rowptr[n], kEll3)] = ent[j];
+            }
+            if ((rc = dupload(h, &h->d_ell3, ell))) return rc;
+        }
     }
     {
         int rc;

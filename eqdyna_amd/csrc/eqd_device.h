@@ -25,6 +25,8 @@ namespace eqd {
 constexpr uint32_t kEntPml = 0x80000000u;
 constexpr uint32_t kEntIdxMask = 0x0FFFFFFFu;
 constexpr int kEntLocalShift = 28;
+constexpr uint32_t kEntNone = 0xFFFFFFFFu;  // an unused slot of the ELL gather table
+constexpr int kEll3 = 8;
 
 struct StepConsts {
     double dt, w, rdampk, rdampm, gterm, ccosphi, sinphi, exp_dt_tv, kapa_hg, fric_tp_h, pi;
@@ -260,6 +262,11 @@ struct Member {
     StepConsts C;
     const int32_t *node3_free;  // 3-component nodes that are not split nodes
     int64_t n_node3_free;
+    // the free nodes' gather entries slot-major by position in node3_free
+    // (blk_at<kEll3>: entry k of free node i at slot k), unused slots
+    // kEntNone; null when a free node has more than kEll3 entries (then the
+    // node kernel reads the CSR table)
+    const uint32_t *ell3;
     const int32_t *node3_all;   // all 3-component nodes
     int64_t n_node3_all;
     Clock *clk;

@@ -51,6 +51,7 @@ Member<R> make_member(const eqd_engine *e, const eqd_handle *h)
     M.C = h->C;
     M.node3_free = h->d_node3_free;
     M.n_node3_free = (int64_t)h->node3_free.size();
+    M.ell3 = h->d_ell3;
     M.node3_all = h->d_node3_all;
     M.n_node3_all = (int64_t)h->node3_all.size();
     M.clk = h->d_clock;

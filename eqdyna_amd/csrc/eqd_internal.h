@@ -152,6 +152,7 @@ struct eqd_handle {
     eqd::StepConsts C{};
     eqd::Clock *d_clock = nullptr;
     int32_t *d_node3_free = nullptr, *d_node3_all = nullptr;
+    uint32_t *d_ell3 = nullptr;  // slot-major gather entries of the free 3-component nodes (or null)
     // learned Newton guess: float32 parameters and their bfloat16 operands
     bool learn = false;
     float *mlp_p = nullptr, *mlp_w4r = nullptr;

@@ -1114,6 +1114,39 @@ __device__ __forceinline__ void gather3(const NodeState &N, int node, const R *e
     }
 }
 
+// gather3 from the slot-major table (entry k of free node i at
+// blk_at<kEll3>(i, k), unused slots kEntNone after the used ones): the same
+// loads and the same sums in the same order as gather3
+template <typename R>
+__device__ __forceinline__ void gather3_ell(const uint32_t *ell, int64_t i, const R *ef, const R *efp, double F[3])
+{
+    F[0] = 0.0; F[1] = 0.0; F[2] = 0.0;
+    uint32_t e[kEll3];
+#pragma unroll
+    for (int k = 0; k < kEll3; k++) e[k] = ell[blk_at<kEll3>(i, k)];
+    R f[kEll3][3];
+#pragma unroll
+    for (int k = 0; k < kEll3; k++) {
+        if (e[k] != kEntNone) {
+            const int64_t idx = e[k] & kEntIdxMask;
+            const int a = (e[k] >> kEntLocalShift) & 7;
+            const bool pml = (e[k] & kEntPml) != 0;
+            const R *b = pml ? efp + blk_at<kEfPml>(idx, a * 6) : ef + blk_at<kEfReg>(idx, a * 3);
+            f[k][0] = b[0];
+            f[k][1] = b[64];
+            f[k][2] = b[128];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < kEll3; k++) {
+        if (e[k] != kEntNone) {
+            F[0] = F[0] + (double)f[k][0];
+            F[1] = F[1] + (double)f[k][1];
+            F[2] = F[2] + (double)f[k][2];
+        }
+    }
+}
+
 // First NaN velocity: the reference stops at the first node, in node order, of
 // the first step whose update gives a NaN (src/driver.f90:147-152).  The key
 // (step << 32 | node + 1) is lowered atomically, so the earliest step and then
@@ -1145,7 +1178,8 @@ __global__ void __launch_bounds__(256) node3_kernel(const Member<R> *__restrict_
     const int node = M.node3_free[i];
     const int64_t nn = N.nn;
     double F[3];
-    gather3<R>(N, node, ef, efp, F);
+    if (M.ell3) gather3_ell<R>(M.ell3, i, ef, efp, F);
+    else gather3<R>(N, node, ef, efp, F);
     const double m = node_mass<R>(N, node);
     if (mode == 1) {
 #pragma unroll

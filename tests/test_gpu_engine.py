@@ -64,6 +64,23 @@ def test_graph_replay_matches_eager(name, precision):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("name,precision", [("box_rsf4", "fp64"), ("box_dip", "fp64"), ("box_rough", "fp32"),
+                                            ("box_q", "fp32")])
+def test_slot_major_node_gather_matches_csr(name, precision, monkeypatch):
+    # the free nodes' gather from the slot-major table (default) and from the
+    # CSR table (EQD_NODE_ELL=0) add the same forces in the same order
+    res = []
+    for ell in ("1", "0"):
+        monkeypatch.setenv("EQD_NODE_ELL", ell)
+        bundle, sc, _ = load_golden(name)
+        s = Solver(bundle, sc, precision)
+        s.run(int(sc["nstep"]))
+        res.append(outputs(s))
+        s.close()
+    assert_same(res[0], res[1], f"{name} {precision} slot-major vs CSR gather")
+
+
+@pytest.mark.gpu
 def test_sampled_timing_counts_and_keeps_results():
     # per-kernel events on every 4th step of a run: the timers count exactly
     # those launches, and timing never changes what the steps compute
