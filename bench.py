@@ -193,7 +193,11 @@ def kernel_bytes(bundle, sc, layout: dict, precision: str) -> dict:
     out["element_pml"] = (npl * (32 + ops("pml") + 2 * 21 * r + 3 * r + 1 + 6 * r + (24 * r if mass_term else 0))
                           + pml_slots * r + int((occ_pml > 0).sum()) * 3 * (nv + nu_r))
     n3, e3 = int(node3.sum()), int(occ[node3].sum())
-    out["node"] = n3 * (4 + 4 + mb + 3 * 2 * nv + 3 * 2 * nu_w) + e3 * (4 + 3 * r)
+    if layout.get("node3_slot_major", 0):
+        # node id, mass, state; all 8 entry slots of the slot-major table; the forces
+        out["node"] = n3 * (4 + mb + 3 * 2 * nv + 3 * 2 * nu_w + 8 * 4) + e3 * 3 * r
+    else:
+        out["node"] = n3 * (4 + 4 + mb + 3 * 2 * nv + 3 * 2 * nu_w) + e3 * (4 + 3 * r)
     npn, ep = int(nodepml.sum()), int(occ[nodepml].sum())
     out["node_pml"] = (npn * (4 + 4 + mb + 3 * mb + 12 * 2 * nv + 3 * 2 * nv + 3 * 2 * nu_w)
                        + ep * (4 + 6 * r) + npl * 6 * r)

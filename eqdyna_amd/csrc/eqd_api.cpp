@@ -1692,6 +1692,7 @@ int eqd_layout_info(eqd_handle *h, const char *key, int64_t *value)
     else if (k == "phi_compact_pml") *value = h->phi_compact_pml;
     else if (k == "geom_ops_regular") *value = h->s.precision == EQD_PREC_FP32 ? h->E32.opb_geom : 0;
     else if (k == "geom_ops_pml") *value = h->s.precision == EQD_PREC_FP32 ? h->P32.opb_geom : 0;
+    else if (k == "node3_slot_major") *value = h->d_ell3 ? 1 : 0;
     else return fail(h, EQD_EINVAL, "unknown layout key %s", key);
     return EQD_OK;
 }

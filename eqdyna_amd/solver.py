@@ -259,7 +259,7 @@ class Solver:
         out = {}
         for k in ("regular_elements", "pml_elements", "op_classes_regular", "phi_classes_regular",
                   "op_classes_pml", "phi_classes_pml", "phi_compact_regular", "phi_compact_pml",
-                  "geom_ops_regular", "geom_ops_pml"):
+                  "geom_ops_regular", "geom_ops_pml", "node3_slot_major"):
             v = C.c_int64()
             self._check(self.L.eqd_layout_info(self.h, k.encode(), C.byref(v)))
             out[k] = v.value

@@ -162,7 +162,10 @@ int64_t eqd_device_bytes(eqd_handle *h);
  * "geom_ops_regular", "geom_ops_pml": 1 when the FP32 bulk mode rebuilds the
  * class's operators from the elements' corner coordinates in the kernels
  * (meshes whose elements all differ; the given operators must be those of
- * the coordinates, EQD_GEOM_OPS=0 in the environment keeps them stored).
+ * the coordinates, EQD_GEOM_OPS=0 in the environment keeps them stored);
+ * "node3_slot_major": 1 when the 3-component nodes gather their element
+ * forces from a slot-major table of 8 entries per node (every such node has
+ * at most 8 elements; EQD_NODE_ELL=0 keeps the CSR table).
  * Returns EQD_EINVAL for an unknown key. */
 int eqd_layout_info(eqd_handle *h, const char *key, int64_t *value);
 
