@@ -34,6 +34,52 @@ __device__ __forceinline__ void store_nt(R *p, R v)
     __builtin_nontemporal_store(v, p);
 }
 
+// two values of a PML slot or summed-stress row (pml_at) as one access in
+// FP64, as two in FP32
+template <typename R>
+struct Pair;
+template <>
+struct Pair<double> {
+    typedef double T __attribute__((ext_vector_type(2)));
+};
+
+template <typename R>
+__device__ __forceinline__ void store_pair_nt(R *p, R a, R b)
+{
+    if constexpr (sizeof(R) == sizeof(double)) {
+        const typename Pair<R>::T v = {a, b};
+        __builtin_nontemporal_store(v, (typename Pair<R>::T *)p);
+    } else {
+        __builtin_nontemporal_store(a, p);
+        __builtin_nontemporal_store(b, p + 64);
+    }
+}
+
+template <typename R>
+__device__ __forceinline__ void store_pair(R *p, R a, R b)
+{
+    if constexpr (sizeof(R) == sizeof(double)) {
+        const typename Pair<R>::T v = {a, b};
+        *(typename Pair<R>::T *)p = v;
+    } else {
+        p[0] = a;
+        p[64] = b;
+    }
+}
+
+template <typename R>
+__device__ __forceinline__ void load_pair(const R *p, R &a, R &b)
+{
+    if constexpr (sizeof(R) == sizeof(double)) {
+        const typename Pair<R>::T v = *(const typename Pair<R>::T *)p;
+        a = v.x;
+        b = v.y;
+    } else {
+        a = p[0];
+        b = p[64];
+    }
+}
+
 // The element kernels' other once-per-step streams (IEN, stresses, operator
 // rows) are loaded non-temporal for the same reason: c4 FP32 step 0.688 ->
 // 0.664 ms, FP64 3.2 % faster, c2 FP32 1.1 %.
@@ -968,12 +1014,9 @@ constexpr int kU1 = (kKeep || kGeo) ? 8 : EQD_PREFETCH;
     const R sxz = sv[11] + sv[12];
     const R syz = sv[13] + sv[14];
     // the summed stresses, for the PML nodes' split forces (node_pml_kernel)
-    E.ssum[blk_at<6>(p, 0)] = sxx;
-    E.ssum[blk_at<6>(p, 1)] = syy;
-    E.ssum[blk_at<6>(p, 2)] = szz;
-    E.ssum[blk_at<6>(p, 3)] = sxy;
-    E.ssum[blk_at<6>(p, 4)] = sxz;
-    E.ssum[blk_at<6>(p, 5)] = syz;
+    store_pair(E.ssum + pml_at<R, 6>(p, 0), sxx, syy);
+    store_pair(E.ssum + pml_at<R, 6>(p, 2), szz, sxy);
+    store_pair(E.ssum + pml_at<R, 6>(p, 4), sxz, syz);
     R s0[6];
 #pragma unroll
     for (int k = 0; k < 6; k++) s0[k] = load_nt(E.stress + blk_at<21>(p, 15 + k)) + (R)C.rdampk * srt[k];
@@ -1059,15 +1102,14 @@ constexpr int kU2 = (kKeep || kGeo) ? 8 : EQD_ELEM_UNROLL;
             // a PML node: its 9 split components are (0 - (dw g_i) s_j) with
             // the summed stresses stored above, so the node kernel forms them
             // from dw g (3 values) bit for bit; 10-12 are stored whole
-#pragma unroll
-            for (int k = 0; k < 3; k++) store_nt(E.ef + blk_at<kEfPml>(p, a * 6 + k), dw * (k == 0 ? g1 : (k == 1 ? g2 : g3)));
-#pragma unroll
-            for (int k = 0; k < 3; k++) store_nt(E.ef + blk_at<kEfPml>(p, a * 6 + 3 + k), f[9 + k] + hg[k]);
+            store_pair_nt(E.ef + pml_at<R, kEfPml>(p, a * 6 + 0), dw * g1, dw * g2);
+            store_pair_nt(E.ef + pml_at<R, kEfPml>(p, a * 6 + 2), dw * g3, f[9] + hg[0]);
+            store_pair_nt(E.ef + pml_at<R, kEfPml>(p, a * 6 + 4), f[10] + hg[1], f[11] + hg[2]);
         } else {
             // a 3-component node takes the collapsed sums (src/assembleGlobalKU.f90:55-61)
-            store_nt(E.ef + blk_at<kEfPml>(p, a * 6 + 0), f[0] + f[1] + f[2] + f[9] + hg[0]);
-            store_nt(E.ef + blk_at<kEfPml>(p, a * 6 + 1), f[3] + f[4] + f[5] + f[10] + hg[1]);
-            store_nt(E.ef + blk_at<kEfPml>(p, a * 6 + 2), f[6] + f[7] + f[8] + f[11] + hg[2]);
+            store_pair_nt(E.ef + pml_at<R, kEfPml>(p, a * 6 + 0), f[0] + f[1] + f[2] + f[9] + hg[0],
+                          f[3] + f[4] + f[5] + f[10] + hg[1]);
+            store_nt(E.ef + pml_at<R, kEfPml>(p, a * 6 + 2), f[6] + f[7] + f[8] + f[11] + hg[2]);
         }
         if constexpr (!kLds && !kKeep) {
 #pragma unroll
@@ -1096,11 +1138,15 @@ __device__ __forceinline__ void gather3(const NodeState &N, int node, const R *e
             if (j0 + k < end) {
                 const int64_t idx = e[k] & kEntIdxMask;
                 const int a = (e[k] >> kEntLocalShift) & 7;
-                const bool pml = (e[k] & kEntPml) != 0;
-                const R *b = pml ? efp + blk_at<kEfPml>(idx, a * 6) : ef + blk_at<kEfReg>(idx, a * 3);
-                f[k][0] = b[0];
-                f[k][1] = b[64];
-                f[k][2] = b[128];
+                if (e[k] & kEntPml) {
+                    load_pair(efp + pml_at<R, kEfPml>(idx, a * 6), f[k][0], f[k][1]);
+                    f[k][2] = efp[pml_at<R, kEfPml>(idx, a * 6 + 2)];
+                } else {
+                    const R *b = ef + blk_at<kEfReg>(idx, a * 3);
+                    f[k][0] = b[0];
+                    f[k][1] = b[64];
+                    f[k][2] = b[128];
+                }
             }
         }
 #pragma unroll
@@ -1130,11 +1176,15 @@ __device__ __forceinline__ void gather3_ell(const uint32_t *ell, int64_t i, cons
         if (e[k] != kEntNone) {
             const int64_t idx = e[k] & kEntIdxMask;
             const int a = (e[k] >> kEntLocalShift) & 7;
-            const bool pml = (e[k] & kEntPml) != 0;
-            const R *b = pml ? efp + blk_at<kEfPml>(idx, a * 6) : ef + blk_at<kEfReg>(idx, a * 3);
-            f[k][0] = b[0];
-            f[k][1] = b[64];
-            f[k][2] = b[128];
+            if (e[k] & kEntPml) {
+                load_pair(efp + pml_at<R, kEfPml>(idx, a * 6), f[k][0], f[k][1]);
+                f[k][2] = efp[pml_at<R, kEfPml>(idx, a * 6 + 2)];
+            } else {
+                const R *b = ef + blk_at<kEfReg>(idx, a * 3);
+                f[k][0] = b[0];
+                f[k][1] = b[64];
+                f[k][2] = b[128];
+            }
         }
     }
 #pragma unroll
@@ -1233,10 +1283,13 @@ __global__ void __launch_bounds__(256) node_pml_kernel(const Member<R> *__restri
                 const int a = (e[k] >> kEntLocalShift) & 7;
                 // the 9 split components from dw g and the element's summed
                 // stresses, as elem_pml_kernel forms them; 10-12 as stored
-                const R q1 = efp[blk_at<kEfPml>(idx, a * 6 + 0)], q2 = efp[blk_at<kEfPml>(idx, a * 6 + 1)],
-                        q3 = efp[blk_at<kEfPml>(idx, a * 6 + 2)];
-                const R sxx = ssum[blk_at<6>(idx, 0)], syy = ssum[blk_at<6>(idx, 1)], szz = ssum[blk_at<6>(idx, 2)],
-                        sxy = ssum[blk_at<6>(idx, 3)], sxz = ssum[blk_at<6>(idx, 4)], syz = ssum[blk_at<6>(idx, 5)];
+                R q1, q2, q3, d1, d2, d3, sxx, syy, szz, sxy, sxz, syz;
+                load_pair(efp + pml_at<R, kEfPml>(idx, a * 6 + 0), q1, q2);
+                load_pair(efp + pml_at<R, kEfPml>(idx, a * 6 + 2), q3, d1);
+                load_pair(efp + pml_at<R, kEfPml>(idx, a * 6 + 4), d2, d3);
+                load_pair(ssum + pml_at<R, 6>(idx, 0), sxx, syy);
+                load_pair(ssum + pml_at<R, 6>(idx, 2), szz, sxy);
+                load_pair(ssum + pml_at<R, 6>(idx, 4), sxz, syz);
                 f[k][0] = (R)0 - q1 * sxx;
                 f[k][1] = (R)0 - q2 * sxy;
                 f[k][2] = (R)0 - q3 * sxz;
@@ -1246,8 +1299,9 @@ __global__ void __launch_bounds__(256) node_pml_kernel(const Member<R> *__restri
                 f[k][6] = (R)0 - q1 * sxz;
                 f[k][7] = (R)0 - q2 * syz;
                 f[k][8] = (R)0 - q3 * szz;
-#pragma unroll
-                for (int c = 0; c < 3; c++) f[k][9 + c] = efp[blk_at<kEfPml>(idx, a * 6 + 3 + c)];
+                f[k][9] = d1;
+                f[k][10] = d2;
+                f[k][11] = d3;
             }
         }
 #pragma unroll
