@@ -99,6 +99,22 @@ __host__ __device__ __forceinline__ int64_t blk_at(int64_t t, int s)
 // c4 FP64 PML element kernel 0.288 -> 0.252 ms, step 1.006 -> 0.966 ms; in
 // FP32 (8-byte pairs) the same layout measured no faster on c4 and 1.4 %
 // slower on the 64-scenario ensemble, so the FP32 mode keeps blk_at.
+// The regular elements' forces on their nodes, component c of local node a
+// of element t: FP64 keeps x and y of a node as a 16-byte pair (rows 2a,
+// 2a+1 of a 64-element block interleaved) and z in rows 16 + a, so a node's
+// force is one paired and one single access; FP32 keeps blk_at<24> slot
+// a*3+c.  Both fill 24 rows of 64 values per block.
+template <typename R>
+__host__ __device__ __forceinline__ int64_t reg_ef_at(int64_t t, int a, int c)
+{
+    if constexpr (sizeof(R) == sizeof(double)) {
+        if (c < 2) return ((t >> 6) * 24 + 2 * a) * 64 + (t & 63) * 2 + c;
+        return ((t >> 6) * 24 + 16 + a) * 64 + (t & 63);
+    } else {
+        return blk_at<24>(t, a * 3 + c);
+    }
+}
+
 template <typename R, int W>
 __host__ __device__ __forceinline__ int64_t pml_at(int64_t t, int s)
 {

@@ -34,7 +34,7 @@ __device__ __forceinline__ void store_nt(R *p, R v)
     __builtin_nontemporal_store(v, p);
 }
 
-// two values of a PML slot or summed-stress row (pml_at) as one access in
+// two values of a force slot or summed-stress row (pml_at, reg_ef_at) as one access in
 // FP64, as two in FP32
 template <typename R>
 struct Pair;
@@ -751,8 +751,8 @@ constexpr int kU2 = (kKeep || kGeo) ? 8 : EQD_ELEM_UNROLL;
                 f[i] = f[i] + fv;
             }
         }
-#pragma unroll
-        for (int k = 0; k < 3; k++) store_nt(E.ef + blk_at<kEfReg>(t, a * 3 + k), f[k]);
+        store_pair_nt(E.ef + reg_ef_at<R>(t, a, 0), f[0], f[1]);
+        store_nt(E.ef + reg_ef_at<R>(t, a, 2), f[2]);
         if constexpr (!kLds && !kKeep) {
 #pragma unroll
             for (int k = 0; k < kNo; k++) opc_[k] = opn_[k];
@@ -1142,10 +1142,8 @@ __device__ __forceinline__ void gather3(const NodeState &N, int node, const R *e
                     load_pair(efp + pml_at<R, kEfPml>(idx, a * 6), f[k][0], f[k][1]);
                     f[k][2] = efp[pml_at<R, kEfPml>(idx, a * 6 + 2)];
                 } else {
-                    const R *b = ef + blk_at<kEfReg>(idx, a * 3);
-                    f[k][0] = b[0];
-                    f[k][1] = b[64];
-                    f[k][2] = b[128];
+                    load_pair(ef + reg_ef_at<R>(idx, a, 0), f[k][0], f[k][1]);
+                    f[k][2] = ef[reg_ef_at<R>(idx, a, 2)];
                 }
             }
         }
@@ -1180,10 +1178,8 @@ __device__ __forceinline__ void gather3_ell(const uint32_t *ell, int64_t i, cons
                 load_pair(efp + pml_at<R, kEfPml>(idx, a * 6), f[k][0], f[k][1]);
                 f[k][2] = efp[pml_at<R, kEfPml>(idx, a * 6 + 2)];
             } else {
-                const R *b = ef + blk_at<kEfReg>(idx, a * 3);
-                f[k][0] = b[0];
-                f[k][1] = b[64];
-                f[k][2] = b[128];
+                load_pair(ef + reg_ef_at<R>(idx, a, 0), f[k][0], f[k][1]);
+                f[k][2] = ef[reg_ef_at<R>(idx, a, 2)];
             }
         }
     }
