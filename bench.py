@@ -431,8 +431,9 @@ def main() -> int:
     copy_gbs = measure_copy_gbs()
     log(f"device copy: {copy_gbs:.1f} GB/s (read + write)")
     roof = roofline(run, kbytes, precision, args.workload, run["wall"] / steps * 1e3, copy_gbs)
-    roof["kernel_times_from"] = (f"HIP events of the kernels' own dispatches on every {args.timing_every}-th "
-                                 f"timed step ({run['ktimes'][KERNEL_KEY[roof['kernel']]][1]} launches)")
+    every = "every timed step" if args.timing_every == 1 else f"one timed step in {args.timing_every}"
+    roof["kernel_times_from"] = (f"HIP events of the kernels' own dispatches on {every} "
+                                 f"({run['ktimes'][KERNEL_KEY[roof['kernel']]][1]} launches)")
     del bundle
 
     cpu = None
