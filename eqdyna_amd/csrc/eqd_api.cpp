@@ -1146,6 +1146,32 @@ int eqd_commit(eqd_handle *h)
         if ((rc = dalloc(h, &d, non * (int64_t)s.nstep * 12))) return rc;
         HIPCHK(h, hipMemset(d, 0, sizeof(double) * non * s.nstep * 12));
         F.hist = d;
+        // hypocentre pairs, as showSourceDynamics finds them
+        // (src/faulting.f90:348-349): the slave node within tol = 1e-5 m of
+        // xsource along x and of zsource along z
+        {
+            std::vector<int32_t> hy(nf, -1);
+            int32_t nh = 0;
+            for (int64_t i = 0; i < nf; i++) {
+                const int64_t n = fn[i];
+                if (std::fabs(coor[n * 3] - s.xsource) < 1.0e-5 && std::fabs(coor[n * 3 + 2] - s.zsource) < 1.0e-5)
+                    hy[i] = nh++;
+            }
+            F.nhyp = nh;
+            F.hyp = nullptr;
+            F.hyp_hist = nullptr;
+            h->hyp_pairs.clear();
+            for (int64_t i = 0; i < nf; i++)
+                if (hy[i] >= 0) h->hyp_pairs.push_back((int32_t)i);
+            if (nh > 0) {
+                int32_t *hp;
+                if ((rc = dupload(h, &hp, hy))) return rc;
+                F.hyp = hp;
+                if ((rc = dalloc(h, &d, (int64_t)s.nstep * nh * kHypVals))) return rc;
+                HIPCHK(h, hipMemset(d, 0, sizeof(double) * s.nstep * nh * kHypVals));
+                F.hyp_hist = d;
+            }
+        }
         F.tph = nullptr;
         if (s.friclaw == 5) {
             if ((rc = dalloc(h, &d, 2 * nf * (int64_t)s.nstep))) return rc;
@@ -1657,6 +1683,31 @@ int eqd_record_get(eqd_handle *h, double *vel, double *slip_rate, int32_t max_re
     if (dropped > 0)
         return fail(h, EQD_EINVAL, "%d record steps since the last eqd_record_get, capacity %d: the last %d dropped",
                     n + dropped, R.capacity, dropped);
+    return EQD_OK;
+}
+
+int eqd_hypocentre_pairs(eqd_handle *h, int32_t *pairs, int32_t cap, int32_t *n)
+{
+    if (!h || !n || cap < 0 || (cap > 0 && !pairs)) return EQD_EINVAL;
+    if (!h->committed) return fail(h, EQD_ESTATE, "eqd_hypocentre_pairs before eqd_commit");
+    *n = (int32_t)h->hyp_pairs.size();
+    for (int32_t k = 0; k < *n && k < cap; k++) pairs[k] = h->hyp_pairs[k] + 1;
+    return EQD_OK;
+}
+
+int eqd_hypocentre_get(eqd_handle *h, int32_t first_step, int32_t count, double *out)
+{
+    if (!h || count < 0 || (count > 0 && !out)) return EQD_EINVAL;
+    if (!h->committed) return fail(h, EQD_ESTATE, "eqd_hypocentre_get before eqd_commit");
+    if (count == 0 || h->F.nhyp == 0) return EQD_OK;
+    // only steps of runs that have ended: their rows are final, whatever a
+    // run in flight writes to the rows of later steps
+    if (first_step < 1 || first_step + count - 1 > h->nt)
+        return fail(h, EQD_EINVAL, "steps %d..%d are not all done (clock %d)", first_step, first_step + count - 1, h->nt);
+    HIPCHK(h, hipSetDevice(h->s.device));
+    const size_t row = (size_t)h->F.nhyp * kHypVals;
+    HIPCHK(h, hipMemcpy(out, h->F.hyp_hist + (size_t)(first_step - 1) * row, sizeof(double) * row * count,
+                        hipMemcpyDeviceToHost));
     return EQD_OK;
 }
 

@@ -27,6 +27,10 @@ constexpr uint32_t kEntIdxMask = 0x0FFFFFFFu;
 constexpr int kEntLocalShift = 28;
 constexpr uint32_t kEntNone = 0xFFFFFFFFu;  // an unused slot of the ELL gather table
 constexpr int kEll3 = 8;
+// values recorded per step for a hypocentre pair (showSourceDynamics,
+// src/faulting.f90:343-365): timeElapsed, fric 78, 79, 80, 23, 73, 71, 72,
+// 74, 75, 76, 77, 1, 2, 3, 9, 10, 20, 81
+constexpr int kHypVals = 19;
 
 struct StepConsts {
     double dt, w, rdampk, rdampm, gterm, ccosphi, sinphi, exp_dt_tv, kapa_hg, fric_tp_h, pi;
@@ -234,6 +238,11 @@ struct FaultState {
     double *fnft;           // [nf]
     const int32_t *station; // [nf] on-fault station (0-based) or -1
     double *hist;           // [non][nstep][12]
+    // hypocentre pairs (showSourceDynamics): row of the pair or -1, and per
+    // step and row the kHypVals values the reference prints
+    const int32_t *hyp;     // [nf] or null
+    double *hyp_hist;       // [nstep][nhyp][kHypVals]
+    int32_t nhyp;
     int32_t nstep;
     // thermal pressurization (friclaw 5): slip rate and shear traction of
     // every pair at every step, [2][nf][nstep] (onFaultTPHist), else null

@@ -529,6 +529,7 @@ std::vector<Region> mutable_regions(const eqd_engine *e)
             add(h->F.fnft, (size_t)h->F.nf * sizeof(double));
             add(h->F.nstat, 5 * (size_t)h->F.ld * sizeof(int32_t));
             add(h->F.hist, (size_t)(h->s.non > 0 ? h->s.non : 1) * h->s.nstep * 12 * sizeof(double));
+            add(h->F.hyp_hist, (size_t)h->F.nhyp * h->s.nstep * kHypVals * sizeof(double));
         }
         if (h->S.nrow > 0) add(h->S.hist, (size_t)(h->S.nrow + 1) * h->s.nstep * sizeof(double));
         if (h->F.tph) add(h->F.tph, 2 * (size_t)h->F.nf * h->s.nstep * sizeof(double));

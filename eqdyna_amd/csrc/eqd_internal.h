@@ -138,6 +138,7 @@ struct eqd_handle {
     std::vector<int32_t> node3_all, node3_free, pml_node_ids;
     std::vector<int32_t> pml_node_index;       // node -> PML node index or -1
     int32_t n_reg = 0, n_pml = 0, nreg_stress = 6;
+    std::vector<int32_t> hyp_pairs;  // pairs (0-based) at the hypocentre, in pair order
     // device objects
     eqd::NodeState N{};
     eqd::RegElems<double> E64{};

@@ -1797,6 +1797,18 @@ __global__ void __launch_bounds__(128, EQD_FAULT_WAVES) fault_kernel(const Membe
         h[10] = FR(51) + FR(42);
         h[11] = FR(52);
     }
+    // showSourceDynamics, src/faulting.f90:343-365: the hypocentre pair's
+    // state after the friction solve, kept per step for the host to print
+    if (Fs.hyp) {
+        const int hy = Fs.hyp[i];
+        if (hy >= 0) {
+            double *o = Fs.hyp_hist + ((int64_t)(nt - 1) * Fs.nhyp + hy) * kHypVals;
+            constexpr int slots[kHypVals - 1] = {78, 79, 80, 23, 73, 71, 72, 74, 75, 76, 77, 1, 2, 3, 9, 10, 20, 81};
+            o[0] = time;
+#pragma unroll
+            for (int k = 0; k < kHypVals - 1; k++) o[k + 1] = FR(slots[k]);
+        }
+    }
     // storeRuptureTime, src/faulting.f90:329-341
     if (Fs.fnft[i] > 5000.0 && rate[3] >= C.slipRateThres) Fs.fnft[i] = time;
     // source-evolution record (output_src_evol, src/library_output.f90:297-312)

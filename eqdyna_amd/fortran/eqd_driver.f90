@@ -38,6 +38,7 @@ module eqd_abi
         EQD_IDHIST = 23, EQD_STRESSARR = 24, EQD_PSTRAIN = 25, EQD_V1 = 26, EQD_VELARR = 27, &
         EQD_DISPARR = 28, EQD_NODALFORCEARR = 29, EQD_FRIC = 30, EQD_FNFT = 31, EQD_ONFAULTHIST = 32, &
         EQD_OFFFAULTHIST = 33
+    integer(c_int32_t), parameter :: EQD_HYP_VALS = 19
 
     interface
         integer(c_int) function eqd_create(s, h) bind(C, name='eqd_create')
@@ -90,6 +91,19 @@ module eqd_abi
             integer(c_int32_t), value :: max_records
             integer(c_int32_t), intent(out) :: cnt
         end function eqd_record_get
+        integer(c_int) function eqd_hypocentre_pairs(h, pairs, cap, n) bind(C, name='eqd_hypocentre_pairs')
+            import :: c_ptr, c_int, c_int32_t
+            type(c_ptr), value :: h
+            integer(c_int32_t), dimension(*) :: pairs
+            integer(c_int32_t), value :: cap
+            integer(c_int32_t), intent(out) :: n
+        end function eqd_hypocentre_pairs
+        integer(c_int) function eqd_hypocentre_get(h, first_step, cnt, out) bind(C, name='eqd_hypocentre_get')
+            import :: c_ptr, c_int, c_int32_t, c_double
+            type(c_ptr), value :: h
+            integer(c_int32_t), value :: first_step, cnt
+            real(c_double), dimension(*) :: out
+        end function eqd_hypocentre_get
         integer(c_int) function eqd_clock(h, nt, t) bind(C, name='eqd_clock')
             import :: c_ptr, c_int, c_int32_t, c_double
             type(c_ptr), value :: h
@@ -141,10 +155,12 @@ subroutine driver
     type(eqd_scalars) :: s
     type(c_ptr) :: h
     integer(c_int) :: rc
-    integer(c_int32_t) :: ntc, badnode, badnt, chunk, done, nrec, irec, inode
+    integer(c_int32_t) :: ntc, badnode, badnt, chunk, done, nrec, irec, inode, nhyp
     integer(c_int32_t), parameter :: maxrec = 32  ! the records of two chunks
     real(c_double) :: tc
-    real(c_double), allocatable :: recv(:, :, :), recs(:, :)
+    real(c_double), allocatable :: recv(:, :, :), recs(:, :), hypv(:, :, :)
+    integer(c_int32_t), allocatable :: hypp(:)
+    integer(c_int32_t) :: nopairs(1)
     integer(c_int64_t) :: nf, non, noff
     character(len=8) :: envval
 
@@ -216,6 +232,11 @@ subroutine driver
         call eqd_check(h, eqd_record(h, surfaceNodeIdArr, int(surface_nnode, c_int32_t), 10_c_int32_t, maxrec), &
                        'record')
     endif
+    ! showSourceDynamics (src/faulting.f90:343-365): the library keeps the
+    ! hypocentre pairs' values per step; they are printed after each chunk.
+    call eqd_check(h, eqd_hypocentre_pairs(h, nopairs, 0_c_int32_t, nhyp), 'hypocentre pairs')
+    allocate(hypp(max(nhyp, 1)), hypv(EQD_HYP_VALS, max(nhyp, 1), 100))
+    if (nhyp > 0) call eqd_check(h, eqd_hypocentre_pairs(h, hypp, nhyp, nhyp), 'hypocentre pairs')
     ! Advance in chunks that end where the reference prints progress
     ! (src/driver.f90:13-17), at most 100 steps (10 records).  Each chunk goes
     ! to the device before the records of the one before it are written, so
@@ -234,6 +255,8 @@ subroutine driver
         endif
         call eqd_check(h, rc, 'run')
         done = done + chunk
+        ! before the next chunk's progress lines, where the reference prints them
+        if (nhyp > 0) call source_dynamics(done - chunk + 1, chunk)
         rc = eqd_clock(h, ntc, tc)
         nt = ntc
         timeElapsed = tc
@@ -292,6 +315,31 @@ contains
             write(*,'(X,A,40X,f7.3,4X,A)') '=', timeElapsed + dt, 's'
         endif
     end subroutine progress
+
+    ! the 'Hypocenter dynamics' blocks of src/faulting.f90:350-362 for steps
+    ! first .. first + cnt - 1, in step order and pair order
+    subroutine source_dynamics(first, cnt)
+        integer(c_int32_t), intent(in) :: first, cnt
+        integer(c_int32_t) :: k, j, n
+        call eqd_check(h, eqd_hypocentre_get(h, first, cnt, hypv), 'hypocentre get')
+        do k = 1, cnt
+            do j = 1, nhyp
+                n = nsmp(1, hypp(j), 1)
+                write(*,*) 'Hypocenter dynamics'
+                write(*,'(X,A,3E15.7)') 'X, Y, Z      (m)   ', meshCoor(1, n), meshCoor(2, n), meshCoor(3, n)
+                write(*,'(X,A,E15.7)') 'TimeElapsed  (s)   ', hypv(1, j, k)
+                write(*,'(X,A,3E15.7)') 'n,s,d tract  (MPa) ', hypv(2, j, k)/1.d6, hypv(3, j, k)/1.d6, hypv(4, j, k)/1.d6
+                write(*,'(X,A,E15.7)') 'state_normal (MPa) ', hypv(5, j, k)/1.d6
+                write(*,'(X,A,3E15.7)') 'n,s,d slip   (m)   ', hypv(6, j, k), hypv(7, j, k), hypv(8, j, k)
+                write(*,'(X,A,3E15.7)') 's,d, peak sr (m/s) ', hypv(9, j, k), hypv(10, j, k), hypv(11, j, k)
+                write(*,'(X,A,E15.7)') 'cummul slip  (m)   ', hypv(12, j, k)
+                write(*,'(X,A,3E15.7)') 'sw_fs, sw_fd, sw_D0', hypv(13, j, k), hypv(14, j, k), hypv(15, j, k)
+                write(*,'(X,A,2E15.7)') 'rsf_a, rsf_b       ', hypv(16, j, k), hypv(17, j, k)
+                write(*,'(X,A,E15.7)') 'rsf_state          ', hypv(18, j, k)
+                write(*,'(X,A,E15.7)') 'Nuc add tau0 (MPa) ', hypv(19, j, k)/1.d6
+            enddo
+        enddo
+    end subroutine source_dynamics
 end subroutine driver
 
 ! Full-precision companions of the reference's text outputs, written when

@@ -24,11 +24,13 @@ EXPORTS = ["eqd_create", "eqd_set", "eqd_commit", "eqd_run", "eqd_run_async", "e
            "eqd_ensemble_create", "eqd_ensemble_run", "eqd_ensemble_tune", "eqd_ensemble_set_launch",
            "eqd_ensemble_set_graph", "eqd_ensemble_set_timing", "eqd_ensemble_kernel_time",
            "eqd_ensemble_last_error", "eqd_ensemble_destroy", "eqd_precompute", "eqd_record", "eqd_record_get",
-           "eqd_mlp_forward_noalloc"]
+           "eqd_mlp_forward_noalloc", "eqd_hypocentre_pairs", "eqd_hypocentre_get"]
 
 # launch kinds (EQD_KIND_*)
 KINDS = {"element": 0, "element_pml": 1, "node": 2, "node_pml": 3, "fault": 4}
 
+HYP_VALS = 19
+HYP_SLOTS = [78, 79, 80, 23, 73, 71, 72, 74, 75, 76, 77, 1, 2, 3, 9, 10, 20, 81]  # after timeElapsed
 MLP_NTRAIN = 8833
 MLP_NPARAMS = 8847
 
@@ -99,6 +101,8 @@ def lib():
         L.eqd_commit.argtypes = [C.c_void_p]
         L.eqd_run.argtypes = [C.c_void_p, C.c_int32]
         L.eqd_run_async.argtypes = [C.c_void_p, C.c_int32]
+        L.eqd_hypocentre_pairs.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.POINTER(C.c_int32)]
+        L.eqd_hypocentre_get.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p]
         L.eqd_wait.argtypes = [C.c_void_p]
         L.eqd_get.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_int64]
         L.eqd_clock.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_double)]
@@ -297,6 +301,23 @@ class Solver:
 
     def device_bytes(self) -> int:
         return int(self.L.eqd_device_bytes(self.h))
+
+    def hypocentre_pairs(self) -> np.ndarray:
+        """1-based pairs at the hypocentre (showSourceDynamics)."""
+        n = C.c_int32()
+        self._check(self.L.eqd_hypocentre_pairs(self.h, None, 0, C.byref(n)))
+        out = np.zeros(max(n.value, 1), dtype="<i4")
+        self._check(self.L.eqd_hypocentre_pairs(self.h, out.ctypes.data, out.size, C.byref(n)))
+        return out[:n.value]
+
+    def hypocentre_get(self, first_step: int, count: int) -> np.ndarray:
+        """[count, pairs, 19]: timeElapsed and fric 78, 79, 80, 23, 73, 71, 72, 74,
+        75, 76, 77, 1, 2, 3, 9, 10, 20, 81 of each hypocentre pair at each step."""
+        npairs = len(self.hypocentre_pairs())
+        out = np.zeros((count, npairs, HYP_VALS))
+        if npairs and count:
+            self._check(self.L.eqd_hypocentre_get(self.h, int(first_step), int(count), out.ctypes.data))
+        return out
 
     def record(self, node_ids, every: int = 10, capacity: int = 64) -> None:
         """Record the velocities of node_ids (1-based) and fric(47) at every

@@ -274,6 +274,21 @@ int eqd_mlp_train(const float *X, const float *r, int64_t n, const eqd_mlp_train
 int eqd_record(eqd_handle *h, const int32_t *node_ids, int32_t n_nodes, int32_t every, int32_t capacity);
 int eqd_record_get(eqd_handle *h, double *vel, double *slip_rate, int32_t max_records, int32_t *count);
 
+/* The hypocentre dynamics that showSourceDynamics (src/faulting.f90:343-365)
+ * prints every step for each pair whose slave node lies within 1e-5 m of
+ * (xsource, zsource) in x and z: the library keeps, per step and pair,
+ * EQD_HYP_VALS values taken where the reference prints them (after the
+ * friction solve): timeElapsed, fric 78, 79, 80 (normal, strike, dip
+ * traction), 23, 73, 71, 72 (slips), 74, 75, 76 (slip rates), 77, 1, 2, 3,
+ * 9, 10, 20, 81.  eqd_hypocentre_pairs gives the pairs (1-based, pair
+ * order, up to cap of them; *n = how many there are).  eqd_hypocentre_get
+ * copies steps first_step .. first_step + count - 1 into out
+ * [count][n][EQD_HYP_VALS]; the steps must be done (a run in flight is not
+ * waited for: its steps are not yet asked for). */
+#define EQD_HYP_VALS 19
+int eqd_hypocentre_pairs(eqd_handle *h, int32_t *pairs, int32_t cap, int32_t *n);
+int eqd_hypocentre_get(eqd_handle *h, int32_t first_step, int32_t count, double *out);
+
 /* ---------------------------------------------------------------------------
  * Device-side one-time precompute: replaces the element loop of
  * assembleGlobalMass (src/assembleGlobalMass.f90:283-406, with

@@ -24,6 +24,9 @@ def available() -> bool:
 LAST_PHASES = {}
 
 
+LAST_STDOUT = [""]  # standard output of the last run (its printed diagnostics)
+
+
 def run_reference(case, workdir: str, nranks: int = 1, dump: bool = True, timeout: int = 3600, binary: str = REF_BIN):
     """Write the case inputs into ``workdir``, run the reference (``binary``:
     the parity build, or REF_BIN_O3 for timing) on ``nranks`` MPI ranks
@@ -44,6 +47,7 @@ def run_reference(case, workdir: str, nranks: int = 1, dump: bool = True, timeou
     p = subprocess.run(cmd, cwd=workdir, env=env, capture_output=True, text=True, timeout=timeout)
     if p.returncode != 0 or "EQD_REF" not in p.stdout:
         raise RuntimeError(f"reference run failed ({p.returncode}):\n{p.stdout[-3000:]}\n{p.stderr[-3000:]}")
+    LAST_STDOUT[0] = p.stdout
     m = re.search(r"t_loop_s\s+(\S+)", p.stdout)
     t_loop = float(m.group(1)) if m else float("nan")
     # start-up phases of the last run (seconds, max over ranks)

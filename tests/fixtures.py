@@ -99,3 +99,94 @@ def channel_errors(ch, ch_ref):
         sc = scale[GROUPS[k]]
         err[k] = d / sc if sc > 0 else d
     return err
+
+
+# showSourceDynamics (src/faulting.f90:350-362) prints, per step and
+# hypocentre pair, 'Hypocenter dynamics' and eleven labelled lines of E15.7
+# numbers: X, Y, Z; timeElapsed; fric 78, 79, 80 / 1e6; fric 23 / 1e6;
+# fric 73, 71, 72; fric 74, 75, 76; fric 77; fric 1, 2, 3; fric 9, 10;
+# fric 20; fric 81 / 1e6.
+HYP_LINES = 11
+HYP_MPA = [1, 2, 3, 4, 18]  # columns of hypocentre_values printed in MPa
+
+
+def fortran_real(tok: str) -> float:
+    """A number in Fortran E format, also where the runtime drops the 'E' of
+    a three-digit exponent (0.1234567-123)."""
+    import re
+    m = re.fullmatch(r"([-+]?\d*\.\d*)([-+]\d+)", tok)
+    return float(m.group(1) + "E" + m.group(2)) if m else float(tok)
+
+
+def hypocentre_blocks(text: str):
+    """(rows, progress): the printed blocks as rows of 22 numbers (X, Y, Z,
+    then the 19 quantities of hypocentre_values), and for each progress line
+    of the time loop ('Current time in dynamic rupture') how many blocks came
+    before it."""
+    lines = text.splitlines()
+    rows, progress = [], []
+    i = 0
+    while i < len(lines):
+        t = lines[i].strip()
+        if t.startswith("=") and "Current time in dynamic rupture" in t:
+            progress.append(len(rows))
+        if t == "Hypocenter dynamics":
+            vals = []
+            for ln in lines[i + 1:i + 1 + HYP_LINES]:
+                body = ln[20:]  # one blank and a 19-character label
+                vals += [fortran_real(body[k:k + 15]) for k in range(0, len(body.rstrip()), 15)]
+            rows.append(vals)
+            i += HYP_LINES
+        i += 1
+    return np.array(rows, dtype=np.float64).reshape(-1, 22), progress
+
+
+def hypocentre_values(rec):
+    """Records of eqd_hypocentre_get (time, fric 78, 79, 80, 23, 73, 71, 72,
+    74, 75, 76, 77, 1, 2, 3, 9, 10, 20, 81) as showSourceDynamics prints them
+    (stresses in MPa)."""
+    v = np.array(rec, dtype=np.float64, copy=True)
+    v[..., HYP_MPA] /= 1.0e6
+    return v
+
+
+def hypocentre_of_fric(fric, pair, time):
+    """The 19 printed quantities of one pair (0-based) from a fric array
+    [pairs, 100]."""
+    from eqdyna_amd.solver import HYP_SLOTS
+    return hypocentre_values(np.concatenate([[time], fric[pair, np.array(HYP_SLOTS) - 1]]))
+
+
+def printed_unit(p):
+    """One unit of the 7th significant digit of E15.7 numbers p (0.ddddddd
+    times a power of ten); 0 where p is 0."""
+    p = np.abs(np.asarray(p, dtype=np.float64))
+    e = np.floor(np.log10(np.where(p > 0, p, 1.0))) + 1.0
+    return np.where(p > 0, 10.0 ** (e - 7.0), 0.0)
+
+
+def hypocentre_columns(sc):
+    """Columns of hypocentre_values that the reference defines for this
+    friction law (fric 23 is left undefined under thermal pressurization)."""
+    from eqdyna_amd.solver import HYP_SLOTS
+    skip = undefined_slots(sc)
+    return np.array([0] + [k + 1 for k, slot in enumerate(HYP_SLOTS) if slot not in skip])
+
+
+# quantity groups of hypocentre_values: time; stresses (n, s, d traction,
+# state_normal, nucleation); slips (n, s, d, cumulative); slip rates; slip
+# weakening parameters; rsf a, b; state
+HYP_GROUPS = [[0], [1, 2, 3, 4, 18], [5, 6, 7, 11], [8, 9, 10], [12, 13, 14], [15, 16], [17]]
+
+
+def hypocentre_scale(vals, skip_cols=()):
+    """Per column, the largest magnitude in its quantity group over the
+    series vals [steps, 19]: a pair at the hypocentre of a strike-slip fault
+    holds rounding noise in its normal and dip components, which only make
+    sense against the strike components."""
+    scale = np.zeros(vals.shape[-1])
+    for grp in HYP_GROUPS:
+        g = [k for k in grp if k not in skip_cols]
+        if g:
+            scale[g] = max(np.abs(vals[..., g]).max(), 1e-300)
+    return scale

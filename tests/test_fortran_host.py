@@ -12,6 +12,7 @@ import pytest
 
 import refrun
 from eqdyna_amd import case
+from fixtures import hypocentre_blocks, printed_unit
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 GPU_HOST = os.path.join(ROOT, "eqdyna_amd", "fortran", "_build", "eqdyna_gpu")
@@ -32,6 +33,23 @@ def test_reference_host_with_gpu_driver_matches_reference():
         g = np.loadtxt(os.path.join(dg, "frt.txt0"))
         r = np.loadtxt(os.path.join(dr, "frt.txt0"))
         gs = sorted(f for f in os.listdir(dg) if f.startswith("faultst"))
+    # the 'Hypocenter dynamics' blocks (showSourceDynamics,
+    # src/faulting.f90:343-365): one per step, between the same progress
+    # lines; each number within one unit of its 7th printed digit or 2e-7 of
+    # the largest value of its quantity: coordinates, time, stresses (n, s, d
+    # traction, state_normal, nucleation), slips (n, s, d, cumulative; the
+    # normal slip is rounding noise next to the others), slip rates, slip
+    # weakening parameters, rsf a and b, state
+    hg, pg = hypocentre_blocks(p.stdout)
+    hr, pr = hypocentre_blocks(refrun.LAST_STDOUT[0])
+    assert hr.shape[0] == c.nstep and hg.shape == hr.shape and pg == pr
+    assert np.array_equal(hg[:, :3], hr[:, :3])
+    hscale = np.zeros(hr.shape[1])
+    for grp in [[0, 1, 2], [3], [4, 5, 6, 7, 21], [8, 9, 10, 14], [11, 12, 13], [15, 16, 17], [18, 19], [20]]:
+        hscale[grp] = max(np.abs(hr[:, grp]).max(), 1e-300)
+    herr = np.abs(hg - hr)
+    assert np.all(herr <= np.maximum(printed_unit(hr) * 1.0000001, 2e-7 * hscale)), \
+        np.array2string((herr / hscale).max(axis=0), precision=2)
     assert g.shape == r.shape
     # rupture times (column 4) identical; every other column within 2e-6 (one
     # unit of the 7th printed digit) of the largest value of its quantity:

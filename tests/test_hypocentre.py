@@ -1,0 +1,111 @@
+"""Hypocentre dynamics (showSourceDynamics, src/faulting.f90:343-365): per
+step, the values of the pair whose slave node lies within 1e-5 m of
+(xsource, zsource) in x and z, taken after the friction solve.
+
+Pins: the reference build's own printed blocks against the oracle's fric
+after k steps (the slot list, the MPa scaling and the step's time; the
+oracle matches the reference bit for bit, so every printed number must be
+the oracle's value rounded to 7 digits; fric 23 is left out under thermal
+pressurization, where the reference leaves it undefined).  The GPU's series against the
+oracle at full precision (FP64 bar 1e-9 of the largest value of each
+quantity group: stresses, slips, slip rates, ...)
+and, bit for bit, against the GPU's own fric after the last step."""
+import os
+import tempfile
+
+import numpy as np
+import pytest
+
+import oracle
+import refrun
+from fixtures import (hypocentre_blocks, hypocentre_columns, hypocentre_of_fric, hypocentre_scale, hypocentre_values,
+                      load_golden, printed_unit)
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _golden_cases():
+    import sys
+    sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
+    import make_fixtures
+    return {c.name: c for c in make_fixtures.cases()}
+
+
+def _times(dt, n):
+    # timeElapsed = timeElapsed + dt once per step (src/driver.f90:11)
+    t, out = 0.0, []
+    for _ in range(n):
+        t += dt
+        out.append(t)
+    return np.array(out)
+
+
+def _hyp_pair(bundle, sc):
+    sl = bundle["nsmp"][:, 0] - 1
+    x, z = bundle["meshCoor"][sl, 0], bundle["meshCoor"][sl, 2]
+    return np.nonzero((np.abs(x - sc["xsource"]) < 1e-5) & (np.abs(z - sc["zsource"]) < 1e-5))[0]
+
+
+@pytest.mark.parametrize("name", ["box_sw", "box_rsf4", "box_tp", "box_tpv2802"])
+def test_reference_prints_oracle_fric(name):
+    if not refrun.available():
+        pytest.skip("needs the reference build (built where the reference sources are present)")
+    c = _golden_cases()[name]
+    with tempfile.TemporaryDirectory() as d:
+        b, sc, _, _ = refrun.run_reference(c, d, 1, dump=True, timeout=300)
+    rows, progress = hypocentre_blocks(refrun.LAST_STDOUT[0])
+    nstep = int(sc["nstep"])
+    pairs = _hyp_pair(b, sc)
+    assert len(pairs) == 1 and rows.shape[0] == nstep
+    # progress lines before steps 1, 101, ...
+    assert progress == list(range(0, nstep, 100))
+    sl = b["nsmp"][pairs[0], 0] - 1
+    xyz = b["meshCoor"][sl]
+    assert np.all(np.abs(rows[:, :3] - xyz) <= printed_unit(rows[:, :3]) * 0.5000001)
+    times = _times(sc["dt"], nstep)
+    for k in sorted({1, 2, nstep // 3, nstep}):
+        cols = hypocentre_columns(sc)
+        want = hypocentre_of_fric(oracle.run_oracle(b, sc, k)["fric"], pairs[0], times[k - 1])[cols]
+        got = rows[k - 1, 3:][cols]
+        # the printed number is the oracle's value rounded to 7 digits
+        bad = np.abs(want - got) > printed_unit(got) * 0.5000001
+        assert not bad.any(), (name, k, np.nonzero(bad)[0], want[bad], got[bad])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["box_sw", "box_rsf4", "box_tp", "box_dip", "box_c5rsf", "box_tpv2802"])
+def test_gpu_hypocentre_series_matches_oracle(name):
+    from eqdyna_amd.solver import EqdError, Solver
+    print(f"case {name}", flush=True)
+    b, sc, _ = load_golden(name)
+    nstep = int(sc["nstep"])
+    pair = _hyp_pair(b, sc)
+    s = Solver(b, sc, "fp64")
+    try:
+        assert list(s.hypocentre_pairs()) == list(pair + 1)
+        done = 0
+        while done < nstep:  # uneven chunks: the series must not depend on them
+            n = min(7, nstep - done)
+            s.run(n)
+            done += n
+            with pytest.raises(EqdError):
+                s.hypocentre_get(1, done + 1)  # a step not yet taken
+        series = s.hypocentre_get(1, nstep)[:, 0, :]
+        fric = s.outputs()["fric"]
+    finally:
+        s.close()
+    times = _times(sc["dt"], nstep)
+    assert np.allclose(series[:, 0], times, rtol=1e-13, atol=0)
+    # the last step's record is the device's own fric, bit for bit
+    assert np.array_equal(hypocentre_values(series[-1]), hypocentre_of_fric(fric, pair[0], series[-1, 0]))
+    cols = hypocentre_columns(sc)
+    vals = hypocentre_values(series)
+    scale = hypocentre_scale(vals, [k for k in range(vals.shape[1]) if k not in cols])[cols]
+    got = vals[:, cols]
+    err = np.zeros(len(cols))
+    for k in sorted({1, 2, nstep // 3, 2 * nstep // 3, nstep}):
+        want = hypocentre_of_fric(oracle.run_oracle(b, sc, k)["fric"], pair[0], times[k - 1])[cols]
+        err = np.maximum(err, np.abs(got[k - 1] - want) / scale)
+    print(f"[{name}] hypocentre series vs oracle: worst {err.max():.2e} per column "
+          + np.array2string(err, precision=1), flush=True)
+    assert err.max() <= 1e-9
