@@ -1702,8 +1702,9 @@ int eqd_hypocentre_get(eqd_handle *h, int32_t first_step, int32_t count, double 
     if (count == 0 || h->F.nhyp == 0) return EQD_OK;
     // only steps of runs that have ended: their rows are final, whatever a
     // run in flight writes to the rows of later steps
-    if (first_step < 1 || first_step + count - 1 > h->nt)
-        return fail(h, EQD_EINVAL, "steps %d..%d are not all done (clock %d)", first_step, first_step + count - 1, h->nt);
+    const int64_t last = (int64_t)first_step + count - 1;
+    if (first_step < 1 || last > h->nt)
+        return fail(h, EQD_EINVAL, "steps %d..%lld are not all done (clock %d)", first_step, (long long)last, h->nt);
     HIPCHK(h, hipSetDevice(h->s.device));
     const size_t row = (size_t)h->F.nhyp * kHypVals;
     HIPCHK(h, hipMemcpy(out, h->F.hyp_hist + (size_t)(first_step - 1) * row, sizeof(double) * row * count,
