@@ -922,6 +922,162 @@ def rough_fractal(c: Case, alpha: float, hurst: float, seed: int) -> None:
     c.rough = np.ascontiguousarray(rows)
 
 
+def reference_fractal_surface(n: int, hurst: float, seed: int) -> np.ndarray:
+    """The reference's own fractal surface, value for value: the spectrum of
+    scripts/generateFaultInterface:11-52 built from numpy's legacy
+    RandomState stream (seeded once; n*n normal deviates for the real parts,
+    then n*n for the imaginary parts, both times sqrt(2)/2), amplitude
+    |k|^-(H+1) with the Butterworth roll-off 1/(1+(|k|^2/(n/4)^2)^4), the
+    wavenumbers with |k|^2 <= 2 removed, and a quarter of the real part of
+    the inverse transform.  Unlike fractal_surface it keeps the reference's
+    RNG, wavenumber indexing and scaling, so a case the reference generated
+    with a seed can be rebuilt."""
+    rs = np.random.RandomState(seed)
+    re_part = rs.normal(size=(n, n))
+    im_part = rs.normal(size=(n, n))
+    spec = 0.5 * math.sqrt(2.0) * (re_part + 1j * im_part)
+    half = n // 2
+    k = np.arange(n)
+    k = np.where(k > half, k - n, k).astype(np.float64)
+    ksq = k[:, None] ** 2 + k[None, :] ** 2
+    kcut_sq = (n / 4) * (n / 4)
+    amp = np.zeros((n, n))
+    keep = ksq > 2.0
+    amp[keep] = np.sqrt(1.0 / ksq[keep] ** (hurst + 1.0) * (1.0 / (1.0 + (ksq[keep] / kcut_sq) ** 4)))
+    shaped = amp * spec
+    shaped[~keep] = 0.0
+    return 0.25 * np.real(np.fft.ifft2(shaped))
+
+
+def rough_reference_fractal(c: Case, seed: int = 1, hurst: float = 1.0, alpha: float = 0.005) -> None:
+    """bFault_Rough_Geometry.txt as the reference's generator writes it for
+    ``c`` (scripts/generateFaultInterface:54-115): the surface on an
+    nfx x nfx grid scaled so that its standard deviation is alpha times
+    nfx * dx, its first nfz rows, the dip term (nz-1-j) dx cos(dip), slopes
+    by numpy's gradient, every value rounded to six decimals (the file's
+    '%f' format)."""
+    nx, nz = c.nfx, c.nfz
+    w = reference_fractal_surface(nx, hurst, seed)
+    surf = (w * alpha / (np.std(w) / nx / c.dx))[:nz, :nx]
+    surf = surf + (nz - 1 - np.arange(nz))[:, None] * c.dx * math.cos(c.dip / 180.0 * math.pi)
+    dhdx = np.gradient(surf / c.dx, axis=1)
+    dhdz = np.gradient(surf / c.dz, axis=0)
+    rows = np.stack([surf.T.ravel(), dhdx.T.ravel(), dhdz.T.ravel()], axis=1)  # x-major, z fastest
+    c.rough_nx, c.rough_nz = nx, nz
+    c.rough_origin = (float(c.fxmin), float(c.fzmin), float(c.dx))
+    c.rough = np.vectorize(lambda v: float("%f" % v))(rows)
+
+
+def _taper(x: float, z: float, wwx: float, wx: float, wwz1: float, wz1: float, wwz2: float, wz2: float):
+    """Along-strike and down-dip scale factors of case_input/test.drv.a6
+    (getScaleCoeff): 1 inside |x| <= wwx and wwz1 <= |z| <= wwz2, falling
+    linearly to 0 over wx along strike, wz1 above and wz2 below."""
+    ax, az = abs(x), abs(z)
+    if ax <= wwx:
+        sx = 1.0
+    elif ax < wwx + wx:
+        sx = 1.0 - (ax - wwx) / wx
+    else:
+        sx = 0.0
+    sz = 1.0
+    if az <= wwz1 - wz1 or az >= wwz2 + wz2:
+        sz = 0.0
+    elif wwz1 - wz1 <= az <= wwz1:
+        sz = 1.0 - (wwz1 - az) / wz1
+    elif wwz2 <= az <= wwz2 + wz2:
+        sz = 1.0 - (az - wwz2) / wz2
+    if wwz1 <= az <= wwz2:
+        sz = 1.0
+    return sx, sz
+
+
+def drv_a6() -> Case:
+    """case_input/test.drv.a6/user_defined_params.py: a Drucker-Prager medium
+    (C_elastic 0) on a 17-layer velocity profile, the reference's seed-1
+    fractal rough fault (insertFaultType 2), slip-law rate and state with
+    strong velocity weakening under tapered a and Vw, TPV2802 nucleation,
+    500 m cells, 2 x 2 x 1 ranks.  dt, roumax and vmaxPML keep the class
+    defaults (6000 m/s, 2670 kg/m3) because scripts/defaultParameters.py
+    evaluates them before the case sets its layers."""
+    c = Case(name="drv.a6")
+    c.xmin, c.xmax = -40.0e3, 40.0e3
+    c.ymin, c.ymax = -31.0e3, 30.0e3
+    c.zmin, c.zmax = -40.0e3, 0.0e3
+    c.fxmin, c.fxmax = -25.0e3, 25.0e3
+    c.fymin, c.fymax = 0.0, 0.0
+    c.fzmin, c.fzmax = -25.0e3, 0.0e3
+    c.xsource, c.ysource, c.zsource = -12.0e3, 0.0, -12.0e3
+    c.C_elastic = 0
+    c.str1ToFaultAngle, c.devStrToStrVertRatio = 45.0, 0.33
+    c.C_nuclea = 1
+    c.friclaw = 4
+    c.insertFaultType = 2
+    c.enlarging_ratio = 1.0
+    c.outputGroundMotion = 0
+    c.output_plastic = 1
+    c.term = 5.0
+    c.dx = c.dy = c.dz = 500.0
+    c.tpv = 2802
+    # layers: at each 500 m cell centre down to 8 km, Vs from the power laws
+    # of the case file, Vp = max(1.4 + 1.14 Vs, 1.68 Vs), rho = 2.4405 + 0.10271 Vs
+    rows = []
+    nz_max = round((c.zmax - c.zmin) / c.dz + 1)
+    for i in range(nz_max):
+        zk = (i + 0.5) * c.dz / 1.0e3
+        if zk < 0.03:
+            vs = 2.206 * zk ** 0.272
+        elif zk < 0.19:
+            vs = 3.542 * zk ** 0.407
+        elif zk < 4.0:
+            vs = 2.505 * zk ** 0.199
+        elif zk < 8.0:
+            vs = 2.927 * zk ** 0.086
+        else:
+            vs = 2.927 * 8.0 ** 0.086
+        vp = max(1.4 + 1.14 * vs, 1.68 * vs)
+        rho = 2.4405 + 0.10271 * vs
+        rows.append([zk * 1e3 + 0.5 * c.dz, vp * 1e3, vs * 1e3, rho * 1e3])
+        if zk > 8.0:
+            break
+    rows[-1][0] = 1.e6
+    c.nmat, c.n2mat = len(rows), 4
+    c.mat = np.array(rows)
+    c.dt = 0.5 * c.dx / 6.0e3
+    c.nucR = 4.0e3
+    c.nucdtau0 = 50.0e6
+    c.nx, c.ny, c.nz = 2, 2, 1
+    c.st_on_fault = [list(s) for s in TPV1053D_ON]
+    c.st_off_fault = [list(s) for s in TPV1053D_OFF]
+    rough_reference_fractal(c, seed=1)
+    o = c.new_ofv()
+    a0, da, b, dc, r0, v0, fw, vw, dvw = 0.01, 0.01, 0.014, 0.4, 0.6, 1e-6, 0.2, 0.1, 0.9
+    for ix, x in enumerate(c.fx):
+        for iz, z in enumerate(c.fz):
+            o[1, iz, ix], o[2, iz, ix], o[3, iz, ix] = 0.18, 0.12, 0.3
+            o[7, iz, ix] = -120.0e6
+            o[8, iz, ix] = 40.0e6
+            t1, t2 = _taper(x, z, 20.e3, 2.e3, 2.e3, 2.e3, 14.e3, 1.e3)
+            dai = 2.0 * da if abs(z) > 15.e3 else da
+            o[9, iz, ix] = a0 + (1. - t1 * t2) * dai
+            o[10, iz, ix] = b
+            o[11, iz, ix] = dc
+            o[12, iz, ix] = v0
+            o[13, iz, ix] = r0
+            o[14, iz, ix] = fw
+            o[15, iz, ix] = vw + dvw * (1. - t1 * t2)
+            o[16, iz, ix] = 4.0e-4 + 1.0 * (1. - t1 * t2)
+            o[17, iz, ix] = 1.0e-6
+            o[18, iz, ix] = 2.7e6
+            o[19, iz, ix] = 0.1e6
+            o[40, iz, ix] = 0.02
+            o[41, iz, ix] = 483.15
+            o[42, iz, ix] = 80.0e6
+            o[46, iz, ix] = 1e-16
+            o[20, iz, ix] = state_steady_state(o[9, iz, ix], b, dc, v0, r0, o[8, iz, ix], o[7, iz, ix],
+                                               o[46, iz, ix], c.friclaw)
+    return c
+
+
 def c3(steps: int = 3000, seed: int = 3, tau0: float = 10.5e6, vs_top: float = 2000.0, sn: float = -30.0e6,
        nucdtau0: float = 18.0e6) -> Case:
     """c3, the rough-fault rate-and-state box at 100 m: 240 x 120 x 120 = 3,456,000 hexahedra

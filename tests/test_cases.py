@@ -1,7 +1,11 @@
 """Seeded workload generators (CPU): the c5 ensemble draws scenarios in the
 stated ranges, deterministically, and its random fields have the stated
-statistics."""
+statistics; the restated seed-1 fractal fault of test.drv.a6 reproduces the
+fault the reference published."""
+import os
+
 import numpy as np
+import pytest
 
 from eqdyna_amd import case
 
@@ -55,3 +59,27 @@ def test_von_karman_field_statistics():
     assert near > 0.8 and abs(far) < 0.3, (near, far)
     g = case.von_karman_field(128, 96, 100.0, 2.0e3, 0.75, seed=1)
     assert np.array_equal(f, g)
+
+
+DRV_A6 = "/root/reference/test.reference.results/test.drv.a6"
+
+
+def test_reference_fractal_surface_matches_published_drv_a6_fault():
+    # frt.txt1 and frt.txt3 (src/library_output.f90:160-205) print x, y, z of
+    # every fault node of the two ranks that hold the fault; y is the rough
+    # surface of scripts/generateFaultInterface (seed 1) at that (x, z).
+    if not os.path.isdir(DRV_A6):
+        pytest.skip("needs the reference's published results (CPU container only)")
+    c = case.drv_a6()
+    n = 0
+    for f in ("frt.txt1", "frt.txt3"):
+        p = np.loadtxt(os.path.join(DRV_A6, f))
+        ix = np.rint((p[:, 0] - c.fxmin) / c.dx).astype(int)
+        iz = np.rint((p[:, 2] - c.fzmin) / c.dz).astype(int)
+        y = c.rough[ix * c.nfz + iz, 0]
+        # within half a unit of the 7th printed digit (E18.7); exactly half
+        # where the six-decimal value is itself a decimal tie
+        unit = 10.0 ** (np.floor(np.log10(np.maximum(np.abs(p[:, 1]), 1e-300))) - 6.0)
+        assert np.all(np.abs(y - p[:, 1]) <= 0.5 * unit * (1 + 1e-6)), f
+        n += len(p)
+    assert n == c.nfx * c.nfz + c.nfz  # the pairs of both ranks, the column at x = 0 in each
