@@ -660,8 +660,18 @@ int engine_init(eqd_engine *e, const std::vector<eqd_handle *> &hs)
     // node kernels within a step; events join the two every step
     HIPCHK(e, hipStreamCreateWithFlags(&e->st, hipStreamNonBlocking));
     HIPCHK(e, hipStreamCreateWithFlags(&e->st2, hipStreamNonBlocking));
-    HIPCHK(e, hipEventCreateWithFlags(&e->ev_fork, hipEventDisableTiming));
-    HIPCHK(e, hipEventCreateWithFlags(&e->ev_join, hipEventDisableTiming));
+    // The fork and join events order two streams of this device and nothing
+    // else, so they skip the system-scope fence (a cache writeback and
+    // invalidation at every record: each kernel's own device-scope release
+    // and acquire already order the data); EQD_EVENT_SYSTEM_FENCE=1 keeps it.
+    // The record event orders a copy into host memory and keeps its fence.
+    unsigned sync_flags = hipEventDisableTiming | hipEventDisableSystemFence;
+    {
+        const char *s = std::getenv("EQD_EVENT_SYSTEM_FENCE");
+        if (s && s[0] == '1') sync_flags = hipEventDisableTiming;
+    }
+    HIPCHK(e, hipEventCreateWithFlags(&e->ev_fork, sync_flags));
+    HIPCHK(e, hipEventCreateWithFlags(&e->ev_join, sync_flags));
     HIPCHK(e, hipEventCreateWithFlags(&e->ev_rec, hipEventDisableTiming));
     const size_t msize = e->fp32 ? sizeof(Member<float>) : sizeof(Member<double>);
     HIPCHK(e, hipMalloc(&e->d_members, msize * hs.size()));
