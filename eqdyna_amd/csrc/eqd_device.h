@@ -328,14 +328,19 @@ struct LaunchGrid {
 // With remap the blocks are renumbered so that each XCD takes one contiguous
 // range of the launch's work: neighbouring elements, which share nodes, are
 // then served by the same L2.  A bijection of [0, gridDim.x).
-__device__ __forceinline__ unsigned launch_block(int remap)
+// skip: leading blocks of the launch that do other work (a multiple of 8).
+__device__ __forceinline__ unsigned launch_block(int remap, unsigned skip = 0)
 {
-    const unsigned b = blockIdx.x, g = gridDim.x;
+    const unsigned b = blockIdx.x - skip, g = gridDim.x - skip;
     if (!remap) return b;
     const unsigned x = b & 7u, i = b >> 3, q = g >> 3, r = g & 7u;
     // XCD x runs q + 1 blocks when x < r, else q
     return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
 }
+
+// leading blocks of an element launch that carries the step start of
+// nstart scenarios
+__host__ __device__ __forceinline__ int step_start_skip(int nstart) { return (nstart + 7) & ~7; }
 
 // scenario of logical block gb (a batched launch has a table; a single
 // scenario's launch has none)

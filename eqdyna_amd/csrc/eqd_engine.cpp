@@ -1,7 +1,9 @@
 // The step engine of libeqd: advances one scenario (a handle) or an ensemble
 // of scenarios.  One step is (src/driver.f90:9-34)
-//   clock -> stations -> elements (regular, PML) -> fault pairs | nodes (3-comp, PML)
-// with the fault pairs on a second stream beside the node kernels.  Every
+//   step start (clock, stations) + regular elements -> records -> PML elements
+//     -> fault pairs | nodes (3-comp, PML)
+// with the step start in the first blocks of the regular-element launch and
+// the fault pairs on a second stream beside the node kernels.  Every
 // launch covers all scenarios that share its kernel variant: the kernels find
 // their scenario through a block-to-scenario table and read its arrays from a
 // per-scenario descriptor (Member in eqd_device.h), so a 64-scenario ensemble
@@ -67,7 +69,7 @@ int64_t count_of(const eqd_handle *h, int kind)
     case kKNode3: return (int64_t)h->node3_free.size();
     case kKNodePml: return h->PN.n;
     case kKFault: return h->F.nf;
-    case kKStation: return 0;  // launched with the clock, one block per scenario (launch_step_start)
+    case kKStation: return 0;  // the step start: its own launch or blocks of the first element launch
     case kKPend3: return (int64_t)h->node3_all.size();
     case kKPendPml: return h->PN.n;
     case kKTp: return h->F.tph ? h->F.nf * 64 : 0;  // one wavefront per pair
@@ -256,24 +258,30 @@ int enqueue_step(eqd_engine *e)
 {
     const Member<R> *dM = (const Member<R> *)e->d_members;
     hipStream_t st = e->st;
-    {
-        // clock advance and off-fault stations, one block per scenario
+    // The step start (clock advance and off-fault stations, one block per
+    // scenario) rides in the first blocks of the first regular-element
+    // launch, which needs no clock itself; a step without regular elements
+    // launches it on its own (and so does EQD_STEP_START_LAUNCH=1).
+    const auto &ge = e->groups[kKElem];
+    const int nstart = ge.empty() || e->step_start_launch ? 0 : (int)e->hs.size();
+    if (nstart == 0) {
         const Stamp p = take_timer(e, kKStation, 1);
         HIPCHK(e, launch_step_start<R>(dM, (int)e->hs.size(), st, p.start(0), p.stop(0)));
     }
     {
+        const Stamp p = take_timer(e, kKElem, ge.size());
+        for (size_t i = 0; i < ge.size(); i++) {
+            const Group &g = ge[i];
+            HIPCHK(e, launch_elem<R>(dM, g.g, g.sig & 1, (g.sig >> 1) & 1, (g.sig >> 2) & 3, (g.sig >> 4) & 1,
+                                     (g.sig >> 5) & 3, i == 0 ? nstart : 0, st, p.start(i), p.stop(i)));
+        }
+    }
+    {
+        // records read the advanced clock; the element kernels leave the node
+        // state they record untouched
         const auto &gr = e->groups[kKRecord];
         const Stamp p = take_timer(e, kKRecord, gr.size());
         for (size_t i = 0; i < gr.size(); i++) HIPCHK(e, launch_record<R>(dM, gr[i].g, st, p.start(i), p.stop(i)));
-    }
-    {
-        const auto &gs = e->groups[kKElem];
-        const Stamp p = take_timer(e, kKElem, gs.size());
-        for (size_t i = 0; i < gs.size(); i++) {
-            const Group &g = gs[i];
-            HIPCHK(e, launch_elem<R>(dM, g.g, g.sig & 1, (g.sig >> 1) & 1, (g.sig >> 2) & 3, (g.sig >> 4) & 1,
-                                     (g.sig >> 5) & 3, st, p.start(i), p.stop(i)));
-        }
     }
     {
         const auto &gs = e->groups[kKPml];
@@ -654,6 +662,8 @@ int engine_init(eqd_engine *e, const std::vector<eqd_handle *> &hs)
         if (s && s[0] == '1') e->tune.remap[kKElem] = e->tune.remap[kKPml] = 1;
         s = std::getenv("EQD_SINGLE_STREAM");
         e->single_stream = s && s[0] == '1';
+        s = std::getenv("EQD_STEP_START_LAUNCH");
+        e->step_start_launch = s && s[0] == '1';
     }
     HIPCHK(e, hipSetDevice(e->device));
     // a second stream (default priority) for the fault pairs, which overlap the

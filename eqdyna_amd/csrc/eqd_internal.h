@@ -16,7 +16,7 @@
 
 namespace eqd {
 template <typename R>
-hipError_t launch_elem(const Member<R> *, const LaunchGrid &, int q, int dp, int hg, int mass, int ops,
+hipError_t launch_elem(const Member<R> *, const LaunchGrid &, int q, int dp, int hg, int mass, int ops, int nstart,
                        hipStream_t, hipEvent_t t0 = nullptr, hipEvent_t t1 = nullptr);
 template <typename R>
 hipError_t launch_pml(const Member<R> *, const LaunchGrid &, int hg, int mass, int ops, hipStream_t,
@@ -80,6 +80,7 @@ struct eqd_engine {
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     hipEvent_t ev_rec = nullptr;  // a run's end, for the record copies
     bool single_stream = false;
+    bool step_start_launch = false;  // the step start as a launch of its own
     void *d_members = nullptr;  // Member<R>[hs.size()]
     std::vector<eqd::Group> groups[eqd::kKinds];
     eqd::Tune tune{};

@@ -320,6 +320,44 @@ __device__ __forceinline__ double exact_u(const Member<R> &M, int node, int k)
     return node_u<R>(M.N, node, k);
 }
 
+// The start of a step for one scenario, by one block: the clock advance of
+// src/driver.f90:9-11 (nt += 1, timeElapsed += dt with the reference's FP64
+// addition, and the run's last-step flag), then storeOffFaultStData
+// (src/driver.f90:157-180) at the new clock: time and the recorded
+// displacement or velocity of every off-fault station row.  Every thread
+// reads the old clock before the barrier and thread 0 writes the new one
+// after it.  Later kernels of the step read the advanced clock from device
+// memory; the element kernels do not read it.
+template <typename R>
+__device__ __forceinline__ void step_start_block(const Member<R> &M)
+{
+    Clock *c = M.clk;
+    const int nt = c->nt + 1;
+    const double time = c->time + M.C.dt;
+    const int run_end = c->run_end;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        c->nt = nt;
+        c->time = time;
+        c->mode = nt == run_end ? 1 : 0;
+    }
+    const OffStations S = M.S;
+    if (S.nrow <= 0) return;
+    double *col = S.hist + (int64_t)(nt - 1) * (S.nrow + 1);
+    for (int r = threadIdx.x; r <= S.nrow; r += blockDim.x) {
+        if (r == 0) {
+            col[0] = time;
+            continue;
+        }
+        const int i = r - 1;
+        const int node = S.idh[i], k = S.idh[S.nrow + i], kind = S.idh[2 * S.nrow + i];
+        if (kind == 1)
+            col[r] = exact_u(M, node, k);
+        else if (kind == 2)
+            col[r] = exact_v(M, node, k);
+    }
+}
+
 // store a node's new state at full precision (and its rounded copy)
 template <typename R>
 __device__ __forceinline__ void exact_store(const Member<R> &M, int node, int k, double v, double u)
@@ -410,15 +448,23 @@ __device__ __forceinline__ void exact_store(const Member<R> &M, int node, int k,
 // operators are rebuilt from the element's corner coordinates (geo_ops).
 template <typename R, bool kQ, bool kDP, int kHG, bool kMass, int kOps>
 __global__ void __launch_bounds__(256, (kOps == kOpsGeom && !kQ && !kDP && !kMass ? EQD_REG_GEO_WAVES : 1))
-    elem_regular_kernel(const Member<R> *__restrict__ Ms, const uint16_t *__restrict__ bmem, int remap)
+    elem_regular_kernel(const Member<R> *__restrict__ Ms, const uint16_t *__restrict__ bmem, int remap, int nstart)
 {
+    // With nstart > 0 the launch carries the step start of scenarios
+    // 0 .. nstart-1 in its first blocks, rounded up to a multiple of 8 so
+    // that the element blocks keep their XCD assignment.
+    const int nskip = step_start_skip(nstart);
+    if ((int)blockIdx.x < nskip) {
+        if ((int)blockIdx.x < nstart) step_start_block<R>(Ms[blockIdx.x]);
+        return;
+    }
     constexpr bool kGeo = kOps == kOpsGeom;
     constexpr bool kLds = kOps == kOpsLds, kCmp = kOps == kOpsCompact || kGeo;
     constexpr int kW = kGeo ? kOpG : (kCmp ? kOpBc : kOpB);
     constexpr int kNst = kQ ? 12 : 6;  // stress values per element (with the Q memory variables)
     extern __shared__ __attribute__((aligned(16))) unsigned char eqd_smem[];
     typedef __attribute__((address_space(3))) R lds_R;
-    const unsigned gb = launch_block(remap);
+    const unsigned gb = launch_block(remap, (unsigned)nskip);
     const Member<R> &M = Ms[member_of(bmem, gb)];
     const RegElems<R> E = M.E;
     const NodeState N = M.N;
@@ -1848,45 +1894,12 @@ __global__ void __launch_bounds__(128, EQD_FAULT_WAVES) fault_kernel(const Membe
     flag_nan(N, nm, nt + 1, vm[0], vm[1], vm[2]);
 }
 
-// The start of a step, one block per scenario: the clock advance of
-// src/driver.f90:9-11 (nt += 1, timeElapsed += dt with the reference's FP64
-// addition, and the run's last-step flag), then storeOffFaultStData
-// (src/driver.f90:157-180) at the new clock: time and the recorded
-// displacement or velocity of every off-fault station row.  One kernel
-// instead of two saves a kernel boundary per step; every later kernel of the
-// step reads the advanced clock from device memory.
+// The start of a step as a launch of its own, one block per scenario (used
+// when a step has no regular-element launch to carry it).
 template <typename R>
 __global__ void __launch_bounds__(64) step_start_kernel(const Member<R> *__restrict__ Ms)
 {
-    const Member<R> &M = Ms[blockIdx.x];
-    __shared__ double s_time;
-    __shared__ int s_nt;
-    if (threadIdx.x == 0) {
-        Clock *c = M.clk;
-        const int nt = c->nt + 1;
-        const double time = c->time + M.C.dt;
-        c->nt = nt;
-        c->time = time;
-        c->mode = nt == c->run_end ? 1 : 0;
-        s_nt = nt;
-        s_time = time;
-    }
-    __syncthreads();
-    const OffStations S = M.S;
-    if (S.nrow <= 0) return;
-    double *col = S.hist + (int64_t)(s_nt - 1) * (S.nrow + 1);
-    for (int r = threadIdx.x; r <= S.nrow; r += blockDim.x) {
-        if (r == 0) {
-            col[0] = s_time;
-            continue;
-        }
-        const int i = r - 1;
-        const int node = S.idh[i], k = S.idh[S.nrow + i], kind = S.idh[2 * S.nrow + i];
-        if (kind == 1)
-            col[r] = exact_u(M, node, k);
-        else if (kind == 2)
-            col[r] = exact_v(M, node, k);
-    }
+    step_start_block<R>(Ms[blockIdx.x]);
 }
 
 // thermop, src/thermop.f90:1-40: pore pressure change fric(51) and
@@ -1977,9 +1990,12 @@ static inline unsigned grid_for(int64_t n, int block) { return (unsigned)((n + b
 
 template <typename R>
 hipError_t launch_elem(const Member<R> *M, const LaunchGrid &g, int q, int dp, int hg, int mass, int ops,
-                       hipStream_t s, hipEvent_t t0, hipEvent_t t1)
+                       int nstart, hipStream_t s, hipEvent_t t0, hipEvent_t t1)
 {
-    if (g.grid == 0) return hipSuccess;
+    if (nstart < 0) return hipErrorInvalidValue;
+    if (g.grid == 0 && nstart == 0) return hipSuccess;
+    // leading blocks for the step start of nstart scenarios (see elem_regular_kernel)
+    const int grid = g.grid + step_start_skip(nstart);
     if (g.block < 64 || g.block > 256 || g.block % 64) return hipErrorInvalidValue;
     if (ops < kOpsRows || ops > kOpsGeom) return hipErrorInvalidValue;
     // geometry rows exist in the FP32 bulk mode only
@@ -1990,13 +2006,17 @@ hipError_t launch_elem(const Member<R> *M, const LaunchGrid &g, int q, int dp, i
 #define EQD_LAUNCH(Q, DP, HG, MASS)                                                                               \
     do {                                                                                                          \
         if (ops == kOpsLds)                                                                                       \
-            EQD_GO((elem_regular_kernel<R, Q, DP, HG, MASS, kOpsLds>), g.grid, g.block, lds, s, M, g.bmem, g.remap); \
+            EQD_GO((elem_regular_kernel<R, Q, DP, HG, MASS, kOpsLds>), grid, g.block, lds, s, M, g.bmem, g.remap,  \
+                   nstart);                                                                                       \
         else if (ops == kOpsGeom)                                                                                 \
-            EQD_GO((elem_regular_kernel<R, Q, DP, HG, MASS, kGeoOr>), g.grid, g.block, 0, s, M, g.bmem, g.remap); \
+            EQD_GO((elem_regular_kernel<R, Q, DP, HG, MASS, kGeoOr>), grid, g.block, 0, s, M, g.bmem, g.remap,     \
+                   nstart);                                                                                       \
         else if (ops == kOpsCompact)                                                                              \
-            EQD_GO((elem_regular_kernel<R, Q, DP, HG, MASS, kOpsCompact>), g.grid, g.block, 0, s, M, g.bmem, g.remap); \
+            EQD_GO((elem_regular_kernel<R, Q, DP, HG, MASS, kOpsCompact>), grid, g.block, 0, s, M, g.bmem,         \
+                   g.remap, nstart);                                                                              \
         else                                                                                                      \
-            EQD_GO((elem_regular_kernel<R, Q, DP, HG, MASS, kOpsRows>), g.grid, g.block, 0, s, M, g.bmem, g.remap); \
+            EQD_GO((elem_regular_kernel<R, Q, DP, HG, MASS, kOpsRows>), grid, g.block, 0, s, M, g.bmem, g.remap,   \
+                   nstart);                                                                                       \
     } while (0)
 #define EQD_HG(Q, DP, MASS)                         \
     do {                                            \
@@ -2118,7 +2138,7 @@ hipError_t launch_step_start(const Member<R> *M, int nmem, hipStream_t s, hipEve
 }
 
 #define EQD_INSTANTIATE(R)                                                                                     \
-    template hipError_t launch_elem<R>(const Member<R> *, const LaunchGrid &, int, int, int, int, int, hipStream_t,    \
+    template hipError_t launch_elem<R>(const Member<R> *, const LaunchGrid &, int, int, int, int, int, int, hipStream_t, \
                                        hipEvent_t, hipEvent_t);                                                       \
     template hipError_t launch_pml<R>(const Member<R> *, const LaunchGrid &, int, int, int, hipStream_t, hipEvent_t,  \
                                       hipEvent_t);                                                                    \

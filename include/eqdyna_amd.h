@@ -139,7 +139,9 @@ int eqd_nan_info(eqd_handle *h, int32_t *node, int32_t *nt);
 /* Average device time per launch of each kernel over the last eqd_run,
  * measured with HIP events on the handle's stream (milliseconds), and the
  * number of launches.  names: "element", "element_pml", "fault", "node",
- * "node_pml", "station". */
+ * "node_pml", "station" (the step start: clock and off-fault stations; it
+ * has launches of its own only in steps without regular elements, otherwise
+ * it runs in the first blocks of the "element" launch). */
 int eqd_kernel_time(eqd_handle *h, const char *name, double *avg_ms, int64_t *launches);
 
 /* Per-launch event timing used by eqd_kernel_time: 0 off, k >= 1 on every

@@ -64,6 +64,28 @@ def test_graph_replay_matches_eager(name, precision):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("name,precision", [("box_rsf4", "fp64"), ("box_q", "fp32"), ("box_tp", "fp64")])
+def test_step_start_in_element_launch_matches_own_launch(name, precision, monkeypatch):
+    # the clock advance and the off-fault stations carried by the first
+    # blocks of the regular-element launch (default) and launched on their
+    # own (EQD_STEP_START_LAUNCH=1): the same clock, station rows and state,
+    # over runs split so that the last-step flag is set more than once
+    res = []
+    for own in ("0", "1"):
+        monkeypatch.setenv("EQD_STEP_START_LAUNCH", own)
+        bundle, sc, _ = load_golden(name)
+        s = Solver(bundle, sc, precision)
+        n = int(sc["nstep"])
+        s.run(n // 3)
+        s.run(n - n // 3)
+        assert s.clock()[0] == n
+        res.append(dict(outputs(s), stations=s.get("OffFaultStGramSCEC")))
+        s.close()
+    assert_same(res[0], res[1], f"{name} {precision} step start in the element launch vs its own launch")
+    assert np.any(res[0]["stations"] != 0) and np.array_equal(res[0]["stations"], res[1]["stations"])
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("name,precision", [("box_rsf4", "fp64"), ("box_dip", "fp64"), ("box_rough", "fp32"),
                                             ("box_q", "fp32")])
 def test_slot_major_node_gather_matches_csr(name, precision, monkeypatch):
