@@ -343,8 +343,8 @@ int eqd_tune(eqd_handle *h, int32_t steps, char *report, int64_t report_cap);
 
 /* on = 1: eqd_run captures one step as a HIP graph (once) and replays it;
  * on = 0 (the default): eqd_run launches every step's kernels, which measured
- * faster on this ROCm for one scenario (432,000 hexahedra: 0.100 against
- * 0.114 ms per FP32 step; 3.46 M hexahedra: 0.498 against 0.511 ms).
+ * faster on this ROCm for one scenario (432,000 hexahedra: 0.094 against
+ * 0.105 ms per FP32 step; 3.46 M hexahedra: 0.491 against 0.502 ms).
  * Per-kernel timing (eqd_set_timing) runs steps uncaptured.
  * Replayed and launched steps compute the same bits. */
 int eqd_set_graph(eqd_handle *h, int32_t on);
