@@ -13,11 +13,15 @@ KEYS = ["fric", "fnft", "velArr", "dispArr", "v1", "stressArr", "onFaultQuantHis
 
 
 def outputs(s):
-    return {k: s.get(k) for k in KEYS}
+    out = {k: s.get(k) for k in KEYS}
+    # the per-step hypocentre record (showSourceDynamics) of every step run
+    out["hypocentre"] = s.hypocentre_get(1, s.clock()[0])
+    return out
 
 
 def assert_same(a, b, tag):
-    for k in KEYS:
+    assert a.keys() == b.keys(), tag
+    for k in a:
         x, y = a[k], b[k]
         same = np.array_equal(x, y, equal_nan=True)
         if not same:
