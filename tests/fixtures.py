@@ -114,6 +114,7 @@ def fortran_real(tok: str) -> float:
     """A number in Fortran E format, also where the runtime drops the 'E' of
     a three-digit exponent (0.1234567-123)."""
     import re
+    tok = tok.strip()
     m = re.fullmatch(r"([-+]?\d*\.\d*)([-+]\d+)", tok)
     return float(m.group(1) + "E" + m.group(2)) if m else float(tok)
 

@@ -109,3 +109,31 @@ def test_gpu_hypocentre_series_matches_oracle(name):
     print(f"[{name}] hypocentre series vs oracle: worst {err.max():.2e} per column "
           + np.array2string(err, precision=1), flush=True)
     assert err.max() <= 1e-9
+
+
+def test_block_parser_reads_the_reference_format():
+    # helpers of the tests above on a hand-written stdout: two progress lines
+    # around one block; a Fortran E15.7 number whose three-digit exponent
+    # drops the 'E'; the unit of the 7th printed digit
+    from fixtures import fortran_real
+    assert fortran_real("0.1234567E+02") == 12.34567
+    assert fortran_real("-0.4166667-300") == -0.4166667e-300
+    assert np.allclose(printed_unit([12.34567, 0.1e-2, 0.0]), [1e-5, 1e-9, 0.0], rtol=1e-12)
+    lines = [" =     Current time in dynamic rupture                               =",
+             " Hypocenter dynamics",
+             " X, Y, Z      (m)     0.1000000E+01 -0.2000000E+01  0.3000000E+01",
+             " TimeElapsed  (s)     0.5000000E-02",
+             " n,s,d tract  (MPa)  -0.6000000E+02  0.3000000E+02  0.0000000E+00",
+             " state_normal (MPa)   0.0000000E+00",
+             " n,s,d slip   (m)     0.0000000E+00  0.1000000E-01  0.0000000E+00",
+             " s,d, peak sr (m/s)   0.2000000E+01  0.0000000E+00  0.2000000E+01",
+             " cummul slip  (m)     0.1000000E-01",
+             " sw_fs, sw_fd, sw_D0  0.6000000E+00  0.4500000E+00  0.3000000E+00",
+             " rsf_a, rsf_b         0.1000000E-01  0.1200000E-01",
+             " rsf_state            0.5000000-300",
+             " Nuc add tau0 (MPa)   0.4000000E+02",
+             " =     Current time in dynamic rupture (GPU)                         ="]
+    rows, progress = hypocentre_blocks("\n".join(lines))
+    assert progress == [0, 1] and rows.shape == (1, 22)
+    assert list(rows[0, :5]) == [1.0, -2.0, 3.0, 0.005, -60.0]
+    assert rows[0, 20] == 0.5e-300 and rows[0, 21] == 40.0
