@@ -20,7 +20,10 @@ A "step" is one explicit time step (src/driver.f90:9-34) over the whole mesh,
 with the mesh resident in HBM.  Prints one JSON line:
   value       Mdof-steps/s = 3 * N_nodes * steps / t / 1e6 (N_nodes = grid nodes +
               split master nodes; PML split components are not counted)
-  roofline    the run's dominant kernel (largest total time): its algorithmic
+  roofline    the run's dominant bulk kernel (largest total time among the
+              HBM-bound element and node kernels; the fault kernel is
+              latency-bound, runs beside the node kernels and has no byte
+              model): its algorithmic
               bytes per launch (kernel_bytes: every array it needs, in the layout
               the library chose for the mesh, moved once) divided by its average
               launch time, measured with HIP events its own dispatches record
@@ -222,7 +225,8 @@ def measure_copy_gbs(nbytes: int = 1 << 32, reps: int = 5) -> float:
 
 
 def roofline(run, kbytes: dict, precision: str, workload: str, ms_per_step: float, copy_gbs: float):
-    """Roofline of the run's dominant kernel (largest total time): its
+    """Roofline of the run's dominant bulk kernel (largest total time among
+    the kernels with a byte model: element, PML element, node, PML node): its
     algorithmic bytes per launch (kernel_bytes) over its average launch time
     from the HIP events its own dispatches record; beside it the whole step's
     algorithmic bytes over the measured time per step, and the measured copy
