@@ -23,10 +23,9 @@ with the mesh resident in HBM.  Prints one JSON line:
   roofline    the run's dominant bulk kernel (largest total time among the
               HBM-bound element and node kernels; the fault kernel is
               latency-bound, runs beside the node kernels and has no byte
-              model): its algorithmic
-              bytes per launch (kernel_bytes: every array it needs, in the layout
-              the library chose for the mesh, moved once) divided by its average
-              launch time, measured with HIP events its own dispatches record
+              model): its algorithmic bytes per launch (kernel_bytes: every
+              array it needs, in the layout the library chose for the mesh,
+              moved once) divided by its average launch time, measured with HIP events its own dispatches record
               on every --timing-every-th timed step; peak = 8 TB/s, beside a
               measured device copy;
               and the whole step's algorithmic bytes over the time per step.
