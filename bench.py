@@ -25,10 +25,11 @@ with the mesh resident in HBM.  Prints one JSON line:
               latency-bound, runs beside the node kernels and has no byte
               model): its algorithmic bytes per launch (kernel_bytes: every
               array it needs, in the layout the library chose for the mesh,
-              moved once) divided by its average launch time, measured with HIP events its own dispatches record
-              on every --timing-every-th timed step; peak = 8 TB/s, beside a
-              measured device copy;
-              and the whole step's algorithmic bytes over the time per step.
+              moved once) divided by its average launch time, measured with
+              HIP events its own dispatches record on every
+              --timing-every-th timed step; peak = 8 TB/s, beside a measured
+              device copy; and the whole step's algorithmic bytes over the
+              time per step.
   cpu_baseline the reference Fortran (oracle/_ref/eqd_ref) on MPI ranks on the
               host's cores, on a bounded sample of the same workload.
 """
